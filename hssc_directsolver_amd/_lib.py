@@ -1,0 +1,94 @@
+"""ctypes binding of libhssc.so (the C ABI declared in include/hssc.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhssc.so"
+
+HSSC_OK = 0
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "SINGULAR", 4: "RANK_OVERFLOW",
+          5: "HIP_ERROR", 6: "NOT_CONVERGED", 7: "UNSUPPORTED"}
+POT = {"SL_L_2D": 0, "SL_H_2D": 1}
+PREC = {"fp64": 0, "fp32": 1}
+
+
+class HsscParams(C.Structure):
+    """Mirror of struct hssc_params (include/hssc.h)."""
+    _fields_ = [
+        ("flag_pot", C.c_int32), ("kh", C.c_double), ("sym", C.c_int32), ("Ng", C.c_int32),
+        ("np", C.c_int32), ("acc", C.c_double), ("layers", C.c_int32), ("n_cut", C.c_int32),
+        ("transinv", C.c_int32), ("depth", C.c_int32), ("h", C.c_double),
+        ("max_particles", C.c_int32), ("interform", C.c_int32), ("k_cut", C.c_int32),
+        ("dr_weights_re", C.c_double), ("dr_weights_im", C.c_double),
+        ("factor_precision", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+# (name, restype, argtypes) of every entry point of include/hssc.h
+_P = C.c_void_p
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+_PP = C.POINTER(HsscParams)
+SIGNATURES = [
+    ("hssc_last_error", C.c_char_p, []),
+    ("hssc_version", C.c_char_p, []),
+    ("hssc_params_init", C.c_int, [_PP, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_double, C.c_int32, C.c_int32, C.c_int32]),
+    ("hssc_params_set_dr_weights", C.c_int, [_PP, C.c_double, C.c_double]),
+    ("hssc_params_npoints", C.c_int64, [_PP]),
+    ("hssc_source_points", C.c_int, [_PP, _DP]),
+    ("hssc_kernel_eval", C.c_int, [_PP, _DP, C.c_int64, _DP, C.c_int64, _DP, C.c_int64]),
+    ("hssc_hss2d_build_tree", C.c_int, [_PP, C.POINTER(_P)]),
+    ("hssc_hss2d_build_index_sets", C.c_int, [_PP, C.POINTER(_P)]),
+    ("hssc_tree_depth", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("hssc_tree_level_info", C.c_int, [_P, C.c_int32, _I64P, _I64P, _I64P, _I64P, _I64P, _I64P]),
+    ("hssc_tree_get_isrc", C.c_int, [_P, C.c_int64, _I64P]),
+    ("hssc_tree_get_isk", C.c_int, [_P, C.c_int64, _I64P]),
+    ("hssc_tree_get_J", C.c_int, [_P, C.c_int64, _I64P]),
+    ("hssc_tree_get_proxy", C.c_int, [_P, C.c_int64, _DP]),
+    ("hssc_tree_get_T", C.c_int, [_P, C.c_int64, _DP]),
+    ("hssc_tree_destroy", None, [_P]),
+    ("hssc_levhss", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    ("hssc_hss2d_invert", C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    ("hssc_inverse_get_FI", C.c_int, [_P, C.c_int64, _DP]),
+    ("hssc_inverse_get_E", C.c_int, [_P, C.c_int64, _DP]),
+    ("hssc_inverse_destroy", None, [_P]),
+    ("hssc_hss2d_compress", C.c_int, [_P, C.POINTER(_P)]),
+    ("hssc_operator_destroy", None, [_P]),
+    ("hssc_hss2d_applyinv", C.c_int, [_P, _DP, _DP, C.c_int64]),
+    ("hssc_hss2d_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),
+    ("hssc_hss2d_applyinv_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("hssc_hss2d_apply_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("hssc_hss2d_solve_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int32,
+                                          C.POINTER(C.c_int32), _DP, _P]),
+]
+
+_lib = None
+
+
+class HsscError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"hssc error {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Loads libhssc.so once; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: run hssc_directsolver_amd.build.build()")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != HSSC_OK:
+        raise HsscError(status, lib().hssc_last_error().decode())
+    return status

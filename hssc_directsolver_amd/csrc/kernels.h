@@ -1,0 +1,77 @@
+// Host-side launchers of the HIP kernels (kernels.hip).  All launches are asynchronous on
+// the given stream; arrays are column-major; s* arguments are per-batch element strides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hssc {
+namespace k {
+
+// A(r,c) = K(R_r, C_c) for the 2D Laplace single-layer kernel with per-point weights.
+template <typename T>
+void gen_block(hipStream_t st, int64_t m, int64_t n, int64_t batch, const double* Rx,
+               const double* Ry, const double* Rw, int64_t sR, const double* Cx, const double* Cy,
+               const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, double h);
+
+// Parent F = [E1 K12; K12^T E2] assembled directly in the parent's J order.
+template <typename T>
+void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int32_t* J,
+                int64_t sJ, const double* Px, const double* Py, const double* Pw, int64_t sP,
+                const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF, double h);
+
+template <typename T>
+void gemm(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
+          const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C,
+          int64_t ldc, int64_t sC, int64_t batch);
+
+template <typename T>
+void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
+               const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta,
+               T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
+               int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn);
+
+// In-place inverse of a batch of n x n matrices.
+template <typename T>
+void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
+
+constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
+
+template <typename T>
+void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
+              T* Tf, int64_t sT, int64_t batch);
+template <typename T>
+void qr_copyV(hipStream_t st, int64_t M, int64_t j0, int jb, const T* A, int64_t lda, int64_t sA,
+              T* V, int64_t ldv, int64_t sV, int64_t batch);
+template <typename T>
+void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
+                 int64_t sW, int64_t batch);
+
+// y = alpha op(A) x + gamma z ; work needs batch * ceil(inner/512) * out elements.
+template <typename T>
+void gemv(hipStream_t st, bool trans, int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
+          int64_t sA, const T* x, int64_t sx, T gamma, const T* z, int64_t sz, T* y, int64_t sy,
+          int64_t batch, T* work);
+constexpr int64_t kGemvChunk = 512;
+
+template <typename TS, typename TD>
+void gather(hipStream_t st, int64_t n, int64_t m, const int32_t* idx, int64_t sidx, const TS* src,
+            int64_t ld_src, int64_t s_src, TD* dst, int64_t ld_dst, int64_t s_dst, int64_t batch);
+template <typename TS, typename TD>
+void scatter(hipStream_t st, int64_t n, int64_t m, const int32_t* idx, int64_t sidx, const TS* src,
+             int64_t ld_src, int64_t s_src, TD* dst, int64_t ld_dst, int64_t s_dst, int64_t batch,
+             bool accumulate);
+template <typename TX, typename TY>
+void axpby(hipStream_t st, int64_t len, TY a, const TX* x, TY b, TY* y);
+template <typename TS, typename TD>
+void copy2d(hipStream_t st, int64_t m, int64_t n, const TS* src, int64_t lds, int64_t ss, TD* dst,
+            int64_t ldd, int64_t sd, int64_t batch);
+template <typename T>
+void add2d(hipStream_t st, int64_t m, int64_t n, T a, const T* s1, int64_t ld1, int64_t ss1, T b,
+           const T* s2, int64_t ld2, int64_t ss2, T* dst, int64_t ldd, int64_t sd, int64_t batch);
+template <typename T>
+void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks);
+
+}  // namespace k
+}  // namespace hssc

@@ -1,0 +1,989 @@
+// GPU orchestration of the HSS2D compress / invert / apply path and the C ABI (hssc.h).
+//
+// Level-by-level batched launches: every box of a level (or the single per-level operator
+// of a translation-invariant tree) is processed by one launch of each kernel.  Matrices of
+// a level are stored as strided batches in J order (skeleton first), so the skeleton /
+// residual blocks of F^-1 used by E = inv(R F^-1 R^T) and by the solve sweeps are plain
+// sub-blocks.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <type_traits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace hssc {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+namespace {
+
+struct HipError {
+  int status;
+};
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      g_last_error = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+      throw HipError{e_ == hipErrorOutOfMemory ? HSSC_ERR_OUT_OF_MEMORY : HSSC_ERR_HIP};  \
+    }                                                                                     \
+  } while (0)
+
+template <typename T>
+T* dalloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dupload(const std::vector<T>& h) {
+  T* d = dalloc<T>(h.size());
+  if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Runs f and maps exceptions to status codes.
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return fail(HSSC_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+}
+
+int check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(HSSC_ERR_HIP, "no HIP device is available");
+  return HSSC_OK;
+}
+
+// y = alpha * op(A_key) x_box + gamma * z_box for every box of a level.
+//   A: per-key matrix (rows x cols, leading dimension lda, key stride sA; sA = 0 for TI)
+//   x, y, z: per-box blocks with leading dimensions ldx, ldy, ldz and box stride ld*m.
+// TI levels share A, so all boxes form one GEMM with nbox*m columns.
+template <typename T>
+void level_matvec(hipStream_t st, bool trans, int64_t rows, int64_t cols, T alpha, const T* A,
+                  int64_t lda, int64_t sA, const T* x, int64_t ldx, T gamma, const T* z,
+                  int64_t ldz, T* y, int64_t ldy, int64_t nbox, int64_t m, bool shared,
+                  T* work) {
+  const int64_t out = trans ? cols : rows, inner = trans ? rows : cols;
+  if (out <= 0) return;
+  if (!shared && m == 1) {
+    k::gemv<T>(st, trans, rows, cols, alpha, A, lda, sA, x, ldx, gamma, z, ldz, y, ldy, nbox,
+               work);
+    return;
+  }
+  if (gamma != T(0) || inner <= 0) {
+    if (z != y || gamma != T(1))
+      k::add2d<T>(st, out, nbox * m, gamma, z, ldz, 0, T(0), z, ldz, 0, y, ldy, 0, 1);
+  }
+  if (inner <= 0) return;
+  const T beta = gamma != T(0) ? T(1) : T(0);
+  if (shared) {
+    k::gemm<T>(st, trans, false, out, nbox * m, inner, alpha, A, lda, 0, x, ldx, 0, beta, y, ldy,
+               0, 1);
+  } else {
+    k::gemm<T>(st, trans, false, out, m, inner, alpha, A, lda, sA, x, ldx, ldx * m, beta, y, ldy,
+               ldy * m, nbox);
+  }
+}
+
+int64_t gemv_work_elems(int64_t out, int64_t inner, int64_t batch) {
+  return batch * ((inner + k::kGemvChunk - 1) / k::kGemvChunk + 1) * out;
+}
+
+}  // namespace
+}  // namespace hssc
+
+using namespace hssc;
+
+// ---------------------------------------------------------------------------------------
+// Object definitions
+// ---------------------------------------------------------------------------------------
+
+hssc_tree::~hssc_tree() {
+  for (auto& L : levels) {
+    dfree(L.dJ), dfree(L.dJinv), dfree(L.dPx), dfree(L.dPy), dfree(L.dPw);
+    dfree(L.dPNx), dfree(L.dPNy), dfree(L.dPNw);
+    dfree(L.dXpx), dfree(L.dXpy), dfree(L.dXpw), dfree(L.dT), dfree(L.dT32);
+  }
+  dfree(dLeafG);
+}
+
+namespace hssc {
+// Per-RHS-count solve workspace (levels x {Q, PU, PD}) plus scratch.
+template <typename T>
+struct SolveWork {
+  int64_t m = 0;
+  std::vector<T*> Q, PU, PD;
+  T* S1 = nullptr;
+  T* S2 = nullptr;
+  T* work = nullptr;
+  void release() {
+    for (T* p : Q) dfree(p);
+    for (T* p : PU) dfree(p);
+    for (T* p : PD) dfree(p);
+    Q.clear(), PU.clear(), PD.clear();
+    dfree(S1), dfree(S2), dfree(work);
+    S1 = S2 = work = nullptr;
+    m = 0;
+  }
+  void ensure(const hssc_tree* t, int64_t mm) {
+    if (m == mm) return;
+    release();
+    const int D = t->depth;
+    Q.assign(D + 1, nullptr), PU.assign(D + 1, nullptr), PD.assign(D + 1, nullptr);
+    int64_t smax = 0, wmax = 0;
+    for (int l = 0; l <= D; ++l) {
+      const Level& L = t->levels[l];
+      Q[l] = dalloc<T>(L.nbox * L.n * mm);
+      PU[l] = dalloc<T>(L.nbox * L.k * mm);
+      PD[l] = dalloc<T>(L.nbox * L.k * mm);
+      smax = std::max(smax, L.nbox * L.n * mm);
+      wmax = std::max(wmax, gemv_work_elems(L.n, L.n, L.nbox));
+    }
+    S1 = dalloc<T>(smax);
+    S2 = dalloc<T>(smax);
+    work = dalloc<T>(wmax);
+    m = mm;
+  }
+};
+}  // namespace hssc
+
+struct hssc_inverse {
+  hssc_tree* tree = nullptr;
+  int prec = HSSC_PREC_FP64;
+  std::vector<void*> FI, E;  // per level, nkey x n x n and nkey x k x k (J order)
+  mutable SolveWork<double> w64;
+  mutable SolveWork<float> w32;
+  ~hssc_inverse() {
+    for (void* p : FI) dfree(p);
+    for (void* p : E) dfree(p);
+    w64.release();
+    w32.release();
+  }
+};
+
+struct hssc_operator {
+  hssc_tree* tree = nullptr;
+  std::vector<double*> B;  // level depth: Bself (nkey x n x n, J order); others: B12 (kc x kc)
+  mutable SolveWork<double> w;
+  ~hssc_operator() {
+    for (double* p : B) dfree(p);
+    w.release();
+  }
+};
+
+namespace hssc {
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Tree upload and interpolation operators
+// ---------------------------------------------------------------------------------------
+
+void upload_tree(hssc_tree* t) {
+  HIPCHK(hipGetDevice(&t->device));
+  for (auto& L : t->levels) {
+    L.dJ = dupload(L.J), L.dJinv = dupload(L.Jinv);
+    L.dPx = dupload(L.Px), L.dPy = dupload(L.Py), L.dPw = dupload(L.Pw);
+    L.dPNx = dupload(L.PNx), L.dPNy = dupload(L.PNy), L.dPNw = dupload(L.PNw);
+    L.dXpx = dupload(L.Xpx), L.dXpy = dupload(L.Xpy), L.dXpw = dupload(L.Xpw);
+  }
+  t->dLeafG = dupload(t->leafG);
+  t->on_device = true;
+}
+
+// T = K(Xp, X_sk) \ K(Xp, X_rs) for every key of every level
+// (HSS2D/interpolation_operator.m:57-66 and the same operator to tolerance for :80-127),
+// by blocked Householder QR of the augmented M x n block [K11 K21] on its first k
+// columns, then back substitution with R11.  The root's T is never used by invert or the
+// sweeps (the root has no E), so it is not formed.
+void compute_interpolation(hssc_tree* t, hipStream_t st) {
+  const double h = t->p.h;
+  int64_t amax = 0, vmax = 0, wmax = 0;
+  for (int l = 1; l <= t->depth; ++l) {
+    const Level& L = t->levels[l];
+    if (L.n == L.k) continue;
+    amax = std::max(amax, L.nkey * L.M * L.n);
+    vmax = std::max(vmax, L.nkey * L.M * k::kQrNb);
+    wmax = std::max(wmax, L.nkey * k::kQrNb * L.n);
+  }
+  double* A = dalloc<double>(amax);
+  double* V = dalloc<double>(vmax);
+  double* W1 = dalloc<double>(wmax);
+  double* W2 = dalloc<double>(wmax);
+  const int64_t nb = k::kQrNb;
+  int64_t kmax = 1;
+  for (const Level& L : t->levels) kmax = std::max(kmax, L.nkey);
+  double* Tf = dalloc<double>(kmax * nb * nb);
+  for (int l = 1; l <= t->depth; ++l) {
+    Level& L = t->levels[l];
+    const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+    L.dT = dalloc<double>(E * kk * std::max<int64_t>(r, 1));
+    if (r == 0) continue;
+    if (M < kk) {
+      throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
+    }
+    const int64_t sA = M * n;
+    // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
+    k::gen_block<double>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
+                         h);
+    for (int64_t j0 = 0; j0 < kk; j0 += nb) {
+      const int jb = int(std::min(nb, kk - j0));
+      k::qr_panel<double>(st, M, j0, jb, A, M, sA, Tf, nb * nb, E);
+      const int64_t nc = n - j0 - jb;
+      if (nc <= 0) continue;
+      const int64_t Mp = M - j0;
+      k::qr_copyV<double>(st, M, j0, jb, A, M, sA, V, Mp, Mp * nb, E);
+      double* C = A + j0 + (j0 + jb) * M;
+      // W1 = V^T C ; W2 = Tf^T W1 ; C -= V W2
+      k::gemm<double>(st, true, false, jb, nc, Mp, 1.0, V, Mp, Mp * nb, C, M, sA, 0.0, W1, nb,
+                      nb * nc, E);
+      k::gemm<double>(st, true, false, jb, nc, jb, 1.0, Tf, nb, nb * nb, W1, nb, nb * nc, 0.0, W2,
+                      nb, nb * nc, E);
+      k::gemm<double>(st, false, false, Mp, nc, jb, -1.0, V, Mp, Mp * nb, W2, nb, nb * nc, 1.0, C,
+                      M, sA, E);
+    }
+    // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
+    double* C1 = A + kk * M;
+    const int64_t last = ((kk - 1) / nb) * nb;
+    for (int64_t p = last; p >= 0; p -= nb) {
+      const int bsz = int(std::min(nb, kk - p));
+      k::trinv_upper<double>(st, p, bsz, A, M, sA, Tf, nb * nb, E);
+      // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
+      k::gemm<double>(st, false, false, bsz, r, bsz, 1.0, Tf, bsz, nb * nb, C1 + p, M, sA, 0.0,
+                      C1 + p, M, sA, E);
+      if (p > 0)
+        k::gemm<double>(st, false, false, p, r, bsz, -1.0, A + p * M, M, sA, C1 + p, M, sA, 1.0,
+                        C1, M, sA, E);
+    }
+    k::copy2d<double, double>(st, kk, r, C1, M, sA, L.dT, kk, kk * r, E);
+    L.q = r;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  dfree(A), dfree(V), dfree(W1), dfree(W2), dfree(Tf);
+  t->has_T = true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Inversion (HSS2D/HSS2D_invert_fsym.m:73-195, dense F^-1 and E)
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+const T* level_T(const Level& L);
+template <>
+const double* level_T<double>(const Level& L) {
+  return L.dT;
+}
+template <>
+const float* level_T<float>(const Level& L) {
+  return L.dT32;
+}
+
+template <typename T>
+void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
+  const int D = t->depth;
+  const double h = t->p.h;
+  const bool TI = t->p.transinv != 0;
+  inv->FI.assign(D + 1, nullptr);
+  inv->E.assign(D + 1, nullptr);
+  if (std::is_same<T, float>::value) {
+    for (int l = 1; l <= D; ++l) {
+      Level& L = t->levels[l];
+      const int64_t r = L.n - L.k;
+      if (!L.dT32) L.dT32 = dalloc<float>(L.nkey * L.k * std::max<int64_t>(r, 1));
+      if (r > 0) k::axpby<double, float>(st, L.nkey * L.k * r, 1.0f, L.dT, 0.0f, L.dT32);
+    }
+  }
+  int64_t wmax = 0;
+  for (int l = 1; l <= D; ++l) {
+    const Level& L = t->levels[l];
+    wmax = std::max(wmax, L.nkey * L.k * (L.n - L.k));
+  }
+  T* Wk = dalloc<T>(wmax);
+  for (int l = D; l >= 0; --l) {
+    const Level& L = t->levels[l];
+    const int64_t n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+    T* FI = dalloc<T>(E * n * n);
+    inv->FI[l] = FI;
+    if (l == D) {
+      k::gen_block<T>(st, n, n, E, L.dPx, L.dPy, L.dPw, n, L.dPx, L.dPy, L.dPw, n, FI, n, n * n, h);
+    } else {
+      const Level& C = t->levels[l + 1];
+      const int64_t kc = C.k;
+      const int64_t sEbox = TI ? 0 : kc * kc, sEpar = TI ? 0 : 2 * kc * kc;
+      k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
+                       static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, h);
+    }
+    k::invert_batched<T>(st, n, FI, n, n * n, E);
+    if (l > 0) {
+      // EI = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T ;  E = inv(EI)
+      T* EI = dalloc<T>(E * kk * kk);
+      inv->E[l] = EI;
+      k::copy2d<T, T>(st, kk, kk, FI, n, n * n, EI, kk, kk * kk, E);
+      if (r > 0) {
+        const T* Tm = level_T<T>(L);
+        k::copy2d<T, T>(st, kk, r, FI + kk * n, n, n * n, Wk, kk, kk * r, E);
+        k::gemm<T>(st, false, false, kk, r, r, T(1), Tm, kk, kk * r, FI + kk + kk * n, n, n * n,
+                   T(1), Wk, kk, kk * r, E);
+        k::gemm<T>(st, false, false, kk, kk, r, T(1), Tm, kk, kk * r, FI + kk, n, n * n, T(1), EI,
+                   kk, kk * kk, E);
+        k::gemm<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI, kk,
+                   kk * kk, E);
+      }
+      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  dfree(Wk);
+}
+
+// ---------------------------------------------------------------------------------------
+// Tree sweeps
+// ---------------------------------------------------------------------------------------
+
+// Forms the J-ordered input of every box of level l (< depth) from the children's
+// upward vectors PU[l+1] (child blocks kc x m, two per parent) into Q (n x m per box).
+template <typename T>
+void form_parent_q(const hssc_tree* t, int l, const T* PUc, T* Q, T* S, int64_t m,
+                   hipStream_t st) {
+  const Level& L = t->levels[l];
+  const int64_t n = L.n, kc = t->levels[l + 1].k;
+  const bool TI = t->p.transinv != 0;
+  // natural order: rows [0,kc) = child 1, [kc,n) = child 2
+  k::copy2d<T, T>(st, kc, m, PUc, kc, 2 * kc * m, S, n, n * m, L.nbox);
+  k::copy2d<T, T>(st, kc, m, PUc + kc * m, kc, 2 * kc * m, S + kc, n, n * m, L.nbox);
+  k::gather<T, T>(st, n, m, L.dJ, TI ? 0 : n, S, n, n * m, Q, n, n * m, L.nbox);
+}
+
+// Splits J-ordered box vectors PHI (n x m) into the children's downward vectors PD[l+1].
+template <typename T>
+void split_to_children(const hssc_tree* t, int l, const T* PHI, T* PDc, T* S, int64_t m,
+                       hipStream_t st, bool accumulate_into_S = false) {
+  const Level& L = t->levels[l];
+  const int64_t n = L.n, kc = t->levels[l + 1].k;
+  const bool TI = t->p.transinv != 0;
+  k::scatter<T, T>(st, n, m, L.dJ, TI ? 0 : n, PHI, n, n * m, S, n, n * m, L.nbox,
+                   accumulate_into_S);
+  k::copy2d<T, T>(st, kc, m, S, n, n * m, PDc, kc, 2 * kc * m, L.nbox);
+  k::copy2d<T, T>(st, kc, m, S + kc, n, n * m, PDc + kc * m, kc, 2 * kc * m, L.nbox);
+}
+
+// HSS2D/HSS2D_applyinv_TI.m:63-225 and HSS2D_applyinv_fsym.m (Algorithm 6 of the paper).
+template <typename T>
+void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const double* X, double* Y,
+                   int64_t m, hipStream_t st) {
+  const hssc_tree* t = inv->tree;
+  const int D = t->depth;
+  const bool TI = t->p.transinv != 0;
+  const int64_t N = t->N;
+  w.ensure(t, m);
+  // upward pass
+  for (int l = D; l >= 1; --l) {
+    const Level& L = t->levels[l];
+    const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
+    const T* FI = static_cast<const T*>(inv->FI[l]);
+    const T* E = static_cast<const T*>(inv->E[l]);
+    const T* Tm = level_T<T>(L);
+    T* Q = w.Q[l];
+    if (l == D) k::gather<double, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, nb);
+    else form_parent_q<T>(t, l, w.PU[l + 1], Q, w.S1, m, st);
+    // psi = FI q
+    T* PSI = w.S2;
+    level_matvec<T>(st, false, n, n, T(1), FI, n, n * n, Q, n, T(0), nullptr, n, PSI, n, nb, m,
+                    TI, w.work);
+    // u = psi_s + T psi_r
+    T* U = w.S1;
+    if (r > 0)
+      level_matvec<T>(st, false, kk, r, T(1), Tm, kk, kk * r, PSI + kk, n, T(1), PSI, n, U, kk,
+                      nb, m, TI, w.work);
+    else
+      k::copy2d<T, T>(st, kk, nb * m, PSI, n, 0, U, kk, 0, 1);
+    // phi_up = E u
+    level_matvec<T>(st, false, kk, kk, T(1), E, kk, kk * kk, U, kk, T(0), nullptr, kk, w.PU[l],
+                    kk, nb, m, TI, w.work);
+  }
+  // root: phi = FI q
+  {
+    const Level& L = t->levels[0];
+    const int64_t n = L.n;
+    T* Q = w.Q[0];
+    if (D == 0) k::gather<double, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, 1);
+    else form_parent_q<T>(t, 0, w.PU[1], Q, w.S1, m, st);
+    T* PHI = w.S2;
+    level_matvec<T>(st, false, n, n, T(1), static_cast<const T*>(inv->FI[0]), n, n * n, Q, n,
+                    T(0), nullptr, n, PHI, n, 1, m, TI, w.work);
+    if (D == 0) k::scatter<T, double>(st, n, m, t->dLeafG, n, PHI, n, n * m, Y, N, 0, 1, false);
+    else split_to_children<T>(t, 0, PHI, w.PD[1], w.S1, m, st);
+  }
+  // downward pass
+  for (int l = 1; l <= D; ++l) {
+    const Level& L = t->levels[l];
+    const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
+    const T* FI = static_cast<const T*>(inv->FI[l]);
+    const T* E = static_cast<const T*>(inv->E[l]);
+    const T* Tm = level_T<T>(L);
+    T* Q = w.Q[l];
+    // wv = E phi_dn - phi_up
+    T* WV = w.S1;
+    level_matvec<T>(st, false, kk, kk, T(1), E, kk, kk * kk, w.PD[l], kk, T(-1), w.PU[l], kk, WV,
+                    kk, nb, m, TI, w.work);
+    // v = q + [wv ; T^T wv]
+    T* V = w.S2;
+    k::add2d<T>(st, kk, nb * m, T(1), Q, n, 0, T(1), WV, kk, 0, V, n, 0, 1);
+    if (r > 0)
+      level_matvec<T>(st, true, kk, r, T(1), Tm, kk, kk * r, WV, kk, T(1), Q + kk, n, V + kk, n,
+                      nb, m, TI, w.work);
+    // phi = FI v  (into Q, which is no longer needed)
+    level_matvec<T>(st, false, n, n, T(1), FI, n, n * n, V, n, T(0), nullptr, n, Q, n, nb, m, TI,
+                    w.work);
+    if (l == D) k::scatter<T, double>(st, n, m, t->dLeafG, n, Q, n, n * m, Y, N, 0, nb, false);
+    else split_to_children<T>(t, l, Q, w.PD[l + 1], w.S1, m, st);
+  }
+}
+
+// HSS2D/HSS2D_apply_TI.m:74-189 and HSS2D_apply_fsym.m: telescoping forward operator.
+void apply_impl(const hssc_operator* op, const double* X, double* Y, int64_t m, hipStream_t st) {
+  const hssc_tree* t = op->tree;
+  SolveWork<double>& w = op->w;
+  const int D = t->depth;
+  const bool TI = t->p.transinv != 0;
+  const int64_t N = t->N;
+  w.ensure(t, m);
+  // upward: phi_up = q_s + T q_r
+  for (int l = D; l >= 1; --l) {
+    const Level& L = t->levels[l];
+    const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
+    double* Qd = w.Q[l];
+    if (l == D) k::gather<double, double>(st, n, m, t->dLeafG, n, X, N, 0, Qd, n, n * m, nb);
+    else form_parent_q<double>(t, l, w.PU[l + 1], Qd, w.S1, m, st);
+    if (r > 0)
+      level_matvec<double>(st, false, kk, r, 1.0, L.dT, kk, kk * r, Qd + kk, n, 1.0, Qd, n,
+                           w.PU[l], kk, nb, m, TI, w.work);
+    else
+      k::copy2d<double, double>(st, kk, nb * m, Qd, n, 0, w.PU[l], kk, 0, 1);
+  }
+  if (D == 0) {
+    const Level& L = t->levels[0];
+    k::gather<double, double>(st, L.n, m, t->dLeafG, L.n, X, N, 0, w.Q[0], L.n, L.n * m, 1);
+    level_matvec<double>(st, false, L.n, L.n, 1.0, op->B[0], L.n, L.n * L.n, w.Q[0], L.n, 0.0,
+                         nullptr, L.n, w.S1, L.n, 1, m, TI, w.work);
+    k::scatter<double, double>(st, L.n, m, t->dLeafG, L.n, w.S1, L.n, L.n * m, Y, N, 0, 1, false);
+    return;
+  }
+  // downward
+  for (int l = 0; l < D; ++l) {
+    const Level& L = t->levels[l];
+    const Level& C = t->levels[l + 1];
+    const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox, kc = C.k;
+    const double* B12 = op->B[l];
+    const int64_t sB = TI ? 0 : kc * kc;
+    double* S = w.S1;  // natural-order eta (n x m per box)
+    if (l > 0) {
+      // eta_J = [phi_dn ; T^T phi_dn], then natural order
+      double* EJ = w.S2;
+      k::copy2d<double, double>(st, kk, nb * m, w.PD[l], kk, 0, EJ, n, 0, 1);
+      if (r > 0)
+        level_matvec<double>(st, true, kk, r, 1.0, L.dT, kk, kk * r, w.PD[l], kk, 0.0, nullptr,
+                             n, EJ + kk, n, nb, m, TI, w.work);
+      k::scatter<double, double>(st, n, m, L.dJ, TI ? 0 : n, EJ, n, n * m, S, n, n * m, nb,
+                                 false);
+    }
+    // children: PD[2e] = B12 PU[2e+1] + eta[0:kc];  PD[2e+1] = B12^T PU[2e] + eta[kc:n]
+    double* PUc = w.PU[l + 1];
+    double* PDc = w.PD[l + 1];
+    for (int s = 0; s < 2; ++s) {
+      const double* xin = PUc + (1 - s) * kc * m;
+      double* yout = PDc + s * kc * m;
+      const double* z = S + s * kc;
+      const double gamma = l > 0 ? 1.0 : 0.0;
+      // per parent box: x block stride 2*kc*m, y block stride 2*kc*m, z stride n*m
+      if (!TI && m == 1) {
+        k::gemv<double>(st, s == 1, kc, kc, 1.0, B12, kc, sB, xin, 2 * kc, gamma, z, n, yout,
+                        2 * kc, nb, w.work);
+      } else {
+        if (l > 0)
+          k::add2d<double>(st, kc, m, 1.0, z, n, n * m, 0.0, z, n, n * m, yout, kc, 2 * kc * m, nb);
+        k::gemm<double>(st, s == 1, false, kc, m, kc, 1.0, B12, kc, sB, xin, kc, 2 * kc * m, gamma,
+                        yout, kc, 2 * kc * m, nb);
+      }
+    }
+  }
+  // leaves: phi_J = Bself q_J + [phi_dn ; T^T phi_dn]
+  {
+    const Level& L = t->levels[D];
+    const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
+    double* EJ = w.S2;
+    k::copy2d<double, double>(st, kk, nb * m, w.PD[D], kk, 0, EJ, n, 0, 1);
+    if (r > 0)
+      level_matvec<double>(st, true, kk, r, 1.0, L.dT, kk, kk * r, w.PD[D], kk, 0.0, nullptr, n,
+                           EJ + kk, n, nb, m, TI, w.work);
+    level_matvec<double>(st, false, n, n, 1.0, op->B[D], n, n * n, w.Q[D], n, 1.0, EJ, n, w.S1,
+                         n, nb, m, TI, w.work);
+    k::scatter<double, double>(st, n, m, t->dLeafG, n, w.S1, n, n * m, Y, N, 0, nb, false);
+  }
+}
+
+double device_norm(const double* x, int64_t len, hipStream_t st, double* part_d) {
+  const int nblk = 256;
+  k::sumsq<double>(st, len, x, part_d, nblk);
+  double part[256];
+  HIPCHK(hipMemcpyAsync(part, part_d, sizeof(part), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double s = 0;
+  for (int i = 0; i < nblk; ++i) s += part[i];
+  return std::sqrt(s);
+}
+
+int applyinv_dispatch(const hssc_inverse* inv, const double* X, double* Y, int64_t m,
+                      hipStream_t st) {
+  if (inv->prec == HSSC_PREC_FP64)
+    applyinv_impl<double>(inv, inv->w64, X, Y, m, st);
+  else
+    applyinv_impl<float>(inv, inv->w32, X, Y, m, st);
+  return HSSC_OK;
+}
+
+}  // namespace
+}  // namespace hssc
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char* hssc_last_error(void) { return hssc::g_last_error.c_str(); }
+
+const char* hssc_version(void) { return "hssc-mi355x 0.1 (gfx950)"; }
+
+int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_t Ng,
+                     int32_t np, double acc, int32_t lay, int32_t n_cut, int32_t TI) {
+  if (!p) return fail(HSSC_ERR_INVALID_ARG, "null params");
+  if (pot != HSSC_POT_SL_L_2D && pot != HSSC_POT_SL_H_2D)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown kernel");
+  if (Ng < 1 || np < 2 || (np & (np - 1)) != 0)
+    return fail(HSSC_ERR_INVALID_ARG, "need Ng >= 1 and np a power of two >= 2");
+  if (!(acc > 0 && acc < 1)) return fail(HSSC_ERR_INVALID_ARG, "acc must be in (0,1)");
+  if (lay < 1) return fail(HSSC_ERR_INVALID_ARG, "layers must be >= 1");
+  std::memset(p, 0, sizeof(*p));
+  p->flag_pot = pot;
+  p->kh = kh;
+  p->sym = sym;
+  p->Ng = Ng;
+  p->np = np;
+  p->acc = acc;
+  p->layers = lay;
+  p->n_cut = n_cut;
+  p->transinv = TI ? 1 : 0;
+  // Common/HSS_tree_parameters.m:113,118: depth = round(2(log2 np - 1) + 3) - 1
+  const int nlevels = int(std::lround(2 * (std::log(double(np)) / std::log(2.0) - 1) + 3));
+  p->depth = nlevels - 1;
+  p->h = 2.0 / (double(Ng) * np);
+  p->max_particles = Ng * Ng;
+  p->interform = HSSC_INTER_LOWRANK;
+  p->k_cut = 2500;
+  p->factor_precision = HSSC_PREC_FP64;
+  return HSSC_OK;
+}
+
+int hssc_params_set_dr_weights(hssc_params* p, double re, double im) {
+  if (!p) return fail(HSSC_ERR_INVALID_ARG, "null params");
+  p->dr_weights_re = re;
+  p->dr_weights_im = im;
+  return HSSC_OK;
+}
+
+int64_t hssc_params_npoints(const hssc_params* p) {
+  if (!p) return -1;
+  return int64_t(p->np) * p->np * p->Ng * p->Ng;
+}
+
+int hssc_source_points(const hssc_params* p, double* X) {
+  if (!p || !X) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  hssc::source_points(p->Ng, p->np, X);
+  return HSSC_OK;
+}
+
+int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const double* X2,
+                     int64_t n, double* A, int64_t lda) {
+  if (!p || !X1 || !X2 || !A || m < 0 || n < 0 || lda < m)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid kernel_eval arguments");
+  if (p->flag_pot != HSSC_POT_SL_L_2D)
+    return fail(HSSC_ERR_UNSUPPORTED, "only SL_L_2D is implemented on the GPU");
+  if (int s = check_device()) return s;
+  return guarded([&]() {
+    std::vector<double> w1(m), w2(n), x1(m), y1(m), x2(n), y2(n);
+    for (int64_t i = 0; i < m; ++i) x1[i] = X1[i], y1[i] = X1[m + i], w1[i] = point_weight(*p, x1[i], y1[i]);
+    for (int64_t j = 0; j < n; ++j) x2[j] = X2[j], y2[j] = X2[n + j], w2[j] = point_weight(*p, x2[j], y2[j]);
+    double *dx1 = dupload(x1), *dy1 = dupload(y1), *dw1 = dupload(w1);
+    double *dx2 = dupload(x2), *dy2 = dupload(y2), *dw2 = dupload(w2);
+    double* dA = dalloc<double>(std::max<int64_t>(1, m * n));
+    k::gen_block<double>(0, m, n, 1, dx1, dy1, dw1, 0, dx2, dy2, dw2, 0, dA, std::max<int64_t>(m, 1),
+                         0, p->h);
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy2D(A, lda * sizeof(double), dA, m * sizeof(double), m * sizeof(double), n,
+                       hipMemcpyDeviceToHost));
+    dfree(dx1), dfree(dy1), dfree(dw1), dfree(dx2), dfree(dy2), dfree(dw2), dfree(dA);
+    return HSSC_OK;
+  });
+}
+
+int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
+  if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *tree = nullptr;
+  if (p->flag_pot != HSSC_POT_SL_L_2D)
+    return fail(HSSC_ERR_UNSUPPORTED, "only SL_L_2D trees are implemented");
+  return guarded([&]() {
+    auto* t = new hssc_tree();
+    const int s = build_index_sets(*p, t);
+    if (s != HSSC_OK) {
+      delete t;
+      return s;
+    }
+    *tree = t;
+    return HSSC_OK;
+  });
+}
+
+int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree) {
+  if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *tree = nullptr;
+  if (int s = check_device()) return s;
+  int s = hssc_hss2d_build_index_sets(p, tree);
+  if (s != HSSC_OK) return s;
+  hssc_tree* t = *tree;
+  s = guarded([&]() {
+    upload_tree(t);
+    compute_interpolation(t, 0);
+    return HSSC_OK;
+  });
+  if (s != HSSC_OK) {
+    delete t;
+    *tree = nullptr;
+  }
+  return s;
+}
+
+int hssc_tree_depth(const hssc_tree* t, int32_t* depth) {
+  if (!t || !depth) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *depth = t->depth;
+  return HSSC_OK;
+}
+
+int hssc_tree_level_info(const hssc_tree* t, int32_t lev, int64_t* n, int64_t* k, int64_t* M,
+                         int64_t* nbox, int64_t* nkey, int64_t* q) {
+  if (!t || lev < 0 || lev > t->depth) return fail(HSSC_ERR_INVALID_ARG, "bad level");
+  const Level& L = t->levels[lev];
+  if (n) *n = L.n;
+  if (k) *k = L.k;
+  if (M) *M = L.M;
+  if (nbox) *nbox = L.nbox;
+  if (nkey) *nkey = L.nkey;
+  if (q) *q = L.q;
+  return HSSC_OK;
+}
+
+static int box_level(const hssc_tree* t, int64_t box, int* lev) {
+  if (!t || box < 1 || box >= (int64_t(1) << (t->depth + 1)))
+    return fail(HSSC_ERR_INVALID_ARG, "box number out of range");
+  int l = 0;
+  while ((int64_t(1) << (l + 1)) <= box) ++l;
+  *lev = l;
+  return HSSC_OK;
+}
+
+static int64_t box_key(const hssc_tree* t, int64_t box, int lev) {
+  return t->p.transinv ? 0 : box - (int64_t(1) << lev);
+}
+
+int hssc_tree_get_isrc(const hssc_tree* t, int64_t box, int64_t* out) {
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  std::copy(t->Isrc[box].begin(), t->Isrc[box].end(), out);
+  return HSSC_OK;
+}
+
+int hssc_tree_get_isk(const hssc_tree* t, int64_t box, int64_t* out) {
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  std::copy(t->Isk[box].begin(), t->Isk[box].end(), out);
+  return HSSC_OK;
+}
+
+int hssc_tree_get_J(const hssc_tree* t, int64_t box, int64_t* out) {
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  const Level& L = t->levels[lev];
+  const int64_t key = box_key(t, box, lev);
+  for (int64_t a = 0; a < L.n; ++a) out[a] = L.J[key * L.n + a];
+  return HSSC_OK;
+}
+
+int hssc_tree_get_proxy(const hssc_tree* t, int64_t box, double* out) {
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  const Level& L = t->levels[lev];
+  const int64_t key = box_key(t, box, lev);
+  for (int64_t r = 0; r < L.M; ++r) {
+    out[r] = L.Xpx[key * L.M + r];
+    out[L.M + r] = L.Xpy[key * L.M + r];
+  }
+  return HSSC_OK;
+}
+
+int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out) {
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
+  const Level& L = t->levels[lev];
+  if (lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no interpolation operator");
+  const int64_t key = box_key(t, box, lev), cnt = L.k * (L.n - L.k);
+  if (cnt == 0) return HSSC_OK;
+  return guarded([&]() {
+    HIPCHK(hipMemcpy(out, L.dT + key * cnt, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return HSSC_OK;
+  });
+}
+
+void hssc_tree_destroy(hssc_tree* t) { delete t; }
+
+int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss) {
+  // HSS2D/levHSS.m:33-65: first level whose skeletons are all <= n_cut (TI: the level k).
+  if (!t || !lhss) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  int lev = 0;
+  int64_t n_min = t->levels[0].k;
+  if (!t->p.transinv && t->depth >= 1) n_min = std::max(n_min, t->levels[1].k);
+  while (n_min > n_cut && lev < t->depth - 1) {
+    ++lev;
+    n_min = t->levels[lev].k;  // uniform tree: every box of a level has the same k
+  }
+  if (lev > 0) *lhss = (lev % 2 == 1) ? lev - 1 : lev;
+  else *lhss = -1;
+  return HSSC_OK;
+}
+
+int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** out) {
+  if (!t || !out) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
+  if (precision != HSSC_PREC_FP64 && precision != HSSC_PREC_FP32)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown precision");
+  return guarded([&]() {
+    auto* inv = new hssc_inverse();
+    inv->tree = t;
+    inv->prec = precision;
+    try {
+      if (precision == HSSC_PREC_FP64) invert_impl<double>(t, inv, 0);
+      else invert_impl<float>(t, inv, 0);
+    } catch (...) {
+      delete inv;
+      throw;
+    }
+    *out = inv;
+    return HSSC_OK;
+  });
+}
+
+static int inverse_block(const hssc_inverse* inv, int64_t box, double* out, bool wantE) {
+  const hssc_tree* t = inv ? inv->tree : nullptr;
+  int lev;
+  if (int s = box_level(t, box, &lev)) return s;
+  if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
+  const Level& L = t->levels[lev];
+  if (wantE && lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no E");
+  const int64_t key = box_key(t, box, lev);
+  const int64_t n = wantE ? L.k : L.n;
+  return guarded([&]() {
+    std::vector<double> J(n * n);
+    const void* src = wantE ? inv->E[lev] : inv->FI[lev];
+    if (inv->prec == HSSC_PREC_FP64) {
+      HIPCHK(hipMemcpy(J.data(), static_cast<const double*>(src) + key * n * n, n * n * 8,
+                       hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(n * n);
+      HIPCHK(hipMemcpy(f.data(), static_cast<const float*>(src) + key * n * n, n * n * 4,
+                       hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n * n; ++i) J[i] = f[i];
+    }
+    if (wantE) {
+      std::copy(J.begin(), J.end(), out);
+    } else {
+      // stored in J order: FI(J[a], J[c]) = stored(a, c)
+      const int32_t* Jp = L.J.data() + key * L.n;
+      for (int64_t c = 0; c < n; ++c)
+        for (int64_t a = 0; a < n; ++a) out[Jp[a] + int64_t(Jp[c]) * n] = J[a + c * n];
+    }
+    return HSSC_OK;
+  });
+}
+
+int hssc_inverse_get_FI(const hssc_inverse* inv, int64_t box, double* out) {
+  return inverse_block(inv, box, out, false);
+}
+
+int hssc_inverse_get_E(const hssc_inverse* inv, int64_t box, double* out) {
+  return inverse_block(inv, box, out, true);
+}
+
+void hssc_inverse_destroy(hssc_inverse* inv) { delete inv; }
+
+int hssc_hss2d_compress(hssc_tree* t, hssc_operator** out) {
+  // HSS2D/HSS2D_compress_fsym.m:73-138: Bself = K(I_src, I_src) at leaves (J order here),
+  // B12 = K(I_sk(c1), I_sk(c2)) at every other box (skeleton order).
+  if (!t || !out) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
+  return guarded([&]() {
+    auto* op = new hssc_operator();
+    op->tree = t;
+    const int D = t->depth;
+    op->B.assign(D + 1, nullptr);
+    try {
+      for (int l = 0; l <= D; ++l) {
+        const Level& L = t->levels[l];
+        if (l == D) {
+          op->B[l] = dalloc<double>(L.nkey * L.n * L.n);
+          k::gen_block<double>(0, L.n, L.n, L.nkey, L.dPx, L.dPy, L.dPw, L.n, L.dPx, L.dPy, L.dPw,
+                               L.n, op->B[l], L.n, L.n * L.n, t->p.h);
+        } else {
+          const int64_t kc = t->levels[l + 1].k;
+          op->B[l] = dalloc<double>(L.nkey * kc * kc);
+          k::gen_block<double>(0, kc, kc, L.nkey, L.dPNx, L.dPNy, L.dPNw, L.n, L.dPNx + kc,
+                               L.dPNy + kc, L.dPNw + kc, L.n, op->B[l], kc, kc * kc, t->p.h);
+        }
+      }
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipGetLastError());
+    } catch (...) {
+      delete op;
+      throw;
+    }
+    *out = op;
+    return HSSC_OK;
+  });
+}
+
+void hssc_operator_destroy(hssc_operator* op) { delete op; }
+
+int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY, int64_t m,
+                               void* stream) {
+  if (!inv || !dX || !dY || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid applyinv arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return guarded([&]() {
+    applyinv_dispatch(inv, static_cast<const double*>(dX), static_cast<double*>(dY), m, st);
+    HIPCHK(hipGetLastError());
+    return HSSC_OK;
+  });
+}
+
+int hssc_hss2d_apply_device(const hssc_operator* op, const void* dX, void* dY, int64_t m,
+                            void* stream) {
+  if (!op || !dX || !dY || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid apply arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return guarded([&]() {
+    apply_impl(op, static_cast<const double*>(dX), static_cast<double*>(dY), m, st);
+    HIPCHK(hipGetLastError());
+    return HSSC_OK;
+  });
+}
+
+static int host_wrap(int64_t N, int64_t m, const double* X, double* Y,
+                     const std::function<int(const double*, double*)>& f) {
+  return guarded([&]() {
+    double* dX = dalloc<double>(N * m);
+    double* dY = dalloc<double>(N * m);
+    HIPCHK(hipMemcpy(dX, X, N * m * 8, hipMemcpyHostToDevice));
+    int s = f(dX, dY);
+    if (s == HSSC_OK) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipMemcpy(Y, dY, N * m * 8, hipMemcpyDeviceToHost));
+    }
+    dfree(dX), dfree(dY);
+    return s;
+  });
+}
+
+int hssc_hss2d_applyinv(const hssc_inverse* inv, const double* X, double* Y, int64_t m) {
+  if (!inv || !X || !Y || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid applyinv arguments");
+  return host_wrap(inv->tree->N, m, X, Y, [&](const double* dX, double* dY) {
+    return hssc_hss2d_applyinv_device(inv, dX, dY, m, nullptr);
+  });
+}
+
+int hssc_hss2d_apply(const hssc_operator* op, const double* X, double* Y, int64_t m) {
+  if (!op || !X || !Y || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid apply arguments");
+  return host_wrap(op->tree->N, m, X, Y, [&](const double* dX, double* dY) {
+    return hssc_hss2d_apply_device(op, dX, dY, m, nullptr);
+  });
+}
+
+int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, const void* dB,
+                            void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
+                            double* relres, void* stream) {
+  // Common/Iterative_Refinement.m:20-35 with A = the FP64 compressed operator.
+  if (!inv || !op || !dB || !dY || m < 1 || maxit < 1 || inv->tree != op->tree)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid solve arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t len = inv->tree->N * m;
+  return guarded([&]() {
+    const double* B = static_cast<const double*>(dB);
+    double* Y = static_cast<double*>(dY);
+    double* R = dalloc<double>(len);
+    double* D = dalloc<double>(len);
+    double* part = dalloc<double>(256);
+    applyinv_dispatch(inv, B, Y, m, st);
+    const double nb = device_norm(B, len, st, part);
+    int it = 1;
+    double rel = 0;
+    for (;;) {
+      apply_impl(op, Y, R, m, st);
+      k::axpby<double, double>(st, len, 1.0, B, -1.0, R);  // R = B - A Y
+      rel = device_norm(R, len, st, part) / (nb > 0 ? nb : 1.0);
+      if (rel <= tol || it >= maxit) break;
+      applyinv_dispatch(inv, R, D, m, st);
+      k::axpby<double, double>(st, len, 1.0, D, 1.0, Y);
+      ++it;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(R), dfree(D), dfree(part);
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+// Host construction of the HSS2D tree, skeleton index sets and proxies.
+//
+// Follows HSS2D/HSS2D_build_tree_fsym.m (uniform tree, 'bdry' skeleton and proxy rules)
+// and HSS2D/build_box_proxy.m.  Everything here is integer / comparison work on at most
+// a few thousand points per box; the numerical work (interpolation operators) runs on the
+// GPU afterwards.  The floating-point operation order matches the oracle so that index
+// sets agree exactly.
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "internal.h"
+
+namespace hssc {
+
+void source_points(int Ng, int np, double* X) {
+  // Common/HSS_tree_parameters.m:144-160: np x np panels, Ng x Ng cell-centred nodes,
+  // panel-major, y fastest inside a panel and across panels.
+  const double hp = 2.0 / np;
+  const double hh = 2.0 / (double(Ng) * np);
+  const int64_t npp = int64_t(Ng) * Ng;
+  const int64_t N = int64_t(np) * np * npp;
+  for (int64_t g = 0; g < N; ++g) {
+    const int64_t p = g / npp, loc = g % npp;
+    const int64_t px = p / np, py = p % np;
+    const int64_t jx = loc / Ng, iy = loc % Ng;
+    X[g] = (-1.0 + double(px) * hp) + (double(jx) + 0.5) * hh;
+    X[N + g] = (-1.0 + double(py) * hp) + (double(iy) + 0.5) * hh;
+  }
+}
+
+double point_weight(const hssc_params& p, double x, double y) {
+  // Common/Kernel_Eval.m:111-159 for params.proxy == 0 and a symmetric kernel:
+  // TI -> 1; NTI Laplace -> b_func(.,0.3,0.6,0.25) (:135,:146); NTI Helmholtz ->
+  // sqrt(Bump_function(.,4,h)) (:133,:143).
+  if (p.transinv) return 1.0;
+  if (p.kh > 0) {
+    const double c = 4 * p.h, D = 10.0;
+    const double f1 = 0.5 * std::tanh(D * (-std::fabs(x) + 1 - c)) + 0.5;
+    const double f2 = 0.5 * std::tanh(D * (-std::fabs(y) + 1 - c)) + 0.5;
+    return std::sqrt(f1 * f2);
+  }
+  const double d1 = x - 0.3, d2 = y - 0.6;
+  return 1.0 + 0.25 * std::exp(-d1 * d1 - d2 * d2);
+}
+
+namespace {
+
+void level_radii(int lev, double* colrad, double* rowrad) {
+  // HSS2D/HSS2D_build_tree_fsym.m:322-326 (colrad = x half-width, rowrad = y).
+  const int scl = lev / 2;
+  *rowrad = std::ldexp(1.0, -scl);
+  *colrad = *rowrad / double(1 << (lev % 2));
+}
+
+// Stable argsort by key (MATLAB sort is stable).
+std::vector<int32_t> argsort_stable(const std::vector<double>& key) {
+  std::vector<int32_t> idx(key.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  return idx;
+}
+
+// LOCAL_Preset_Skeleton_Info, 'bdry' branch (:804-864) with LOCAL_orient_skeleton
+// (:866-874) and LOCAL_orient_residual (:877-883).
+void preset_skeleton(const std::vector<double>& xs, const std::vector<double>& ys, double cx,
+                     double cy, int lev, const hssc_params& p, std::vector<int32_t>* J,
+                     int64_t* k) {
+  double colrad, rowrad;
+  level_radii(lev, &colrad, &rowrad);
+  const double width = p.layers * p.h;
+  const int64_t n = int64_t(xs.size());
+  std::vector<int32_t> sk, rs;
+  for (int64_t i = 0; i < n; ++i) {
+    const bool bdry = (std::fabs(xs[i] - cx) > (colrad - width)) ||
+                      (std::fabs(ys[i] - cy) > (rowrad - width));
+    (bdry ? sk : rs).push_back(int32_t(i));
+  }
+  const bool even = (lev % 2) == 0;
+  const double th0 = even ? M_PI / 2 : 0.0;
+  // Skeleton: stable sort by radius, then stable sort by angle from th0.
+  std::vector<double> th(sk.size()), rho(sk.size());
+  for (size_t i = 0; i < sk.size(); ++i) {
+    const double dx = xs[sk[i]] - cx, dy = ys[sk[i]] - cy;
+    double t = std::atan2(dy, dx);
+    if (t < th0) t = t + 2 * M_PI;
+    th[i] = t;
+    rho[i] = std::hypot(dx, dy);
+  }
+  std::vector<int32_t> J1 = argsort_stable(rho);
+  std::vector<double> thJ1(sk.size());
+  for (size_t i = 0; i < sk.size(); ++i) thJ1[i] = th[J1[i]];
+  std::vector<int32_t> J2 = argsort_stable(thJ1);
+  // Residual: stable sort by the normal coordinate, then by minus the tangential one.
+  std::vector<double> nrm(rs.size()), tang(rs.size());
+  for (size_t i = 0; i < rs.size(); ++i) {
+    nrm[i] = even ? xs[rs[i]] : ys[rs[i]];
+    tang[i] = even ? ys[rs[i]] : xs[rs[i]];
+  }
+  std::vector<int32_t> R1 = argsort_stable(nrm);
+  std::vector<double> tR1(rs.size());
+  for (size_t i = 0; i < rs.size(); ++i) tR1[i] = -tang[R1[i]];
+  std::vector<int32_t> R2 = argsort_stable(tR1);
+
+  J->clear();
+  for (size_t i = 0; i < sk.size(); ++i) J->push_back(sk[J1[J2[i]]]);
+  for (size_t i = 0; i < rs.size(); ++i) J->push_back(rs[R1[R2[i]]]);
+  *k = int64_t(sk.size());
+}
+
+// MATLAB a:d:b for spans that are integer multiples of d up to rounding.
+void colon(double a, double d, double b, std::vector<double>* out) {
+  const int64_t n = int64_t(std::floor((b - a) / d + 1e-10)) + 1;
+  out->clear();
+  for (int64_t i = 0; i < n; ++i) out->push_back(a + d * double(i));
+}
+
+// HSS2D/build_box_proxy.m:52-85: four strips of exterior points at spacing h, `layers`
+// deep, with the corner trimming of :71-84.
+void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, double h,
+               std::vector<double>* px, std::vector<double>* py) {
+  const double width = (layers - 0.5) * h;
+  const double r1 = layers * layers - 0.5, r2 = layers * layers + 0.5;
+  const double ew = colrad + width - h / 2, eh = rowrad + width - h / 2;
+  std::vector<double> xr, yr;
+  px->clear();
+  py->clear();
+  // strip 1 (left): x fastest
+  colon(cx - colrad - width, h, cx - colrad - h / 2, &xr);
+  colon(cy - rowrad - width, h, cy + rowrad + width, &yr);
+  for (double y : yr)
+    for (double x : xr) {
+      const double dx = x - cx, dy = y - cy;
+      const bool rm = ((std::fabs(dx) > ew) &&
+                       ((dy > rowrad + width - r1 * h) || (dy < -(rowrad + width - r2 * h)))) ||
+                      (std::fabs(dy) > eh);
+      if (!rm) px->push_back(x), py->push_back(y);
+    }
+  // strip 2 (top): y fastest
+  colon(cx - colrad + h / 2, h, cx + colrad - h / 2, &xr);
+  colon(cy + rowrad + width, -h, cy + rowrad + h / 2, &yr);
+  for (double x : xr)
+    for (double y : yr) {
+      const double dx = x - cx, dy = y - cy;
+      const bool rm = (std::fabs(dy) > eh) &&
+                      ((dx > colrad + width - r1 * h) || (dx < -(colrad + width - r2 * h)));
+      if (!rm) px->push_back(x), py->push_back(y);
+    }
+  // strip 3 (right): x fastest
+  colon(cx + colrad + h / 2, h, cx + colrad + width, &xr);
+  colon(cy + rowrad + width, -h, cy - rowrad - width, &yr);
+  for (double y : yr)
+    for (double x : xr) {
+      const double dx = x - cx, dy = y - cy;
+      const bool rm = ((std::fabs(dx) > ew) &&
+                       ((dy > rowrad + width - r2 * h) || (dy < -(rowrad + width - r1 * h)))) ||
+                      (std::fabs(dy) > eh);
+      if (!rm) px->push_back(x), py->push_back(y);
+    }
+  // strip 4 (bottom): y fastest
+  colon(cx + colrad - h / 2, -h, cx - colrad + h / 2, &xr);
+  colon(cy - rowrad - h / 2, -h, cy - rowrad - width, &yr);
+  for (double x : xr)
+    for (double y : yr) {
+      const double dx = x - cx, dy = y - cy;
+      const bool rm = (std::fabs(dy) > eh) &&
+                      ((dx > colrad + width - r2 * h) || (dx < -(colrad + width - r1 * h)));
+      if (!rm) px->push_back(x), py->push_back(y);
+    }
+}
+
+int proxy_layers(int lev, const hssc_params& p) {
+  // lay_prox of HSS2D/HSS2D_build_tree_fsym.m:339-347 (params.m = 1).
+  double colrad, rowrad;
+  level_radii(lev, &colrad, &rowrad);
+  if (lev > 0) {
+    if (p.flag_pot == HSSC_POT_SL_L_2D) return p.layers + 1;
+    return 2 * int(std::lround(std::min(rowrad, colrad) / p.h));
+  }
+  return p.layers;
+}
+
+}  // namespace
+
+int build_index_sets(const hssc_params& p, hssc_tree* t) {
+  if (p.Ng < 1 || p.np < 1 || (p.np & (p.np - 1)) != 0)
+    return fail(HSSC_ERR_INVALID_ARG, "np must be a power of two and Ng >= 1");
+  if (p.np < 2) return fail(HSSC_ERR_INVALID_ARG, "np must be at least 2");
+  if (p.sym != 1)
+    return fail(HSSC_ERR_UNSUPPORTED, "only symmetric kernels (HSS2D_*_fsym) are supported");
+  t->p = p;
+  const int depth = p.depth;
+  t->depth = depth;
+  const int64_t N = int64_t(p.np) * p.np * p.Ng * p.Ng;
+  t->N = N;
+  t->X.resize(2 * N);
+  source_points(p.Ng, p.np, t->X.data());
+  const double* Xx = t->X.data();
+  const double* Xy = t->X.data() + N;
+
+  // Heap-numbered centres (bisect_level, :1289-1344).
+  const int64_t nboxes = (int64_t(1) << (depth + 1)) - 1;
+  t->cent.assign(2 * (nboxes + 1), 0.0);
+  for (int lev = 0; lev < depth; ++lev) {
+    const double s = 0.5 * std::ldexp(1.0, -(lev / 2));
+    for (int64_t b = int64_t(1) << lev; b < (int64_t(1) << (lev + 1)); ++b) {
+      const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
+      if (lev % 2 == 0) {
+        t->cent[2 * (2 * b)] = cx + (-s), t->cent[2 * (2 * b) + 1] = cy + 0.0;
+        t->cent[2 * (2 * b + 1)] = cx + s, t->cent[2 * (2 * b + 1) + 1] = cy + 0.0;
+      } else {
+        t->cent[2 * (2 * b)] = cx + 0.0, t->cent[2 * (2 * b) + 1] = cy + s;
+        t->cent[2 * (2 * b + 1)] = cx + 0.0, t->cent[2 * (2 * b + 1) + 1] = cy + (-s);
+      }
+    }
+  }
+  t->Isrc.assign(nboxes + 1, {});
+  t->Isk.assign(nboxes + 1, {});
+  // Leaf slabs (Build_I_source_unif, :1203-1230).
+  const double hp = 2.0 / p.np;
+  const int64_t nmx = int64_t(p.Ng) * p.Ng;
+  for (int64_t b = int64_t(1) << depth; b <= nboxes; ++b) {
+    const int64_t px = int64_t(std::floor((t->cent[2 * b] + 1.0) / hp));
+    const int64_t py = int64_t(std::floor((t->cent[2 * b + 1] + 1.0) / hp));
+    const int64_t pn = py + px * p.np;
+    auto& I = t->Isrc[b];
+    I.resize(nmx);
+    for (int64_t i = 0; i < nmx; ++i) I[i] = pn * nmx + i;
+  }
+
+  t->levels.assign(depth + 1, Level{});
+  const bool TI = p.transinv != 0;
+  // Upward skeleton pass (:321-622).
+  for (int lev = depth; lev >= 0; --lev) {
+    Level& L = t->levels[lev];
+    L.lev = lev;
+    L.nbox = int64_t(1) << lev;
+    L.nkey = TI ? 1 : L.nbox;
+    double colrad, rowrad;
+    level_radii(lev, &colrad, &rowrad);
+    const int lay_prox = proxy_layers(lev, p);
+    const int64_t b0 = int64_t(1) << lev;
+    for (int64_t bi = 0; bi < L.nbox; ++bi) {
+      const int64_t b = b0 + bi;
+      if (lev < depth) {
+        auto& I = t->Isrc[b];
+        I = t->Isk[2 * b];
+        I.insert(I.end(), t->Isk[2 * b + 1].begin(), t->Isk[2 * b + 1].end());
+      }
+      const auto& I = t->Isrc[b];
+      const int64_t n = int64_t(I.size());
+      if (bi == 0) L.n = n;
+      if (n != L.n) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform box sizes on a level");
+      if (!TI || bi == 0) {
+        std::vector<double> xs(n), ys(n);
+        for (int64_t i = 0; i < n; ++i) xs[i] = Xx[I[i]], ys[i] = Xy[I[i]];
+        const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
+        std::vector<int32_t> J;
+        int64_t k;
+        preset_skeleton(xs, ys, cx, cy, lev, p, &J, &k);
+        std::vector<double> qx, qy;
+        box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, &qx, &qy);
+        if (bi == 0) {
+          L.k = k;
+          L.M = int64_t(qx.size());
+          L.J.resize(L.nkey * n);
+          L.Jinv.resize(L.nkey * n);
+          L.Px.resize(L.nkey * n), L.Py.resize(L.nkey * n), L.Pw.resize(L.nkey * n);
+          L.PNx.resize(L.nkey * n), L.PNy.resize(L.nkey * n), L.PNw.resize(L.nkey * n);
+          L.Xpx.resize(L.nkey * L.M), L.Xpy.resize(L.nkey * L.M), L.Xpw.resize(L.nkey * L.M);
+        }
+        if (k != L.k || int64_t(qx.size()) != L.M)
+          return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes on a level");
+        const int64_t key = TI ? 0 : bi;
+        for (int64_t a = 0; a < n; ++a) {
+          L.J[key * n + a] = J[a];
+          L.Jinv[key * n + J[a]] = int32_t(a);
+          L.Px[key * n + a] = xs[J[a]];
+          L.Py[key * n + a] = ys[J[a]];
+          L.Pw[key * n + a] = point_weight(p, xs[J[a]], ys[J[a]]);
+          L.PNx[key * n + a] = xs[a];
+          L.PNy[key * n + a] = ys[a];
+          L.PNw[key * n + a] = point_weight(p, xs[a], ys[a]);
+        }
+        for (int64_t r = 0; r < L.M; ++r) {
+          L.Xpx[key * L.M + r] = qx[r];
+          L.Xpy[key * L.M + r] = qy[r];
+          L.Xpw[key * L.M + r] = point_weight(p, qx[r], qy[r]);
+        }
+      }
+      const int64_t key = TI ? 0 : bi;
+      auto& S = t->Isk[b];
+      S.resize(L.k);
+      for (int64_t a = 0; a < L.k; ++a) S[a] = I[L.J[key * n + a]];
+    }
+    L.q = L.n - L.k;
+  }
+  // Leaf gather map: global index of each J-ordered leaf point.
+  const Level& LL = t->levels[depth];
+  t->leafG.resize(LL.nbox * LL.n);
+  for (int64_t bi = 0; bi < LL.nbox; ++bi) {
+    const int64_t key = TI ? 0 : bi;
+    const auto& I = t->Isrc[(int64_t(1) << depth) + bi];
+    for (int64_t a = 0; a < LL.n; ++a) t->leafG[bi * LL.n + a] = int32_t(I[LL.J[key * LL.n + a]]);
+  }
+  return HSSC_OK;
+}
+
+}  // namespace hssc

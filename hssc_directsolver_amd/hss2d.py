@@ -1,0 +1,264 @@
+"""Python mirror of the reference's HSS2D MATLAB entry points, backed by libhssc.so.
+
+Function names and argument meaning follow HSSC_DirectSolver (DOCUMENTATION_HSS2D.txt):
+
+    params    = HSS_tree_parameters(pot, kh, sym, Ng, np, acc, lay, n_cut, TI)
+    INFO_TREE = HSS2D_build_tree_fsym(params)
+    A_HSS     = HSS2D_invert_fsym(INFO_TREE, params)            # Fopts = 'dense'
+    Y         = HSS2D_applyinv_TI(INFO_TREE, A_HSS, X, params)  # or _fsym for NTI
+    M_HSS     = HSS2D_compress_fsym(INFO_TREE, params)
+    Y         = HSS2D_apply_TI(INFO_TREE, M_HSS, X, params)     # or _fsym
+    lHSS      = levHSS(INFO_TREE, n_cut, trinv)
+
+All numerical work runs on the GPU through the C ABI; X and Y are numpy arrays
+(N x m, X_source ordering) or torch tensors already on the GPU (the *_device helpers).
+Errors raise HsscError (the reference raises MATLAB runtime errors instead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import PREC, POT, HsscParams, check, lib
+
+_DP = C.POINTER(C.c_double)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Params:
+    """The params struct of Common/HSS_tree_parameters.m:116-139."""
+
+    def __init__(self, raw: HsscParams):
+        self.raw = raw
+
+    def __getattr__(self, name):
+        raw = object.__getattribute__(self, "raw")
+        return getattr(raw, name)
+
+    @property
+    def N(self) -> int:
+        return int(lib().hssc_params_npoints(C.byref(self.raw)))
+
+    @property
+    def X_source(self) -> np.ndarray:
+        """N x 2 grid of Common/HSS_tree_parameters.m:144-160."""
+        X = np.empty((2, self.N))
+        check(lib().hssc_source_points(C.byref(self.raw), _dptr(X)))
+        return X.T
+
+
+def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None) -> Params:
+    """Common/HSS_tree_parameters.m:1."""
+    raw = HsscParams()
+    check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
+                                 float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
+                                 int(n_cut), int(TI)))
+    if corr0 is not None:
+        check(lib().hssc_params_set_dr_weights(C.byref(raw), float(np.real(corr0)),
+                                               float(np.imag(corr0))))
+    return Params(raw)
+
+
+def Kernel_Eval(X1, X2, params: Params) -> np.ndarray:
+    """Common/Kernel_Eval.m:1 (computed on the GPU)."""
+    X1 = np.asfortranarray(X1, dtype=np.float64)
+    X2 = np.asfortranarray(X2, dtype=np.float64)
+    m, n = X1.shape[0], X2.shape[0]
+    A = np.empty((m, n), order="F")
+    check(lib().hssc_kernel_eval(C.byref(params.raw), _dptr(X1), m, _dptr(X2), n, _dptr(A),
+                                 max(m, 1)))
+    return A
+
+
+class Tree:
+    """INFO_TREE: owns the device-resident tree (hssc_tree*)."""
+
+    def __init__(self, handle, params: Params):
+        self.handle = handle
+        self.params = params
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().hssc_tree_destroy(h)
+            self.handle = None
+
+    @property
+    def depth(self) -> int:
+        d = C.c_int32()
+        check(lib().hssc_tree_depth(self.handle, C.byref(d)))
+        return d.value
+
+    def level_info(self, lev):
+        vals = [C.c_int64() for _ in range(6)]
+        check(lib().hssc_tree_level_info(self.handle, lev, *[C.byref(v) for v in vals]))
+        return dict(zip(("n", "k", "M", "nbox", "nkey", "q"), (v.value for v in vals)))
+
+    def _i64(self, fn, box, size):
+        out = np.empty(size, dtype=np.int64)
+        check(fn(self.handle, int(box), out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def _box_level(self, box):
+        return int(box).bit_length() - 1
+
+    def I_src(self, box):
+        return self._i64(lib().hssc_tree_get_isrc, box, self.level_info(self._box_level(box))["n"])
+
+    def I_sk(self, box):
+        return self._i64(lib().hssc_tree_get_isk, box, self.level_info(self._box_level(box))["k"])
+
+    def J(self, box):
+        return self._i64(lib().hssc_tree_get_J, box, self.level_info(self._box_level(box))["n"])
+
+    def proxy(self, box):
+        M = self.level_info(self._box_level(box))["M"]
+        out = np.empty((2, M))
+        check(lib().hssc_tree_get_proxy(self.handle, int(box), _dptr(out)))
+        return out.T
+
+    def T(self, box):
+        info = self.level_info(self._box_level(box))
+        k, r = info["k"], info["n"] - info["k"]
+        out = np.empty((r, k)).T  # k x r column-major view
+        buf = np.empty(k * r)
+        check(lib().hssc_tree_get_T(self.handle, int(box), _dptr(buf)))
+        return buf.reshape((r, k)).T.copy() if k * r else out
+
+
+def _tree(params: Params, index_only=False) -> Tree:
+    h = C.c_void_p()
+    fn = lib().hssc_hss2d_build_index_sets if index_only else lib().hssc_hss2d_build_tree
+    check(fn(C.byref(params.raw), C.byref(h)))
+    return Tree(h, params)
+
+
+def HSS2D_build_tree_fsym(params: Params) -> Tree:
+    """HSS2D/HSS2D_build_tree_fsym.m:1 (index sets on the host, T on the GPU)."""
+    return _tree(params)
+
+
+def HSS2D_build_index_sets(params: Params) -> Tree:
+    """Tree geometry and index sets only (no interpolation operators, no GPU needed)."""
+    return _tree(params, index_only=True)
+
+
+def levHSS(tree: Tree, n_cut, trinv=None) -> int:
+    """HSS2D/levHSS.m:1."""
+    out = C.c_int32()
+    check(lib().hssc_levhss(tree.handle, int(n_cut), C.byref(out)))
+    return out.value
+
+
+class Inverse:
+    """A_HSS of HSS2D_invert_fsym (dense F^-1 and E per box or per level)."""
+
+    def __init__(self, handle, tree: Tree, precision: str):
+        self.handle = handle
+        self.tree = tree  # keeps the tree alive
+        self.precision = precision
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().hssc_inverse_destroy(h)
+            self.handle = None
+
+    def FI(self, box):
+        n = self.tree.level_info(self.tree._box_level(box))["n"]
+        buf = np.empty(n * n)
+        check(lib().hssc_inverse_get_FI(self.handle, int(box), _dptr(buf)))
+        return buf.reshape((n, n)).T.copy()
+
+    def E(self, box):
+        k = self.tree.level_info(self.tree._box_level(box))["k"]
+        buf = np.empty(k * k)
+        check(lib().hssc_inverse_get_E(self.handle, int(box), _dptr(buf)))
+        return buf.reshape((k, k)).T.copy()
+
+
+def HSS2D_invert_fsym(tree: Tree, params: Params = None, precision: str = "fp64") -> Inverse:
+    """HSS2D/HSS2D_invert_fsym.m:1 with Fopts = 'dense'."""
+    h = C.c_void_p()
+    check(lib().hssc_hss2d_invert(tree.handle, PREC[precision], C.byref(h)))
+    return Inverse(h, tree, precision)
+
+
+class Operator:
+    """A_HSS of HSS2D_compress_fsym (dense Bself / B12 blocks)."""
+
+    def __init__(self, handle, tree: Tree):
+        self.handle = handle
+        self.tree = tree
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().hssc_operator_destroy(h)
+            self.handle = None
+
+
+def HSS2D_compress_fsym(tree: Tree, params: Params = None) -> Operator:
+    """HSS2D/HSS2D_compress_fsym.m:1 with dense blocks."""
+    h = C.c_void_p()
+    check(lib().hssc_hss2d_compress(tree.handle, C.byref(h)))
+    return Operator(h, tree)
+
+
+def _host_call(fn, handle, X):
+    X = np.asarray(X, dtype=np.float64)
+    shape = X.shape
+    Xf = np.asfortranarray(X.reshape(shape[0], -1))
+    Y = np.empty_like(Xf, order="F")
+    check(fn(handle, _dptr(Xf), _dptr(Y), Xf.shape[1]))
+    return Y.reshape(shape)
+
+
+def HSS2D_applyinv_TI(tree: Tree, A_HSS: Inverse, X, params: Params = None):
+    """HSS2D/HSS2D_applyinv_TI.m:1."""
+    return _host_call(lib().hssc_hss2d_applyinv, A_HSS.handle, X)
+
+
+HSS2D_applyinv_fsym = HSS2D_applyinv_TI  # HSS2D/HSS2D_applyinv_fsym.m:1 (same C entry point)
+
+
+def HSS2D_apply_TI(tree: Tree, M_HSS: Operator, X, params: Params = None):
+    """HSS2D/HSS2D_apply_TI.m:1."""
+    return _host_call(lib().hssc_hss2d_apply, M_HSS.handle, X)
+
+
+HSS2D_apply_fsym = HSS2D_apply_TI  # HSS2D/HSS2D_apply_fsym.m:1
+
+
+# ---- device-resident variants (torch tensors on the GPU, float64, N x m column-major) ----
+
+def _stream_ptr(stream):
+    return None if stream is None else C.c_void_p(stream.cuda_stream)
+
+
+def applyinv_device(A_HSS: Inverse, X, Y, stream=None):
+    """Y = A^-1 X with X, Y torch float64 tensors of shape (m, N) (each row one RHS)."""
+    m = X.shape[0] if X.dim() == 2 else 1
+    check(lib().hssc_hss2d_applyinv_device(A_HSS.handle, C.c_void_p(X.data_ptr()),
+                                           C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
+
+
+def apply_device(M_HSS: Operator, X, Y, stream=None):
+    m = X.shape[0] if X.dim() == 2 else 1
+    check(lib().hssc_hss2d_apply_device(M_HSS.handle, C.c_void_p(X.data_ptr()),
+                                        C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
+
+
+def solve_device(A_HSS: Inverse, M_HSS: Operator, B, Y, tol=1e-10, maxit=10, stream=None):
+    """Common/Iterative_Refinement.m on the GPU; returns (iterations, relative residual)."""
+    m = B.shape[0] if B.dim() == 2 else 1
+    it = C.c_int32()
+    rel = C.c_double()
+    check(lib().hssc_hss2d_solve_device(A_HSS.handle, M_HSS.handle, C.c_void_p(B.data_ptr()),
+                                        C.c_void_p(Y.data_ptr()), m, float(tol), int(maxit),
+                                        C.byref(it), C.byref(rel), _stream_ptr(stream)))
+    return it.value, rel.value

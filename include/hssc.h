@@ -1,0 +1,168 @@
+/*
+ * hssc.h -- C ABI of the MI355X-native HSS compressed direct solver.
+ *
+ * The entry points mirror the MATLAB functions of HSSC_DirectSolver (Corona, Martinsson,
+ * Zorin) that make up the compress -> invert -> apply path.  Each declaration names the
+ * reference function it replaces (paths relative to the reference repository root).
+ *
+ * Conventions (the MEX / ctypes layers convert from MATLAB's):
+ *   - arrays are column-major with explicit sizes; point sets are N x 2 (x column, then y);
+ *   - index vectors are int64 and 0-based (MATLAB is 1-based);
+ *   - every call returns an int status (HSSC_OK on success); the message of the last
+ *     failure on the calling thread is returned by hssc_last_error();
+ *   - handles own device memory on the device that was current when they were created;
+ *     calls on one handle must be serialised by the caller, distinct handles are
+ *     independent.
+ */
+#ifndef HSSC_H
+#define HSSC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+#define HSSC_OK 0
+#define HSSC_ERR_INVALID_ARG 1
+#define HSSC_ERR_OUT_OF_MEMORY 2
+#define HSSC_ERR_SINGULAR 3
+#define HSSC_ERR_RANK_OVERFLOW 4
+#define HSSC_ERR_HIP 5
+#define HSSC_ERR_NOT_CONVERGED 6
+#define HSSC_ERR_UNSUPPORTED 7
+
+/* ---- enumerations ------------------------------------------------------------------ */
+#define HSSC_POT_SL_L_2D 0 /* 2D Laplace single layer, Common/Kernel_Eval.m:60-61     */
+#define HSSC_POT_SL_H_2D 1 /* 2D Helmholtz single layer, Common/Kernel_Eval.m:44-49   */
+
+#define HSSC_INTER_DENSE 0   /* params.INTERform = 'dense'   */
+#define HSSC_INTER_LOWRANK 1 /* params.INTERform = 'lowrank' */
+
+#define HSSC_PREC_FP64 0 /* factorization (F^-1, E) stored and applied in FP64            */
+#define HSSC_PREC_FP32 1 /* factorization in FP32; solves refine against the FP64 operator */
+
+/* The params struct of Common/HSS_tree_parameters.m:116-139 (field names kept). */
+typedef struct hssc_params {
+  int32_t flag_pot;   /* HSSC_POT_*                                   (.flag_pot)   */
+  double kh;          /* wavenumber                                   (.kh)         */
+  int32_t sym;        /* symmetric kernel                             (.sym)        */
+  int32_t Ng;         /* points per panel side                        (.Ng)         */
+  int32_t np;         /* panels per domain side                                     */
+  double acc;         /* ID / interpolation accuracy                  (.acc)        */
+  int32_t layers;     /* boundary layers of the skeleton              (.layers)     */
+  int32_t n_cut;      /* HSS switch threshold (levHSS)                (.n_cut)      */
+  int32_t transinv;   /* translation invariant kernel                 (.transinv)   */
+  int32_t depth;      /* tree depth, leaves are panels                (.depth)      */
+  double h;           /* grid spacing 2/(Ng*np)                       (.h)          */
+  int32_t max_particles; /* Ng*Ng                                     (.max_particles) */
+  int32_t interform;  /* HSSC_INTER_*                                 (.INTERform)  */
+  int32_t k_cut;      /* 2500                                         (.k_cut)      */
+  double dr_weights_re; /* Duan-Rokhlin corr0, real part              (.dr_weights) */
+  double dr_weights_im; /* Duan-Rokhlin corr0, imaginary part                       */
+  int32_t factor_precision; /* HSSC_PREC_* (no reference counterpart: GPU policy)   */
+  int32_t reserved[7];
+} hssc_params;
+
+typedef struct hssc_tree hssc_tree;         /* INFO_TREE + params.X_source            */
+typedef struct hssc_inverse hssc_inverse;   /* A_HSS of HSS2D_invert_* (FI, E, JE)    */
+typedef struct hssc_operator hssc_operator; /* A_HSS of HSS2D_compress_* (Bself, B12) */
+
+/* Thread-local text of the last failure ("" if none). */
+const char* hssc_last_error(void);
+
+/* Library version string. */
+const char* hssc_version(void);
+
+/* Replaces Common/HSS_tree_parameters.m:1 (params = HSS_tree_parameters(pot,kh,sym,Ng,np,
+ * acc,lay,n_cut,TI)), defaults of :116-139.  The Duan-Rokhlin weight for kh>0
+ * (:167-175) is given by hssc_params_set_dr_weights. */
+int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_t Ng,
+                     int32_t np, double acc, int32_t lay, int32_t n_cut, int32_t TI);
+
+/* Sets params.dr_weights = corr0 (Common/DR_stencil.m:10-25 output, order 4). */
+int hssc_params_set_dr_weights(hssc_params* p, double corr0_re, double corr0_im);
+
+/* Number of grid points N = (Ng*np)^2. */
+int64_t hssc_params_npoints(const hssc_params* p);
+
+/* params.X_source of Common/HSS_tree_parameters.m:144-160, written N x 2 column-major. */
+int hssc_source_points(const hssc_params* p, double* X);
+
+/* Replaces Common/Kernel_Eval.m:1 (A = Kernel_Eval(X1,X2,params)) for the 2D plane:
+ * A (m x n, column-major, leading dimension lda >= m) = K[X1, X2].  X1 is m x 2, X2 is
+ * n x 2, both column-major host arrays.  Real kernels write doubles; complex kernels
+ * write interleaved (re, im) pairs.  Computed on the GPU. */
+int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const double* X2,
+                     int64_t n, double* A, int64_t lda);
+
+/* Replaces HSS2D/HSS2D_build_tree_fsym.m:1 (INFO_TREE = HSS2D_build_tree_fsym(params)):
+ * uniform tree, 'bdry' skeletons and proxies, interpolation operators T computed on the
+ * GPU by least squares (rank-revealing randomized compression when interform is
+ * HSSC_INTER_LOWRANK). */
+int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree);
+
+/* Geometry and index sets only (no interpolation operators, no GPU work).  Used by the
+ * same builder; exposed so index-set parity can be checked without a device. */
+int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree);
+
+/* Tree queries (the fields of INFO_TREE, HSS2D/HSS2D_build_tree_fsym.m:655-680). */
+int hssc_tree_depth(const hssc_tree* t, int32_t* depth);
+/* Per level: n = |I_src|, k = |I_sk|, M = proxy size, nbox = boxes, nkey = distinct
+ * (J,T) sets (1 if transinv, nbox otherwise), q = stored rank of T (n-k if dense). */
+int hssc_tree_level_info(const hssc_tree* t, int32_t lev, int64_t* n, int64_t* k,
+                         int64_t* M, int64_t* nbox, int64_t* nkey, int64_t* q);
+/* Heap box number b (1-based, as in the reference) -> I_src / I_sk (0-based global). */
+int hssc_tree_get_isrc(const hssc_tree* t, int64_t box, int64_t* out);
+int hssc_tree_get_isk(const hssc_tree* t, int64_t box, int64_t* out);
+/* J of box b (NTI) or of b's level (TI), 0-based local indices, skeleton first. */
+int hssc_tree_get_J(const hssc_tree* t, int64_t box, int64_t* out);
+/* Proxy points of box b (or its level), M x 2 column-major. */
+int hssc_tree_get_proxy(const hssc_tree* t, int64_t box, double* out);
+/* Interpolation operator of box b (or its level) as a dense k x (n-k) column-major
+ * matrix (T = T.U * T.V.' when stored low-rank). */
+int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out);
+void hssc_tree_destroy(hssc_tree* t);
+
+/* Replaces HSS2D/levHSS.m:1 (lHSS = levHSS(TREE,n_cut,trinv)). */
+int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss);
+
+/* Replaces HSS2D/HSS2D_invert_fsym.m:1 with params.Fopts = 'dense' (dense F^-1 and E per
+ * box, or per level when transinv).  precision = HSSC_PREC_FP64 or HSSC_PREC_FP32. */
+int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** inv);
+/* F^-1 (n x n, rows/cols in I_src order) and E (k x k) of box b (or its level). */
+int hssc_inverse_get_FI(const hssc_inverse* inv, int64_t box, double* out);
+int hssc_inverse_get_E(const hssc_inverse* inv, int64_t box, double* out);
+void hssc_inverse_destroy(hssc_inverse* inv);
+
+/* Replaces HSS2D/HSS2D_compress_fsym.m:1 with dense Bself / B12 blocks. */
+int hssc_hss2d_compress(hssc_tree* t, hssc_operator** op);
+void hssc_operator_destroy(hssc_operator* op);
+
+/* Replaces HSS2D/HSS2D_applyinv_TI.m:1 and HSS2D/HSS2D_applyinv_fsym.m:1
+ * (Y = HSS2D_applyinv_*(INFO_TREE,A_HSS,X,params)).  X, Y: N x m column-major host
+ * arrays in the X_source ordering. */
+int hssc_hss2d_applyinv(const hssc_inverse* inv, const double* X, double* Y, int64_t m);
+
+/* Replaces HSS2D/HSS2D_apply_TI.m:1 and HSS2D/HSS2D_apply_fsym.m:1. */
+int hssc_hss2d_apply(const hssc_operator* op, const double* X, double* Y, int64_t m);
+
+/* Device-pointer variants (X, Y already in device memory; stream = hipStream_t or 0). */
+int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY, int64_t m,
+                               void* stream);
+int hssc_hss2d_apply_device(const hssc_operator* op, const void* dX, void* dY, int64_t m,
+                            void* stream);
+
+/* Replaces Common/Iterative_Refinement.m:1 with A = the FP64 compressed operator and
+ * Ainv = the stored inverse: Y = Ainv(B); while ||B - A(Y)|| > tol*||B|| and
+ * iter < maxit: Y += Ainv(B - A(Y)).  Returns the iterations done and the final relative
+ * residual.  Device pointers. */
+int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, const void* dB,
+                            void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
+                            double* relres, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSSC_H */
