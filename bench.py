@@ -1,0 +1,219 @@
+#!/usr/bin/env python
+"""HSS2D N = 512^2 build + solve on one MI355X.
+
+Workload: the reference's HSS2D volume integral equation (I + h^2 K), K = 1/2 log r^2
+(SL_L_2D, Common/Kernel_Eval.m:60-61) on the uniform 512 x 512 grid of
+HSS_tree_parameters('SL_L_2D', 0, 1, 8, 64, 1e-10, 2, 400, TI) (N = 262144, leaf 8x8,
+depth 12), non-translation-invariant (per-box operators, TI = 0), dense-block inverse,
+seeded unit-norm Gaussian right-hand side.
+
+One step = HSS2D_build_tree_fsym (host index sets + GPU interpolation operators)
+         + HSS2D_compress_fsym (forward operator used by the refinement residual)
+         + HSS2D_invert_fsym + solve to 1e-10 relative residual against the FP64
+           compressed operator (Common/Iterative_Refinement.m).
+Two paths are timed in every step:
+  mixed: FP32 factorization (F^-1, E on MFMA), FP64 refinement  -> reported `value`
+  fp64 : FP64 factorization, FP64 refinement (plain path)
+
+Prints one JSON line.  `value` is the mixed path's build + solve seconds for the whole
+job (time-like, lower is better).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA peaks
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--Ng", type=int, default=8)
+    ap.add_argument("--np", dest="np_", type=int, default=64)
+    ap.add_argument("--TI", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-np", type=int, default=64, help="np of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def stats_snapshot(H):
+    import ctypes as C
+    L = H.lib()
+    out = {}
+    for name in ("gemm_f64", "gemm_f32", "gemv_f64", "gemv_f32", "gemvt_f64", "gemvt_f32",
+                 "gj_small_f64", "gj_small_f32", "gj_diag_f64", "gj_diag_f32", "gen_block_f64",
+                 "gen_block_f32", "gen_parent_f64", "gen_parent_f32", "qr_panel_f64"):
+        n, fl, by, ms = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        L.hssc_stats_get(name.encode(), C.byref(n), C.byref(fl), C.byref(by), C.byref(ms))
+        if n.value:
+            out[name] = dict(launches=n.value, flops=fl.value, bytes=by.value, ms=ms.value)
+    return out
+
+
+def one_step(H, P, torch, b, tol):
+    """Runs both paths once; returns timings (s) and residual info."""
+    sync = torch.cuda.synchronize
+    t = {}
+    sync()
+    t0 = time.perf_counter()
+    tree = H.HSS2D_build_tree_fsym(P)
+    op = H.HSS2D_compress_fsym(tree)
+    sync()
+    t1 = time.perf_counter()
+    t["tree"] = t1 - t0
+    res = {}
+    for prec in ("fp32", "fp64"):
+        sync()
+        a = time.perf_counter()
+        inv = H.HSS2D_invert_fsym(tree, precision=prec)
+        sync()
+        c = time.perf_counter()
+        y = torch.empty_like(b)
+        it, rel = H.solve_device(inv, op, b, y, tol=tol, maxit=12)
+        sync()
+        d = time.perf_counter()
+        res[prec] = dict(invert=c - a, solve=d - c, iters=it, relres=rel)
+        del inv
+    return t, res, tree, op
+
+
+def cpu_baseline(np_cpu):
+    """The oracle (numpy/scipy FP64 restatement of the reference) on the TI tree of the same
+    grid: one box per level, per-level times scaled by the boxes per level give the NTI
+    build; the oracle's solve sweeps every box, as the NTI solve does."""
+    from oracle import hss2d as o
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    p = o.tree_parameters("SL_L_2D", 0, 1, 8, np_cpu, 1e-10, 2, 400, 1)
+    tb, ti = {}, {}
+    t0 = time.perf_counter()
+    tree = o.build_tree(p, interform="dense", timings=tb)
+    inv = o.invert(tree, timings=ti)
+    bvec = o.random_rhs(p["X_source"].shape[0])
+    t1 = time.perf_counter()
+    o.applyinv(tree, inv, bvec)
+    t2 = time.perf_counter()
+    build_nti = sum((tb[l] + ti[l]) * 2 ** l for l in tb)
+    return dict(value=build_nti + (t2 - t1), unit="s", cores=threads, kind="port",
+                sample=(f"oracle (numpy/scipy FP64) on the TI tree of the {8*np_cpu}^2 grid: "
+                        f"one box per level ({t1 - t0:.1f} s build+invert measured, per-level "
+                        f"time x boxes per level = {build_nti:.1f} s NTI build) + one full "
+                        f"solve sweep ({t2 - t1:.2f} s)"),
+                measured_s=round(t2 - t0, 2))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(see DESIGN.md: how the tree would shard)"}))
+        sys.exit(2)
+    import torch
+    import hssc_directsolver_amd as H
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}))
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    P = H.HSS_tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, args.TI)
+    N = P.N
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    b = torch.randn(N, generator=gen, dtype=torch.float64)
+    b = (b / b.norm()).to("cuda:0")
+
+    for _ in range(args.warmup):
+        one_step(H, P, torch, b, args.tol)
+
+    L = H.lib()
+    L.hssc_stats_reset()
+    L.hssc_stats_enable(1)
+    torch.cuda.synchronize()
+    T0 = time.perf_counter()
+    runs = []
+    for _ in range(args.steps):
+        runs.append(one_step(H, P, torch, b, args.tol))
+    torch.cuda.synchronize()
+    T1 = time.perf_counter()
+    L.hssc_stats_enable(0)
+    stats = stats_snapshot(H)
+
+    def avg(f):
+        return sum(f(r) for r in runs) / len(runs)
+
+    tree_s = avg(lambda r: r[0]["tree"])
+    mixed = {k: avg(lambda r, k=k: r[1]["fp32"][k]) for k in ("invert", "solve")}
+    fp64 = {k: avg(lambda r, k=k: r[1]["fp64"][k]) for k in ("invert", "solve")}
+    last = runs[-1][1]
+    build_mixed = tree_s + mixed["invert"]
+    build_fp64 = tree_s + fp64["invert"]
+    total_mixed = build_mixed + mixed["solve"]
+    total_fp64 = build_fp64 + fp64["solve"]
+
+    # dominant kernel class over the timed region (both paths)
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"]) if stats else None
+    roof = None
+    if dom:
+        name, s = dom
+        prec = "f64" if name.endswith("f64") else "f32"
+        mfma = name.startswith(("gemm", "gj_"))
+        if mfma:
+            ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
+            roof = dict(kernel=name, bound="mfma", achieved=round(ach, 3),
+                        peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+                        frac=round(ach / PEAK_TFLOPS[prec], 4), traffic=None,
+                        launches=s["launches"], avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2),
+                        flops_per_launch=s["flops"] / s["launches"])
+        else:
+            ach = s["bytes"] / (s["ms"] * 1e-3) / 1e9
+            roof = dict(kernel=name, bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS,
+                        unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 4), traffic=None,
+                        launches=s["launches"], avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2),
+                        bytes_per_launch=s["bytes"] / s["launches"])
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_np)
+
+    out = {
+        "metric": "HSS2D N=512² build s + solve ms @1e-10 resid: mixed+batched vs FP64 GPU",
+        "value": round(total_mixed, 4),
+        "unit": "s (build + solve, mixed FP32 factorization + FP64 refinement)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (T1 - T0) / args.steps, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factorization / f64 refinement",
+        "data": "synthetic (seeded Gaussian RHS on the reference's uniform grid)",
+        "config": {"workload": f"HSS2D SL_L_2D {'TI' if args.TI else 'NTI'} Ng={args.Ng} "
+                               f"np={args.np_} N={N} acc=1e-10 layers=2 dense blocks",
+                   "N": N},
+        "mixed": {"build_s": round(build_mixed, 4), "tree_s": round(tree_s, 4),
+                  "invert_s": round(mixed["invert"], 4), "solve_ms": round(1e3 * mixed["solve"], 3),
+                  "iters": last["fp32"]["iters"], "relres": last["fp32"]["relres"],
+                  "N_per_s": round(N / total_mixed, 1)},
+        "fp64": {"build_s": round(build_fp64, 4), "invert_s": round(fp64["invert"], 4),
+                 "solve_ms": round(1e3 * fp64["solve"], 3), "iters": last["fp64"]["iters"],
+                 "relres": last["fp64"]["relres"], "total_s": round(total_fp64, 4),
+                 "N_per_s": round(N / total_fp64, 1)},
+        "speedup_mixed_vs_fp64": round(total_fp64 / total_mixed, 3),
+        "roofline": roof,
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"], 3)) for k, v in stats.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhssc.so"
-SOURCES = ["tree.cpp", "kernels.hip", "solver.hip"]
+SOURCES = ["tree.cpp", "stats.cpp", "kernels.hip", "solver.hip"]
 ARCH = os.environ.get("HSSC_OFFLOAD_ARCH", "gfx950")
 
 
@@ -41,7 +41,8 @@ def build(verbose: bool = False) -> Path:
         obj = objdir / (src + ".o")
         cmd = [cc, *common, *dev, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
-            cmd = [cc, *common, "-x", "c++", "-c", str(CSRC / src), "-o", str(obj)]
+            # host-only translation units (still compiled by hipcc for the HIP headers)
+            cmd = [cc, *common, "-c", str(CSRC / src), "-o", str(obj)]
         _run(cmd, verbose)
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")

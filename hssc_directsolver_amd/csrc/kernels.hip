@@ -11,9 +11,15 @@
 #include <climits>
 
 #include "kernels.h"
+#include "stats.h"
 
 namespace hssc {
 namespace k {
+
+template <typename T>
+constexpr const char* tname(const char* f64, const char* f32) {
+  return sizeof(T) == 8 ? f64 : f32;
+}
 
 // ------------------------------------------------------------------------------------
 // Kernel-block generation (Common/Kernel_Eval.m:29-96, 2D Laplace single layer)
@@ -58,8 +64,10 @@ void gen_block(hipStream_t st, int64_t m, int64_t n, int64_t batch, const double
                const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, double h) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   dim3 grid((m + 63) / 64, (n + 15) / 16, batch);
-  hipLaunchKernelGGL(gen_block_kernel<T>, grid, dim3(256), 0, st, m, n, Rx, Ry, Rw, sR, Cx, Cy,
-                     Cw, sC, A, lda, sA, 0.5 * h * h);
+  const double ent = double(m) * n * batch;
+  HSSC_TIMED(tname<T>("gen_block_f64", "gen_block_f32"), st, ent, ent * sizeof(T),
+             hipLaunchKernelGGL(gen_block_kernel<T>, grid, dim3(256), 0, st, m, n, Rx, Ry, Rw, sR,
+                                Cx, Cy, Cw, sC, A, lda, sA, 0.5 * h * h));
 }
 
 // Parent block F = [E1 K12; K12^T E2] in the parent's J order
@@ -105,8 +113,10 @@ void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int3
                 int64_t sJ, const double* Px, const double* Py, const double* Pw, int64_t sP,
                 const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF, double h) {
   dim3 grid((n + 63) / 64, (n + 15) / 16, batch);
-  hipLaunchKernelGGL(gen_parent_kernel<T>, grid, dim3(256), 0, st, n, kc, J, sJ, Px, Py, Pw, sP,
-                     Ech, sEbox, sEparent, F, sF, 0.5 * h * h);
+  const double ent = double(n) * n * batch;
+  HSSC_TIMED(tname<T>("gen_parent_f64", "gen_parent_f32"), st, ent, ent * sizeof(T),
+             hipLaunchKernelGGL(gen_parent_kernel<T>, grid, dim3(256), 0, st, n, kc, J, sJ, Px, Py,
+                                Pw, sP, Ech, sEbox, sEparent, F, sF, 0.5 * h * h));
 }
 
 // ------------------------------------------------------------------------------------
@@ -117,6 +127,9 @@ void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int3
 // D = B^T A^T so that a lane's 4 accumulators run down one column of C with lane%16
 // indexing rows: column-major stores of 16 consecutive rows coalesce.
 
+// Operand layout (both types): lane l supplies A(l%16, l/16) and B(l/16, l%16).
+// Result layout, measured on gfx950: item r of lane l holds D(row(l,r), l%16) with
+//   f64 16x16x4: row = l/16 + 4r        f32 16x16x4: row = 4*(l/16) + r.
 template <typename T>
 struct Mfma;
 template <>
@@ -125,6 +138,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -132,6 +146,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int GB_M = 64, GB_N = 64, GB_K = 16;
@@ -210,8 +225,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
     }
     __syncthreads();
   }
-  // D(jrow, icol) = sum_k B(k, j) A(i, k): lane holds D rows 4*(lane/16)+r -> C column,
-  // D column lane%16 -> C row.
+  // D(jrow, icol) = sum_k B(k, j) A(i, k): the D row of item r is the C column, the D
+  // column lane%16 is the C row.
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -219,7 +234,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gi = i0 + wi + x * 16 + (lane & 15);
-        const int64_t gj = j0 + wj + y * 16 + 4 * (lane >> 4) + r;
+        const int64_t gj = j0 + wj + y * 16 + Mfma<T>::row(lane, r);
         if (gi < m && gj < n) {
           T* c = C + gi + gj * ldc;
           const T v = alpha * acc[x][y][r];
@@ -235,9 +250,15 @@ void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k
                int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn) {
   if (m <= 0 || n <= 0 || batch <= 0 || rtiles <= 0 || ctiles <= 0) return;
   dim3 grid(rtiles, ctiles, batch);
-#define HSSC_GEMM_LAUNCH(TA_, TB_)                                                            \
-  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(256), 0, st, m, n, kdim, alpha, A, \
-                     lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, cs0, csn)
+  // algorithmic work: rows / cols actually produced (the skipped block excluded)
+  const double rows = double(rs0 < m ? m - rsn : m), cols = double(cs0 < n ? n - csn : n);
+  const double fl = 2.0 * rows * cols * double(kdim) * double(batch);
+  const double by = (rows * kdim + kdim * cols + 2 * rows * cols) * sizeof(T) * double(batch);
+#define HSSC_GEMM_LAUNCH(TA_, TB_)                                                              \
+  HSSC_TIMED(tname<T>("gemm_f64", "gemm_f32"), st, fl, by,                                      \
+             hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(256), 0, st, m, n, kdim,  \
+                                alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, cs0, \
+                                csn))
   if (!ta && !tb) HSSC_GEMM_LAUNCH(false, false);
   else if (ta && !tb) HSSC_GEMM_LAUNCH(true, false);
   else if (!ta && tb) HSSC_GEMM_LAUNCH(false, true);
@@ -373,7 +394,10 @@ bool gj_small(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t batc
       granted = lds;
     }
   }
-  hipLaunchKernelGGL(gj_small_kernel<T>, dim3(batch), dim3(256), lds, st, n, A, lda, sA);
+  const double fl = 2.0 * n * double(n) * n * batch;
+  HSSC_TIMED(tname<T>("gj_small_f64", "gj_small_f32"), st, fl, 2.0 * n * n * sizeof(T) * batch,
+             hipLaunchKernelGGL(gj_small_kernel<T>, dim3(batch), dim3(256), lds, st, n, A, lda,
+                                sA));
   return true;
 }
 
@@ -418,7 +442,10 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
   for (int64_t p = 0; p < n; p += GJ_NB) {
     const int nb = int(p + GJ_NB <= n ? GJ_NB : n - p);
     const int64_t nout = n - nb;
-    hipLaunchKernelGGL(gj_diag_kernel<T>, dim3(batch), dim3(256), 0, st, p, nb, A, lda, sA);
+    HSSC_TIMED(tname<T>("gj_diag_f64", "gj_diag_f32"), st, 2.0 * nb * nb * nb * batch,
+               2.0 * nb * nb * sizeof(T) * batch,
+               hipLaunchKernelGGL(gj_diag_kernel<T>, dim3(batch), dim3(256), 0, st, p, nb, A, lda,
+                                  sA));
     if (nout <= 0) continue;
     const int64_t tiles = (nout + 63) / 64;
     T* Apb = A + p;             // rows B
@@ -633,8 +660,11 @@ __global__ void __launch_bounds__(256) trinv_upper_kernel(int64_t p, int nb, con
 template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
               T* Tf, int64_t sT, int64_t batch) {
-  hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(1024), 0, st, M, j0, jb, A, lda, sA,
-                     Tf, sT);
+  const double rows = double(M - j0);
+  HSSC_TIMED(tname<T>("qr_panel_f64", "qr_panel_f32"), st, 4.0 * rows * jb * jb * batch,
+             2.0 * rows * jb * sizeof(T) * batch,
+             hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(1024), 0, st, M, j0, jb, A,
+                                lda, sA, Tf, sT));
 }
 
 template <typename T>
@@ -736,14 +766,18 @@ void gemv(hipStream_t st, bool trans, int64_t m, int64_t n, T alpha, const T* A,
     return;
   }
   const int64_t nsplit = (inner + GV_CHUNK - 1) / GV_CHUNK;
+  const double fl = 2.0 * m * double(n) * batch;
+  const double by = (double(m) * n + m + n) * sizeof(T) * batch;
   if (!trans) {
     dim3 grid((m + GV_ROWS - 1) / GV_ROWS, nsplit, batch);
-    hipLaunchKernelGGL(gemv_n_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x, sx, work,
-                       nsplit);
+    HSSC_TIMED(tname<T>("gemv_f64", "gemv_f32"), st, fl, by,
+               hipLaunchKernelGGL(gemv_n_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x,
+                                  sx, work, nsplit));
   } else {
     dim3 grid((n + 3) / 4, nsplit, batch);
-    hipLaunchKernelGGL(gemv_t_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x, sx, work,
-                       nsplit);
+    HSSC_TIMED(tname<T>("gemvt_f64", "gemvt_f32"), st, fl, by,
+               hipLaunchKernelGGL(gemv_t_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x,
+                                  sx, work, nsplit));
   }
   hipLaunchKernelGGL(gemv_reduce<T>, dim3((out + 255) / 256, batch), dim3(256), 0, st, out, nsplit,
                      work, alpha, z, sz, gamma, y, sy);

@@ -162,6 +162,18 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
                             void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
                             double* relres, void* stream);
 
+/* ---- instrumentation -------------------------------------------------------------
+ * No counterpart in the reference (which times stages with tic/toc,
+ * HSS2D/Test/Test_HSS2D_InverseCompression.m:44-50): when enabled, every launch of a
+ * kernel class ("gemm_f64", "gemv_f64", "gj_small_f64", "gen_block_f64", "qr_panel_f64",
+ * ... and the _f32 variants) is bracketed by HIP events on its own stream, and its
+ * algorithmic flops and bytes are accumulated.  hssc_stats_get synchronises the pending
+ * events and returns the totals since the last reset. */
+int hssc_stats_enable(int32_t on);
+int hssc_stats_reset(void);
+int hssc_stats_get(const char* kernel, int64_t* launches, double* flops, double* bytes,
+                   double* ms);
+
 #ifdef __cplusplus
 }
 #endif

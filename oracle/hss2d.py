@@ -478,7 +478,7 @@ def build_index_sets(params):
     return build_tree(params, with_T=False)
 
 
-def build_tree(params, interform=None, seed=0, with_T=True):
+def build_tree(params, interform=None, seed=0, with_T=True, timings=None):
     """HSS2D_build_tree_fsym (uniform tree, preset 'bdry' skeletons and proxies).
 
     Restates :81-287 (tree), :292-623 (upward skeleton pass) and
@@ -498,7 +498,9 @@ def build_tree(params, interform=None, seed=0, with_T=True):
     tree.I_src.update(leaf_slabs(params, cent))
     q = np.zeros(depth + 5, dtype=int)  # ranks of low-rank T per level (:299)
     TI = params["transinv"]
+    import time as _time
     for lev in range(depth, -1, -1):
+        _t0 = _time.perf_counter()
         colrad, rowrad = level_radii(lev)
         lay_prox = proxy_layers(lev, params)
         for b in range(2 ** lev, 2 ** (lev + 1)):
@@ -528,6 +530,8 @@ def build_tree(params, interform=None, seed=0, with_T=True):
                     q[lev] = max(q[lev], U.shape[1]) if not TI else U.shape[1]
             J, k = tree.J[key], tree.k[key]
             tree.I_sk[b] = tree.I_src[b][J[:k]]
+        if timings is not None:
+            timings[lev] = timings.get(lev, 0.0) + _time.perf_counter() - _t0
     return tree
 
 
@@ -550,7 +554,7 @@ def R_matrix(J, k, T):
     return R
 
 
-def invert(tree):
+def invert(tree, timings=None):
     """HSS2D/HSS2D_invert_fsym.m:73-195 with Fopts='dense'.
 
     Leaf: FI = inv(K(I_src, I_src)) (:102-111).  Non-leaf: FI = inv([E1 K12; K12^T E2])
@@ -558,9 +562,11 @@ def invert(tree):
     LR=1 branch is the same product evaluated through T.U, T.V)."""
     params = tree.params
     X = params["X_source"]
+    import time as _time
     out = Inverse()
     TI = params["transinv"]
     for lev in range(tree.depth, -1, -1):
+        _t0 = _time.perf_counter()
         boxes = [2 ** lev] if TI else range(2 ** lev, 2 ** (lev + 1))
         for b in boxes:
             key = tree.key(b)
@@ -577,6 +583,8 @@ def invert(tree):
             if lev > 0:
                 R = R_matrix(tree.J[key], tree.k[key], tree.Tdense(b))
                 out.E[key] = np.linalg.inv(R @ FI @ R.T)
+        if timings is not None:
+            timings[lev] = timings.get(lev, 0.0) + _time.perf_counter() - _t0
     return out
 
 

@@ -94,7 +94,11 @@ def test_applyinv_and_apply_vs_dense(Ng, np_, TI):
     yor = o.applyinv(ot, o.invert(ot), b)
     # the plain FP64 GPU path matches the oracle's HSS solution and the dense solution
     assert rel(y, yor) < 1e-9
-    assert rel(y, ytrue) < 2e-9
+    # against A\b the error is the HSS compression error (acc = 1e-10 per level; the
+    # reference prints "on the order of 1e-8 or better", INSTALL.txt:30-31): no worse than
+    # the oracle's own
+    assert rel(y, ytrue) <= 1.5 * rel(yor, ytrue) + 1e-10
+    assert rel(y, ytrue) < 1e-8
     # forward operator vs dense matvec
     assert rel(H.HSS2D_apply_TI(tree, op, ytrue), A @ ytrue) < 1e-9
     # the dense-block inverse is the exact inverse of the compressed operator
