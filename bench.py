@@ -134,8 +134,6 @@ def main():
         one_step(H, P, torch, b, args.tol)
 
     L = H.lib()
-    L.hssc_stats_reset()
-    L.hssc_stats_enable(1)
     torch.cuda.synchronize()
     T0 = time.perf_counter()
     runs = []
@@ -143,6 +141,12 @@ def main():
         runs.append(one_step(H, P, torch, b, args.tol))
     torch.cuda.synchronize()
     T1 = time.perf_counter()
+    # One more step of the same workload with per-kernel HIP-event instrumentation (levels
+    # serialised, so event times are kernel durations comparable with rocprofv3's).
+    L.hssc_stats_reset()
+    L.hssc_stats_enable(1)
+    one_step(H, P, torch, b, args.tol)
+    torch.cuda.synchronize()
     L.hssc_stats_enable(0)
     stats = stats_snapshot(H)
 
@@ -158,7 +162,7 @@ def main():
     total_mixed = build_mixed + mixed["solve"]
     total_fp64 = build_fp64 + fp64["solve"]
 
-    # dominant kernel class over the timed region (both paths)
+    # dominant kernel class over the instrumented step (both paths)
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"]) if stats else None
     roof = None
     if dom:
@@ -209,7 +213,10 @@ def main():
                  "N_per_s": round(N / total_fp64, 1)},
         "speedup_mixed_vs_fp64": round(total_fp64 / total_mixed, 3),
         "roofline": roof,
-        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"], 3)) for k, v in stats.items()},
+        "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
+                            tflops=round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else None,
+                            gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
+                    for k, v in stats.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -32,11 +32,19 @@ void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
                int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn);
 
+// C = A^T B with the long K dimension split in chunks of kchunk rows (deterministic sum);
+// part needs batch * ceil(kdim/kchunk) * m * n elements.
+template <typename T>
+void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
+                    int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
+                    int64_t batch, T* part, int64_t kchunk);
+
 // In-place inverse of a batch of n x n matrices.
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
 
 constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
+constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (QR_NT * QR_RS)
 
 template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,

@@ -159,14 +159,23 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
                                                   const T* B, int64_t ldb, int64_t sB,
                                                   T beta, T* C, int64_t ldc,
                                                   int64_t sC, int64_t rs0, int64_t rsn, int64_t cs0,
-                                                  int64_t csn) {
+                                                  int64_t csn, int64_t ksplit, int64_t kchunk) {
   using v4 = typename Mfma<T>::v4;
   __shared__ T As[GB_K][GB_M + 1];  // As[kk][i] = op(A)(i0+i, k0+kk)
   __shared__ T Bs[GB_K][GB_N + 1];  // Bs[kk][j] = op(B)(k0+kk, j0+j)
-  const int64_t e = blockIdx.z;
+  // blockIdx.z = e * ksplit + q: batch element e, K chunk q (rows [q*kchunk, ...) of op(B)
+  // and columns of op(A)); chunk q writes its own output slice C + (e*ksplit + q)*sC.
+  const int64_t z = blockIdx.z;
+  const int64_t e = z / ksplit, q = z % ksplit;
   A += e * sA;
   B += e * sB;
-  C += e * sC;
+  C += z * sC;
+  if (ksplit > 1) {
+    const int64_t k0q = q * kchunk;
+    A += TA ? k0q : k0q * lda;
+    B += TB ? k0q * ldb : k0q;
+    kdim = (kdim - k0q < kchunk) ? kdim - k0q : kchunk;
+  }
   int64_t i0 = int64_t(blockIdx.x) * GB_M, j0 = int64_t(blockIdx.y) * GB_N;
   if (i0 >= rs0) i0 += rsn;
   if (j0 >= cs0) j0 += csn;
@@ -244,12 +253,29 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 }
 
 template <typename T>
+void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
+                     const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
+                     T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
+                     int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
+                     int64_t ksplit, int64_t kchunk);
+
+template <typename T>
 void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
                const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta,
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
                int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn) {
+  gemm_skip_split<T>(st, ta, tb, m, n, kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                     rtiles, ctiles, rs0, rsn, cs0, csn, 1, kdim);
+}
+
+template <typename T>
+void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
+                     const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
+                     T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
+                     int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
+                     int64_t ksplit, int64_t kchunk) {
   if (m <= 0 || n <= 0 || batch <= 0 || rtiles <= 0 || ctiles <= 0) return;
-  dim3 grid(rtiles, ctiles, batch);
+  dim3 grid(rtiles, ctiles, batch * ksplit);
   // algorithmic work: rows / cols actually produced (the skipped block excluded)
   const double rows = double(rs0 < m ? m - rsn : m), cols = double(cs0 < n ? n - csn : n);
   const double fl = 2.0 * rows * cols * double(kdim) * double(batch);
@@ -258,7 +284,7 @@ void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k
   HSSC_TIMED(tname<T>("gemm_f64", "gemm_f32"), st, fl, by,                                      \
              hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(256), 0, st, m, n, kdim,  \
                                 alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, cs0, \
-                                csn))
+                                csn, ksplit, kchunk))
   if (!ta && !tb) HSSC_GEMM_LAUNCH(false, false);
   else if (ta && !tb) HSSC_GEMM_LAUNCH(true, false);
   else if (!ta && tb) HSSC_GEMM_LAUNCH(false, true);
@@ -336,6 +362,133 @@ __device__ void gj_lds(T* S, int n, int ld, int* piv, T* f, int tid, int nthr) {
       }
       __syncthreads();
     }
+  }
+}
+
+// Register-resident Gauss-Jordan for n <= 64*S (n <= 4*CB): 256 threads, lanes are rows
+// (row = lane + 64*s), wave w owns columns [w*CB, (w+1)*CB).  Per pivot column c:
+//   * the wave owning column c finds the pivot row by a shuffle argmax over rows >= c and
+//     publishes it, with the old column c, through double-buffered LDS;
+//   * one barrier;
+//   * every wave reads the old rows c and p of its own columns by lane shuffles, applies
+//     the swap logically and performs the sweep on its registers.
+// Column unscrambling (swaps undone in reverse order) is folded into the final store.
+template <typename T, int S, int CB>
+__global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, int64_t lda,
+                                                     int64_t sA, int64_t off) {
+  __shared__ T colbuf[2][64 * S];
+  __shared__ int spiv[2];
+  __shared__ int pivs[64 * S];
+  __shared__ int dest[64 * S];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  A += int64_t(blockIdx.x) * sA + off;
+  T v[S][CB];
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      const int i = lane + 64 * s, j = wave * CB + t;
+      v[s][t] = (i < n && j < n) ? A[i + int64_t(j) * lda] : T(0);
+    }
+  // candidates for column 0
+  auto publish = [&](int c) {
+    // called by the wave owning column c
+    const int tc = c - wave * CB;
+    T best = T(-1);
+    int bi = n;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      T x = T(0);
+#pragma unroll
+      for (int t = 0; t < CB; ++t)
+        if (t == tc) x = v[s][t];
+      const int i = lane + 64 * s;
+      if (i < n) colbuf[c & 1][i] = x;
+      const T ax = fabs(x);
+      if (i >= c && i < n && ax > best) best = ax, bi = i;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const T ob = __shfl_xor(best, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
+    }
+    if (lane == 0) spiv[c & 1] = bi, pivs[c] = bi;
+  };
+  if (wave == 0) publish(0);
+  for (int c = 0; c < n; ++c) {
+    __syncthreads();
+    const int p = spiv[c & 1];
+    const T* cb = colbuf[c & 1];
+    const T piv = cb[p];
+    const T inv = T(1) / piv;
+    // old rows c and p of this wave's columns
+    T rc[CB], rp[CB];
+    const int sc = c >> 6, lc = c & 63, sp = p >> 6, lp = p & 63;
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      T xc = v[0][t], xp = v[0][t];
+#pragma unroll
+      for (int s = 1; s < S; ++s) {
+        if (s == sc) xc = v[s][t];
+        if (s == sp) xp = v[s][t];
+      }
+      rc[t] = __shfl(xc, lc);
+      rp[t] = __shfl(xp, lp);
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int i = lane + 64 * s;
+      const T f = (i == c) ? cb[p] : (i == p) ? cb[c] : cb[i < n ? i : 0];
+#pragma unroll
+      for (int t = 0; t < CB; ++t) {
+        const int j = wave * CB + t;
+        T old = (i == c) ? rp[t] : (i == p) ? rc[t] : v[s][t];
+        T nv;
+        if (i == c) nv = (j == c) ? inv : rp[t] * inv;
+        else nv = (j == c) ? -f * inv : old - f * (rp[t] * inv);
+        v[s][t] = nv;
+      }
+    }
+    if (c + 1 < n && wave == (c + 1) / CB) publish(c + 1);
+  }
+  __syncthreads();
+  // final column j of the inverse is column sigma(j) of the swept matrix, where sigma
+  // applies the swaps (c, pivs[c]) for c = n-1 .. 0; each column is stored at
+  // dest = sigma^-1(its index).
+  if (threadIdx.x == 0) {
+    int* sig = dest;  // reuse: compute sigma in place, then invert into colbuf-free space
+    for (int j = 0; j < n; ++j) sig[j] = j;
+    for (int c = n - 1; c >= 0; --c) {
+      const int q = pivs[c];
+      const int a = sig[c];
+      sig[c] = sig[q];
+      sig[q] = a;
+    }
+    // invert sigma into pivs (no longer needed)
+    for (int j = 0; j < n; ++j) pivs[sig[j]] = j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      const int i = lane + 64 * s, j = wave * CB + t;
+      if (i < n && j < n) A[i + int64_t(pivs[j]) * lda] = v[s][t];
+    }
+}
+
+template <typename T>
+void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, int64_t batch,
+            const char* label) {
+  const double fl = 2.0 * n * double(n) * n * batch;
+  if (n <= 64) {
+    HSSC_TIMED(label, st, fl, 2.0 * n * n * sizeof(T) * batch,
+               hipLaunchKernelGGL((gj_reg_kernel<T, 1, 16>), dim3(batch), dim3(256), 0, st, n, A,
+                                  lda, sA, off));
+  } else {
+    HSSC_TIMED(label, st, fl, 2.0 * n * n * sizeof(T) * batch,
+               hipLaunchKernelGGL((gj_reg_kernel<T, 2, 32>), dim3(batch), dim3(256), 0, st, n, A,
+                                  lda, sA, off));
   }
 }
 
@@ -442,10 +595,7 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
   for (int64_t p = 0; p < n; p += GJ_NB) {
     const int nb = int(p + GJ_NB <= n ? GJ_NB : n - p);
     const int64_t nout = n - nb;
-    HSSC_TIMED(tname<T>("gj_diag_f64", "gj_diag_f32"), st, 2.0 * nb * nb * nb * batch,
-               2.0 * nb * nb * sizeof(T) * batch,
-               hipLaunchKernelGGL(gj_diag_kernel<T>, dim3(batch), dim3(256), 0, st, p, nb, A, lda,
-                                  sA));
+    gj_reg<T>(st, nb, A, lda, sA, p + p * lda, batch, tname<T>("gj_diag_f64", "gj_diag_f32"));
     if (nout <= 0) continue;
     const int64_t tiles = (nout + 63) / 64;
     T* Apb = A + p;             // rows B
@@ -466,7 +616,10 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
   if (n <= 0 || batch <= 0) return;
-  if (n <= 128 && gj_small<T>(st, int(n), A, lda, sA, batch)) return;
+  if (n <= 128) {
+    gj_reg<T>(st, int(n), A, lda, sA, 0, batch, tname<T>("gj_small_f64", "gj_small_f32"));
+    return;
+  }
   gj_blocked<T>(st, n, A, lda, sA, batch);
 }
 
@@ -491,114 +644,269 @@ __device__ T block_sum(T v, T* red, int tid) {
   return s;
 }
 
-constexpr int QR_NB = 32;
+constexpr int QR_NB = 32;  // outer panel width (the trailing update runs as GEMMs)
+constexpr int QR_IB = 4;   // inner block, held in registers
+constexpr int QR_RS = 16;  // row slots per thread: M - j0 <= QR_NT * QR_RS
+constexpr int QR_NT = 512; // threads per panel workgroup (8 waves)
+constexpr int QR_AC = 4;   // reflectors accumulated per reduction pass
 
 template <typename T>
-__global__ void __launch_bounds__(1024) qr_panel_kernel(int64_t M, int64_t j0, int jb,
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Sums NV per-thread values over the whole (QR_NT-thread) workgroup: wave shuffles, then
+// one LDS pass.  out[v] is valid in every thread after the call.
+template <typename T, int NV>
+__device__ __forceinline__ void qr_block_reduce(T (&vals)[NV], T (*red)[NV], T* out, int tid) {
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) vals[v] = wave_sum(vals[v]);
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) red[wave][v] = vals[v];
+  }
+  __syncthreads();
+  if (tid < NV) {
+    T sum = T(0);
+    for (int w = 0; w < QR_NT / 64; ++w) sum += red[w][tid];
+    out[tid] = sum;
+  }
+  __syncthreads();
+}
+
+// Left-looking panel: for each inner block of QR_IB columns, (1) apply the panel's earlier
+// reflectors through the compact-WY factor, (2) factor the block in registers with one
+// fused reduction per column (norm, dot products and the diagonal row broadcast through
+// double-buffered LDS, one barrier), (3) extend the Gram matrix V^T V and the triangular
+// factor Tf by the block's columns.  Row r = j0 + tid + QR_NT*s of the panel lives in slot
+// s of thread tid.
+template <typename T>
+__global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, int jb,
                                                         T* __restrict__ A, int64_t lda, int64_t sA,
-                                                        T* __restrict__ Tf, int64_t sT) {
-  __shared__ T red[16 * QR_NB];
-  __shared__ T sdot[QR_NB];
-  __shared__ T stau[QR_NB];
+                                                        T* __restrict__ Tfo, int64_t sT) {
+  __shared__ T Tf[QR_NB][QR_NB + 1];
   __shared__ T G[QR_NB][QR_NB + 1];
+  __shared__ T W[QR_NB][QR_IB];
+  __shared__ T Wp[QR_NB][QR_IB];
+  __shared__ T red[2][QR_NT / 64][QR_IB];
+  __shared__ T rowv[2][QR_IB];
+  __shared__ T tau_s[QR_NB];
+  __shared__ T tmp[QR_NB];
+  __shared__ T redW[QR_NT / 64][QR_AC * QR_IB];
+  __shared__ T GW[QR_AC][QR_IB];
   A += int64_t(blockIdx.x) * sA;
-  Tf += int64_t(blockIdx.x) * sT;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nthr = blockDim.x;
-  const int nw = nthr >> 6;
-  for (int c = 0; c < jb; ++c) {
-    const int64_t col = j0 + c;
-    const int64_t r0 = j0 + c;  // diagonal row
-    T* a = A + col * lda;
-    // norm^2 of a(r0+1 .. M-1)
-    T s = T(0);
-    for (int64_t r = r0 + 1 + tid; r < M; r += nthr) s += a[r] * a[r];
-    s = block_sum<T>(s, red, tid);
-    const T alpha = a[r0];
-    T beta, tau, scal;
-    if (s == T(0)) {
-      tau = T(0), beta = alpha, scal = T(0);
-    } else {
-      T nrm = sqrt(alpha * alpha + s);
-      beta = alpha >= T(0) ? -nrm : nrm;
-      tau = (beta - alpha) / beta;
-      scal = T(1) / (alpha - beta);
-    }
-    __syncthreads();
-    for (int64_t r = r0 + 1 + tid; r < M; r += nthr) a[r] *= scal;
-    if (tid == 0) {
-      a[r0] = beta;
-      stau[c] = tau;
-    }
-    __syncthreads();
-    // apply H_c to the remaining panel columns: w_j = v^T a_j, a_j -= tau v w_j
-    const int nrest = jb - c - 1;
-    if (nrest > 0 && tau != T(0)) {
-      T part[QR_NB];
+  Tfo += int64_t(blockIdx.x) * sT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int idx = tid; idx < QR_NB * (QR_NB + 1); idx += QR_NT) {
+    (&Tf[0][0])[idx] = T(0);
+    (&G[0][0])[idx] = T(0);
+  }
+  __syncthreads();
+  // local row index l = r - j0 (int); the panel block A(j0 + l, j0 + c) is P[l + c*lda]
+  T* P = A + j0 + j0 * lda;
+  const int Ml = int(M - j0);
+  T x[QR_RS][QR_IB];
+  int buf = 0;
+  for (int c0 = 0; c0 < jb; c0 += QR_IB) {
+    const int ib = (jb - c0 < QR_IB) ? jb - c0 : QR_IB;
+    T* Pb = P + int64_t(c0) * lda;  // first column of the block
+    // load the raw block
 #pragma unroll
-      for (int j = 0; j < QR_NB; ++j) part[j] = T(0);
-      for (int64_t r = r0 + tid; r < M; r += nthr) {
-        const T v = (r == r0) ? T(1) : a[r];
+    for (int s = 0; s < QR_RS; ++s) {
+      const int l = tid + QR_NT * s;
 #pragma unroll
-        for (int j = 0; j < QR_NB; ++j)
-          if (j < nrest) part[j] += v * A[r + (col + 1 + j) * lda];
+      for (int jj = 0; jj < QR_IB; ++jj)
+        x[s][jj] = (l < Ml && jj < ib) ? Pb[l + int64_t(jj) * lda] : T(0);
+    }
+    // (1) W = V_prev^T x (QR_AC reflectors per pass), W' = Tf^T W, x -= V_prev W'
+    if (c0 > 0) {
+      for (int a0 = 0; a0 < c0; a0 += QR_AC) {
+        T acc[QR_AC * QR_IB];
+#pragma unroll
+        for (int v = 0; v < QR_AC * QR_IB; ++v) acc[v] = T(0);
+#pragma unroll
+        for (int s = 0; s < QR_RS; ++s) {
+          const int l = tid + QR_NT * s;
+          if (l >= Ml || l < a0) continue;
+#pragma unroll
+          for (int u = 0; u < QR_AC; ++u) {
+            const int a = a0 + u;
+            if (a < c0 && l >= a) {
+              const T v = (l == a) ? T(1) : P[l + int64_t(a) * lda];
+#pragma unroll
+              for (int jj = 0; jj < QR_IB; ++jj) acc[u * QR_IB + jj] += v * x[s][jj];
+            }
+          }
+        }
+        qr_block_reduce<T, QR_AC * QR_IB>(acc, redW, &W[a0][0], tid);
       }
-#pragma unroll
-      for (int j = 0; j < QR_NB; ++j) {
-        T v = part[j];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0 && j < nrest) red[wave * QR_NB + j] = v;
+      if (tid < c0 * QR_IB) {
+        const int a = tid / QR_IB, jj = tid % QR_IB;
+        T sum = T(0);
+        for (int b = 0; b <= a; ++b) sum += Tf[b][a] * W[b][jj];
+        Wp[a][jj] = sum;
       }
       __syncthreads();
-      if (tid < nrest) {
-        T t = T(0);
-        for (int w = 0; w < nw; ++w) t += red[w * QR_NB + tid];
-        sdot[tid] = t * tau;
+#pragma unroll
+      for (int s = 0; s < QR_RS; ++s) {
+        const int l = tid + QR_NT * s;
+        if (l >= Ml) continue;
+        const int amax = (c0 < l + 1) ? c0 : l + 1;  // v_a(l) = 0 for l < a
+        for (int a = 0; a < amax; ++a) {
+          const T v = (l == a) ? T(1) : P[l + int64_t(a) * lda];
+#pragma unroll
+          for (int jj = 0; jj < QR_IB; ++jj) x[s][jj] -= v * Wp[a][jj];
+        }
+      }
+    }
+    // (2) factor the block
+#pragma unroll
+    for (int c = 0; c < QR_IB; ++c) {
+      if (c < ib) {
+        const int ld = c0 + c;  // local row of the diagonal
+        const int own = ld % QR_NT, so = ld / QR_NT;
+        T sig = T(0), d[QR_IB];
+#pragma unroll
+        for (int jj = 0; jj < QR_IB; ++jj) d[jj] = T(0);
+#pragma unroll
+        for (int s = 0; s < QR_RS; ++s) {
+          const int l = tid + QR_NT * s;
+          if (l >= ld && l < Ml) {
+            const T xc = x[s][c];
+            if (l > ld) sig += xc * xc;
+#pragma unroll
+            for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] += xc * x[s][jj];
+          }
+        }
+        sig = wave_sum(sig);
+#pragma unroll
+        for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] = wave_sum(d[jj]);
+        if (lane == 0) {
+          red[buf][wave][0] = sig;
+#pragma unroll
+          for (int jj = c + 1; jj < QR_IB; ++jj) red[buf][wave][jj] = d[jj];
+        }
+        if (tid == own) {
+#pragma unroll
+          for (int s = 0; s < QR_RS; ++s)
+            if (s == so) {
+              rowv[buf][0] = x[s][c];
+#pragma unroll
+              for (int jj = c + 1; jj < QR_IB; ++jj) rowv[buf][jj] = x[s][jj];
+            }
+        }
+        __syncthreads();
+        sig = T(0);
+#pragma unroll
+        for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] = T(0);
+        for (int w = 0; w < QR_NT / 64; ++w) {
+          sig += red[buf][w][0];
+#pragma unroll
+          for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] += red[buf][w][jj];
+        }
+        const T alpha = rowv[buf][0];
+        T beta, tau, scal;
+        if (sig == T(0)) {
+          tau = T(0), beta = alpha, scal = T(0);
+        } else {
+          const T nrm = sqrt(alpha * alpha + sig);
+          beta = alpha >= T(0) ? -nrm : nrm;
+          tau = (beta - alpha) / beta;
+          scal = T(1) / (alpha - beta);
+        }
+        // w_j = v^T a_j = a_j(ld) + (sum_{l>=ld} x_l a_j(l) - alpha a_j(ld)) * scal
+        T tw[QR_IB];
+#pragma unroll
+        for (int jj = 0; jj < QR_IB; ++jj) tw[jj] = T(0);
+#pragma unroll
+        for (int jj = c + 1; jj < QR_IB; ++jj) {
+          const T arow = rowv[buf][jj];
+          tw[jj] = tau * (arow + (d[jj] - alpha * arow) * scal);
+        }
+#pragma unroll
+        for (int s = 0; s < QR_RS; ++s) {
+          const int l = tid + QR_NT * s;
+          if (l >= ld && l < Ml) {
+            const T v = (l == ld) ? T(1) : x[s][c] * scal;
+#pragma unroll
+            for (int jj = c + 1; jj < QR_IB; ++jj) x[s][jj] -= v * tw[jj];
+            x[s][c] = (l == ld) ? beta : v;
+          }
+        }
+        if (tid == 0) tau_s[c0 + c] = tau;
+        buf ^= 1;
+      }
+    }
+    // store the block
+#pragma unroll
+    for (int s = 0; s < QR_RS; ++s) {
+      const int l = tid + QR_NT * s;
+      if (l < Ml) {
+#pragma unroll
+        for (int jj = 0; jj < QR_IB; ++jj)
+          if (jj < ib) Pb[l + int64_t(jj) * lda] = x[s][jj];
+      }
+    }
+    __syncthreads();
+    // (3) Gram columns G(a, c0+c) = v_a^T v_{c0+c} for a < c0+c, accumulated from the
+    // block's registers (v_c: 0 above its diagonal, 1 on it, x below) against the earlier
+    // reflectors (QR_AC per pass) and the block itself.
+    for (int a0 = 0; a0 < c0 + QR_IB; a0 += QR_AC) {
+      T acc[QR_AC * QR_IB];
+#pragma unroll
+      for (int v = 0; v < QR_AC * QR_IB; ++v) acc[v] = T(0);
+#pragma unroll
+      for (int s = 0; s < QR_RS; ++s) {
+        const int l = tid + QR_NT * s;
+        if (l >= Ml || l < c0) continue;
+        T vb[QR_IB];
+#pragma unroll
+        for (int c = 0; c < QR_IB; ++c)
+          vb[c] = (c >= ib || l < c0 + c) ? T(0) : (l == c0 + c ? T(1) : x[s][c]);
+#pragma unroll
+        for (int u = 0; u < QR_AC; ++u) {
+          const int a = a0 + u;
+          T va;
+          if (a < c0) {
+            va = (l < a) ? T(0) : (l == a ? T(1) : P[l + int64_t(a) * lda]);
+          } else {
+            T t = T(0);
+#pragma unroll
+            for (int c = 0; c < QR_IB; ++c)
+              if (a - c0 == c) t = vb[c];
+            va = t;
+          }
+#pragma unroll
+          for (int c = 0; c < QR_IB; ++c) acc[u * QR_IB + c] += va * vb[c];
+        }
+      }
+      qr_block_reduce<T, QR_AC * QR_IB>(acc, redW, &GW[0][0], tid);
+      if (tid < QR_AC * QR_IB) {
+        const int u = tid / QR_IB, c = tid % QR_IB;
+        const int a = a0 + u, ci = c0 + c;
+        if (c < ib && a < ci) G[a][ci] = GW[u][c];
       }
       __syncthreads();
-      for (int64_t r = r0 + tid; r < M; r += nthr) {
-        const T v = (r == r0) ? T(1) : a[r];
-        for (int j = 0; j < nrest; ++j) A[r + (col + 1 + j) * lda] -= v * sdot[j];
+    }
+    // (4) Tf(0:i, i) = -tau_i Tf(0:i, 0:i) G(0:i, i), Tf(i, i) = tau_i
+    for (int c = 0; c < ib; ++c) {
+      const int i = c0 + c;
+      if (tid < i) {
+        T sum = T(0);
+        for (int q = tid; q < i; ++q) sum += Tf[tid][q] * G[q][i];
+        tmp[tid] = sum;
       }
+      __syncthreads();
+      if (tid < i) Tf[tid][i] = -tau_s[i] * tmp[tid];
+      if (tid == i) Tf[i][i] = tau_s[i];
       __syncthreads();
     }
   }
-  // Gram matrix G = V^T V (unit lower trapezoidal V, rows j0..M-1).
-  {
-    for (int idx = tid; idx < jb * jb; idx += nthr) {
-      const int i = idx % jb, j = idx / jb;
-      G[i][j] = T(0);
-    }
-    __syncthreads();
-    // each wave accumulates a subset of (i, j>i) pairs over all rows
-    for (int pair = wave; pair < jb * jb; pair += nw) {
-      const int i = pair % jb, j = pair / jb;
-      if (j <= i) continue;
-      T s = T(0);
-      // v_i(r) for r >= j0+i with v_i(j0+i) = 1; v_j(r) zero for r < j0+j, one at j0+j
-      for (int64_t r = j0 + j + lane; r < M; r += 64) {
-        const T vi = A[r + (j0 + i) * lda];  // r > j0+i always here
-        const T vj = (r == j0 + j) ? T(1) : A[r + (j0 + j) * lda];
-        s += vi * vj;
-      }
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) G[i][j] = s;
-    }
-    __syncthreads();
-    // Tf(0:i, i) = -tau_i * Tf(0:i, 0:i) * G(0:i, i);  Tf(i,i) = tau_i
-    if (tid == 0) {
-      for (int i = 0; i < jb; ++i) {
-        for (int r = 0; r < i; ++r) {
-          T t = T(0);
-          for (int q = r; q < i; ++q) t += Tf[r + q * QR_NB] * G[q][i];
-          red[r] = t;
-        }
-        for (int r = 0; r < i; ++r) Tf[r + i * QR_NB] = -stau[i] * red[r];
-        Tf[i + i * QR_NB] = stau[i];
-        for (int r = i + 1; r < QR_NB; ++r) Tf[r + i * QR_NB] = T(0);
-      }
-      for (int i = jb; i < QR_NB; ++i)
-        for (int r = 0; r < QR_NB; ++r) Tf[r + i * QR_NB] = T(0);
-    }
+  for (int idx = tid; idx < QR_NB * QR_NB; idx += QR_NT) {
+    const int r = idx % QR_NB, i = idx / QR_NB;
+    Tfo[r + i * QR_NB] = Tf[r][i];
   }
 }
 
@@ -663,7 +971,7 @@ void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, 
   const double rows = double(M - j0);
   HSSC_TIMED(tname<T>("qr_panel_f64", "qr_panel_f32"), st, 4.0 * rows * jb * jb * batch,
              2.0 * rows * jb * sizeof(T) * batch,
-             hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(1024), 0, st, M, j0, jb, A,
+             hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(QR_NT), 0, st, M, j0, jb, A,
                                 lda, sA, Tf, sT));
 }
 
@@ -895,6 +1203,39 @@ void add2d(hipStream_t st, int64_t m, int64_t n, T a, const T* s1, int64_t ld1, 
                      dst, ldd, sd);
 }
 
+// out[e][i] = sum_{q < nsplit} part[e*nsplit + q][i]   (split-K reduction, fixed order)
+template <typename T>
+__global__ void __launch_bounds__(256) sum_split_kernel(int64_t len, int64_t nsplit,
+                                                        const T* __restrict__ part,
+                                                        T* __restrict__ out, int64_t sout) {
+  const int64_t e = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= len) return;
+  T s = T(0);
+  for (int64_t q = 0; q < nsplit; ++q) s += part[(e * nsplit + q) * len + i];
+  out[e * sout + i] = s;
+}
+
+// C (m x n) = A^T B for tall A (K x m) and B (K x n) with a long K: the K range is cut in
+// nsplit chunks computed as independent GEMMs into `part`, then summed in a fixed order.
+template <typename T>
+void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
+                    int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
+                    int64_t batch, T* part, int64_t kchunk) {
+  const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
+  if (nsplit <= 1 || ldc != m) {
+    gemm<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
+    return;
+  }
+  const int64_t len = m * n;
+  const int64_t big = INT64_MAX;
+  gemm_skip_split<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), part, m, len,
+                     batch, (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0, nsplit,
+                     kchunk);
+  hipLaunchKernelGGL(sum_split_kernel<T>, dim3((len + 255) / 256, batch), dim3(256), 0, st, len,
+                     nsplit, part, C, sC);
+}
+
 // Sum of squares (for residual norms), deterministic two-phase.
 template <typename T>
 __global__ void __launch_bounds__(256) sumsq_partial(int64_t len, const T* __restrict__ x,
@@ -941,6 +1282,9 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void gemv<T>(hipStream_t, bool, int64_t, int64_t, T, const T*, int64_t, int64_t,        \
                         const T*, int64_t, T, const T*, int64_t, T*, int64_t, int64_t, T*);        \
   template void sumsq<T>(hipStream_t, int64_t, const T*, double*, int);                           \
+  template void gemm_tn_splitk<T>(hipStream_t, int64_t, int64_t, int64_t, const T*, int64_t,       \
+                                  int64_t, const T*, int64_t, int64_t, T*, int64_t, int64_t,       \
+                                  int64_t, T*, int64_t);                           \
   template void add2d<T>(hipStream_t, int64_t, int64_t, T, const T*, int64_t, int64_t, T, const T*, \
                          int64_t, int64_t, T*, int64_t, int64_t, int64_t);
 

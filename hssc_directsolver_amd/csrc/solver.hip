@@ -20,6 +20,7 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "stats.h"
 
 namespace hssc {
 
@@ -224,71 +225,89 @@ void upload_tree(hssc_tree* t) {
 // by blocked Householder QR of the augmented M x n block [K11 K21] on its first k
 // columns, then back substitution with R11.  The root's T is never used by invert or the
 // sweeps (the root has no E), so it is not formed.
-void compute_interpolation(hssc_tree* t, hipStream_t st) {
+void compute_interpolation(hssc_tree* t, hipStream_t st0) {
   const double h = t->p.h;
-  int64_t amax = 0, vmax = 0, wmax = 0;
-  for (int l = 1; l <= t->depth; ++l) {
-    const Level& L = t->levels[l];
-    if (L.n == L.k) continue;
-    amax = std::max(amax, L.nkey * L.M * L.n);
-    vmax = std::max(vmax, L.nkey * L.M * k::kQrNb);
-    wmax = std::max(wmax, L.nkey * k::kQrNb * L.n);
-  }
-  double* A = dalloc<double>(amax);
-  double* V = dalloc<double>(vmax);
-  double* W1 = dalloc<double>(wmax);
-  double* W2 = dalloc<double>(wmax);
   const int64_t nb = k::kQrNb;
-  int64_t kmax = 1;
-  for (const Level& L : t->levels) kmax = std::max(kmax, L.nkey);
-  double* Tf = dalloc<double>(kmax * nb * nb);
-  for (int l = 1; l <= t->depth; ++l) {
-    Level& L = t->levels[l];
-    const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
-    L.dT = dalloc<double>(E * kk * std::max<int64_t>(r, 1));
-    if (r == 0) continue;
-    if (M < kk) {
-      throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
+  const int64_t kchunk = 512;
+  // Levels are independent: each level's QR sequence runs on its own non-blocking stream so
+  // the few large boxes of the top levels overlap with the many small boxes below.
+  struct Ws {
+    double *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
+  };
+  std::vector<Ws> ws(t->depth + 1);
+  // With instrumentation on, the levels run on one stream so that every kernel's event
+  // time is its own duration (no overlap with other levels).
+  const int nstreams = stats::enabled() ? 1 : std::max(1, std::min(t->depth, 8));
+  std::vector<hipStream_t> streams(nstreams, nullptr);
+  HIPCHK(hipStreamSynchronize(st0));  // the tree upload is complete before the level streams start
+  for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  auto cleanup = [&]() {
+    for (auto& s : streams)
+      if (s) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
+    for (auto& w : ws) dfree(w.A), dfree(w.V), dfree(w.W1), dfree(w.W2), dfree(w.Tf), dfree(w.part);
+  };
+  try {
+    for (int l = t->depth; l >= 1; --l) {
+      Level& L = t->levels[l];
+      const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+      L.dT = dalloc<double>(E * kk * std::max<int64_t>(r, 1));
+      L.q = r;
+      if (r == 0) continue;
+      if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
+      if (M > k::kQrMaxRows)
+        throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy larger than the QR panel kernel supports")};
+      hipStream_t st = streams[(l - 1) % nstreams];
+      Ws& w = ws[l];
+      const int64_t sA = M * n;
+      const int64_t nsplit = (M + kchunk - 1) / kchunk;
+      w.A = dalloc<double>(E * sA);
+      w.V = dalloc<double>(E * M * nb);
+      w.W1 = dalloc<double>(E * nb * n);
+      w.W2 = dalloc<double>(E * nb * n);
+      w.Tf = dalloc<double>(E * nb * nb);
+      w.part = dalloc<double>(E * nsplit * nb * n);
+      double* A = w.A;
+      // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
+      k::gen_block<double>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M,
+                           sA, h);
+      for (int64_t j0 = 0; j0 < kk; j0 += nb) {
+        const int jb = int(std::min(nb, kk - j0));
+        k::qr_panel<double>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
+        const int64_t nc = n - j0 - jb;
+        if (nc <= 0) continue;
+        const int64_t Mp = M - j0;
+        k::qr_copyV<double>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
+        double* C = A + j0 + (j0 + jb) * M;
+        // W1 = V^T C ; W2 = Tf^T W1 ; C -= V W2
+        k::gemm_tn_splitk<double>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
+                                  w.part, kchunk);
+        k::gemm<double>(st, true, false, jb, nc, jb, 1.0, w.Tf, nb, nb * nb, w.W1, jb, jb * nc, 0.0,
+                        w.W2, jb, jb * nc, E);
+        k::gemm<double>(st, false, false, Mp, nc, jb, -1.0, w.V, Mp, Mp * nb, w.W2, jb, jb * nc, 1.0,
+                        C, M, sA, E);
+      }
+      // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
+      double* C1 = A + kk * M;
+      const int64_t last = ((kk - 1) / nb) * nb;
+      for (int64_t p = last; p >= 0; p -= nb) {
+        const int bsz = int(std::min(nb, kk - p));
+        k::trinv_upper<double>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
+        // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
+        k::gemm<double>(st, false, false, bsz, r, bsz, 1.0, w.Tf, bsz, nb * nb, C1 + p, M, sA, 0.0,
+                        C1 + p, M, sA, E);
+        if (p > 0)
+          k::gemm<double>(st, false, false, p, r, bsz, -1.0, A + p * M, M, sA, C1 + p, M, sA, 1.0,
+                          C1, M, sA, E);
+      }
+      k::copy2d<double, double>(st, kk, r, C1, M, sA, L.dT, kk, kk * r, E);
     }
-    const int64_t sA = M * n;
-    // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
-    k::gen_block<double>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
-                         h);
-    for (int64_t j0 = 0; j0 < kk; j0 += nb) {
-      const int jb = int(std::min(nb, kk - j0));
-      k::qr_panel<double>(st, M, j0, jb, A, M, sA, Tf, nb * nb, E);
-      const int64_t nc = n - j0 - jb;
-      if (nc <= 0) continue;
-      const int64_t Mp = M - j0;
-      k::qr_copyV<double>(st, M, j0, jb, A, M, sA, V, Mp, Mp * nb, E);
-      double* C = A + j0 + (j0 + jb) * M;
-      // W1 = V^T C ; W2 = Tf^T W1 ; C -= V W2
-      k::gemm<double>(st, true, false, jb, nc, Mp, 1.0, V, Mp, Mp * nb, C, M, sA, 0.0, W1, nb,
-                      nb * nc, E);
-      k::gemm<double>(st, true, false, jb, nc, jb, 1.0, Tf, nb, nb * nb, W1, nb, nb * nc, 0.0, W2,
-                      nb, nb * nc, E);
-      k::gemm<double>(st, false, false, Mp, nc, jb, -1.0, V, Mp, Mp * nb, W2, nb, nb * nc, 1.0, C,
-                      M, sA, E);
-    }
-    // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
-    double* C1 = A + kk * M;
-    const int64_t last = ((kk - 1) / nb) * nb;
-    for (int64_t p = last; p >= 0; p -= nb) {
-      const int bsz = int(std::min(nb, kk - p));
-      k::trinv_upper<double>(st, p, bsz, A, M, sA, Tf, nb * nb, E);
-      // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
-      k::gemm<double>(st, false, false, bsz, r, bsz, 1.0, Tf, bsz, nb * nb, C1 + p, M, sA, 0.0,
-                      C1 + p, M, sA, E);
-      if (p > 0)
-        k::gemm<double>(st, false, false, p, r, bsz, -1.0, A + p * M, M, sA, C1 + p, M, sA, 1.0,
-                        C1, M, sA, E);
-    }
-    k::copy2d<double, double>(st, kk, r, C1, M, sA, L.dT, kk, kk * r, E);
-    L.q = r;
+    for (auto& s : streams) HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+  } catch (...) {
+    cleanup();
+    throw;
   }
-  HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipGetLastError());
-  dfree(A), dfree(V), dfree(W1), dfree(W2), dfree(Tf);
+  cleanup();
   t->has_T = true;
 }
 
