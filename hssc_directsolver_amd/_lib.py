@@ -62,6 +62,17 @@ SIGNATURES = [
     ("hssc_hss2d_apply_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("hssc_hss2d_solve_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int32,
                                           C.POINTER(C.c_int32), _DP, _P]),
+    ("hssc_hss1d_compress", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32, C.c_double,
+                                      C.c_int32, C.POINTER(_P)]),
+    ("hssc_hss1d_invert", C.c_int, [_P]),
+    ("hssc_hss1d_applyinv", C.c_int, [_P, _DP, _DP, C.c_int64]),
+    ("hssc_hss1d_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),
+    ("hssc_hss1d_applyinv_device", C.c_int, [_P, _P, _P, C.c_int64]),
+    ("hssc_hss1d_apply_device", C.c_int, [_P, _P, _P, C.c_int64]),
+    ("hssc_hss1d_info", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("hssc_hss1d_box", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), _I64P, _I64P]),
+    ("hssc_hss1d_destroy", None, [_P]),
     ("hssc_stats_enable", C.c_int, [C.c_int32]),
     ("hssc_stats_reset", C.c_int, []),
     ("hssc_stats_get", C.c_int, [C.c_char_p, _I64P, _DP, _DP, _DP]),

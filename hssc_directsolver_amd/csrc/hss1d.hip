@@ -1,0 +1,1047 @@
+// HSS1D compress / invert / apply for second-kind boundary integral equations on curves
+// (HSS1D/HSS1D_compress_nsym.m 'green', HSS1D_invert_nsym.m, HSS1D_applyinv_nsym.m,
+// HSS1D_apply_nsym.m).
+//
+// Ranks come from adaptive IDs, so every level is a ragged batch: per-box sizes and offsets
+// live in device arrays and each kernel runs one workgroup per box (the boxes of one level
+// are independent).  The host builds the binary tree, neighbour lists and proxy circles
+// (integer / geometry work) exactly as the reference does; the GPU generates every kernel
+// block, runs the interpolative decompositions, the inversion and the sweeps.
+//
+// Operator: A = -1/2 I + D, D the Laplace double layer with per-point normals, weights and
+// curvature (diagonal -kappa w / 4pi); proxy sources carry log charges (1/2pi) log|x - z|.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "stats.h"
+
+namespace hssc {
+namespace {
+
+struct HipErr {
+  int status;
+};
+
+#define H1CHK(expr)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fail(e_ == hipErrorOutOfMemory ? HSSC_ERR_OUT_OF_MEMORY : HSSC_ERR_HIP,              \
+           std::string(#expr) + ": " + hipGetErrorString(e_));                             \
+      throw HipErr{e_ == hipErrorOutOfMemory ? HSSC_ERR_OUT_OF_MEMORY : HSSC_ERR_HIP};     \
+    }                                                                                      \
+  } while (0)
+
+template <typename T>
+T* dmalloc(size_t n) {
+  void* p = nullptr;
+  H1CHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dput(const std::vector<T>& h) {
+  T* d = dmalloc<T>(h.size());
+  if (!h.empty()) H1CHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename T>
+std::vector<T> dget(const T* d, size_t n) {
+  std::vector<T> h(n);
+  if (n) H1CHK(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+  return h;
+}
+
+void dfree1(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kRadiusRel = 1.75;  // HSS1D/HSS1D_compress_nsym.m:51
+
+// ------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------
+
+struct CurveDev {
+  const double *x, *y, *nx, *ny, *w, *kap;
+};
+
+// Double layer at target (tx,ty) from curve source j; tgt = index of the target if it is a
+// curve point (self term), -1 otherwise.
+__device__ __forceinline__ double dl_entry(const CurveDev& C, double tx, double ty, int tgt, int j) {
+  if (tgt == j) return -0.5 - C.kap[j] * C.w[j] / (4 * kPi);
+  const double dx = tx - C.x[j], dy = ty - C.y[j];
+  const double r2 = dx * dx + dy * dy;
+  return (dx * C.nx[j] + dy * C.ny[j]) / r2 / (2 * kPi) * C.w[j];
+}
+
+// Ragged block generation.  Matrix b is m x n at A + off[b] (leading dimension m).
+//   mode 0 (A_out rows x sources): rows r < mc[b] are curve targets ridx[ro[b]+r], the other
+//          rows proxy targets (px, py at pofs[b] + r - mc); columns are curve sources cidx.
+//   mode 1 (A_in^T): rows r < mc[b] are curve sources ridx[..], the other rows proxy
+//          sources (log charges); columns are curve targets cidx.
+__global__ void __launch_bounds__(256) gen1d_kernel(
+    CurveDev C, int mode, const int64_t* __restrict__ off, const int* __restrict__ mr,
+    const int* __restrict__ nc, const int* __restrict__ mcv, const int* __restrict__ ridx,
+    const int64_t* __restrict__ ro, const int* __restrict__ cidx, const int64_t* __restrict__ co,
+    const double* __restrict__ px, const double* __restrict__ py, const int64_t* __restrict__ pofs,
+    double* __restrict__ A) {
+  const int b = blockIdx.y;
+  const int m = mr[b], n = nc[b], mc = mcv[b];
+  double* M = A + off[b];
+  for (int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x; idx < int64_t(m) * n;
+       idx += int64_t(gridDim.x) * 256) {
+    const int r = int(idx % m), c = int(idx / m);
+    const int cj = cidx[co[b] + c];
+    double v;
+    if (mode == 0) {
+      if (r < mc) {
+        const int ti = ridx[ro[b] + r];
+        v = dl_entry(C, C.x[ti], C.y[ti], ti, cj);
+      } else {
+        const int64_t q = pofs[b] + r - mc;
+        v = dl_entry(C, px[q], py[q], -1, cj);
+      }
+    } else {
+      if (r < mc) {
+        const int sj = ridx[ro[b] + r];
+        v = dl_entry(C, C.x[cj], C.y[cj], cj, sj);
+      } else {
+        const int64_t q = pofs[b] + r - mc;
+        const double dx = C.x[cj] - px[q], dy = C.y[cj] - py[q];
+        v = log(dx * dx + dy * dy) / (4 * kPi);
+      }
+    }
+    M[idx] = v;
+  }
+}
+
+// Per-box telescoping inverse (HSS1D/HSS1D_invert_nsym.m:44-83), one workgroup per box,
+// all blocks in global scratch (they are at most 128 x 128 and stay in L2).
+struct BoxInv {
+  int n, k;
+  int64_t F;      // F (n x n) -> Finv in place (assembled by the kernel)
+  int64_t Jup, Jdn, Tup, Tdn;  // permutations (ints) and T (k x (n-k))
+  int64_t E;      // E (k x k)
+  int64_t RF;     // R Finv (k x n)
+  int64_t FL;     // Finv L (n x k)
+  int64_t Linv, Rinv, Dinv;
+  // assembly: leaf -> Mself offset; else children E offsets and M_sib offsets
+  int leaf;
+  int64_t Mself, E1, E2, S1, S2;
+  int k1, k2;
+};
+
+__device__ void gj_inplace(double* A, int n, int* piv, double* f, int* sp) {
+  // Gauss-Jordan with partial pivoting on an n x n column-major matrix in global memory
+  // (one workgroup); swaps undone by columns at the end.
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int c = 0; c < n; ++c) {
+    if (tid < 64) {
+      double best = -1;
+      int bi = c;
+      for (int r = c + tid; r < n; r += 64) {
+        const double v = fabs(A[r + c * n]);
+        if (v > best) best = v, bi = r;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
+      }
+      if (tid == 0) *sp = bi, piv[c] = bi;
+    }
+    __syncthreads();
+    const int p = *sp;
+    if (p != c)
+      for (int j = tid; j < n; j += nt) {
+        const double t = A[c + j * n];
+        A[c + j * n] = A[p + j * n];
+        A[p + j * n] = t;
+      }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) f[i] = A[i + c * n];
+    __syncthreads();
+    const double inv = 1.0 / f[c];
+    for (int j = tid; j < n; j += nt) A[c + j * n] = (j == c) ? inv : A[c + j * n] * inv;
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += nt) {
+      const int i = idx % n, j = idx / n;
+      if (i == c) continue;
+      if (j == c) A[i + j * n] = -f[i] * inv;
+      else A[i + j * n] -= f[i] * A[c + j * n];
+    }
+    __syncthreads();
+  }
+  for (int c = n - 1; c >= 0; --c) {
+    const int p = piv[c];
+    if (p != c) {
+      for (int i = tid; i < n; i += nt) {
+        const double t = A[i + c * n];
+        A[i + c * n] = A[i + p * n];
+        A[i + p * n] = t;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) hss1d_invert_kernel(const BoxInv* __restrict__ boxes,
+                                                           double* __restrict__ S,
+                                                           const int* __restrict__ I) {
+  __shared__ double f[256];
+  __shared__ int piv[256];
+  __shared__ int sp;
+  const BoxInv B = boxes[blockIdx.x];
+  const int n = B.n, k = B.k, tid = threadIdx.x, nt = blockDim.x;
+  double* F = S + B.F;
+  // assemble F
+  for (int idx = tid; idx < n * n; idx += nt) {
+    const int i = idx % n, j = idx / n;
+    double v;
+    if (B.leaf) {
+      v = S[B.Mself + idx];
+    } else if (i < B.k1 && j < B.k1) {
+      v = S[B.E1 + i + j * B.k1];
+    } else if (i >= B.k1 && j >= B.k1) {
+      v = S[B.E2 + (i - B.k1) + (j - B.k1) * B.k2];
+    } else if (i < B.k1) {
+      v = S[B.S1 + i + (j - B.k1) * B.k1];  // M_sib(c1): k1 x k2
+    } else {
+      v = S[B.S2 + (i - B.k1) + j * B.k2];  // M_sib(c2): k2 x k1
+    }
+    F[idx] = v;
+  }
+  __syncthreads();
+  gj_inplace(F, n, piv, f, &sp);
+  __syncthreads();
+  const int* Ju = I + B.Jup;
+  const int* Jd = I + B.Jdn;
+  const double* Tu = S + B.Tup;  // k x (n-k)
+  const double* Td = S + B.Tdn;
+  double* RF = S + B.RF;
+  double* FL = S + B.FL;
+  // RF = R Finv: row i = Finv(Ju[i], :) + sum_c Tu(i, c) Finv(Ju[k+c], :)
+  for (int idx = tid; idx < k * n; idx += nt) {
+    const int i = idx % k, j = idx / k;
+    double s = F[Ju[i] + j * n];
+    for (int c = 0; c < n - k; ++c) s += Tu[i + c * k] * F[Ju[k + c] + j * n];
+    RF[idx] = s;
+  }
+  // FL = Finv L: column j = Finv(:, Jd[j]) + sum_c Finv(:, Jd[k+c]) Td(j, c)
+  for (int idx = tid; idx < n * k; idx += nt) {
+    const int i = idx % n, j = idx / n;
+    double s = F[i + Jd[j] * n];
+    for (int c = 0; c < n - k; ++c) s += F[i + Jd[k + c] * n] * Td[j + c * k];
+    FL[idx] = s;
+  }
+  __syncthreads();
+  // E^-1 = RF L = R Finv L
+  double* E = S + B.E;
+  for (int idx = tid; idx < k * k; idx += nt) {
+    const int i = idx % k, j = idx / k;
+    double s = RF[i + Jd[j] * k];
+    for (int c = 0; c < n - k; ++c) s += RF[i + Jd[k + c] * k] * Td[j + c * k];
+    E[idx] = s;
+  }
+  __syncthreads();
+  gj_inplace(E, k, piv, f, &sp);
+  __syncthreads();
+  double* Li = S + B.Linv;
+  double* Ri = S + B.Rinv;
+  double* Di = S + B.Dinv;
+  for (int idx = tid; idx < n * k; idx += nt) {
+    const int i = idx % n, j = idx / n;
+    double s = 0;
+    for (int q = 0; q < k; ++q) s += FL[i + q * n] * E[q + j * k];
+    Li[idx] = s;  // Linv = Finv L E
+  }
+  for (int idx = tid; idx < k * n; idx += nt) {
+    const int i = idx % k, j = idx / k;
+    double s = 0;
+    for (int q = 0; q < k; ++q) s += E[i + q * k] * RF[q + j * k];
+    Ri[idx] = s;  // Rinv = E R Finv
+  }
+  __syncthreads();
+  for (int idx = tid; idx < n * n; idx += nt) {
+    const int i = idx % n, j = idx / n;
+    double s = F[idx];
+    for (int q = 0; q < k; ++q) s -= Li[i + q * n] * RF[q + j * k];
+    Di[idx] = s;  // Dinv = Finv - Linv R Finv
+  }
+}
+
+__global__ void __launch_bounds__(256) hss1d_top_kernel(int64_t FT, int k2, int k3, int64_t E2,
+                                                        int64_t E3, int64_t S2, int64_t S3,
+                                                        double* __restrict__ S) {
+  __shared__ double f[256];
+  __shared__ int piv[256];
+  __shared__ int sp;
+  const int n = k2 + k3, tid = threadIdx.x;
+  double* F = S + FT;
+  for (int idx = tid; idx < n * n; idx += blockDim.x) {
+    const int i = idx % n, j = idx / n;
+    double v;
+    if (i < k2 && j < k2) v = S[E2 + i + j * k2];
+    else if (i >= k2 && j >= k2) v = S[E3 + (i - k2) + (j - k2) * k3];
+    else if (i < k2) v = S[S2 + i + (j - k2) * k2];
+    else v = S[S3 + (i - k2) + j * k3];
+    F[idx] = v;
+  }
+  __syncthreads();
+  gj_inplace(F, n, piv, f, &sp);
+}
+
+// One level of a sweep: out_box = sum_t M_t x_t (up to three terms), one workgroup per
+// box.  Term t reads M_t(rofs + i, c) = S[mat + rofs + i + c*ld]; its input is either the
+// global vector at gofs (src_global) or the concatenation V[in .. in+len1) ++ V[in2 ..).
+// The output goes to V[out ..] (rows >= split to V[out2 ..] when out2 >= 0) or to the
+// global vector at gofs (scatter).
+struct SweepTerm {
+  int64_t mat;
+  int ld, rofs, cols, len1, src_global;
+  int64_t in, in2;
+};
+struct SweepOp {
+  int rows, nterm;
+  SweepTerm t[3];
+  int64_t out, out2;
+  int split, scatter;
+  int64_t gofs;
+};
+
+__global__ void __launch_bounds__(256) hss1d_sweep_kernel(const SweepOp* __restrict__ ops,
+                                                          const double* __restrict__ S,
+                                                          double* __restrict__ V,
+                                                          const double* __restrict__ gin,
+                                                          double* __restrict__ gout) {
+  const SweepOp O = ops[blockIdx.x];
+  for (int i = threadIdx.x; i < O.rows; i += blockDim.x) {
+    double s = 0;
+    for (int t = 0; t < O.nterm; ++t) {
+      const SweepTerm& T = O.t[t];
+      const double* M = S + T.mat + T.rofs + i;
+      for (int c = 0; c < T.cols; ++c) {
+        double x;
+        if (T.src_global) x = gin[O.gofs + c];
+        else if (c < T.len1) x = V[T.in + c];
+        else x = V[T.in2 + c - T.len1];
+        s += M[int64_t(c) * T.ld] * x;
+      }
+    }
+    if (O.scatter) gout[O.gofs + i] = s;
+    else if (O.out2 >= 0 && i >= O.split) V[O.out2 + i - O.split] = s;
+    else V[O.out + i] = s;
+  }
+}
+
+}  // namespace
+}  // namespace hssc
+
+using namespace hssc;
+
+// ---------------------------------------------------------------------------------------
+// Host object
+// ---------------------------------------------------------------------------------------
+
+struct hssc_hss1d {
+  int64_t N = 0;
+  double acc = 0;
+  int nproxy = 0;
+  std::vector<double> X, Y, NX, NY, W, KAP;
+  // tree (1-based boxes, index 0 unused)
+  std::vector<int> level, parent, c1, c2, start, count;
+  int nboxes = 0, depth = 0;
+  std::vector<std::vector<int>> nei;
+  // compression
+  std::vector<std::vector<int>> isk_up, isk_dn, J_up, J_dn;
+  std::vector<int> nskel, kskel;
+  // device scratch: all matrices (T, M_self, M_sib, inverse blocks) in one arena, all ints
+  // (permutations) in another; offsets per box
+  std::vector<int64_t> oTup, oTdn, oJup, oJdn, oMself, oMsib, oR, oL;
+  std::vector<double> arena_h;  // unused on host; sizes tracked
+  int64_t arena_n = 0, iarena_n = 0;
+  double* S = nullptr;
+  int* I = nullptr;
+  double *dx = nullptr, *dy = nullptr, *dnx = nullptr, *dny = nullptr, *dw = nullptr,
+         *dkap = nullptr;
+  // inverse
+  bool inverted = false;
+  std::vector<int64_t> oE, oLinv, oRinv, oDinv;
+  int64_t oFTinv = -1;
+  // per-box vector offsets for the sweeps
+  std::vector<int64_t> vUp, vDn;
+  int64_t vlen = 0;
+  ~hssc_hss1d() {
+    dfree1(S), dfree1(I), dfree1(dx), dfree1(dy), dfree1(dnx), dfree1(dny), dfree1(dw),
+        dfree1(dkap);
+  }
+  bool leaf(int b) const { return c1[b] <= 0 && c2[b] <= 0; }
+  CurveDev curve() const { return CurveDev{dx, dy, dnx, dny, dw, dkap}; }
+};
+
+namespace hssc {
+namespace {
+
+// LOCAL_get_tree (HSS1D_compress_nsym.m:277-334)
+void build_tree1d(hssc_hss1d* h, int nbox_max) {
+  auto push = [&](int lev, int par, int st, int cnt) {
+    h->level.push_back(lev), h->parent.push_back(par), h->c1.push_back(-1),
+        h->c2.push_back(-1), h->start.push_back(st), h->count.push_back(cnt);
+  };
+  h->level.clear(), h->parent.clear(), h->c1.clear(), h->c2.clear(), h->start.clear(),
+      h->count.clear();
+  push(0, 0, 0, 0);  // index 0 unused
+  push(0, 0, 0, int(h->N));
+  int first, last = 0, created = 1, lev = 0;
+  while (created > 0) {
+    first = last + 1;
+    last = last + created;
+    created = 0;
+    for (int b = first; b <= last; ++b) {
+      const int nb = h->count[b];
+      if (nb > nbox_max) {
+        const int nh = (nb + 1) / 2;
+        const int s1 = last + created + 1, s2 = last + created + 2;
+        h->c1[b] = s1, h->c2[b] = s2;
+        push(lev + 1, b, h->start[b], nh);
+        push(lev + 1, b, h->start[b] + nh, nb - nh);
+        created += 2;
+      }
+    }
+    ++lev;
+  }
+  h->nboxes = last;
+  h->depth = 0;
+  for (int b = 1; b <= last; ++b) h->depth = std::max(h->depth, h->level[b]);
+}
+
+// LOCAL_get_circum_circle (:339-359) on a list of point indices.
+void circum(const hssc_hss1d* h, const std::vector<int>& idx, double* cx, double* cy, double* R) {
+  const size_t n = idx.size();
+  double xc = 0.5 * (h->X[idx[0]] + h->X[idx[n - 1]]);
+  double yc = 0.5 * (h->Y[idx[0]] + h->Y[idx[n - 1]]);
+  double m = 0;
+  for (int i : idx) m = std::max(m, (h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc));
+  double r = std::sqrt(m);
+  const double ex = h->X[idx[0]] - h->X[idx[n - 1]], ey = h->Y[idx[0]] - h->Y[idx[n - 1]];
+  if (1.2 * r * r > ex * ex + ey * ey) {
+    xc = 0, yc = 0;
+    for (int i : idx) xc += h->X[i], yc += h->Y[i];
+    xc *= 1.0 / double(n), yc *= 1.0 / double(n);
+    m = 0;
+    for (int i : idx) m = std::max(m, (h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc));
+    r = std::sqrt(m);
+  }
+  *cx = xc, *cy = yc, *R = r;
+}
+
+std::vector<int> range_of(const hssc_hss1d* h, int b) {
+  std::vector<int> v(h->count[b]);
+  for (int i = 0; i < h->count[b]; ++i) v[i] = h->start[b] + i;
+  return v;
+}
+
+// LOCAL_construct_potential_neighborlist (:376-412)
+void neighbours(hssc_hss1d* h) {
+  const int nb = h->nboxes;
+  h->nei.assign(nb + 1, {});
+  h->nei[1] = {1};
+  for (int b = 2; b <= nb; ++b) {
+    double xc, yc, R;
+    circum(h, range_of(h, b), &xc, &yc, &R);
+    std::vector<int> pot;
+    for (int j : h->nei[h->parent[b]]) {
+      if (h->c1[j] > 0) pot.push_back(h->c1[j]);
+      if (h->c2[j] > 0) pot.push_back(h->c2[j]);
+    }
+    for (int j : pot) {
+      double mn = 1e300;
+      for (int i = h->start[j]; i < h->start[j] + h->count[j]; ++i)
+        mn = std::min(mn, (h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc));
+      if (mn < kRadiusRel * kRadiusRel * R * R) h->nei[b].push_back(j);
+    }
+  }
+  for (int b = 2; b <= nb; ++b) {
+    auto& v = h->nei[b];
+    v.erase(std::remove(v.begin(), v.end(), b), v.end());
+    std::sort(v.begin(), v.end());
+  }
+}
+
+// Appends a block to the double arena (host-side bookkeeping of offsets).
+int64_t take(int64_t* top, int64_t n) {
+  const int64_t o = *top;
+  *top += std::max<int64_t>(n, 0);
+  return o;
+}
+
+// Grows a device arena, keeping its contents.
+template <typename T>
+void grow(T** p, int64_t* cap, int64_t need) {
+  if (need <= *cap) return;
+  int64_t nc = std::max<int64_t>(need, *cap * 2);
+  T* q = dmalloc<T>(nc);
+  if (*p && *cap) H1CHK(hipMemcpy(q, *p, *cap * sizeof(T), hipMemcpyDeviceToDevice));
+  dfree1(*p);
+  *p = q;
+  *cap = nc;
+}
+
+struct Ragged {
+  std::vector<int64_t> off;  // matrix offsets (doubles)
+  std::vector<int> m, n, mc;
+  std::vector<int> ridx, cidx;
+  std::vector<int64_t> ro, co, pofs;
+  std::vector<double> px, py;
+  int64_t total = 0;
+  void add(const std::vector<int>& rows, const std::vector<double>& zx,
+           const std::vector<double>& zy, const std::vector<int>& cols) {
+    off.push_back(total);
+    const int mm = int(rows.size() + zx.size()), nn = int(cols.size());
+    m.push_back(mm), n.push_back(nn), mc.push_back(int(rows.size()));
+    ro.push_back(int64_t(ridx.size())), co.push_back(int64_t(cidx.size()));
+    pofs.push_back(int64_t(px.size()));
+    ridx.insert(ridx.end(), rows.begin(), rows.end());
+    cidx.insert(cidx.end(), cols.begin(), cols.end());
+    px.insert(px.end(), zx.begin(), zx.end());
+    py.insert(py.end(), zy.begin(), zy.end());
+    total += int64_t(mm) * nn;
+  }
+};
+
+// Generates ragged blocks on the device; returns the device matrix array (caller frees).
+double* gen_ragged(const hssc_hss1d* h, const Ragged& R, int mode, std::vector<void*>* tmp) {
+  auto keep = [&](void* p) { tmp->push_back(p); return p; };
+  const int nm = int(R.off.size());
+  int64_t* doff = static_cast<int64_t*>(keep(dput(R.off)));
+  int* dm = static_cast<int*>(keep(dput(R.m)));
+  int* dn = static_cast<int*>(keep(dput(R.n)));
+  int* dmc = static_cast<int*>(keep(dput(R.mc)));
+  int* dri = static_cast<int*>(keep(dput(R.ridx)));
+  int* dci = static_cast<int*>(keep(dput(R.cidx)));
+  int64_t* dro = static_cast<int64_t*>(keep(dput(R.ro)));
+  int64_t* dco = static_cast<int64_t*>(keep(dput(R.co)));
+  int64_t* dpo = static_cast<int64_t*>(keep(dput(R.pofs)));
+  double* dpx = static_cast<double*>(keep(dput(R.px)));
+  double* dpy = static_cast<double*>(keep(dput(R.py)));
+  double* A = dmalloc<double>(R.total);
+  if (nm > 0) {
+    int64_t mx = 1;
+    for (int i = 0; i < nm; ++i) mx = std::max<int64_t>(mx, int64_t(R.m[i]) * R.n[i]);
+    dim3 grid(unsigned(std::min<int64_t>((mx + 255) / 256, 64)), nm);
+    HSSC_TIMED("gen1d_f64", 0, double(R.total), double(R.total) * 8,
+               hipLaunchKernelGGL(gen1d_kernel, grid, dim3(256), 0, 0, h->curve(), mode, doff, dm,
+                                  dn, dmc, dri, dro, dci, dco, dpx, dpy, dpo, A));
+  }
+  return A;
+}
+
+// Runs the acc-mode ID on a ragged batch, then the fixed-rank ID with k = kfix (per matrix)
+// on a fresh copy.  Returns perms (host) and T (device arena offsets appended).
+struct IdResult {
+  std::vector<int> k;
+  std::vector<std::vector<int>> perm;
+  std::vector<std::vector<double>> T;  // k x (n-k) column-major
+};
+
+IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* kfix,
+                uint64_t seed) {
+  const int nm = int(R.off.size());
+  IdResult out;
+  out.k.resize(nm), out.perm.resize(nm), out.T.resize(nm);
+  if (nm == 0) return out;
+  std::vector<int64_t> poff(nm), toff(nm), woff(nm);
+  int64_t pt = 0, tt = 0, wt = 0;
+  for (int i = 0; i < nm; ++i) {
+    poff[i] = pt, pt += R.n[i];
+    toff[i] = tt, tt += int64_t(R.n[i]) * R.n[i];
+    woff[i] = wt, wt += 6 * int64_t(R.n[i]) + R.m[i];
+  }
+  std::vector<void*> tmp;
+  double* Ac = dmalloc<double>(R.total);
+  tmp.push_back(Ac);
+  H1CHK(hipMemcpy(Ac, A, R.total * 8, hipMemcpyDeviceToDevice));
+  int64_t* doff = dput(R.off);
+  int* dm = dput(R.m);
+  int* dn = dput(R.n);
+  int64_t* dpo = dput(poff);
+  int64_t* dto = dput(toff);
+  int64_t* dwo = dput(woff);
+  int* dperm = dmalloc<int>(pt);
+  int* dk = dmalloc<int>(nm);
+  double* dT = dmalloc<double>(tt);
+  double* dwork = dmalloc<double>(wt);
+  int* dkf = kfix ? dput(*kfix) : nullptr;
+  tmp.insert(tmp.end(), {doff, dm, dn, dpo, dto, dwo, dperm, dk, dT, dwork, dkf});
+  k::cpqr_id<double>(0, nm, doff, dm, dn, A, Ac, kfix ? -1.0 : acc, dkf, dperm, dpo, dk, dT, dto,
+                     dwork, dwo, seed);
+  H1CHK(hipDeviceSynchronize());
+  H1CHK(hipGetLastError());
+  std::vector<int> hk = dget(dk, nm);
+  std::vector<int> hp = dget(dperm, pt);
+  std::vector<double> hT = dget(dT, tt);
+  for (int i = 0; i < nm; ++i) {
+    out.k[i] = hk[i];
+    out.perm[i].assign(hp.begin() + poff[i], hp.begin() + poff[i] + R.n[i]);
+    const int64_t cnt = int64_t(hk[i]) * (R.n[i] - hk[i]);
+    out.T[i].assign(hT.begin() + toff[i], hT.begin() + toff[i] + cnt);
+  }
+  // restore A from the copy so a second pass can reuse it
+  H1CHK(hipMemcpy(A, Ac, R.total * 8, hipMemcpyDeviceToDevice));
+  for (void* p : tmp) dfree1(p);
+  return out;
+}
+
+}  // namespace
+}  // namespace hssc
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+
+extern "C" {
+
+int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, const double* kappa,
+                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
+                        hssc_hss1d** out) {
+  if (!Xc || !nrm || !w || !kappa || !out || N < 4 || nbox_max < 2 || !(acc > 0 && acc < 1) ||
+      nproxy < 4)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d_compress arguments");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(HSSC_ERR_HIP, "no HIP device is available");
+  hssc_hss1d* h = nullptr;
+  try {
+    h = new hssc_hss1d();
+    h->N = N, h->acc = acc, h->nproxy = nproxy;
+    h->X.assign(Xc, Xc + N), h->Y.assign(Xc + N, Xc + 2 * N);
+    h->NX.assign(nrm, nrm + N), h->NY.assign(nrm + N, nrm + 2 * N);
+    h->W.assign(w, w + N), h->KAP.assign(kappa, kappa + N);
+    h->dx = dput(h->X), h->dy = dput(h->Y), h->dnx = dput(h->NX), h->dny = dput(h->NY);
+    h->dw = dput(h->W), h->dkap = dput(h->KAP);
+    build_tree1d(h, nbox_max);
+    if (h->nboxes < 3) throw HipErr{fail(HSSC_ERR_INVALID_ARG, "tree needs at least 2 levels")};
+    neighbours(h);
+    const int nb = h->nboxes;
+    h->isk_up.assign(nb + 1, {}), h->isk_dn.assign(nb + 1, {});
+    h->J_up.assign(nb + 1, {}), h->J_dn.assign(nb + 1, {});
+    h->nskel.assign(nb + 1, 0), h->kskel.assign(nb + 1, 0);
+    h->oTup.assign(nb + 1, -1), h->oTdn.assign(nb + 1, -1), h->oJup.assign(nb + 1, -1),
+        h->oJdn.assign(nb + 1, -1), h->oMself.assign(nb + 1, -1), h->oMsib.assign(nb + 1, -1);
+    h->oR.assign(nb + 1, -1), h->oL.assign(nb + 1, -1);
+    std::vector<double> arena;  // host staging of T blocks (uploaded with the M blocks)
+    std::vector<int> iarena;
+    // compress level by level, deepest first (HSS1D_compress_nsym.m:84-216)
+    for (int lev = h->depth; lev >= 1; --lev) {
+      std::vector<int> boxes;
+      for (int b = 2; b <= nb; ++b)
+        if (h->level[b] == lev) boxes.push_back(b);
+      Ragged Rout, Rin;
+      std::vector<std::vector<int>> outs(boxes.size()), inns(boxes.size());
+      for (size_t q = 0; q < boxes.size(); ++q) {
+        const int b = boxes[q];
+        std::vector<int> out, inn;
+        if (h->leaf(b)) {
+          out = range_of(h, b);
+          inn = out;
+        } else {
+          out = h->isk_up[h->c1[b]];
+          out.insert(out.end(), h->isk_up[h->c2[b]].begin(), h->isk_up[h->c2[b]].end());
+          inn = h->isk_dn[h->c1[b]];
+          inn.insert(inn.end(), h->isk_dn[h->c2[b]].begin(), h->isk_dn[h->c2[b]].end());
+        }
+        std::vector<int> uni = out;
+        uni.insert(uni.end(), inn.begin(), inn.end());
+        std::sort(uni.begin(), uni.end());
+        double xc, yc, R;
+        circum(h, uni, &xc, &yc, &R);
+        const double rp = kRadiusRel * R;
+        std::vector<double> zx(nproxy), zy(nproxy);
+        for (int t = 0; t < nproxy; ++t) {
+          // linspace(0, 2pi(1-1/n), n)
+          const double th = (nproxy > 1) ? (2 * kPi * (1 - 1.0 / nproxy)) * t / (nproxy - 1) : 0;
+          zx[t] = xc + rp * std::cos(th), zy[t] = yc + rp * std::sin(th);
+        }
+        std::vector<int> omay, imay;
+        for (int j : h->nei[b]) {
+          if (h->leaf(j)) {
+            auto r = range_of(h, j);
+            omay.insert(omay.end(), r.begin(), r.end());
+            imay.insert(imay.end(), r.begin(), r.end());
+          } else {
+            for (int s : {h->c1[j], h->c2[j]}) {
+              omay.insert(omay.end(), h->isk_up[s].begin(), h->isk_up[s].end());
+              imay.insert(imay.end(), h->isk_dn[s].begin(), h->isk_dn[s].end());
+            }
+          }
+        }
+        std::vector<int> oin, iin;
+        for (int i : omay)
+          if ((h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc) < rp * rp)
+            oin.push_back(i);
+        for (int i : imay)
+          if ((h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc) < rp * rp)
+            iin.push_back(i);
+        Rout.add(iin, zx, zy, out);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
+        Rin.add(oin, zx, zy, inn);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
+        outs[q] = out, inns[q] = inn;
+      }
+      std::vector<void*> tmp;
+      double* Aout = gen_ragged(h, Rout, 0, &tmp);
+      double* Ain = gen_ragged(h, Rin, 1, &tmp);
+      tmp.push_back(Aout), tmp.push_back(Ain);
+      const uint64_t seed = 0x1234ull + uint64_t(lev) * 7919ull;
+      IdResult io = run_id(Rout, Aout, acc, nullptr, seed);
+      IdResult ii = run_id(Rin, Ain, acc, nullptr, seed ^ 0xABCDull);
+      // rank equalisation (HSS1D_compress_nsym.m:190-196): the smaller side is redone with
+      // the fixed rank k = max(k_out, k_in)
+      std::vector<int> kf(boxes.size());
+      bool need_out = false, need_in = false;
+      for (size_t q = 0; q < boxes.size(); ++q) {
+        kf[q] = std::max(io.k[q], ii.k[q]);
+        need_out |= kf[q] != io.k[q];
+        need_in |= kf[q] != ii.k[q];
+      }
+      if (need_out) {
+        IdResult f = run_id(Rout, Aout, acc, &kf, seed);
+        for (size_t q = 0; q < boxes.size(); ++q)
+          if (io.k[q] != kf[q]) io.k[q] = f.k[q], io.perm[q] = f.perm[q], io.T[q] = f.T[q];
+      }
+      if (need_in) {
+        IdResult f = run_id(Rin, Ain, acc, &kf, seed ^ 0xABCDull);
+        for (size_t q = 0; q < boxes.size(); ++q)
+          if (ii.k[q] != kf[q]) ii.k[q] = f.k[q], ii.perm[q] = f.perm[q], ii.T[q] = f.T[q];
+      }
+      for (void* p : tmp) dfree1(p);
+      for (size_t q = 0; q < boxes.size(); ++q) {
+        const int b = boxes[q], k = kf[q];
+        h->nskel[b] = int(outs[q].size());
+        h->kskel[b] = k;
+        h->J_up[b] = io.perm[q];
+        h->J_dn[b] = ii.perm[q];
+        h->isk_up[b].resize(k), h->isk_dn[b].resize(k);
+        for (int i = 0; i < k; ++i)
+          h->isk_up[b][i] = outs[q][io.perm[q][i]], h->isk_dn[b][i] = inns[q][ii.perm[q][i]];
+        h->oTup[b] = int64_t(arena.size());
+        arena.insert(arena.end(), io.T[q].begin(), io.T[q].end());
+        h->oTdn[b] = int64_t(arena.size());
+        arena.insert(arena.end(), ii.T[q].begin(), ii.T[q].end());
+        h->oJup[b] = int64_t(iarena.size());
+        iarena.insert(iarena.end(), io.perm[q].begin(), io.perm[q].end());
+        h->oJdn[b] = int64_t(iarena.size());
+        iarena.insert(iarena.end(), ii.perm[q].begin(), ii.perm[q].end());
+        // explicit R = [I T_up] (columns permuted by J_up) and L = [I; T_dn^T] (rows by J_dn)
+        const int n = h->nskel[b];
+        std::vector<double> Rm(int64_t(k) * n, 0.0), Lm(int64_t(n) * k, 0.0);
+        for (int i = 0; i < k; ++i) {
+          Rm[i + int64_t(io.perm[q][i]) * k] = 1.0;
+          Lm[ii.perm[q][i] + int64_t(i) * n] = 1.0;
+          for (int c = 0; c < n - k; ++c) {
+            Rm[i + int64_t(io.perm[q][k + c]) * k] = io.T[q][i + int64_t(c) * k];
+            Lm[ii.perm[q][k + c] + int64_t(i) * n] = ii.T[q][i + int64_t(c) * k];
+          }
+        }
+        h->oR[b] = int64_t(arena.size());
+        arena.insert(arena.end(), Rm.begin(), Rm.end());
+        h->oL[b] = int64_t(arena.size());
+        arena.insert(arena.end(), Lm.begin(), Lm.end());
+      }
+    }
+    // M_self at leaves, M_sib at children of every parent (HSS1D_compress_nsym.m:220-271)
+    Ragged Rm;
+    std::vector<std::pair<int, int>> which;  // (box, 0 = self / 1 = sib)
+    for (int b = nb; b >= 1; --b) {
+      if (h->leaf(b)) {
+        auto r = range_of(h, b);
+        Rm.add(r, {}, {}, r);
+        which.push_back({b, 0});
+      } else {
+        Rm.add(h->isk_dn[h->c1[b]], {}, {}, h->isk_up[h->c2[b]]);
+        which.push_back({h->c1[b], 1});
+        Rm.add(h->isk_dn[h->c2[b]], {}, {}, h->isk_up[h->c1[b]]);
+        which.push_back({h->c2[b], 1});
+      }
+    }
+    std::vector<void*> tmp;
+    double* Mblk = gen_ragged(h, Rm, 0, &tmp);
+    tmp.push_back(Mblk);
+    const int64_t base = int64_t(arena.size());
+    for (size_t q = 0; q < which.size(); ++q) {
+      const int b = which[q].first;
+      (which[q].second == 0 ? h->oMself[b] : h->oMsib[b]) = base + Rm.off[q];
+    }
+    arena.resize(base + Rm.total);
+    h->arena_n = int64_t(arena.size());
+    h->S = dmalloc<double>(h->arena_n);
+    H1CHK(hipMemcpy(h->S, arena.data(), base * 8, hipMemcpyHostToDevice));
+    H1CHK(hipMemcpy(h->S + base, Mblk, Rm.total * 8, hipMemcpyDeviceToDevice));
+    h->iarena_n = int64_t(iarena.size());
+    h->I = dput(iarena);
+    h->vUp.assign(nb + 1, 0), h->vDn.assign(nb + 1, 0);
+    int64_t vv = 0;
+    for (int b = 2; b <= nb; ++b) h->vUp[b] = vv, vv += h->kskel[b];
+    for (int b = 2; b <= nb; ++b) h->vDn[b] = vv, vv += h->kskel[b];
+    h->vlen = vv;
+    H1CHK(hipDeviceSynchronize());
+    for (void* p : tmp) dfree1(p);
+    *out = h;
+    return HSSC_OK;
+  } catch (const HipErr& e) {
+    delete h;
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    delete h;
+    return fail(HSSC_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+}
+
+int hssc_hss1d_invert(hssc_hss1d* h) {
+  // HSS1D/HSS1D_invert_nsym.m:44-91, one launch per level (deepest first) + the top block.
+  if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
+  try {
+    const int nb = h->nboxes;
+    for (int b = 2; b <= nb; ++b)
+      if (h->nskel[b] > 256)
+        throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D block larger than 256")};
+    int64_t top = h->arena_n;
+    h->oE.assign(nb + 1, -1), h->oLinv.assign(nb + 1, -1), h->oRinv.assign(nb + 1, -1),
+        h->oDinv.assign(nb + 1, -1);
+    std::vector<BoxInv> all(nb + 1);
+    for (int b = 2; b <= nb; ++b) {
+      const int n = h->nskel[b], k = h->kskel[b];
+      BoxInv B{};
+      B.n = n, B.k = k;
+      B.F = take(&top, int64_t(n) * n);
+      B.Jup = h->oJup[b], B.Jdn = h->oJdn[b], B.Tup = h->oTup[b], B.Tdn = h->oTdn[b];
+      B.E = h->oE[b] = take(&top, int64_t(k) * k);
+      B.RF = take(&top, int64_t(k) * n);
+      B.FL = take(&top, int64_t(n) * k);
+      B.Linv = h->oLinv[b] = take(&top, int64_t(n) * k);
+      B.Rinv = h->oRinv[b] = take(&top, int64_t(k) * n);
+      B.Dinv = h->oDinv[b] = take(&top, int64_t(n) * n);
+      B.leaf = h->leaf(b) ? 1 : 0;
+      B.Mself = h->oMself[b];
+      all[b] = B;
+    }
+    for (int b = 2; b <= nb; ++b)
+      if (!h->leaf(b)) {
+        all[b].E1 = h->oE[h->c1[b]], all[b].E2 = h->oE[h->c2[b]];
+        all[b].S1 = h->oMsib[h->c1[b]], all[b].S2 = h->oMsib[h->c2[b]];
+        all[b].k1 = h->kskel[h->c1[b]], all[b].k2 = h->kskel[h->c2[b]];
+      }
+    const int k2 = h->kskel[2], k3 = h->kskel[3];
+    h->oFTinv = take(&top, int64_t(k2 + k3) * (k2 + k3));
+    grow(&h->S, &h->arena_n, top);
+    for (int lev = h->depth; lev >= 1; --lev) {
+      std::vector<BoxInv> ops;
+      for (int b = 2; b <= nb; ++b)
+        if (h->level[b] == lev) ops.push_back(all[b]);
+      if (ops.empty()) continue;
+      BoxInv* d = dput(ops);
+      HSSC_TIMED("hss1d_invert_f64", 0, 0.0, 0.0,
+                 hipLaunchKernelGGL(hss1d_invert_kernel, dim3(ops.size()), dim3(256), 0, 0, d,
+                                    h->S, h->I));
+      H1CHK(hipDeviceSynchronize());
+      dfree1(d);
+    }
+    hipLaunchKernelGGL(hss1d_top_kernel, dim3(1), dim3(256), 0, 0, h->oFTinv, k2, k3, h->oE[2],
+                       h->oE[3], h->oMsib[2], h->oMsib[3], h->S);
+    H1CHK(hipDeviceSynchronize());
+    H1CHK(hipGetLastError());
+    h->inverted = true;
+    return HSSC_OK;
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+static SweepTerm term(int64_t mat, int ld, int cols, int64_t in, int len1, int64_t in2,
+                      int rofs = 0, int src_global = 0) {
+  SweepTerm T{};
+  T.mat = mat, T.ld = ld, T.cols = cols, T.in = in, T.len1 = len1, T.in2 = in2, T.rofs = rofs;
+  T.src_global = src_global;
+  return T;
+}
+
+static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool inverse) {
+  // inverse: HSS1D/HSS1D_applyinv_nsym.m:43-86; forward: HSS1D/HSS1D_apply_nsym.m:38-101.
+  // One launch per level and direction.
+  const int nb = h->nboxes;
+  double* V = dmalloc<double>(h->vlen + 1);
+  std::vector<void*> tmp{V};
+  auto launch = [&](const std::vector<SweepOp>& ops) {
+    if (ops.empty()) return;
+    SweepOp* d = dput(ops);
+    tmp.push_back(d);
+    HSSC_TIMED("hss1d_sweep_f64", 0, 0.0, 0.0,
+               hipLaunchKernelGGL(hss1d_sweep_kernel, dim3(ops.size()), dim3(256), 0, 0, d, h->S,
+                                  V, dIn, dOut));
+  };
+  auto up_op = [&](int b, int64_t mat) {
+    // up[b] = M [f_b or (up c1; up c2)]  with M = Rinv (inverse) or R (forward)
+    SweepOp O{};
+    O.rows = h->kskel[b], O.nterm = 1, O.out = h->vUp[b], O.out2 = -1;
+    if (h->leaf(b)) O.gofs = h->start[b], O.t[0] = term(mat, h->kskel[b], h->nskel[b], 0, 0, 0, 0, 1);
+    else
+      O.t[0] = term(mat, h->kskel[b], h->nskel[b], h->vUp[h->c1[b]], h->kskel[h->c1[b]],
+                    h->vUp[h->c2[b]]);
+    return O;
+  };
+  for (int lev = h->depth; lev >= 1; --lev) {
+    std::vector<SweepOp> ops;
+    for (int b = 2; b <= nb; ++b)
+      if (h->level[b] == lev) ops.push_back(up_op(b, inverse ? h->oRinv[b] : h->oR[b]));
+    launch(ops);
+  }
+  const int k2 = h->kskel[2], k3 = h->kskel[3];
+  if (inverse) {
+    // [dn2; dn3] = FTinv [up2; up3]
+    SweepOp O{};
+    O.rows = k2 + k3, O.nterm = 1, O.out = h->vDn[2], O.out2 = h->vDn[3], O.split = k2;
+    O.t[0] = term(h->oFTinv, k2 + k3, k2 + k3, h->vUp[2], k2, h->vUp[3]);
+    launch({O});
+    for (int lev = 1; lev <= h->depth; ++lev) {
+      std::vector<SweepOp> ops;
+      for (int b = 2; b <= nb; ++b) {
+        if (h->level[b] != lev) continue;
+        const int n = h->nskel[b], k = h->kskel[b];
+        SweepOp O{};
+        O.rows = n, O.nterm = 2;
+        O.t[1] = term(h->oLinv[b], n, k, h->vDn[b], k, 0);
+        if (h->leaf(b)) {
+          O.scatter = 1, O.gofs = h->start[b];
+          O.t[0] = term(h->oDinv[b], n, n, 0, 0, 0, 0, 1);
+        } else {
+          O.t[0] = term(h->oDinv[b], n, n, h->vUp[h->c1[b]], h->kskel[h->c1[b]], h->vUp[h->c2[b]]);
+          O.out = h->vDn[h->c1[b]], O.out2 = h->vDn[h->c2[b]], O.split = h->kskel[h->c1[b]];
+        }
+        ops.push_back(O);
+      }
+      launch(ops);
+    }
+  } else {
+    // dn2 = M_sib2 up3, dn3 = M_sib3 up2
+    SweepOp A2{}, A3{};
+    A2.rows = k2, A2.nterm = 1, A2.out = h->vDn[2], A2.out2 = -1;
+    A2.t[0] = term(h->oMsib[2], k2, k3, h->vUp[3], k3, 0);
+    A3.rows = k3, A3.nterm = 1, A3.out = h->vDn[3], A3.out2 = -1;
+    A3.t[0] = term(h->oMsib[3], k3, k2, h->vUp[2], k2, 0);
+    launch({A2, A3});
+    for (int lev = 1; lev <= h->depth; ++lev) {
+      std::vector<SweepOp> ops;
+      for (int b = 2; b <= nb; ++b) {
+        if (h->level[b] != lev) continue;
+        const int n = h->nskel[b], k = h->kskel[b];
+        if (h->leaf(b)) {
+          // u_b = L dn_b + M_self q_b
+          SweepOp O{};
+          O.rows = n, O.nterm = 2, O.scatter = 1, O.gofs = h->start[b];
+          O.t[0] = term(h->oMself[b], n, n, 0, 0, 0, 0, 1);
+          O.t[1] = term(h->oL[b], n, k, h->vDn[b], k, 0);
+          ops.push_back(O);
+        } else {
+          // dn c1 = (L dn)(0:n1) + M_sib(c1) up c2 ; dn c2 = (L dn)(n1:) + M_sib(c2) up c1
+          const int c1 = h->c1[b], c2 = h->c2[b], n1 = h->kskel[c1], n2 = h->kskel[c2];
+          SweepOp O1{}, O2{};
+          O1.rows = n1, O1.nterm = 2, O1.out = h->vDn[c1], O1.out2 = -1;
+          O1.t[0] = term(h->oL[b], n, k, h->vDn[b], k, 0, 0);
+          O1.t[1] = term(h->oMsib[c1], n1, n2, h->vUp[c2], n2, 0);
+          O2.rows = n2, O2.nterm = 2, O2.out = h->vDn[c2], O2.out2 = -1;
+          O2.t[0] = term(h->oL[b], n, k, h->vDn[b], k, 0, n1);
+          O2.t[1] = term(h->oMsib[c2], n2, n1, h->vUp[c1], n1, 0);
+          ops.push_back(O1), ops.push_back(O2);
+        }
+      }
+      launch(ops);
+    }
+  }
+  H1CHK(hipDeviceSynchronize());
+  H1CHK(hipGetLastError());
+  for (void* p : tmp) dfree1(p);
+  return HSSC_OK;
+}
+
+int hssc_hss1d_applyinv_device(const hssc_hss1d* h, const void* dF, void* dX, int64_t m) {
+  if (!h || !dF || !dX || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
+  if (!h->inverted) return fail(HSSC_ERR_INVALID_ARG, "hss1d operator is not inverted");
+  try {
+    return sweep1d(h, static_cast<const double*>(dF), static_cast<double*>(dX), true);
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t m) {
+  if (!h || !F || !X || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
+  try {
+    double* dF = dmalloc<double>(h->N);
+    double* dX = dmalloc<double>(h->N);
+    H1CHK(hipMemcpy(dF, F, h->N * 8, hipMemcpyHostToDevice));
+    int s = hssc_hss1d_applyinv_device(h, dF, dX, m);
+    if (s == HSSC_OK) H1CHK(hipMemcpy(X, dX, h->N * 8, hipMemcpyDeviceToHost));
+    dfree1(dF), dfree1(dX);
+    return s;
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_apply_device(const hssc_hss1d* h, const void* dQ, void* dU, int64_t m) {
+  if (!h || !dQ || !dU || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
+  try {
+    return sweep1d(h, static_cast<const double*>(dQ), static_cast<double*>(dU), false);
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_apply(const hssc_hss1d* h, const double* Q, double* U, int64_t m) {
+  if (!h || !Q || !U || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
+  try {
+    double* dQ = dmalloc<double>(h->N);
+    double* dU = dmalloc<double>(h->N);
+    H1CHK(hipMemcpy(dQ, Q, h->N * 8, hipMemcpyHostToDevice));
+    int s = hssc_hss1d_apply_device(h, dQ, dU, m);
+    if (s == HSSC_OK) H1CHK(hipMemcpy(U, dU, h->N * 8, hipMemcpyDeviceToHost));
+    dfree1(dQ), dfree1(dU);
+    return s;
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_info(const hssc_hss1d* h, int32_t* nboxes, int32_t* depth) {
+  if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
+  if (nboxes) *nboxes = h->nboxes;
+  if (depth) *depth = h->depth;
+  return HSSC_OK;
+}
+
+int hssc_hss1d_box(const hssc_hss1d* h, int32_t box, int32_t* level, int32_t* nskel,
+                   int32_t* kskel, int64_t* isk_up, int64_t* isk_dn) {
+  if (!h || box < 1 || box > h->nboxes) return fail(HSSC_ERR_INVALID_ARG, "box out of range");
+  if (level) *level = h->level[box];
+  if (nskel) *nskel = h->nskel[box];
+  if (kskel) *kskel = h->kskel[box];
+  if (isk_up) std::copy(h->isk_up[box].begin(), h->isk_up[box].end(), isk_up);
+  if (isk_dn) std::copy(h->isk_dn[box].begin(), h->isk_dn[box].end(), isk_dn);
+  return HSSC_OK;
+}
+
+void hssc_hss1d_destroy(hssc_hss1d* h) { delete h; }
+
+}  // extern "C"

@@ -81,5 +81,11 @@ void add2d(hipStream_t st, int64_t m, int64_t n, T a, const T* s1, int64_t ld1, 
 template <typename T>
 void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks);
 
+// Batched ragged interpolative decomposition (id.hip).  Work per matrix: 5 n + m.
+template <typename T>
+void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc, T* A,
+             const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
+             T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed);
+
 }  // namespace k
 }  // namespace hssc

@@ -1,0 +1,120 @@
+"""Python mirror of the reference's HSS1D MATLAB entry points, backed by libhssc.so.
+
+    K_HSS = HSS1D_compress_nsym('green', curve, nbox_max, acc)   # HSS1D_compress_nsym.m
+    HSS1D_invert_nsym(K_HSS)                                      # HSS1D_invert_nsym.m
+    x = HSS1D_applyinv_nsym(K_HSS, f)                             # HSS1D_applyinv_nsym.m
+    u = HSS1D_apply_nsym(K_HSS, q)                                # HSS1D_apply_nsym.m
+
+``curve`` is a Curve (points, outward normals, quadrature weights, curvature); the
+operator is the second-kind Laplace double layer -1/2 I + D (see include/hssc.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, lib
+
+_DP = C.POINTER(C.c_double)
+
+
+@dataclass
+class Curve:
+    X: np.ndarray      # N x 2
+    nrm: np.ndarray    # N x 2 outward unit normals
+    w: np.ndarray      # N quadrature weights
+    kappa: np.ndarray  # N curvatures
+
+    @property
+    def N(self):
+        return self.X.shape[0]
+
+
+def star_contour(N, a=0.3, m=5) -> Curve:
+    """r(t) = 1 + a cos(m t) sampled at t_j = 2 pi j / N (trapezoidal rule)."""
+    t = 2 * np.pi * np.arange(N) / N
+    r = 1 + a * np.cos(m * t)
+    rp = -a * m * np.sin(m * t)
+    rpp = -a * m * m * np.cos(m * t)
+    xp = rp * np.cos(t) - r * np.sin(t)
+    yp = rp * np.sin(t) + r * np.cos(t)
+    xpp = rpp * np.cos(t) - 2 * rp * np.sin(t) - r * np.cos(t)
+    ypp = rpp * np.sin(t) + 2 * rp * np.cos(t) - r * np.sin(t)
+    s = np.hypot(xp, yp)
+    return Curve(np.stack([r * np.cos(t), r * np.sin(t)], 1), np.stack([yp / s, -xp / s], 1),
+                 s * 2 * np.pi / N, (xp * ypp - yp * xpp) / s ** 3)
+
+
+class HSS1D:
+    def __init__(self, handle, N):
+        self.handle = handle
+        self.N = N
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().hssc_hss1d_destroy(h)
+            self.handle = None
+
+    @property
+    def nboxes(self):
+        nb, d = C.c_int32(), C.c_int32()
+        check(lib().hssc_hss1d_info(self.handle, C.byref(nb), C.byref(d)))
+        return nb.value
+
+    @property
+    def depth(self):
+        nb, d = C.c_int32(), C.c_int32()
+        check(lib().hssc_hss1d_info(self.handle, C.byref(nb), C.byref(d)))
+        return d.value
+
+    def box(self, b):
+        lev, n, k = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().hssc_hss1d_box(self.handle, int(b), C.byref(lev), C.byref(n), C.byref(k),
+                                   None, None))
+        up = np.empty(k.value, dtype=np.int64)
+        dn = np.empty(k.value, dtype=np.int64)
+        check(lib().hssc_hss1d_box(self.handle, int(b), None, None, None,
+                                   up.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   dn.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(level=lev.value, nskel=n.value, kskel=k.value, isk_up=up, isk_dn=dn)
+
+
+def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64) -> HSS1D:
+    """HSS1D/HSS1D_compress_nsym.m:1 with type 'green'."""
+    if type_ != "green":
+        raise ValueError("only the 'green' (proxy) compression is implemented")
+    X = np.asfortranarray(curve.X, dtype=np.float64)
+    Nm = np.asfortranarray(curve.nrm, dtype=np.float64)
+    w = np.ascontiguousarray(curve.w, dtype=np.float64)
+    k = np.ascontiguousarray(curve.kappa, dtype=np.float64)
+    h = C.c_void_p()
+    check(lib().hssc_hss1d_compress(X.ctypes.data_as(_DP), Nm.ctypes.data_as(_DP),
+                                    w.ctypes.data_as(_DP), k.ctypes.data_as(_DP), X.shape[0],
+                                    int(nbox_max), float(acc), int(nproxy), C.byref(h)))
+    return HSS1D(h, X.shape[0])
+
+
+def HSS1D_invert_nsym(K_HSS: HSS1D, acc=None) -> HSS1D:
+    """HSS1D/HSS1D_invert_nsym.m:1 (the inverse factors are stored in the same handle)."""
+    check(lib().hssc_hss1d_invert(K_HSS.handle))
+    return K_HSS
+
+
+def _vec(fn, K_HSS, v):
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    out = np.empty_like(v)
+    check(fn(K_HSS.handle, v.ctypes.data_as(_DP), out.ctypes.data_as(_DP), 1))
+    return out
+
+
+def HSS1D_applyinv_nsym(K_HSS: HSS1D, f):
+    """HSS1D/HSS1D_applyinv_nsym.m:1 (one right-hand side)."""
+    return _vec(lib().hssc_hss1d_applyinv, K_HSS, f)
+
+
+def HSS1D_apply_nsym(K_HSS: HSS1D, q):
+    """HSS1D/HSS1D_apply_nsym.m:1 (one vector)."""
+    return _vec(lib().hssc_hss1d_apply, K_HSS, q)

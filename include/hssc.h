@@ -162,6 +162,34 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
                             void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
                             double* relres, void* stream);
 
+/* ---- HSS1D: boundary integral equations on curves ----------------------------------
+ * The operator is the second-kind Laplace double layer A = -1/2 I + D on a closed curve
+ * given by its N nodes (X, N x 2), outward unit normals (nrm, N x 2), quadrature weights
+ * w and curvature kappa (D_jj = -kappa_j w_j / 4pi).  The reference's Kernel_Eval takes
+ * positions only; its 'green' compression with this kernel is the BASELINE configuration
+ * of the HSS1D path.  Proxy circles carry log charges. */
+typedef struct hssc_hss1d hssc_hss1d;
+
+/* Replaces HSS1D/HSS1D_compress_nsym.m:1 (K_HSS = HSS1D_compress_nsym('green',X,nbox_max,
+ * acc,params)): binary tree (halve until <= nbox_max points), proxy circles of nproxy
+ * points at 1.75 x the circumradius, skeletons by GPU interpolative decompositions. */
+int hssc_hss1d_compress(const double* X, const double* nrm, const double* w, const double* kappa,
+                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
+                        hssc_hss1d** out);
+/* Replaces HSS1D/HSS1D_invert_nsym.m:1 ([HSSinv,FTinv] = HSS1D_invert_nsym(K_HSS,acc)). */
+int hssc_hss1d_invert(hssc_hss1d* h);
+/* Replaces HSS1D/HSS1D_applyinv_nsym.m:1 and HSS1D/HSS1D_apply_nsym.m:1 (one RHS, m = 1). */
+int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t m);
+int hssc_hss1d_apply(const hssc_hss1d* h, const double* Q, double* U, int64_t m);
+int hssc_hss1d_applyinv_device(const hssc_hss1d* h, const void* dF, void* dX, int64_t m);
+int hssc_hss1d_apply_device(const hssc_hss1d* h, const void* dQ, void* dU, int64_t m);
+/* Tree size and per-box skeleton data (the K_HSS cell rows LEVEL, NSKEL, KSKEL, I_SKUP,
+ * I_SKDN of Common/box_constants.m:33-52; indices 0-based). */
+int hssc_hss1d_info(const hssc_hss1d* h, int32_t* nboxes, int32_t* depth);
+int hssc_hss1d_box(const hssc_hss1d* h, int32_t box, int32_t* level, int32_t* nskel,
+                   int32_t* kskel, int64_t* isk_up, int64_t* isk_dn);
+void hssc_hss1d_destroy(hssc_hss1d* h);
+
 /* ---- instrumentation -------------------------------------------------------------
  * No counterpart in the reference (which times stages with tic/toc,
  * HSS2D/Test/Test_HSS2D_InverseCompression.m:44-50): when enabled, every launch of a
