@@ -11,6 +11,7 @@ from .hss2d import (  # noqa: F401
     HSS2D_build_index_sets, HSS2D_build_tree_fsym, HSS2D_compress_fsym, HSS2D_invert_fsym,
     HSS_tree_parameters, Kernel_Eval, apply_device, applyinv_device, levHSS, solve_device,
 )
+from .interp import ID, LRID_rand  # noqa: F401
 from .hss1d import (  # noqa: F401
     Curve, HSS1D_apply_nsym, HSS1D_applyinv_nsym, HSS1D_compress_nsym, HSS1D_invert_nsym,
     star_contour,

@@ -62,6 +62,9 @@ SIGNATURES = [
     ("hssc_hss2d_apply_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("hssc_hss2d_solve_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int32,
                                           C.POINTER(C.c_int32), _DP, _P]),
+    ("hssc_id", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int64, C.c_double, _I64P, _DP, _I64P]),
+    ("hssc_lrid_rand", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
+                                 C.c_int64, C.c_uint64, _I64P, _I64P, _DP, _DP]),
     ("hssc_hss1d_compress", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32, C.c_double,
                                       C.c_int32, C.POINTER(_P)]),
     ("hssc_hss1d_invert", C.c_int, [_P]),

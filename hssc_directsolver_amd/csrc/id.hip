@@ -9,7 +9,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <cstdint>
+#include <string>
+#include <vector>
+
+#include "internal.h"
 
 #include "kernels.h"
 #include "stats.h"
@@ -282,5 +288,205 @@ template void cpqr_id<double>(hipStream_t, int, const int64_t*, const int*, cons
                               const double*, double, const int*, int*, const int64_t*, int*,
                               double*, const int64_t*, double*, const int64_t*, uint64_t);
 
+// Standard normal numbers (Box-Muller on the counter-based uniforms): G[i], i < len,
+// element i drawn from (seed, base + i).
+__global__ void __launch_bounds__(256) randn_kernel(int64_t len, uint64_t seed, uint64_t base,
+                                                    double* __restrict__ G) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= len) return;
+  const uint64_t c = base + uint64_t(i);
+  double u1 = urand(seed, 2 * c), u2 = urand(seed, 2 * c + 1);
+  if (u1 < 1e-300) u1 = 1e-300;
+  G[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+void randn(hipStream_t st, int64_t len, uint64_t seed, uint64_t base, double* G) {
+  if (len <= 0) return;
+  hipLaunchKernelGGL(randn_kernel, dim3((len + 255) / 256), dim3(256), 0, st, len, seed, base, G);
+}
+
 }  // namespace k
+
+namespace {
+
+struct IdErr {
+  int status;
+};
+
+#define IDCHK(expr)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      throw IdErr{fail(e_ == hipErrorOutOfMemory ? HSSC_ERR_OUT_OF_MEMORY : HSSC_ERR_HIP, \
+                       std::string(#expr) + ": " + hipGetErrorString(e_))};        \
+    }                                                                               \
+  } while (0)
+
+template <typename T>
+T* idalloc(size_t n) {
+  void* p = nullptr;
+  IDCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+// One matrix m x n (column-major, device, ld m) through the batched kernel.  mode acc > 0:
+// adaptive rank; acc <= 0: fixed rank kfix.  Returns rank, permutation, T (k x (n-k)) and
+// the |R_ii| of the pivoted QR.
+struct IdOut {
+  int k = 0;
+  std::vector<int> perm;
+  std::vector<double> T, rdiag;
+};
+
+IdOut id_device(double* dA, int m, int n, double acc, int kfix, uint64_t seed) {
+  std::vector<void*> tmp;
+  auto al = [&](size_t bytes) {
+    void* p = nullptr;
+    IDCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+    tmp.push_back(p);
+    return p;
+  };
+  IdOut out;
+  try {
+    const int64_t zero = 0;
+    int64_t* doff = static_cast<int64_t*>(al(8));
+    int* dm = static_cast<int*>(al(4));
+    int* dn = static_cast<int*>(al(4));
+    int* dkf = static_cast<int*>(al(4));
+    int* dk = static_cast<int*>(al(4));
+    int* dperm = static_cast<int*>(al(size_t(n) * 4));
+    double* dT = static_cast<double*>(al(size_t(n) * n * 8));
+    double* dW = static_cast<double*>(al((6 * size_t(n) + m) * 8));
+    double* dC = static_cast<double*>(al(size_t(m) * n * 8));
+    IDCHK(hipMemcpy(doff, &zero, 8, hipMemcpyHostToDevice));
+    IDCHK(hipMemcpy(dm, &m, 4, hipMemcpyHostToDevice));
+    IDCHK(hipMemcpy(dn, &n, 4, hipMemcpyHostToDevice));
+    IDCHK(hipMemcpy(dkf, &kfix, 4, hipMemcpyHostToDevice));
+    IDCHK(hipMemcpy(dC, dA, size_t(m) * n * 8, hipMemcpyDeviceToDevice));
+    k::cpqr_id<double>(0, 1, doff, dm, dn, dA, dC, acc > 0 ? acc : -1.0, dkf, dperm, doff, dk, dT,
+                       doff, dW, doff, seed);
+    IDCHK(hipDeviceSynchronize());
+    IDCHK(hipGetLastError());
+    IDCHK(hipMemcpy(&out.k, dk, 4, hipMemcpyDeviceToHost));
+    out.perm.resize(n);
+    IDCHK(hipMemcpy(out.perm.data(), dperm, size_t(n) * 4, hipMemcpyDeviceToHost));
+    out.T.resize(size_t(out.k) * (n - out.k));
+    if (!out.T.empty())
+      IDCHK(hipMemcpy(out.T.data(), dT, out.T.size() * 8, hipMemcpyDeviceToHost));
+    const int kmax = std::min(m, n);
+    out.rdiag.resize(kmax);
+    if (kmax) IDCHK(hipMemcpy(out.rdiag.data(), dW + n, size_t(kmax) * 8, hipMemcpyDeviceToHost));
+  } catch (...) {
+    for (void* p : tmp) (void)hipFree(p);
+    throw;
+  }
+  for (void* p : tmp) (void)hipFree(p);
+  return out;
+}
+
+double* upload_matrix(const double* A, int64_t m, int64_t n, int64_t lda) {
+  double* d = idalloc<double>(size_t(m) * n);
+  IDCHK(hipMemcpy2D(d, size_t(m) * 8, A, size_t(lda) * 8, size_t(m) * 8, n, hipMemcpyHostToDevice));
+  return d;
+}
+
+}  // namespace
 }  // namespace hssc
+
+using namespace hssc;
+
+extern "C" {
+
+int hssc_id(const double* A, int64_t m, int64_t n, int64_t lda, double par, int64_t* perm,
+            double* T, int64_t* krank) {
+  // Common/ID.m:1 ([T,I] = ID(A,par)): par < 1 is the accuracy, par >= 1 the fixed rank.
+  if (!A || !perm || !T || !krank || m < 1 || n < 1 || lda < m || m > INT32_MAX || n > 65536)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid ID arguments");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(HSSC_ERR_HIP, "no HIP device is available");
+  double* dA = nullptr;
+  try {
+    dA = upload_matrix(A, m, n, lda);
+    const bool fixed = par >= 1;
+    IdOut o = id_device(dA, int(m), int(n), fixed ? -1.0 : par, fixed ? int(par) : 0, 0x1D5EEDull);
+    (void)hipFree(dA);
+    *krank = o.k;
+    for (int64_t i = 0; i < n; ++i) perm[i] = o.perm[i];
+    std::copy(o.T.begin(), o.T.end(), T);
+    return HSSC_OK;
+  } catch (const IdErr& e) {
+    if (dA) (void)hipFree(dA);
+    return e.status;
+  }
+}
+
+int hssc_lrid_rand(const double* A, int64_t m, int64_t n, int64_t lda, double acc, int64_t k0,
+                   int64_t Cinc, uint64_t seed, int64_t* krank, int64_t* J, double* U, double* V) {
+  // Common/LRID_rand.m:10-40 with dense matvecs: Y = A^T G (Gaussian G, QR_findrank.m
+  // with the sketch grown by C columns while the last pivoted-QR diagonal of Y exceeds acc),
+  // then the fixed-rank ID of Z = Y(:,1:k)^T (ID_rand.m:41-49); A ~ U V^T with
+  // U = A(:, J(1:k)) and V(J,:) = [I T]^T.  U (m x k) and V (n x k) need min(m,n) columns.
+  if (!A || !krank || !J || !U || !V || m < 1 || n < 1 || lda < m || !(acc > 0) || k0 < 1 ||
+      Cinc < 1 || n > 65536)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid LRID_rand arguments");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(HSSC_ERR_HIP, "no HIP device is available");
+  std::vector<void*> tmp;
+  try {
+    double* dA = upload_matrix(A, m, n, lda);
+    tmp.push_back(dA);
+    const int64_t kmax = std::min(m, n);
+    int64_t kk = std::min(k0, kmax);
+    double* G = idalloc<double>(size_t(m) * kmax);
+    double* Y = idalloc<double>(size_t(n) * kmax);
+    double* Yc = idalloc<double>(size_t(n) * kmax);
+    tmp.push_back(G), tmp.push_back(Y), tmp.push_back(Yc);
+    int64_t have = 0, krk = 0;
+    for (;;) {
+      // extend the sketch to kk columns: G(:, have:kk) ~ N(0,1), Y(:, have:kk) = A^T G
+      k::randn(0, m * (kk - have), seed, uint64_t(m) * have, G + m * have);
+      k::gemm<double>(0, true, false, n, kk - have, m, 1.0, dA, m, 0, G + m * have, m, 0, 0.0,
+                      Y + n * have, n, 0, 1);
+      have = kk;
+      IDCHK(hipMemcpy(Yc, Y, size_t(n) * kk * 8, hipMemcpyDeviceToDevice));
+      IdOut q = id_device(Yc, int(n), int(kk), -1.0, int(kk), seed ^ 0x51ull);
+      krk = 0;
+      for (double d : q.rdiag) krk += d > acc ? 1 : 0;
+      const double last = q.rdiag.empty() ? 0.0 : q.rdiag.back();
+      if (!(last > acc) || kk >= kmax) break;
+      kk = std::min(kk + Cinc, kmax);
+    }
+    if (krk == 0) {
+      *krank = 0;
+      return HSSC_OK;
+    }
+    // Z = Y(:, 1:krk)^T  (krk x n), fixed-rank ID
+    std::vector<double> hY(size_t(n) * krk), hZ(size_t(krk) * n);
+    IDCHK(hipMemcpy(hY.data(), Y, hY.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = 0; j < krk; ++j) hZ[j + i * krk] = hY[i + j * n];
+    double* dZ = idalloc<double>(hZ.size());
+    tmp.push_back(dZ);
+    IDCHK(hipMemcpy(dZ, hZ.data(), hZ.size() * 8, hipMemcpyHostToDevice));
+    IdOut z = id_device(dZ, int(krk), int(n), -1.0, int(krk), seed ^ 0x77ull);
+    const int64_t k = z.k;
+    *krank = k;
+    for (int64_t i = 0; i < n; ++i) J[i] = z.perm[i];
+    for (int64_t j = 0; j < k; ++j)
+      for (int64_t i = 0; i < m; ++i) U[i + j * m] = A[i + int64_t(z.perm[j]) * lda];
+    for (int64_t i = 0; i < n * k; ++i) V[i] = 0;
+    for (int64_t j = 0; j < k; ++j) {
+      V[z.perm[j] + j * n] = 1.0;
+      for (int64_t c = 0; c < n - k; ++c) V[z.perm[k + c] + j * n] = z.T[j + c * k];
+    }
+    for (void* p : tmp) (void)hipFree(p);
+    return HSSC_OK;
+  } catch (const IdErr& e) {
+    for (void* p : tmp) (void)hipFree(p);
+    return e.status;
+  }
+}
+
+}  // extern "C"

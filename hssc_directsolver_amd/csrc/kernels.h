@@ -87,5 +87,8 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
              T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed);
 
+// G[i] ~ N(0,1), deterministic in (seed, base + i).
+void randn(hipStream_t st, int64_t len, uint64_t seed, uint64_t base, double* G);
+
 }  // namespace k
 }  // namespace hssc

@@ -162,6 +162,22 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
                             void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
                             double* relres, void* stream);
 
+/* ---- interpolative decompositions ------------------------------------------------- */
+/* Replaces Common/ID.m:1 ([T,I] = ID(A,par)): column-pivoted QR on the GPU; par < 1 is the
+ * accuracy (rank = #{|R_ii| > par |R_11|} plus the reference's randomized check),
+ * par >= 1 the fixed rank.  A: m x n host array (leading dimension lda).  Outputs: the
+ * rank k, the column permutation perm (n, 0-based, skeleton first) and T (k x (n-k),
+ * column-major, leading dimension k; room for n*n doubles). */
+int hssc_id(const double* A, int64_t m, int64_t n, int64_t lda, double par, int64_t* perm,
+            double* T, int64_t* k);
+
+/* Replaces Common/LRID_rand.m:1 with dense matvecs: A ~ U V^T with U = A(:, J(1:k)) and
+ * V(J,:) = [I T]^T, from a Gaussian sketch Y = A^T G of k0 columns grown by C columns while
+ * the last pivoted-QR diagonal of Y exceeds acc (QR_findrank.m), then the fixed-rank ID of
+ * Y^T (ID_rand.m).  U: m x min(m,n), V: n x min(m,n) host buffers; J: n entries. */
+int hssc_lrid_rand(const double* A, int64_t m, int64_t n, int64_t lda, double acc, int64_t k0,
+                   int64_t C, uint64_t seed, int64_t* k, int64_t* J, double* U, double* V);
+
 /* ---- HSS1D: boundary integral equations on curves ----------------------------------
  * The operator is the second-kind Laplace double layer A = -1/2 I + D on a closed curve
  * given by its N nodes (X, N x 2), outward unit normals (nrm, N x 2), quadrature weights

@@ -363,15 +363,16 @@ def _rand_error(A, T, I, k, rng):
     return math.sqrt(np.linalg.norm(y))
 
 
-def qr_findrank(m, n, matvecAt, k0, C, rng):
+def qr_findrank(m, n, matvecAt, k0, C, rng, eps):
     """Common/QR_findrank.m:45-92: adaptive Gaussian range finder on A^T.
 
+    ``eps`` is the routine's tolerance argument (the caller passes acc,
+    interpolation_operator.m:119 -> Rand_AT_sample.m:26): |R_ii| > eps counts toward the rank.
     Returns (krank, Y) with Y = A^T G (n x krank)."""
-    eps = np.finfo(float).eps
     k0 = min(k0, min(m, n))
     G = rng.standard_normal((m, k0))
     Y = matvecAt(G)
-    Q, R = np.linalg.qr(Y)
+    Q, R, _ = sla.qr(Y, mode="economic", pivoting=True)
     D = np.abs(np.diag(R))
     residual = D[-1] if D.size else 0.0
     krank = int(np.sum(D > eps))
@@ -384,7 +385,7 @@ def qr_findrank(m, n, matvecAt, k0, C, rng):
                                                        dtype=Y.dtype)], axis=1)
         Y[:, krank:krank + num] = Ynew
         M = Ynew - Q @ (Q.T @ Ynew)
-        Qc, R = np.linalg.qr(M)
+        Qc, R, _ = sla.qr(M, mode="economic", pivoting=True)
         D = np.abs(np.diag(R))
         residual = D[-1] if D.size else 0.0
         krank += int(np.sum(D > eps))
@@ -402,12 +403,12 @@ def skel_colk(Z, k):
     return T, I
 
 
-def lrid_rand(K21, k0, C, rng):
+def lrid_rand(K21, k0, C, rng, tol):
     """Common/LRID_rand.m:10-40 with ID_rand.m:26-49 and Rand_AT_sample.m:24-41.
 
     K21 ~ U V^T with U = K21(:, J(1:k)) and V(J,:) = [I T]^T.  Returns (U, V)."""
     m, n = K21.shape
-    krank, Y = qr_findrank(m, n, lambda G: K21.T @ G, k0, C, rng)
+    krank, Y = qr_findrank(m, n, lambda G: K21.T @ G, k0, C, rng, tol)
     Z = Y.T
     T, J = skel_colk(Z, krank)
     k = T.shape[0]
@@ -437,7 +438,7 @@ def interpolation_operator(params, Xp, X_sk, X_rs, lowrank, q, lev, rng):
         k0 = min(max(int(math.ceil(2 * q[lev + 2] - q[lev + 4] + 2)), q[lev + 2]), min(m, n))
         C = int(math.ceil(abs(q[lev + 2] - q[lev + 4]) / 2)) + 1
     k0 = max(k0, 1)
-    U, V = lrid_rand(K21, k0, C, rng)
+    U, V = lrid_rand(K21, k0, C, rng, params["acc"])
     return ls_solve(K11, U), V
 
 
