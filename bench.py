@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-np", type=int, default=64, help="np of the CPU baseline sample")
+    ap.add_argument("--no-hss1d", action="store_true", help="skip the HSS1D side case")
     return ap.parse_args()
 
 
@@ -86,6 +87,44 @@ def one_step(H, P, torch, b, tol):
         res[prec] = dict(invert=c - a, solve=d - c, iters=it, relres=rel)
         del inv
     return t, res, tree, op
+
+
+def hss1d_case(H, torch, N=4096, reps=3):
+    """HSS1D: second-kind Laplace double layer on the star contour
+    r(t) = 1 + 0.3 cos(5t), N trapezoidal nodes, leaf 64, ID tol 1e-10, 64 proxy points;
+    Gaussian RHS from seed 0.  Returns GPU times and the oracle's CPU time on the same
+    problem (all of it: the CPU oracle finishes N = 4096 in about a second)."""
+    import numpy as np
+    from oracle import hss1d as o1
+    curve = H.star_contour(N)
+    f = np.random.default_rng(0).standard_normal(N)
+    best = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K = H.HSS1D_compress_nsym("green", curve, 64, 1e-10, nproxy=64)
+        t1 = time.perf_counter()
+        H.HSS1D_invert_nsym(K)
+        t2 = time.perf_counter()
+        x = H.HSS1D_applyinv_nsym(K, f)
+        t3 = time.perf_counter()
+        r = (t1 - t0, t2 - t1, t3 - t2)
+        best = r if best is None or sum(r) < sum(best) else best
+    resid = float(np.linalg.norm(H.HSS1D_apply_nsym(K, x) - f) / np.linalg.norm(f))
+    C = o1.star_contour(N)
+    c0 = time.perf_counter()
+    Ho = o1.compress(C, 64, 1e-10, nproxy=64)
+    c1 = time.perf_counter()
+    inv = o1.invert(Ho)
+    c2 = time.perf_counter()
+    o1.applyinv(Ho, inv, f)
+    c3 = time.perf_counter()
+    return dict(workload=f"HSS1D DL star contour N={N} leaf 64 tol 1e-10 nproxy 64",
+                compress_s=round(best[0], 4), invert_s=round(best[1], 4),
+                solve_ms=round(1e3 * best[2], 3), total_s=round(sum(best), 4),
+                relres_vs_hss=resid,
+                cpu_oracle=dict(compress_s=round(c1 - c0, 3), invert_s=round(c2 - c1, 3),
+                                solve_ms=round(1e3 * (c3 - c2), 2), total_s=round(c3 - c0, 3)))
 
 
 def cpu_baseline(np_cpu):
@@ -186,6 +225,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_np)
+    h1 = hss1d_case(H, torch) if not args.no_hss1d else None
 
     out = {
         "metric": "HSS2D N=512² build s + solve ms @1e-10 resid: mixed+batched vs FP64 GPU",
@@ -218,6 +258,7 @@ def main():
                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
                     for k, v in stats.items()},
         "cpu_baseline": cpu,
+        "hss1d": h1,
     }
     print(json.dumps(out), flush=True)
 
