@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> Path:
         _run(cmd, verbose)
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([cc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs], verbose)
+    _run([cc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -2,7 +2,9 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/hssc.h"
@@ -11,6 +13,30 @@ namespace hssc {
 
 // Records a failure message for hssc_last_error() and returns the status.
 int fail(int status, const std::string& msg);
+
+// Allocator whose value-initialisation is a no-op: resize() leaves new elements
+// uninitialised, so large host arrays are first touched (and paged in) by the threads
+// that fill them instead of being zeroed serially.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+template <class T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
 
 // Per-level data.  In a uniform tree every box of a level has the same n, k and M, so the
 // GPU works on strided batches of equal-size blocks.  Matrices (J, T, F^-1, E) are stored
@@ -22,12 +48,12 @@ struct Level {
   int64_t q = 0;                // columns of the stored T (n-k for dense T)
 
   // Host copies (index sets are built on the host).
-  std::vector<int32_t> J;      // nkey x n   local permutation, skeleton first
-  std::vector<int32_t> Jinv;   // nkey x n   inverse permutation
-  std::vector<double> Px, Py;  // nkey x n   points of I_src in J order
-  std::vector<double> Pw;      // nkey x n   per-point kernel weight b(x) = c(x)
-  std::vector<double> PNx, PNy, PNw;  // nkey x n   the same points in natural I_src order
-  std::vector<double> Xpx, Xpy, Xpw;  // nkey x M proxy points and weights
+  hvec<int32_t> J;      // nkey x n   local permutation, skeleton first
+  hvec<int32_t> Jinv;   // nkey x n   inverse permutation
+  hvec<double> Px, Py;  // nkey x n   points of I_src in J order
+  hvec<double> Pw;      // nkey x n   per-point kernel weight b(x) = c(x)
+  hvec<double> PNx, PNy, PNw;  // nkey x n   the same points in natural I_src order
+  hvec<double> Xpx, Xpy, Xpw;  // nkey x M proxy points and weights
 
   // Device copies (owned by the tree).
   int32_t* dJ = nullptr;

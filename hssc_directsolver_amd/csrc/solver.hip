@@ -54,8 +54,8 @@ T* dalloc(size_t count) {
   return static_cast<T*>(p);
 }
 
-template <typename T>
-T* dupload(const std::vector<T>& h) {
+template <class T, class A>
+T* dupload(const std::vector<T, A>& h) {
   T* d = dalloc<T>(h.size());
   if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return d;

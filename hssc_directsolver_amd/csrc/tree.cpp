@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <thread>
 
 #include "internal.h"
 
@@ -199,6 +200,12 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
   source_points(p.Ng, p.np, t->X.data());
   const double* Xx = t->X.data();
   const double* Xy = t->X.data() + N;
+  // Power-of-two Ng and np make h, the grid, the box centres and the proxy rings dyadic
+  // rationals with short mantissas: every coordinate and every difference is exact.
+  const bool dyadic = (p.Ng & (p.Ng - 1)) == 0 && (p.np & (p.np - 1)) == 0;
+  // per-point kernel weights b(x) = c(x), evaluated once per grid point
+  std::vector<double> wgt(N);
+  for (int64_t g = 0; g < N; ++g) wgt[g] = point_weight(p, Xx[g], Xy[g]);
 
   // Heap-numbered centres (bisect_level, :1289-1344).
   const int64_t nboxes = (int64_t(1) << (depth + 1)) - 1;
@@ -242,6 +249,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
     level_radii(lev, &colrad, &rowrad);
     const int lay_prox = proxy_layers(lev, p);
     const int64_t b0 = int64_t(1) << lev;
+    // I_src of every box of the level (children's skeletons, :358-363)
     for (int64_t bi = 0; bi < L.nbox; ++bi) {
       const int64_t b = b0 + bi;
       if (lev < depth) {
@@ -249,51 +257,109 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
         I = t->Isk[2 * b];
         I.insert(I.end(), t->Isk[2 * b + 1].begin(), t->Isk[2 * b + 1].end());
       }
-      const auto& I = t->Isrc[b];
-      const int64_t n = int64_t(I.size());
+      const int64_t n = int64_t(t->Isrc[b].size());
       if (bi == 0) L.n = n;
       if (n != L.n) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform box sizes on a level");
-      if (!TI || bi == 0) {
-        std::vector<double> xs(n), ys(n);
-        for (int64_t i = 0; i < n; ++i) xs[i] = Xx[I[i]], ys[i] = Xy[I[i]];
-        const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
-        std::vector<int32_t> J;
-        int64_t k;
-        preset_skeleton(xs, ys, cx, cy, lev, p, &J, &k);
-        std::vector<double> qx, qy;
-        box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, &qx, &qy);
-        if (bi == 0) {
-          L.k = k;
-          L.M = int64_t(qx.size());
-          L.J.resize(L.nkey * n);
-          L.Jinv.resize(L.nkey * n);
-          L.Px.resize(L.nkey * n), L.Py.resize(L.nkey * n), L.Pw.resize(L.nkey * n);
-          L.PNx.resize(L.nkey * n), L.PNy.resize(L.nkey * n), L.PNw.resize(L.nkey * n);
-          L.Xpx.resize(L.nkey * L.M), L.Xpy.resize(L.nkey * L.M), L.Xpw.resize(L.nkey * L.M);
-        }
-        if (k != L.k || int64_t(qx.size()) != L.M)
-          return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes on a level");
-        const int64_t key = TI ? 0 : bi;
-        for (int64_t a = 0; a < n; ++a) {
-          L.J[key * n + a] = J[a];
-          L.Jinv[key * n + J[a]] = int32_t(a);
-          L.Px[key * n + a] = xs[J[a]];
-          L.Py[key * n + a] = ys[J[a]];
-          L.Pw[key * n + a] = point_weight(p, xs[J[a]], ys[J[a]]);
-          L.PNx[key * n + a] = xs[a];
-          L.PNy[key * n + a] = ys[a];
-          L.PNw[key * n + a] = point_weight(p, xs[a], ys[a]);
-        }
-        for (int64_t r = 0; r < L.M; ++r) {
-          L.Xpx[key * L.M + r] = qx[r];
-          L.Xpy[key * L.M + r] = qy[r];
-          L.Xpw[key * L.M + r] = point_weight(p, qx[r], qy[r]);
-        }
+    }
+    // skeleton, J order and proxy of one box (key slot = bi for NTI, 0 for TI)
+    auto process = [&](int64_t bi) -> bool {
+      const int64_t b = b0 + bi;
+      const auto& I = t->Isrc[b];
+      const int64_t n = L.n;
+      std::vector<double> xs(n), ys(n);
+      for (int64_t i = 0; i < n; ++i) xs[i] = Xx[I[i]], ys[i] = Xy[I[i]];
+      const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
+      std::vector<int32_t> J;
+      int64_t k;
+      preset_skeleton(xs, ys, cx, cy, lev, p, &J, &k);
+      std::vector<double> qx, qy;
+      box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, &qx, &qy);
+      if (bi == 0) {
+        L.k = k;
+        L.M = int64_t(qx.size());
+        L.J.resize(L.nkey * n);
+        L.Jinv.resize(L.nkey * n);
+        L.Px.resize(L.nkey * n), L.Py.resize(L.nkey * n), L.Pw.resize(L.nkey * n);
+        L.PNx.resize(L.nkey * n), L.PNy.resize(L.nkey * n), L.PNw.resize(L.nkey * n);
+        L.Xpx.resize(L.nkey * L.M), L.Xpy.resize(L.nkey * L.M), L.Xpw.resize(L.nkey * L.M);
       }
+      if (k != L.k || int64_t(qx.size()) != L.M) return false;
+      const int64_t key = TI ? 0 : bi;
+      for (int64_t a = 0; a < n; ++a) {
+        L.J[key * n + a] = J[a];
+        L.Jinv[key * n + J[a]] = int32_t(a);
+        L.Px[key * n + a] = xs[J[a]];
+        L.Py[key * n + a] = ys[J[a]];
+        L.Pw[key * n + a] = wgt[I[J[a]]];
+        L.PNx[key * n + a] = xs[a];
+        L.PNy[key * n + a] = ys[a];
+        L.PNw[key * n + a] = wgt[I[a]];
+      }
+      for (int64_t r = 0; r < L.M; ++r) {
+        L.Xpx[key * L.M + r] = qx[r];
+        L.Xpy[key * L.M + r] = qy[r];
+        L.Xpw[key * L.M + r] = point_weight(p, qx[r], qy[r]);
+      }
+      return true;
+    };
+    if (!process(0)) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
+    if (!TI && L.nbox > 1 && dyadic) {
+      // All coordinates are exact dyadic rationals, so every box of the level is an exact
+      // translate of box 0: the same J (same comparisons on bit-identical relative
+      // coordinates) and the same proxy ring shifted by the centre difference.
+      const int64_t n = L.n, M = L.M;
+      const int64_t c0 = b0;
+      auto copy_box = [&](int64_t bi) {
+        const int64_t b = b0 + bi, key = bi;
+        const auto& I = t->Isrc[b];
+        const double dx = t->cent[2 * b] - t->cent[2 * c0], dy = t->cent[2 * b + 1] - t->cent[2 * c0 + 1];
+        for (int64_t a = 0; a < n; ++a) {
+          const int32_t j = L.J[a];
+          L.J[key * n + a] = j;
+          L.Jinv[key * n + j] = int32_t(a);
+          L.Px[key * n + a] = Xx[I[j]];
+          L.Py[key * n + a] = Xy[I[j]];
+          L.Pw[key * n + a] = wgt[I[j]];
+          L.PNx[key * n + a] = Xx[I[a]];
+          L.PNy[key * n + a] = Xy[I[a]];
+          L.PNw[key * n + a] = wgt[I[a]];
+        }
+        for (int64_t r = 0; r < M; ++r) {
+          const double qx = L.Xpx[r] + dx, qy = L.Xpy[r] + dy;
+          L.Xpx[key * M + r] = qx;
+          L.Xpy[key * M + r] = qy;
+          L.Xpw[key * M + r] = point_weight(p, qx, qy);
+        }
+      };
+      const int64_t nthr = std::max<int64_t>(
+          1, std::min<int64_t>({(L.nbox - 1 + 63) / 64, int64_t(std::thread::hardware_concurrency()), 16}));
+      std::vector<std::thread> pool;
+      for (int64_t w = 0; w < nthr; ++w)
+        pool.emplace_back([&, w]() {
+          for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr) copy_box(bi);
+        });
+      for (auto& th : pool) th.join();
+    } else if (!TI && L.nbox > 1) {
+      // boxes are independent and write disjoint key slots: split them over threads
+      const int64_t nthr = std::max<int64_t>(
+          1, std::min<int64_t>({L.nbox - 1, int64_t(std::thread::hardware_concurrency()), 16}));
+      std::vector<std::thread> pool;
+      std::vector<char> ok(nthr, 1);
+      for (int64_t w = 0; w < nthr; ++w)
+        pool.emplace_back([&, w]() {
+          for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr)
+            if (!process(bi)) ok[w] = 0;
+        });
+      for (auto& th : pool) th.join();
+      for (char c : ok)
+        if (!c) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
+    }
+    for (int64_t bi = 0; bi < L.nbox; ++bi) {
+      const int64_t b = b0 + bi;
       const int64_t key = TI ? 0 : bi;
       auto& S = t->Isk[b];
       S.resize(L.k);
-      for (int64_t a = 0; a < L.k; ++a) S[a] = I[L.J[key * n + a]];
+      for (int64_t a = 0; a < L.k; ++a) S[a] = t->Isrc[b][L.J[key * L.n + a]];
     }
     L.q = L.n - L.k;
   }
