@@ -21,7 +21,7 @@ class HsscParams(C.Structure):
         ("transinv", C.c_int32), ("depth", C.c_int32), ("h", C.c_double),
         ("max_particles", C.c_int32), ("interform", C.c_int32), ("k_cut", C.c_int32),
         ("dr_weights_re", C.c_double), ("dr_weights_im", C.c_double),
-        ("factor_precision", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -35,6 +35,7 @@ SIGNATURES = [
     ("hssc_version", C.c_char_p, []),
     ("hssc_params_init", C.c_int, [_PP, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                    C.c_double, C.c_int32, C.c_int32, C.c_int32]),
+    ("hssc_dr_corr0", C.c_int, [C.c_double, C.c_double, _DP, _DP]),
     ("hssc_params_set_dr_weights", C.c_int, [_PP, C.c_double, C.c_double]),
     ("hssc_params_npoints", C.c_int64, [_PP]),
     ("hssc_source_points", C.c_int, [_PP, _DP]),

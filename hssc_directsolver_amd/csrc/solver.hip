@@ -631,6 +631,10 @@ int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_
   p->interform = HSSC_INTER_LOWRANK;
   p->k_cut = 2500;
   p->factor_precision = HSSC_PREC_FP64;
+  if (kh > 0) {
+    // :167-175, params.order = 4: dr_weights = corr0(h, kh)
+    if (int s = hssc_dr_corr0(p->h, kh, &p->dr_weights_re, &p->dr_weights_im)) return s;
+  }
   return HSSC_OK;
 }
 
@@ -679,8 +683,8 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
 int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
-  if (p->flag_pot != HSSC_POT_SL_L_2D)
-    return fail(HSSC_ERR_UNSUPPORTED, "only SL_L_2D trees are implemented");
+  if (p->flag_pot == HSSC_POT_SL_H_2D && !(p->kh > 0))
+    return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
   return guarded([&]() {
     auto* t = new hssc_tree();
     const int s = build_index_sets(*p, t);

@@ -119,13 +119,15 @@ void colon(double a, double d, double b, std::vector<double>* out) {
 }
 
 // HSS2D/build_box_proxy.m:52-85: four strips of exterior points at spacing h, `layers`
-// deep, with the corner trimming of :71-84.
+// deep, split by the corner trimming of :71-84 into X_proxy and X_extra.  The 'bdry' rule
+// of LOCAL_Build_Proxy (HSS2D_build_tree_fsym.m:962-965) uses [X_proxy; X_extra]: the kept
+// points of strips 1-4 first, then the trimmed points of strips 1-4.
 void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, double h,
                std::vector<double>* px, std::vector<double>* py) {
   const double width = (layers - 0.5) * h;
   const double r1 = layers * layers - 0.5, r2 = layers * layers + 0.5;
   const double ew = colrad + width - h / 2, eh = rowrad + width - h / 2;
-  std::vector<double> xr, yr;
+  std::vector<double> xr, yr, ex, ey;
   px->clear();
   py->clear();
   // strip 1 (left): x fastest
@@ -138,6 +140,7 @@ void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, d
                        ((dy > rowrad + width - r1 * h) || (dy < -(rowrad + width - r2 * h)))) ||
                       (std::fabs(dy) > eh);
       if (!rm) px->push_back(x), py->push_back(y);
+      else ex.push_back(x), ey.push_back(y);
     }
   // strip 2 (top): y fastest
   colon(cx - colrad + h / 2, h, cx + colrad - h / 2, &xr);
@@ -148,6 +151,7 @@ void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, d
       const bool rm = (std::fabs(dy) > eh) &&
                       ((dx > colrad + width - r1 * h) || (dx < -(colrad + width - r2 * h)));
       if (!rm) px->push_back(x), py->push_back(y);
+      else ex.push_back(x), ey.push_back(y);
     }
   // strip 3 (right): x fastest
   colon(cx + colrad + h / 2, h, cx + colrad + width, &xr);
@@ -159,6 +163,7 @@ void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, d
                        ((dy > rowrad + width - r2 * h) || (dy < -(rowrad + width - r1 * h)))) ||
                       (std::fabs(dy) > eh);
       if (!rm) px->push_back(x), py->push_back(y);
+      else ex.push_back(x), ey.push_back(y);
     }
   // strip 4 (bottom): y fastest
   colon(cx + colrad - h / 2, -h, cx - colrad + h / 2, &xr);
@@ -169,7 +174,10 @@ void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, d
       const bool rm = (std::fabs(dy) > eh) &&
                       ((dx > colrad + width - r2 * h) || (dx < -(colrad + width - r1 * h)));
       if (!rm) px->push_back(x), py->push_back(y);
+      else ex.push_back(x), ey.push_back(y);
     }
+  px->insert(px->end(), ex.begin(), ex.end());
+  py->insert(py->end(), ey.begin(), ey.end());
 }
 
 int proxy_layers(int lev, const hssc_params& p) {
@@ -177,6 +185,7 @@ int proxy_layers(int lev, const hssc_params& p) {
   double colrad, rowrad;
   level_radii(lev, &colrad, &rowrad);
   if (lev > 0) {
+    if (p.proxy_layers > 0) return p.proxy_layers;  // caller override of the rule below
     if (p.flag_pot == HSSC_POT_SL_L_2D) return p.layers + 1;
     return 2 * int(std::lround(std::min(rowrad, colrad) / p.h));
   }

@@ -62,7 +62,9 @@ typedef struct hssc_params {
   double dr_weights_re; /* Duan-Rokhlin corr0, real part              (.dr_weights) */
   double dr_weights_im; /* Duan-Rokhlin corr0, imaginary part                       */
   int32_t factor_precision; /* HSSC_PREC_* (no reference counterpart: GPU policy)   */
-  int32_t reserved[7];
+  int32_t proxy_layers; /* > 0: proxy ring depth at lev > 0 instead of lay_prox
+                           (HSS2D_build_tree_fsym.m:339-347); 0 keeps the reference rule */
+  int32_t reserved[6];
 } hssc_params;
 
 typedef struct hssc_tree hssc_tree;         /* INFO_TREE + params.X_source            */
@@ -76,10 +78,18 @@ const char* hssc_last_error(void);
 const char* hssc_version(void);
 
 /* Replaces Common/HSS_tree_parameters.m:1 (params = HSS_tree_parameters(pot,kh,sym,Ng,np,
- * acc,lay,n_cut,TI)), defaults of :116-139.  The Duan-Rokhlin weight for kh>0
- * (:167-175) is given by hssc_params_set_dr_weights. */
+ * acc,lay,n_cut,TI)), defaults of :116-139.  For kh > 0 it also sets the order-4
+ * Duan-Rokhlin weight (:167-175) from hssc_dr_corr0(h, kh). */
 int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_t Ng,
                      int32_t np, double acc, int32_t lay, int32_t n_cut, int32_t TI);
+
+/* Order-4 Duan-Rokhlin centre weight corr0(h, k) of the 2D Helmholtz kernel H0(k r):
+ * replaces Common/DR_stencil.m:64-107 (LOCAL_compute_duan_rokhlin_weights, DVEC(1)), which
+ * runs the Fortran side-car Common/FORTRAN_get_DR_weights.f:85-490 (corr_hank).  Computed
+ * from the lattice-sum closed form corr0 = h^2 + i (2h^2/pi)[log(hk/2) + gamma + Z'(0) +
+ * sum_m (-1)^m (hk/2)^2m / (m!)^2 Z'(-2m)], Z the Epstein zeta function of Z^2.  Like the
+ * side-car, it accepts hk in [1e-4, 1.01] only (HSSC_ERR_INVALID_ARG otherwise). */
+int hssc_dr_corr0(double h, double k, double* corr0_re, double* corr0_im);
 
 /* Sets params.dr_weights = corr0 (Common/DR_stencil.m:10-25 output, order 4). */
 int hssc_params_set_dr_weights(hssc_params* p, double corr0_re, double corr0_im);

@@ -257,7 +257,8 @@ def _colon(a, d, b):
 
 def box_proxy(cB, colrad, rowrad, layers, h):
     """HSS2D/build_box_proxy.m:52-85: four strips of exterior points at spacing h,
-    ``layers`` deep, corners trimmed (Remark 3.4 of the paper).  Returns X_proxy."""
+    ``layers`` deep, split into X_proxy (corners trimmed, Remark 3.4 of the paper) and the
+    trimmed X_extra.  Returns [X_proxy; X_extra], the proxy LOCAL_Build_Proxy uses."""
     cx, cy = cB
     width = (layers - 0.5) * h
     # strip 1 (left): meshgrid then transpose -> x fastest
@@ -293,14 +294,22 @@ def box_proxy(cB, colrad, rowrad, layers, h):
     d = X4 - (cx, cy)
     I4 = (np.abs(d[:, 1]) > eh) & ((d[:, 0] > colrad + width - r2 * h) |
                                    (d[:, 0] < -(colrad + width - r1 * h)))
-    return np.concatenate([X1[~I1], X2[~I2], X3[~I3], X4[~I4]])
+    # LOCAL_Build_Proxy (HSS2D_build_tree_fsym.m:962-965, proxy_rule 'bdry') keeps the
+    # trimmed corner points too, appended after X_proxy: X_ext = [X_proxy; X_extra].
+    return np.concatenate([X1[~I1], X2[~I2], X3[~I3], X4[~I4], X1[I1], X2[I2], X3[I3], X4[I4]])
 
 
 def proxy_layers(lev, params):
-    """lay_prox of HSS2D/HSS2D_build_tree_fsym.m:339-347."""
+    """lay_prox of HSS2D/HSS2D_build_tree_fsym.m:339-347.
+
+    ``params['proxy_layers'] > 0`` replaces the rule for lev > 0 (the reference's
+    2*round(rad/h) for non-Laplace kernels grows with the box: 256 layers at level 1 of a
+    512^2 grid)."""
     colrad, rowrad = level_radii(lev)
     rad = min(rowrad, colrad)
     if lev > 0 or params["m"] < 1:
+        if params.get("proxy_layers", 0) > 0 and lev > 0:
+            return params["proxy_layers"]
         if params["flag_pot"] == "SL_L_2D":
             return params["layers"] + 1
         return 2 * int(round(rad / params["h"]))
@@ -479,17 +488,20 @@ def build_index_sets(params):
     return build_tree(params, with_T=False)
 
 
-def build_tree(params, interform=None, seed=0, with_T=True, timings=None):
+def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False):
     """HSS2D_build_tree_fsym (uniform tree, preset 'bdry' skeletons and proxies).
 
     Restates :81-287 (tree), :292-623 (upward skeleton pass) and
     LOCAL_Interpolation_Operators :1010-1064.  ``interform`` defaults to params
     INTERform ('lowrank' in the reference; 'dense' gives exact LS T at every level).
+    For kh > 0 the reference selects the skeletons of the two bottom levels by an ID
+    (:376-443); ``preset_all=True`` keeps the 'bdry' rule there as well (same operator
+    to tolerance, uniform level shapes).
     """
     if params["skel_rule"] != "bdry" or params["proxy_rule"] != "bdry":
         raise NotImplementedError("only the 'bdry' skeleton/proxy rules are restated")
-    if params["kh"] > 0:
-        raise NotImplementedError("kh>0 (ID skeletons at the bottom levels) is not restated")
+    if params["kh"] > 0 and not preset_all:
+        raise NotImplementedError("kh>0 with ID skeletons at the bottom levels: pass preset_all")
     interform = interform or params["INTERform"]
     rng = np.random.default_rng(seed)
     depth = params["depth"]

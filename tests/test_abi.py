@@ -3,6 +3,8 @@ sets agree exactly with the oracle (no GPU needed)."""
 import re
 from pathlib import Path
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -61,14 +63,52 @@ def test_index_sets_bit_exact(TI, Ng, np_):
 
 
 def test_appendix_level_table_512():
-    # per-level (n, k) of the reference's 'bdry' rule at N = 512^2 (Ng=8, np=64, layers 2)
-    want = {12: (64, 48), 11: (96, 80), 10: (160, 112), 9: (224, 176), 8: (352, 240),
-            7: (480, 368), 6: (736, 496), 5: (992, 752), 4: (1504, 1008), 3: (2016, 1520),
-            2: (3040, 2032), 1: (4064, 3056), 0: (6112, 4080)}
+    # per-level (n, k, M) of the reference's 'bdry' rules at N = 512^2 (Ng=8, np=64,
+    # layers 2): M is the full proxy ring (corner points kept, LOCAL_Build_Proxy), i.e.
+    # (w + 2L)(h + 2L) - w h points for a w x h box and L = layers + 1 (L = layers at the root).
+    want = {12: (64, 48, 132), 11: (96, 80, 180), 10: (160, 112, 228), 9: (224, 176, 324),
+            8: (352, 240, 420), 7: (480, 368, 612), 6: (736, 496, 804), 5: (992, 752, 1188),
+            4: (1504, 1008, 1572), 3: (2016, 1520, 2340), 2: (3040, 2032, 3108),
+            1: (4064, 3056, 4644), 0: (6112, 4080, 4112)}
     t = H.HSS2D_build_index_sets(H.HSS_tree_parameters("SL_L_2D", 0, 1, 8, 64, 1e-10, 2, 400, 1))
-    for lev, (n, k) in want.items():
+    for lev, (n, k, M) in want.items():
         info = t.level_info(lev)
-        assert (info["n"], info["k"]) == (n, k), lev
+        assert (info["n"], info["k"], info["M"]) == (n, k, M), lev
+
+
+def test_helmholtz_level_table_512():
+    # Lippmann-Schwinger Helmholtz at N = 512^2 (k = 20, layers 3, NTI) with 4-layer proxy
+    # rings: the same geometry rules with layers = 3 (every box of a level equal)
+    want = {12: (64, 60, 192), 11: (120, 108, 256), 10: (216, 156, 320), 9: (312, 252, 448),
+            8: (504, 348, 576), 7: (696, 540, 832), 6: (1080, 732, 1088), 5: (1464, 1116, 1600),
+            4: (2232, 1500, 2112), 3: (3000, 2268, 3136), 2: (4536, 3036, 4160),
+            1: (6072, 4572, 6208), 0: (9144, 6108, 6180)}
+    P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 64, 1e-10, 3, 800, 0)
+    P.raw.proxy_layers = 4
+    t = H.HSS2D_build_index_sets(P)
+    for lev, (n, k, M) in want.items():
+        info = t.level_info(lev)
+        assert (info["n"], info["k"], info["M"]) == (n, k, M), lev
+
+
+def test_dr_corr0_known_answer_and_oracle():
+    from oracle.dr_weights import corr0
+    # INSTALL.txt:96-115 (the reference's Fortran side-car, h = 1/32, rk = 1)
+    ref = complex(0.976562500000008e-03, -0.304146129302758e-02)
+    c = corr0(1 / 32, 1.0)
+    assert abs(c - ref) < 2e-17
+    assert abs(c.real - (1 / 32) ** 2) == 0
+    L = _lib.lib()
+    cr, ci = ctypes.c_double(), ctypes.c_double()
+    for h, k in [(1 / 32, 1.0), (1 / 256, 20.0), (2 / 512, 20.0), (1 / 64, 50.0), (0.01, 100.0)]:
+        _lib.check(L.hssc_dr_corr0(h, k, ctypes.byref(cr), ctypes.byref(ci)))
+        want = corr0(h, k)
+        assert abs(complex(cr.value, ci.value) - want) <= 1e-15 * abs(want), (h, k)
+    # the side-car's accepted range of h*k
+    assert L.hssc_dr_corr0(1 / 32, 64.0, ctypes.byref(cr), ctypes.byref(ci)) != 0
+    # params_init sets corr0 for kh > 0 (HSS_tree_parameters.m:167-175)
+    P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 64, 1e-10, 3, 800, 0)
+    assert abs(complex(P.dr_weights_re, P.dr_weights_im) - corr0(1 / 256, 20.0)) < 1e-18
 
 
 def test_gpu_entry_points_fail_loudly_without_device():
