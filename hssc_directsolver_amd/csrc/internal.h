@@ -67,8 +67,10 @@ struct Level {
   double* dXpx = nullptr;
   double* dXpy = nullptr;
   double* dXpw = nullptr;
-  double* dT = nullptr;    // nkey x (k x (n-k)) column-major, FP64
-  float* dT32 = nullptr;   // FP32 copy for the FP32 factorization
+  // T: nkey x (k x (n-k)) column-major, FP64 (double, or interleaved complex for the
+  // Helmholtz kernel), plus the FP32 copy (float / complex float) of the FP32 factorization
+  void* dT = nullptr;
+  void* dT32 = nullptr;
 };
 
 }  // namespace hssc
@@ -86,6 +88,7 @@ struct hssc_tree {
   int32_t* dLeafG = nullptr;
   bool has_T = false;
   bool on_device = false;
+  bool cx = false;  // complex (Helmholtz) kernel: every matrix and vector is complex
   int device = 0;
   ~hssc_tree();
 };

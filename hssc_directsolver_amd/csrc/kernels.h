@@ -6,25 +6,44 @@
 
 #include <cstdint>
 
+#include "../../include/hssc.h"
+#include "scalar.h"
+
 namespace hssc {
 namespace k {
 
-// A(r,c) = K(R_r, C_c) for the 2D Laplace single-layer kernel with per-point weights.
+// The kernel of Common/Kernel_Eval.m:29-96 as evaluated on the GPU (2D plane, per-point
+// weights b(x) = c(x) passed with the points).
+struct KernelSpec {
+  int pot = HSSC_POT_SL_L_2D;  // HSSC_POT_*
+  double scale = 0;            // SL_L_2D: h^2 / 2 (times log r^2); SL_H_2D: h^2 k^2 / 4
+  double kh = 0;               // wavenumber
+  double wre = 0, wim = 0;     // SL_H_2D diagonal w = 1 + k^2 (i/4) corr0
+};
+
+// A(r,c) = K(R_r, C_c) with per-point weights (real T for SL_L_2D, complex for SL_H_2D).
 template <typename T>
 void gen_block(hipStream_t st, int64_t m, int64_t n, int64_t batch, const double* Rx,
                const double* Ry, const double* Rw, int64_t sR, const double* Cx, const double* Cy,
-               const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, double h);
+               const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, const KernelSpec& ks);
 
 // Parent F = [E1 K12; K12^T E2] assembled directly in the parent's J order.
 template <typename T>
 void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int32_t* J,
                 int64_t sJ, const double* Px, const double* Py, const double* Pw, int64_t sP,
-                const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF, double h);
+                const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF,
+                const KernelSpec& ks);
 
 template <typename T>
 void gemm(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
           const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C,
           int64_t ldc, int64_t sC, int64_t batch);
+
+// C = alpha A^H B + beta C (A^T for real data)
+template <typename T>
+void gemm_h(hipStream_t st, int64_t m, int64_t n, int64_t kdim, T alpha, const T* A, int64_t lda,
+            int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc, int64_t sC,
+            int64_t batch);
 
 template <typename T>
 void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
@@ -32,7 +51,7 @@ void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
                int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn);
 
-// C = A^T B with the long K dimension split in chunks of kchunk rows (deterministic sum);
+// C = A^H B with the long K dimension split in chunks of kchunk rows (deterministic sum);
 // part needs batch * ceil(kdim/kchunk) * m * n elements.
 template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,

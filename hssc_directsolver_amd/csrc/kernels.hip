@@ -2,13 +2,15 @@
 //
 // All matrices are column-major.  Batched kernels take a batch index in blockIdx.z (or
 // blockIdx.y) and element strides per batch; a stride of 0 shares one operand across the
-// batch (translation-invariant levels).  Kernels are templated on the working scalar
-// (double for the FP64 path, float for the FP32 factorization).
+// batch (translation-invariant levels).  Kernels are templated on the working scalar:
+// double / float for the real kernels (FP64 path, FP32 factorization) and interleaved
+// complex cplx<double> / cplx<float> for the Helmholtz kernel.
 
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
 #include <climits>
+#include <cstdint>
+#include <string>
 
 #include "kernels.h"
 #include "stats.h"
@@ -17,23 +19,42 @@ namespace hssc {
 namespace k {
 
 template <typename T>
-constexpr const char* tname(const char* f64, const char* f32) {
-  return sizeof(T) == 8 ? f64 : f32;
+constexpr const char* type_suffix() {
+  return is_complex_v<T> ? (sizeof(real_t<T>) == 8 ? "_c128" : "_c64")
+                         : (sizeof(T) == 8 ? "_f64" : "_f32");
+}
+// Instrumentation label of a kernel class, e.g. "gemm_f64", "gemm_c64".
+template <typename T>
+std::string lbl(const char* base) {
+  return std::string(base) + type_suffix<T>();
 }
 
 // ------------------------------------------------------------------------------------
-// Kernel-block generation (Common/Kernel_Eval.m:29-96, 2D Laplace single layer)
+// Kernel-block generation (Common/Kernel_Eval.m:29-96, 2D plane)
 // ------------------------------------------------------------------------------------
-// A(r,c) = (den==0) + h^2 * 0.5 * w_r * w_c * log(den),  den = |x_r - x_c|^2,
-// with the log term dropped when den == 0 (log(den + (den==0)) = 0 in the reference).
-
+// SL_L_2D: A(r,c) = (den==0) + h^2 * 0.5 * w_r * w_c * log(den),  den = |x_r - x_c|^2
+//          (the log term vanishes at den == 0: log(den + (den==0)) = 0 in the reference).
+// SL_H_2D: A(r,c) = w (den==0) + h^2 k^2 (i/4) w_r w_c H0(k sqrt(den)), w = 1 + k^2 (i/4)
+//          corr0 (:44-49; b and c are not applied on the diagonal).  With H0 = J0 + i Y0,
+//          (i c)(J0 + i Y0) = -c Y0 + i c J0.
 template <typename T>
-__device__ __forceinline__ T laplace_entry(double xr, double yr, double wr, double xc, double yc,
-                                           double wc, double hh2) {
+__device__ __forceinline__ T kernel_entry(const KernelSpec& ks, double xr, double yr, double wr,
+                                          double xc, double yc, double wc) {
   const double dx = xr - xc, dy = yr - yc;
   const double den = dx * dx + dy * dy;
-  if (den == 0.0) return T(1.0);
-  return T(hh2 * wr * wc * log(den));
+  if constexpr (is_complex_v<T>) {
+    using R = real_t<T>;
+    if (ks.pot == HSSC_POT_SL_H_2D) {
+      if (den == 0.0) return T(R(ks.wre), R(ks.wim));
+      const double z = ks.kh * sqrt(den), c = ks.scale * wr * wc;
+      return T(R(-c * y0(z)), R(c * j0(z)));
+    }
+    if (den == 0.0) return T(R(1));
+    return T(R(ks.scale * wr * wc * log(den)));
+  } else {
+    if (den == 0.0) return T(1.0);
+    return T(ks.scale * wr * wc * log(den));
+  }
 }
 
 // One thread per entry; threadIdx.x runs along rows so column-major stores coalesce.
@@ -42,7 +63,7 @@ __global__ void __launch_bounds__(256) gen_block_kernel(
     int64_t m, int64_t n, const double* __restrict__ Rx, const double* __restrict__ Ry,
     const double* __restrict__ Rw, int64_t sR, const double* __restrict__ Cx,
     const double* __restrict__ Cy, const double* __restrict__ Cw, int64_t sC, T* __restrict__ A,
-    int64_t lda, int64_t sA, double hh2) {
+    int64_t lda, int64_t sA, KernelSpec ks) {
   const int64_t e = blockIdx.z;
   const int64_t r = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
   const int64_t c0 = int64_t(blockIdx.y) * 16 + (threadIdx.x >> 6) * 4;
@@ -54,20 +75,20 @@ __global__ void __launch_bounds__(256) gen_block_kernel(
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int64_t c = c0 + j;
-    if (c < n) A[r + c * lda] = laplace_entry<T>(xr, yr, wr, Cx[c], Cy[c], Cw[c], hh2);
+    if (c < n) A[r + c * lda] = kernel_entry<T>(ks, xr, yr, wr, Cx[c], Cy[c], Cw[c]);
   }
 }
 
 template <typename T>
 void gen_block(hipStream_t st, int64_t m, int64_t n, int64_t batch, const double* Rx,
                const double* Ry, const double* Rw, int64_t sR, const double* Cx, const double* Cy,
-               const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, double h) {
+               const double* Cw, int64_t sC, T* A, int64_t lda, int64_t sA, const KernelSpec& ks) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   dim3 grid((m + 63) / 64, (n + 15) / 16, batch);
   const double ent = double(m) * n * batch;
-  HSSC_TIMED(tname<T>("gen_block_f64", "gen_block_f32"), st, ent, ent * sizeof(T),
+  HSSC_TIMED(lbl<T>("gen_block").c_str(), st, ent, ent * sizeof(T),
              hipLaunchKernelGGL(gen_block_kernel<T>, grid, dim3(256), 0, st, m, n, Rx, Ry, Rw, sR,
-                                Cx, Cy, Cw, sC, A, lda, sA, 0.5 * h * h));
+                                Cx, Cy, Cw, sC, A, lda, sA, ks));
 }
 
 // Parent block F = [E1 K12; K12^T E2] in the parent's J order
@@ -77,8 +98,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) gen_parent_kernel(
     int64_t n, int64_t kc, const int32_t* __restrict__ J, int64_t sJ,
     const double* __restrict__ Px, const double* __restrict__ Py, const double* __restrict__ Pw,
-    int64_t sP, const T* __restrict__ Ech, int64_t sEbox, int64_t sEkey_per_parent, T* __restrict__ F,
-    int64_t sF, double hh2) {
+    int64_t sP, const T* __restrict__ Ech, int64_t sEbox, int64_t sEkey_per_parent,
+    T* __restrict__ F, int64_t sF, KernelSpec ks) {
   const int64_t e = blockIdx.z;
   const int64_t a = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
   const int64_t c0 = int64_t(blockIdx.y) * 16 + (threadIdx.x >> 6) * 4;
@@ -102,7 +123,7 @@ __global__ void __launch_bounds__(256) gen_parent_kernel(
     } else if (i >= kc && j >= kc) {
       v = E2[(i - kc) + int64_t(j - kc) * kc];
     } else {
-      v = laplace_entry<T>(xa, ya, wa, Px[c], Py[c], Pw[c], hh2);
+      v = kernel_entry<T>(ks, xa, ya, wa, Px[c], Py[c], Pw[c]);
     }
     F[a + c * n] = v;
   }
@@ -111,12 +132,13 @@ __global__ void __launch_bounds__(256) gen_parent_kernel(
 template <typename T>
 void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int32_t* J,
                 int64_t sJ, const double* Px, const double* Py, const double* Pw, int64_t sP,
-                const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF, double h) {
+                const T* Ech, int64_t sEbox, int64_t sEparent, T* F, int64_t sF,
+                const KernelSpec& ks) {
   dim3 grid((n + 63) / 64, (n + 15) / 16, batch);
   const double ent = double(n) * n * batch;
-  HSSC_TIMED(tname<T>("gen_parent_f64", "gen_parent_f32"), st, ent, ent * sizeof(T),
+  HSSC_TIMED(lbl<T>("gen_parent").c_str(), st, ent, ent * sizeof(T),
              hipLaunchKernelGGL(gen_parent_kernel<T>, grid, dim3(256), 0, st, n, kc, J, sJ, Px, Py,
-                                Pw, sP, Ech, sEbox, sEparent, F, sF, 0.5 * h * h));
+                                Pw, sP, Ech, sEbox, sEparent, F, sF, ks));
 }
 
 // ------------------------------------------------------------------------------------
@@ -126,6 +148,8 @@ void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int3
 // Each wave owns a 32x32 quarter = 2x2 MFMA 16x16 tiles.  The MFMA is issued as
 // D = B^T A^T so that a lane's 4 accumulators run down one column of C with lane%16
 // indexing rows: column-major stores of 16 consecutive rows coalesce.
+// Complex operands are split into real and imaginary LDS planes and each k-step issues
+// four real MFMAs per tile (re*re - im*im, re*im + im*re).
 
 // Operand layout (both types): lane l supplies A(l%16, l/16) and B(l/16, l%16).
 // Result layout, measured on gfx950: item r of lane l holds D(row(l,r), l%16) with
@@ -153,16 +177,20 @@ constexpr int GB_M = 64, GB_N = 64, GB_K = 16;
 
 // Tile origins at or beyond rs0 (cs0) are shifted by rsn (csn): the blocked Gauss-Jordan
 // steps use this to run over the index set [0, p) U [p+nb, n) with 64-aligned p.
-template <typename T, bool TA, bool TB>
+// CA conjugates op(A) (complex only: C = A^H B for the Householder updates).
+template <typename T, bool TA, bool CA, bool TB>
 __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t kdim, T alpha,
                                                   const T* A, int64_t lda, int64_t sA,
                                                   const T* B, int64_t ldb, int64_t sB,
                                                   T beta, T* C, int64_t ldc,
                                                   int64_t sC, int64_t rs0, int64_t rsn, int64_t cs0,
                                                   int64_t csn, int64_t ksplit, int64_t kchunk) {
-  using v4 = typename Mfma<T>::v4;
-  __shared__ T As[GB_K][GB_M + 1];  // As[kk][i] = op(A)(i0+i, k0+kk)
-  __shared__ T Bs[GB_K][GB_N + 1];  // Bs[kk][j] = op(B)(k0+kk, j0+j)
+  using R = real_t<T>;
+  constexpr bool CX = is_complex_v<T>;
+  constexpr int NP = CX ? 2 : 1;  // LDS planes per operand
+  using v4 = typename Mfma<R>::v4;
+  __shared__ R As[NP][GB_K][GB_M + 1];  // As[p][kk][i] = plane p of op(A)(i0+i, k0+kk)
+  __shared__ R Bs[NP][GB_K][GB_N + 1];  // Bs[p][kk][j] = plane p of op(B)(k0+kk, j0+j)
   // blockIdx.z = e * ksplit + q: batch element e, K chunk q (rows [q*kchunk, ...) of op(B)
   // and columns of op(A)); chunk q writes its own output slice C + (e*ksplit + q)*sC.
   const int64_t z = blockIdx.z;
@@ -182,11 +210,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
   if (i0 >= m || j0 >= n) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
-  v4 acc[2][2];
+  v4 acc[NP][2][2];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] = v4{0, 0, 0, 0};
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[p][x][y] = v4{0, 0, 0, 0};
 
   for (int64_t k0 = 0; k0 < kdim; k0 += GB_K) {
     // Load op(A) tile (64 x 16): 1024 elements, 4 per thread.
@@ -202,7 +232,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
       const int64_t gi = i0 + ii, gk = k0 + kk;
       T v = T(0);
       if (gi < m && gk < kdim) v = TA ? A[gk + gi * lda] : A[gi + gk * lda];
-      As[kk][ii] = v;
+      As[0][kk][ii] = re_s(v);
+      if constexpr (CX) As[NP - 1][kk][ii] = CA ? -im_s(v) : im_s(v);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -216,21 +247,32 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
       const int64_t gj = j0 + jj, gk = k0 + kk;
       T v = T(0);
       if (gj < n && gk < kdim) v = TB ? B[gj + gk * ldb] : B[gk + gj * ldb];
-      Bs[kk][jj] = v;
+      Bs[0][kk][jj] = re_s(v);
+      if constexpr (CX) Bs[NP - 1][kk][jj] = im_s(v);
     }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < GB_K; kk += 4) {
       const int kq = kk + (lane >> 4);
-      T av[2], bv[2];
+      R av[NP][2], bv[NP][2];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) av[x] = As[kq][wi + x * 16 + (lane & 15)];
+      for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int y = 0; y < 2; ++y) bv[y] = Bs[kq][wj + y * 16 + (lane & 15)];
+        for (int x = 0; x < 2; ++x) av[p][x] = As[p][kq][wi + x * 16 + (lane & 15)];
+#pragma unroll
+        for (int y = 0; y < 2; ++y) bv[p][y] = Bs[p][kq][wj + y * 16 + (lane & 15)];
+      }
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mma(bv[y], av[x], acc[x][y]);
+        for (int y = 0; y < 2; ++y) {
+          acc[0][x][y] = Mfma<R>::mma(bv[0][y], av[0][x], acc[0][x][y]);
+          if constexpr (CX) {
+            acc[0][x][y] = Mfma<R>::mma(-bv[NP - 1][y], av[NP - 1][x], acc[0][x][y]);
+            acc[NP - 1][x][y] = Mfma<R>::mma(bv[NP - 1][y], av[0][x], acc[NP - 1][x][y]);
+            acc[NP - 1][x][y] = Mfma<R>::mma(bv[0][y], av[NP - 1][x], acc[NP - 1][x][y]);
+          }
+        }
     }
     __syncthreads();
   }
@@ -243,10 +285,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gi = i0 + wi + x * 16 + (lane & 15);
-        const int64_t gj = j0 + wj + y * 16 + Mfma<T>::row(lane, r);
+        const int64_t gj = j0 + wj + y * 16 + Mfma<R>::row(lane, r);
         if (gi < m && gj < n) {
           T* c = C + gi + gj * ldc;
-          const T v = alpha * acc[x][y][r];
+          T a;
+          if constexpr (CX) a = T(acc[0][x][y][r], acc[NP - 1][x][y][r]);
+          else a = acc[0][x][y][r];
+          const T v = alpha * a;
           *c = (beta == T(0)) ? v : v + beta * *c;
         }
       }
@@ -257,7 +302,7 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
                      const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
                      T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
                      int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
-                     int64_t ksplit, int64_t kchunk);
+                     int64_t ksplit, int64_t kchunk, bool ca = false);
 
 template <typename T>
 void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
@@ -273,22 +318,31 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
                      const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
                      T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
                      int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
-                     int64_t ksplit, int64_t kchunk) {
+                     int64_t ksplit, int64_t kchunk, bool ca) {
   if (m <= 0 || n <= 0 || batch <= 0 || rtiles <= 0 || ctiles <= 0) return;
   dim3 grid(rtiles, ctiles, batch * ksplit);
-  // algorithmic work: rows / cols actually produced (the skipped block excluded)
+  // algorithmic work: rows / cols actually produced (the skipped block excluded); a complex
+  // multiply-add is 8 real flops
   const double rows = double(rs0 < m ? m - rsn : m), cols = double(cs0 < n ? n - csn : n);
-  const double fl = 2.0 * rows * cols * double(kdim) * double(batch);
+  const double fl = (is_complex_v<T> ? 8.0 : 2.0) * rows * cols * double(kdim) * double(batch);
   const double by = (rows * kdim + kdim * cols + 2 * rows * cols) * sizeof(T) * double(batch);
-#define HSSC_GEMM_LAUNCH(TA_, TB_)                                                              \
-  HSSC_TIMED(tname<T>("gemm_f64", "gemm_f32"), st, fl, by,                                      \
-             hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(256), 0, st, m, n, kdim,  \
-                                alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, cs0, \
-                                csn, ksplit, kchunk))
-  if (!ta && !tb) HSSC_GEMM_LAUNCH(false, false);
-  else if (ta && !tb) HSSC_GEMM_LAUNCH(true, false);
-  else if (!ta && tb) HSSC_GEMM_LAUNCH(false, true);
-  else HSSC_GEMM_LAUNCH(true, true);
+  const bool conjA = is_complex_v<T> && ca;
+#define HSSC_GEMM_LAUNCH(TA_, CA_, TB_)                                                         \
+  HSSC_TIMED(lbl<T>("gemm").c_str(), st, fl, by,                                                \
+             hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_>), grid, dim3(256), 0, st, m, n,  \
+                                kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, \
+                                cs0, csn, ksplit, kchunk))
+  if (conjA) {
+    if (ta && !tb) HSSC_GEMM_LAUNCH(true, true, false);
+    else if (ta && tb) HSSC_GEMM_LAUNCH(true, true, true);
+    else if (!tb) HSSC_GEMM_LAUNCH(false, true, false);
+    else HSSC_GEMM_LAUNCH(false, true, true);
+  } else {
+    if (!ta && !tb) HSSC_GEMM_LAUNCH(false, false, false);
+    else if (ta && !tb) HSSC_GEMM_LAUNCH(true, false, false);
+    else if (!ta && tb) HSSC_GEMM_LAUNCH(false, false, true);
+    else HSSC_GEMM_LAUNCH(true, false, true);
+  }
 #undef HSSC_GEMM_LAUNCH
 }
 
@@ -301,70 +355,24 @@ void gemm(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, 
                (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0);
 }
 
+template <typename T>
+void gemm_h(hipStream_t st, int64_t m, int64_t n, int64_t kdim, T alpha, const T* A, int64_t lda,
+            int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc, int64_t sC,
+            int64_t batch) {
+  const int64_t big = INT64_MAX;
+  gemm_skip_split<T>(st, true, false, m, n, kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                     batch, (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0, 1, kdim,
+                     true);
+}
+
 // ------------------------------------------------------------------------------------
 // Gauss-Jordan inversion (MATLAB inv in HSS2D_invert_fsym.m:108, build_Finv_fsym.m:101,
 // build_E_fsym.m:51,64)
 // ------------------------------------------------------------------------------------
-// In-place GJ sweep with partial (row) pivoting.  After pivot c: A(c,c) = 1/p,
+// In-place GJ sweep with partial (row) pivoting: after pivot c, A(c,c) = 1/p,
 // A(c,j) /= p, A(i,j) -= A(i,c)_old * A(c,j), A(i,c) = -A(i,c)_old / p.  Row swaps are
-// undone at the end by swapping columns in reverse order.  The matrix lives in LDS.
-
-template <typename T>
-__device__ void gj_lds(T* S, int n, int ld, int* piv, T* f, int tid, int nthr) {
-  const int lane = tid & 63, wave = tid >> 6;
-  __shared__ int s_p;
-  for (int c = 0; c < n; ++c) {
-    if (wave == 0) {
-      // argmax |S(r,c)| over r >= c; ties -> lowest row (LAPACK idamax convention)
-      T best = T(-1);
-      int bi = c;
-      for (int r = c + lane; r < n; r += 64) {
-        const T v = fabs(S[r + c * ld]);
-        if (v > best) best = v, bi = r;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
-      }
-      if (lane == 0) s_p = bi, piv[c] = bi;
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != c) {
-      for (int j = tid; j < n; j += nthr) {
-        const T t = S[c + j * ld];
-        S[c + j * ld] = S[p + j * ld];
-        S[p + j * ld] = t;
-      }
-      __syncthreads();
-    }
-    for (int i = tid; i < n; i += nthr) f[i] = S[i + c * ld];
-    __syncthreads();
-    const T inv = T(1) / f[c];
-    for (int j = tid; j < n; j += nthr) S[c + j * ld] = (j == c) ? inv : S[c + j * ld] * inv;
-    __syncthreads();
-    for (int idx = tid; idx < n * n; idx += nthr) {
-      const int i = idx % n, j = idx / n;
-      if (i == c) continue;
-      if (j == c) S[i + j * ld] = -f[i] * inv;
-      else S[i + j * ld] -= f[i] * S[c + j * ld];
-    }
-    __syncthreads();
-  }
-  for (int c = n - 1; c >= 0; --c) {
-    const int p = piv[c];
-    if (p != c) {
-      for (int i = tid; i < n; i += nthr) {
-        const T t = S[i + c * ld];
-        S[i + c * ld] = S[i + p * ld];
-        S[i + p * ld] = t;
-      }
-      __syncthreads();
-    }
-  }
-}
-
+// undone at the end by permuting columns.  Pivot magnitude |re| + |im| (LAPACK izamax).
+//
 // Register-resident Gauss-Jordan for n <= 64*S (n <= 4*CB): 256 threads, lanes are rows
 // (row = lane + 64*s), wave w owns columns [w*CB, (w+1)*CB).  Per pivot column c:
 //   * the wave owning column c finds the pivot row by a shuffle argmax over rows >= c and
@@ -376,6 +384,7 @@ __device__ void gj_lds(T* S, int n, int ld, int* piv, T* f, int tid, int nthr) {
 template <typename T, int S, int CB>
 __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, int64_t lda,
                                                      int64_t sA, int64_t off) {
+  using R = real_t<T>;
   __shared__ T colbuf[2][64 * S];
   __shared__ int spiv[2];
   __shared__ int pivs[64 * S];
@@ -390,11 +399,10 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
       const int i = lane + 64 * s, j = wave * CB + t;
       v[s][t] = (i < n && j < n) ? A[i + int64_t(j) * lda] : T(0);
     }
-  // candidates for column 0
   auto publish = [&](int c) {
     // called by the wave owning column c
     const int tc = c - wave * CB;
-    T best = T(-1);
+    R best = R(-1);
     int bi = n;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -404,11 +412,11 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
         if (t == tc) x = v[s][t];
       const int i = lane + 64 * s;
       if (i < n) colbuf[c & 1][i] = x;
-      const T ax = fabs(x);
+      const R ax = abs1(x);
       if (i >= c && i < n && ax > best) best = ax, bi = i;
     }
     for (int o = 32; o > 0; o >>= 1) {
-      const T ob = __shfl_xor(best, o);
+      const R ob = shfl_xor_s(best, o);
       const int oi = __shfl_xor(bi, o);
       if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
     }
@@ -432,8 +440,8 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
         if (s == sc) xc = v[s][t];
         if (s == sp) xp = v[s][t];
       }
-      rc[t] = __shfl(xc, lc);
-      rp[t] = __shfl(xp, lp);
+      rc[t] = shfl_s(xc, lc);
+      rp[t] = shfl_s(xp, lp);
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -445,7 +453,7 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
         T old = (i == c) ? rp[t] : (i == p) ? rc[t] : v[s][t];
         T nv;
         if (i == c) nv = (j == c) ? inv : rp[t] * inv;
-        else nv = (j == c) ? -f * inv : old - f * (rp[t] * inv);
+        else nv = (j == c) ? -(f * inv) : old - f * (rp[t] * inv);
         v[s][t] = nv;
       }
     }
@@ -456,7 +464,7 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
   // applies the swaps (c, pivs[c]) for c = n-1 .. 0; each column is stored at
   // dest = sigma^-1(its index).
   if (threadIdx.x == 0) {
-    int* sig = dest;  // reuse: compute sigma in place, then invert into colbuf-free space
+    int* sig = dest;
     for (int j = 0; j < n; ++j) sig[j] = j;
     for (int c = n - 1; c >= 0; --c) {
       const int q = pivs[c];
@@ -464,7 +472,6 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
       sig[c] = sig[q];
       sig[q] = a;
     }
-    // invert sigma into pivs (no longer needed)
     for (int j = 0; j < n; ++j) pivs[sig[j]] = j;
   }
   __syncthreads();
@@ -479,111 +486,28 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
 
 template <typename T>
 void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, int64_t batch,
-            const char* label) {
-  const double fl = 2.0 * n * double(n) * n * batch;
+            const std::string& label) {
+  const double fl = (is_complex_v<T> ? 8.0 : 2.0) * n * double(n) * n * batch;
   if (n <= 64) {
-    HSSC_TIMED(label, st, fl, 2.0 * n * n * sizeof(T) * batch,
+    HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
                hipLaunchKernelGGL((gj_reg_kernel<T, 1, 16>), dim3(batch), dim3(256), 0, st, n, A,
                                   lda, sA, off));
-  } else {
-    HSSC_TIMED(label, st, fl, 2.0 * n * n * sizeof(T) * batch,
+  } else if constexpr (!is_complex_v<T>) {
+    HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
                hipLaunchKernelGGL((gj_reg_kernel<T, 2, 32>), dim3(batch), dim3(256), 0, st, n, A,
                                   lda, sA, off));
   }
 }
 
-// Small matrices: one workgroup per matrix, whole matrix in dynamic LDS.
-template <typename T>
-__global__ void __launch_bounds__(256) gj_small_kernel(int n, T* __restrict__ A, int64_t lda,
-                                                       int64_t sA) {
-  extern __shared__ unsigned char smem_raw[];
-  T* S = reinterpret_cast<T*>(smem_raw);
-  const int ld = n + 1;
-  T* f = S + int64_t(ld) * n;
-  int* piv = reinterpret_cast<int*>(f + n);
-  A += int64_t(blockIdx.x) * sA;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < n * n; idx += 256) {
-    const int i = idx % n, j = idx / n;
-    S[i + j * ld] = A[i + j * lda];
-  }
-  __syncthreads();
-  gj_lds<T>(S, n, ld, piv, f, tid, 256);
-  __syncthreads();
-  for (int idx = tid; idx < n * n; idx += 256) {
-    const int i = idx % n, j = idx / n;
-    A[i + j * lda] = S[i + j * ld];
-  }
-}
-
-template <typename T>
-size_t gj_small_lds(int n) {
-  return sizeof(T) * (size_t(n + 1) * n + n) + sizeof(int) * n;
-}
-
-// Largest dynamic LDS a workgroup may request on the current device (160 KiB on gfx950).
-static int max_lds_per_block() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      v = 65536;
-  }
-  return v;
-}
-
-template <typename T>
-bool gj_small(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t batch) {
-  if (batch <= 0 || n <= 0) return true;
-  const size_t lds = gj_small_lds<T>(n);
-  if (lds > size_t(max_lds_per_block())) return false;
-  if (lds > 65536) {
-    static size_t granted = 65536;
-    if (lds > granted) {
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&gj_small_kernel<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess)
-        return false;
-      granted = lds;
-    }
-  }
-  const double fl = 2.0 * n * double(n) * n * batch;
-  HSSC_TIMED(tname<T>("gj_small_f64", "gj_small_f32"), st, fl, 2.0 * n * n * sizeof(T) * batch,
-             hipLaunchKernelGGL(gj_small_kernel<T>, dim3(batch), dim3(256), lds, st, n, A, lda,
-                                sA));
-  return true;
-}
-
 // Blocked GJ without pivoting across diagonal blocks (pivoting inside each nb x nb block).
 // Step p with block rows/cols B = [p, p+nb):
-//   (1) A_BB <- inv(A_BB)                                   gj_diag_kernel
-//   (2) A_XB <- -A_XB * A_BB          (rows X outside B)    gj_col_kernel
-//   (3) A_XY <- A_XY + A_XB * A_BY    (X, Y outside B)      gj_rank_kernel
-//   (4) A_BY <- A_BB * A_BY           (cols Y outside B)    gj_row_kernel
+//   (1) A_BB <- inv(A_BB)                                   gj_reg on the block
+//   (2) A_XB <- -A_XB * A_BB          (rows X outside B)    gemm_skip
+//   (3) A_XY <- A_XY + A_XB * A_BY    (X, Y outside B)      gemm_skip
+//   (4) A_BY <- A_BB * A_BY           (cols Y outside B)    gemm_skip
 // which is the block sweep A11' = A11^-1, A12' = A11^-1 A12, A21' = -A21 A11^-1,
 // A22' = A22 - A21 A11^-1 A12.
 constexpr int GJ_NB = 64;
-
-template <typename T>
-__global__ void __launch_bounds__(256) gj_diag_kernel(int64_t p, int nb, T* __restrict__ A,
-                                                      int64_t lda, int64_t sA) {
-  __shared__ T S[GJ_NB * (GJ_NB + 1)];
-  __shared__ T f[GJ_NB];
-  __shared__ int piv[GJ_NB];
-  A += int64_t(blockIdx.x) * sA + p + p * lda;
-  const int tid = threadIdx.x, ld = GJ_NB + 1;
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int i = idx % nb, j = idx / nb;
-    S[i + j * ld] = A[i + j * lda];
-  }
-  __syncthreads();
-  gj_lds<T>(S, nb, ld, piv, f, tid, 256);
-  __syncthreads();
-  for (int idx = tid; idx < nb * nb; idx += 256) {
-    const int i = idx % nb, j = idx / nb;
-    A[i + j * lda] = S[i + j * ld];
-  }
-}
 
 template <typename T>
 void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
@@ -592,10 +516,11 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
   // (4) write in place: each output tile is produced by one workgroup that reads all of
   // its inputs (K = nb <= 64 fits one tile) before it stores.
   static_assert(GJ_NB == GB_M && GJ_NB == GB_N, "block size must equal the GEMM tile");
+  const std::string diag = lbl<T>("gj_diag");
   for (int64_t p = 0; p < n; p += GJ_NB) {
     const int nb = int(p + GJ_NB <= n ? GJ_NB : n - p);
     const int64_t nout = n - nb;
-    gj_reg<T>(st, nb, A, lda, sA, p + p * lda, batch, tname<T>("gj_diag_f64", "gj_diag_f32"));
+    gj_reg<T>(st, nb, A, lda, sA, p + p * lda, batch, diag);
     if (nout <= 0) continue;
     const int64_t tiles = (nout + 63) / 64;
     T* Apb = A + p;             // rows B
@@ -616,8 +541,9 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
   if (n <= 0 || batch <= 0) return;
-  if (n <= 128) {
-    gj_reg<T>(st, int(n), A, lda, sA, 0, batch, tname<T>("gj_small_f64", "gj_small_f32"));
+  // register-resident GJ: n <= 128 real, n <= 64 complex (twice the registers per entry)
+  if (n <= (is_complex_v<T> ? 64 : 128)) {
+    gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
     return;
   }
   gj_blocked<T>(st, n, A, lda, sA, batch);
@@ -627,32 +553,30 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 // Householder QR least squares (MATLAB K11\K21 in HSS2D/interpolation_operator.m:66)
 // ------------------------------------------------------------------------------------
 // Panel kernel: one workgroup factors columns [j0, j0+jb) of A (rows j0..M-1), streaming
-// the panel through L2.  Reflector i: H_i = I - tau_i v_i v_i^T with v_i(0) = 1 stored
-// below the diagonal, beta on the diagonal (LAPACK dgeqr2 convention, beta = -sign(a) *
-// norm).  It then forms the compact-WY triangular factor Tf (jb x jb, upper) with
-// H_0 ... H_{jb-1} = I - V Tf V^T (dlarft, forward, columnwise).
-
-template <typename T>
-__device__ T block_sum(T v, T* red, int tid) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  __syncthreads();
-  if ((tid & 63) == 0) red[tid >> 6] = v;
-  __syncthreads();
-  T s = T(0);
-  const int nw = blockDim.x >> 6;
-  for (int w = 0; w < nw; ++w) s += red[w];
-  return s;
-}
+// the panel through L2.  Reflector i: H_i = I - tau_i v_i v_i^H with v_i(0) = 1 stored
+// below the diagonal and the real beta on the diagonal (LAPACK xgeqr2 / xlarfg
+// conventions: beta = -sign(Re a) * norm, tau = (beta - a) / beta; the columns are updated
+// by H_i^H).  It then forms the compact-WY triangular factor Tf (jb x jb, upper) with
+// H_0 ... H_{jb-1} = I - V Tf V^H (xlarft, forward, columnwise).  For real data conj is
+// the identity and this is dgeqr2 / dlarft.
 
 constexpr int QR_NB = 32;  // outer panel width (the trailing update runs as GEMMs)
-constexpr int QR_IB = 4;   // inner block, held in registers
 constexpr int QR_RS = 16;  // row slots per thread: M - j0 <= QR_NT * QR_RS
 constexpr int QR_NT = 512; // threads per panel workgroup (8 waves)
-constexpr int QR_AC = 4;   // reflectors accumulated per reduction pass
+// inner block held in registers and reflectors accumulated per reduction pass: half as
+// many for complex data (twice the registers per entry)
+template <typename T>
+constexpr int qr_ib() {
+  return is_complex_v<T> ? 2 : 4;
+}
+template <typename T>
+constexpr int qr_ac() {
+  return is_complex_v<T> ? 2 : 4;
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  for (int o = 32; o > 0; o >>= 1) v += shfl_xor_s(v, o);
   return v;
 }
 
@@ -676,26 +600,28 @@ __device__ __forceinline__ void qr_block_reduce(T (&vals)[NV], T (*red)[NV], T* 
   __syncthreads();
 }
 
-// Left-looking panel: for each inner block of QR_IB columns, (1) apply the panel's earlier
-// reflectors through the compact-WY factor, (2) factor the block in registers with one
-// fused reduction per column (norm, dot products and the diagonal row broadcast through
-// double-buffered LDS, one barrier), (3) extend the Gram matrix V^T V and the triangular
-// factor Tf by the block's columns.  Row r = j0 + tid + QR_NT*s of the panel lives in slot
-// s of thread tid.
+// Left-looking panel: for each inner block of IB columns, (1) apply the panel's earlier
+// reflectors through the compact-WY factor (x -= V Tf^H V^H x), (2) factor the block in
+// registers with one fused reduction per column (norm, dot products and the diagonal row
+// broadcast through double-buffered LDS, one barrier), (3) extend the Gram matrix V^H V and
+// the triangular factor Tf by the block's columns.  Row r = j0 + tid + QR_NT*s of the
+// panel lives in slot s of thread tid.
 template <typename T>
 __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, int jb,
                                                         T* __restrict__ A, int64_t lda, int64_t sA,
                                                         T* __restrict__ Tfo, int64_t sT) {
+  using R = real_t<T>;
+  constexpr int IB = qr_ib<T>(), AC = qr_ac<T>();
   __shared__ T Tf[QR_NB][QR_NB + 1];
   __shared__ T G[QR_NB][QR_NB + 1];
-  __shared__ T W[QR_NB][QR_IB];
-  __shared__ T Wp[QR_NB][QR_IB];
-  __shared__ T red[2][QR_NT / 64][QR_IB];
-  __shared__ T rowv[2][QR_IB];
+  __shared__ T W[QR_NB][IB];
+  __shared__ T Wp[QR_NB][IB];
+  __shared__ T red[2][QR_NT / 64][IB];
+  __shared__ T rowv[2][IB];
   __shared__ T tau_s[QR_NB];
   __shared__ T tmp[QR_NB];
-  __shared__ T redW[QR_NT / 64][QR_AC * QR_IB];
-  __shared__ T GW[QR_AC][QR_IB];
+  __shared__ T redW[QR_NT / 64][AC * IB];
+  __shared__ T GW[AC][IB];
   A += int64_t(blockIdx.x) * sA;
   Tfo += int64_t(blockIdx.x) * sT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -707,45 +633,45 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
   // local row index l = r - j0 (int); the panel block A(j0 + l, j0 + c) is P[l + c*lda]
   T* P = A + j0 + j0 * lda;
   const int Ml = int(M - j0);
-  T x[QR_RS][QR_IB];
+  T x[QR_RS][IB];
   int buf = 0;
-  for (int c0 = 0; c0 < jb; c0 += QR_IB) {
-    const int ib = (jb - c0 < QR_IB) ? jb - c0 : QR_IB;
+  for (int c0 = 0; c0 < jb; c0 += IB) {
+    const int ib = (jb - c0 < IB) ? jb - c0 : IB;
     T* Pb = P + int64_t(c0) * lda;  // first column of the block
     // load the raw block
 #pragma unroll
     for (int s = 0; s < QR_RS; ++s) {
       const int l = tid + QR_NT * s;
 #pragma unroll
-      for (int jj = 0; jj < QR_IB; ++jj)
+      for (int jj = 0; jj < IB; ++jj)
         x[s][jj] = (l < Ml && jj < ib) ? Pb[l + int64_t(jj) * lda] : T(0);
     }
-    // (1) W = V_prev^T x (QR_AC reflectors per pass), W' = Tf^T W, x -= V_prev W'
+    // (1) W = V_prev^H x (AC reflectors per pass), W' = Tf^H W, x -= V_prev W'
     if (c0 > 0) {
-      for (int a0 = 0; a0 < c0; a0 += QR_AC) {
-        T acc[QR_AC * QR_IB];
+      for (int a0 = 0; a0 < c0; a0 += AC) {
+        T acc[AC * IB];
 #pragma unroll
-        for (int v = 0; v < QR_AC * QR_IB; ++v) acc[v] = T(0);
+        for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
 #pragma unroll
         for (int s = 0; s < QR_RS; ++s) {
           const int l = tid + QR_NT * s;
           if (l >= Ml || l < a0) continue;
 #pragma unroll
-          for (int u = 0; u < QR_AC; ++u) {
+          for (int u = 0; u < AC; ++u) {
             const int a = a0 + u;
             if (a < c0 && l >= a) {
-              const T v = (l == a) ? T(1) : P[l + int64_t(a) * lda];
+              const T v = conj_s((l == a) ? T(1) : P[l + int64_t(a) * lda]);
 #pragma unroll
-              for (int jj = 0; jj < QR_IB; ++jj) acc[u * QR_IB + jj] += v * x[s][jj];
+              for (int jj = 0; jj < IB; ++jj) acc[u * IB + jj] += v * x[s][jj];
             }
           }
         }
-        qr_block_reduce<T, QR_AC * QR_IB>(acc, redW, &W[a0][0], tid);
+        qr_block_reduce<T, AC * IB>(acc, redW, &W[a0][0], tid);
       }
-      if (tid < c0 * QR_IB) {
-        const int a = tid / QR_IB, jj = tid % QR_IB;
+      if (tid < c0 * IB) {
+        const int a = tid / IB, jj = tid % IB;
         T sum = T(0);
-        for (int b = 0; b <= a; ++b) sum += Tf[b][a] * W[b][jj];
+        for (int b = 0; b <= a; ++b) sum += conj_s(Tf[b][a]) * W[b][jj];
         Wp[a][jj] = sum;
       }
       __syncthreads();
@@ -757,36 +683,37 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
         for (int a = 0; a < amax; ++a) {
           const T v = (l == a) ? T(1) : P[l + int64_t(a) * lda];
 #pragma unroll
-          for (int jj = 0; jj < QR_IB; ++jj) x[s][jj] -= v * Wp[a][jj];
+          for (int jj = 0; jj < IB; ++jj) x[s][jj] -= v * Wp[a][jj];
         }
       }
     }
     // (2) factor the block
 #pragma unroll
-    for (int c = 0; c < QR_IB; ++c) {
+    for (int c = 0; c < IB; ++c) {
       if (c < ib) {
         const int ld = c0 + c;  // local row of the diagonal
         const int own = ld % QR_NT, so = ld / QR_NT;
-        T sig = T(0), d[QR_IB];
+        R sig = R(0);
+        T d[IB];
 #pragma unroll
-        for (int jj = 0; jj < QR_IB; ++jj) d[jj] = T(0);
+        for (int jj = 0; jj < IB; ++jj) d[jj] = T(0);
 #pragma unroll
         for (int s = 0; s < QR_RS; ++s) {
           const int l = tid + QR_NT * s;
           if (l >= ld && l < Ml) {
             const T xc = x[s][c];
-            if (l > ld) sig += xc * xc;
+            if (l > ld) sig += abs2(xc);
 #pragma unroll
-            for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] += xc * x[s][jj];
+            for (int jj = c + 1; jj < IB; ++jj) d[jj] += conj_s(xc) * x[s][jj];
           }
         }
         sig = wave_sum(sig);
 #pragma unroll
-        for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] = wave_sum(d[jj]);
+        for (int jj = c + 1; jj < IB; ++jj) d[jj] = wave_sum(d[jj]);
         if (lane == 0) {
-          red[buf][wave][0] = sig;
+          red[buf][wave][0] = T(sig);
 #pragma unroll
-          for (int jj = c + 1; jj < QR_IB; ++jj) red[buf][wave][jj] = d[jj];
+          for (int jj = c + 1; jj < IB; ++jj) red[buf][wave][jj] = d[jj];
         }
         if (tid == own) {
 #pragma unroll
@@ -794,36 +721,38 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
             if (s == so) {
               rowv[buf][0] = x[s][c];
 #pragma unroll
-              for (int jj = c + 1; jj < QR_IB; ++jj) rowv[buf][jj] = x[s][jj];
+              for (int jj = c + 1; jj < IB; ++jj) rowv[buf][jj] = x[s][jj];
             }
         }
         __syncthreads();
-        sig = T(0);
+        sig = R(0);
 #pragma unroll
-        for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] = T(0);
+        for (int jj = c + 1; jj < IB; ++jj) d[jj] = T(0);
         for (int w = 0; w < QR_NT / 64; ++w) {
-          sig += red[buf][w][0];
+          sig += re_s(red[buf][w][0]);
 #pragma unroll
-          for (int jj = c + 1; jj < QR_IB; ++jj) d[jj] += red[buf][w][jj];
+          for (int jj = c + 1; jj < IB; ++jj) d[jj] += red[buf][w][jj];
         }
         const T alpha = rowv[buf][0];
-        T beta, tau, scal;
-        if (sig == T(0)) {
-          tau = T(0), beta = alpha, scal = T(0);
+        T tau, scal;
+        R beta;
+        if (sig == R(0) && im_s(alpha) == R(0)) {
+          tau = T(0), beta = re_s(alpha), scal = T(0);
         } else {
-          const T nrm = sqrt(alpha * alpha + sig);
-          beta = alpha >= T(0) ? -nrm : nrm;
-          tau = (beta - alpha) / beta;
-          scal = T(1) / (alpha - beta);
+          const R nrm = sqrt(abs2(alpha) + sig);
+          beta = re_s(alpha) >= R(0) ? -nrm : nrm;
+          tau = (T(beta) - alpha) * T(R(1) / beta);
+          scal = T(1) / (alpha - T(beta));
         }
-        // w_j = v^T a_j = a_j(ld) + (sum_{l>=ld} x_l a_j(l) - alpha a_j(ld)) * scal
-        T tw[QR_IB];
+        // w_j = v^H a_j = a_j(ld) + conj(scal) (sum_{l>=ld} conj(x_l) a_j(l) - conj(alpha)
+        // a_j(ld)); column j is updated by H^H: a_j -= v conj(tau) w_j
+        T tw[IB];
 #pragma unroll
-        for (int jj = 0; jj < QR_IB; ++jj) tw[jj] = T(0);
+        for (int jj = 0; jj < IB; ++jj) tw[jj] = T(0);
 #pragma unroll
-        for (int jj = c + 1; jj < QR_IB; ++jj) {
+        for (int jj = c + 1; jj < IB; ++jj) {
           const T arow = rowv[buf][jj];
-          tw[jj] = tau * (arow + (d[jj] - alpha * arow) * scal);
+          tw[jj] = conj_s(tau) * (arow + conj_s(scal) * (d[jj] - conj_s(alpha) * arow));
         }
 #pragma unroll
         for (int s = 0; s < QR_RS; ++s) {
@@ -831,8 +760,8 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
           if (l >= ld && l < Ml) {
             const T v = (l == ld) ? T(1) : x[s][c] * scal;
 #pragma unroll
-            for (int jj = c + 1; jj < QR_IB; ++jj) x[s][jj] -= v * tw[jj];
-            x[s][c] = (l == ld) ? beta : v;
+            for (int jj = c + 1; jj < IB; ++jj) x[s][jj] -= v * tw[jj];
+            x[s][c] = (l == ld) ? T(beta) : v;
           }
         }
         if (tid == 0) tau_s[c0 + c] = tau;
@@ -845,28 +774,28 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
       const int l = tid + QR_NT * s;
       if (l < Ml) {
 #pragma unroll
-        for (int jj = 0; jj < QR_IB; ++jj)
+        for (int jj = 0; jj < IB; ++jj)
           if (jj < ib) Pb[l + int64_t(jj) * lda] = x[s][jj];
       }
     }
     __syncthreads();
-    // (3) Gram columns G(a, c0+c) = v_a^T v_{c0+c} for a < c0+c, accumulated from the
+    // (3) Gram columns G(a, c0+c) = v_a^H v_{c0+c} for a < c0+c, accumulated from the
     // block's registers (v_c: 0 above its diagonal, 1 on it, x below) against the earlier
-    // reflectors (QR_AC per pass) and the block itself.
-    for (int a0 = 0; a0 < c0 + QR_IB; a0 += QR_AC) {
-      T acc[QR_AC * QR_IB];
+    // reflectors (AC per pass) and the block itself.
+    for (int a0 = 0; a0 < c0 + IB; a0 += AC) {
+      T acc[AC * IB];
 #pragma unroll
-      for (int v = 0; v < QR_AC * QR_IB; ++v) acc[v] = T(0);
+      for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
 #pragma unroll
       for (int s = 0; s < QR_RS; ++s) {
         const int l = tid + QR_NT * s;
         if (l >= Ml || l < c0) continue;
-        T vb[QR_IB];
+        T vb[IB];
 #pragma unroll
-        for (int c = 0; c < QR_IB; ++c)
+        for (int c = 0; c < IB; ++c)
           vb[c] = (c >= ib || l < c0 + c) ? T(0) : (l == c0 + c ? T(1) : x[s][c]);
 #pragma unroll
-        for (int u = 0; u < QR_AC; ++u) {
+        for (int u = 0; u < AC; ++u) {
           const int a = a0 + u;
           T va;
           if (a < c0) {
@@ -874,17 +803,18 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
           } else {
             T t = T(0);
 #pragma unroll
-            for (int c = 0; c < QR_IB; ++c)
+            for (int c = 0; c < IB; ++c)
               if (a - c0 == c) t = vb[c];
             va = t;
           }
+          va = conj_s(va);
 #pragma unroll
-          for (int c = 0; c < QR_IB; ++c) acc[u * QR_IB + c] += va * vb[c];
+          for (int c = 0; c < IB; ++c) acc[u * IB + c] += va * vb[c];
         }
       }
-      qr_block_reduce<T, QR_AC * QR_IB>(acc, redW, &GW[0][0], tid);
-      if (tid < QR_AC * QR_IB) {
-        const int u = tid / QR_IB, c = tid % QR_IB;
+      qr_block_reduce<T, AC * IB>(acc, redW, &GW[0][0], tid);
+      if (tid < AC * IB) {
+        const int u = tid / IB, c = tid % IB;
         const int a = a0 + u, ci = c0 + c;
         if (c < ib && a < ci) G[a][ci] = GW[u][c];
       }
@@ -899,7 +829,7 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
         tmp[tid] = sum;
       }
       __syncthreads();
-      if (tid < i) Tf[tid][i] = -tau_s[i] * tmp[tid];
+      if (tid < i) Tf[tid][i] = -(tau_s[i] * tmp[tid]);
       if (tid == i) Tf[i][i] = tau_s[i];
       __syncthreads();
     }
@@ -969,7 +899,8 @@ template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
               T* Tf, int64_t sT, int64_t batch) {
   const double rows = double(M - j0);
-  HSSC_TIMED(tname<T>("qr_panel_f64", "qr_panel_f32"), st, 4.0 * rows * jb * jb * batch,
+  HSSC_TIMED(lbl<T>("qr_panel").c_str(), st,
+             (is_complex_v<T> ? 16.0 : 4.0) * rows * jb * jb * batch,
              2.0 * rows * jb * sizeof(T) * batch,
              hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(QR_NT), 0, st, M, j0, jb, A,
                                 lda, sA, Tf, sT));
@@ -1040,7 +971,7 @@ __global__ void __launch_bounds__(256) gemv_t_partial(int64_t m, int64_t n, cons
   if (j >= n) return;
   T acc = T(0);
   for (int64_t r = r0 + lane; r < r1; r += 64) acc += A[r + j * lda] * x[r];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  for (int off = 32; off > 0; off >>= 1) acc += shfl_xor_s(acc, off);
   if (lane == 0) part[(e * nsplit + blockIdx.y) * n + j] = acc;
 }
 
@@ -1074,16 +1005,16 @@ void gemv(hipStream_t st, bool trans, int64_t m, int64_t n, T alpha, const T* A,
     return;
   }
   const int64_t nsplit = (inner + GV_CHUNK - 1) / GV_CHUNK;
-  const double fl = 2.0 * m * double(n) * batch;
+  const double fl = (is_complex_v<T> ? 8.0 : 2.0) * m * double(n) * batch;
   const double by = (double(m) * n + m + n) * sizeof(T) * batch;
   if (!trans) {
     dim3 grid((m + GV_ROWS - 1) / GV_ROWS, nsplit, batch);
-    HSSC_TIMED(tname<T>("gemv_f64", "gemv_f32"), st, fl, by,
+    HSSC_TIMED(lbl<T>("gemv").c_str(), st, fl, by,
                hipLaunchKernelGGL(gemv_n_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x,
                                   sx, work, nsplit));
   } else {
     dim3 grid((n + 3) / 4, nsplit, batch);
-    HSSC_TIMED(tname<T>("gemvt_f64", "gemvt_f32"), st, fl, by,
+    HSSC_TIMED(lbl<T>("gemvt").c_str(), st, fl, by,
                hipLaunchKernelGGL(gemv_t_partial<T>, grid, dim3(256), 0, st, m, n, A, lda, sA, x,
                                   sx, work, nsplit));
   }
@@ -1216,22 +1147,23 @@ __global__ void __launch_bounds__(256) sum_split_kernel(int64_t len, int64_t nsp
   out[e * sout + i] = s;
 }
 
-// C (m x n) = A^T B for tall A (K x m) and B (K x n) with a long K: the K range is cut in
+// C (m x n) = A^H B for tall A (K x m) and B (K x n) with a long K: the K range is cut in
 // nsplit chunks computed as independent GEMMs into `part`, then summed in a fixed order.
+// (A^H = A^T for real data.)
 template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                     int64_t batch, T* part, int64_t kchunk) {
   const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
   if (nsplit <= 1 || ldc != m) {
-    gemm<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
+    gemm_h<T>(st, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
     return;
   }
   const int64_t len = m * n;
   const int64_t big = INT64_MAX;
   gemm_skip_split<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), part, m, len,
                      batch, (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0, nsplit,
-                     kchunk);
+                     kchunk, true);
   hipLaunchKernelGGL(sum_split_kernel<T>, dim3((len + 255) / 256, batch), dim3(256), 0, st, len,
                      nsplit, part, C, sC);
 }
@@ -1243,8 +1175,7 @@ __global__ void __launch_bounds__(256) sumsq_partial(int64_t len, const T* __res
   __shared__ double red[4];
   double s = 0;
   for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += int64_t(gridDim.x) * 256) {
-    const double v = double(x[i]);
-    s += v * v;
+    s += double(abs2(x[i]));
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -1263,12 +1194,14 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
 #define HSSC_INST(T)                                                                            \
   template void gen_block<T>(hipStream_t, int64_t, int64_t, int64_t, const double*, const double*, \
                              const double*, int64_t, const double*, const double*, const double*,  \
-                             int64_t, T*, int64_t, int64_t, double);                               \
+                             int64_t, T*, int64_t, int64_t, const KernelSpec&);                    \
   template void gen_parent<T>(hipStream_t, int64_t, int64_t, int64_t, const int32_t*, int64_t,     \
                               const double*, const double*, const double*, int64_t, const T*,      \
-                              int64_t, int64_t, T*, int64_t, double);                              \
+                              int64_t, int64_t, T*, int64_t, const KernelSpec&);                   \
   template void gemm<T>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, T, const T*, int64_t,  \
                         int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, int64_t);    \
+  template void gemm_h<T>(hipStream_t, int64_t, int64_t, int64_t, T, const T*, int64_t, int64_t,   \
+                          const T*, int64_t, int64_t, T, T*, int64_t, int64_t, int64_t);           \
   template void gemm_skip<T>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, T, const T*,    \
                              int64_t, int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, \
                              int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t);        \
@@ -1284,12 +1217,14 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void sumsq<T>(hipStream_t, int64_t, const T*, double*, int);                           \
   template void gemm_tn_splitk<T>(hipStream_t, int64_t, int64_t, int64_t, const T*, int64_t,       \
                                   int64_t, const T*, int64_t, int64_t, T*, int64_t, int64_t,       \
-                                  int64_t, T*, int64_t);                           \
+                                  int64_t, T*, int64_t);                                           \
   template void add2d<T>(hipStream_t, int64_t, int64_t, T, const T*, int64_t, int64_t, T, const T*, \
                          int64_t, int64_t, T*, int64_t, int64_t, int64_t);
 
 HSSC_INST(double)
 HSSC_INST(float)
+HSSC_INST(zdouble)
+HSSC_INST(zfloat)
 
 #define HSSC_INST2(TS, TD)                                                                        \
   template void gather<TS, TD>(hipStream_t, int64_t, int64_t, const int32_t*, int64_t, const TS*,  \
@@ -1304,6 +1239,10 @@ HSSC_INST2(double, double)
 HSSC_INST2(float, float)
 HSSC_INST2(double, float)
 HSSC_INST2(float, double)
+HSSC_INST2(zdouble, zdouble)
+HSSC_INST2(zfloat, zfloat)
+HSSC_INST2(zdouble, zfloat)
+HSSC_INST2(zfloat, zdouble)
 
 }  // namespace k
 }  // namespace hssc

@@ -1,0 +1,138 @@
+// Scalar types of the solver: double and float (real kernels) and interleaved complex
+// cplx<double> / cplx<float> (Helmholtz).  Complex values are stored as (re, im) pairs,
+// the layout of std::complex, MATLAB's interleaved complex API and numpy complex arrays.
+// Complex symmetric operators use plain transposes; conj() appears only inside the
+// Householder QR (Q^H).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+
+namespace hssc {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R re, im;
+  cplx() = default;  // trivial, so cplx arrays may live in LDS
+  __host__ __device__ constexpr cplx(R r) : re(r), im(0) {}  // NOLINT: real -> complex
+  __host__ __device__ constexpr cplx(R r, R i) : re(r), im(i) {}
+  template <typename Q, typename = typename std::enable_if<!std::is_same<Q, R>::value>::type>
+  __host__ __device__ explicit constexpr cplx(const cplx<Q>& o) : re(R(o.re)), im(R(o.im)) {}
+  __host__ __device__ cplx& operator+=(const cplx& o) {
+    re += o.re, im += o.im;
+    return *this;
+  }
+  __host__ __device__ cplx& operator-=(const cplx& o) {
+    re -= o.re, im -= o.im;
+    return *this;
+  }
+  __host__ __device__ cplx& operator*=(const cplx& o) {
+    const R r = re * o.re - im * o.im;
+    im = re * o.im + im * o.re;
+    re = r;
+    return *this;
+  }
+};
+
+template <typename R>
+__host__ __device__ inline cplx<R> operator+(cplx<R> a, const cplx<R>& b) {
+  return a += b;
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator-(cplx<R> a, const cplx<R>& b) {
+  return a -= b;
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator-(const cplx<R>& a) {
+  return cplx<R>(-a.re, -a.im);
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, const cplx<R>& b) {
+  return a *= b;
+}
+// Smith's algorithm (no overflow for |b| near the range limits).
+template <typename R>
+__host__ __device__ inline cplx<R> operator/(const cplx<R>& a, const cplx<R>& b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const R r = b.im / b.re, d = b.re + b.im * r;
+    return cplx<R>((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+  }
+  const R r = b.re / b.im, d = b.im + b.re * r;
+  return cplx<R>((a.re * r + a.im) / d, (a.im * r - a.re) / d);
+}
+template <typename R>
+__host__ __device__ inline bool operator==(const cplx<R>& a, const cplx<R>& b) {
+  return a.re == b.re && a.im == b.im;
+}
+template <typename R>
+__host__ __device__ inline bool operator!=(const cplx<R>& a, const cplx<R>& b) {
+  return !(a == b);
+}
+
+using zdouble = cplx<double>;
+using zfloat = cplx<float>;
+
+template <typename T>
+struct scalar_traits {
+  using real = T;
+  static constexpr bool is_complex = false;
+};
+template <typename R>
+struct scalar_traits<cplx<R>> {
+  using real = R;
+  static constexpr bool is_complex = true;
+};
+template <typename T>
+using real_t = typename scalar_traits<T>::real;
+template <typename T>
+constexpr bool is_complex_v = scalar_traits<T>::is_complex;
+
+// conj, |re| + |im| (LAPACK's cabs1, the pivot magnitude of izamax), |x|^2, Re x
+__host__ __device__ inline double conj_s(double x) { return x; }
+__host__ __device__ inline float conj_s(float x) { return x; }
+template <typename R>
+__host__ __device__ inline cplx<R> conj_s(const cplx<R>& x) {
+  return cplx<R>(x.re, -x.im);
+}
+__host__ __device__ inline double abs1(double x) { return fabs(x); }
+__host__ __device__ inline float abs1(float x) { return fabsf(x); }
+template <typename R>
+__host__ __device__ inline R abs1(const cplx<R>& x) {
+  return fabs(x.re) + fabs(x.im);
+}
+__host__ __device__ inline double abs2(double x) { return x * x; }
+__host__ __device__ inline float abs2(float x) { return x * x; }
+template <typename R>
+__host__ __device__ inline R abs2(const cplx<R>& x) {
+  return x.re * x.re + x.im * x.im;
+}
+__host__ __device__ inline double re_s(double x) { return x; }
+__host__ __device__ inline float re_s(float x) { return x; }
+template <typename R>
+__host__ __device__ inline R re_s(const cplx<R>& x) {
+  return x.re;
+}
+__host__ __device__ inline double im_s(double) { return 0.0; }
+__host__ __device__ inline float im_s(float) { return 0.0f; }
+template <typename R>
+__host__ __device__ inline R im_s(const cplx<R>& x) {
+  return x.im;
+}
+
+// Wave shuffles for every scalar type.
+__device__ inline double shfl_s(double v, int src) { return __shfl(v, src); }
+__device__ inline float shfl_s(float v, int src) { return __shfl(v, src); }
+template <typename R>
+__device__ inline cplx<R> shfl_s(const cplx<R>& v, int src) {
+  return cplx<R>(__shfl(v.re, src), __shfl(v.im, src));
+}
+__device__ inline double shfl_xor_s(double v, int m) { return __shfl_xor(v, m); }
+__device__ inline float shfl_xor_s(float v, int m) { return __shfl_xor(v, m); }
+template <typename R>
+__device__ inline cplx<R> shfl_xor_s(const cplx<R>& v, int m) {
+  return cplx<R>(__shfl_xor(v.re, m), __shfl_xor(v.im, m));
+}
+
+}  // namespace hssc

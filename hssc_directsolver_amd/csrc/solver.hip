@@ -138,6 +138,13 @@ hssc_tree::~hssc_tree() {
 }
 
 namespace hssc {
+// FP64 type of the same field as T: double for real T, complex double for complex T.
+template <typename T>
+using hi_t = typename std::conditional<is_complex_v<T>, zdouble, double>::type;
+// FP32 type of the same field as T.
+template <typename T>
+using lo_t = typename std::conditional<is_complex_v<T>, zfloat, float>::type;
+
 // Per-RHS-count solve workspace (levels x {Q, PU, PD}) plus scratch.
 template <typename T>
 struct SolveWork {
@@ -175,28 +182,72 @@ struct SolveWork {
     m = mm;
   }
 };
+
+// One workspace per scalar type.
+struct SolveWorks {
+  SolveWork<double> d;
+  SolveWork<float> f;
+  SolveWork<zdouble> z;
+  SolveWork<zfloat> c;
+  template <typename T>
+  SolveWork<T>& get();
+  void release() { d.release(), f.release(), z.release(), c.release(); }
+};
+template <>
+SolveWork<double>& SolveWorks::get<double>() {
+  return d;
+}
+template <>
+SolveWork<float>& SolveWorks::get<float>() {
+  return f;
+}
+template <>
+SolveWork<zdouble>& SolveWorks::get<zdouble>() {
+  return z;
+}
+template <>
+SolveWork<zfloat>& SolveWorks::get<zfloat>() {
+  return c;
+}
+
+// The GPU form of Common/Kernel_Eval.m for the tree's kernel.
+k::KernelSpec kernel_spec(const hssc_params& p) {
+  k::KernelSpec ks;
+  ks.pot = p.flag_pot;
+  const double h2 = p.h * p.h;
+  if (p.flag_pot == HSSC_POT_SL_H_2D) {
+    const double k2 = p.kh * p.kh;
+    ks.scale = h2 * k2 * 0.25;
+    ks.kh = p.kh;
+    // w = 1 + k^2 (i/4) corr0
+    ks.wre = 1.0 - 0.25 * k2 * p.dr_weights_im;
+    ks.wim = 0.25 * k2 * p.dr_weights_re;
+  } else {
+    ks.scale = 0.5 * h2;
+  }
+  return ks;
+}
 }  // namespace hssc
 
 struct hssc_inverse {
   hssc_tree* tree = nullptr;
   int prec = HSSC_PREC_FP64;
   std::vector<void*> FI, E;  // per level, nkey x n x n and nkey x k x k (J order)
-  mutable SolveWork<double> w64;
-  mutable SolveWork<float> w32;
+  mutable hssc::SolveWorks w;
   ~hssc_inverse() {
     for (void* p : FI) dfree(p);
     for (void* p : E) dfree(p);
-    w64.release();
-    w32.release();
+    w.release();
   }
 };
 
 struct hssc_operator {
   hssc_tree* tree = nullptr;
-  std::vector<double*> B;  // level depth: Bself (nkey x n x n, J order); others: B12 (kc x kc)
-  mutable SolveWork<double> w;
+  // level depth: Bself (nkey x n x n, J order); others: B12 (kc x kc); FP64 real or complex
+  std::vector<void*> B;
+  mutable hssc::SolveWorks w;
   ~hssc_operator() {
-    for (double* p : B) dfree(p);
+    for (void* p : B) dfree(p);
     w.release();
   }
 };
@@ -224,15 +275,16 @@ void upload_tree(hssc_tree* t) {
 // (HSS2D/interpolation_operator.m:57-66 and the same operator to tolerance for :80-127),
 // by blocked Householder QR of the augmented M x n block [K11 K21] on its first k
 // columns, then back substitution with R11.  The root's T is never used by invert or the
-// sweeps (the root has no E), so it is not formed.
-void compute_interpolation(hssc_tree* t, hipStream_t st0) {
-  const double h = t->p.h;
+// sweeps (the root has no E), so it is not formed.  S = double or complex double.
+template <typename S>
+void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
+  const k::KernelSpec ks = kernel_spec(t->p);
   const int64_t nb = k::kQrNb;
   const int64_t kchunk = 512;
   // Levels are independent: each level's QR sequence runs on its own non-blocking stream so
   // the few large boxes of the top levels overlap with the many small boxes below.
   struct Ws {
-    double *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
+    S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
   };
   std::vector<Ws> ws(t->depth + 1);
   // With instrumentation on, the levels run on one stream so that every kernel's event
@@ -250,7 +302,8 @@ void compute_interpolation(hssc_tree* t, hipStream_t st0) {
     for (int l = t->depth; l >= 1; --l) {
       Level& L = t->levels[l];
       const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
-      L.dT = dalloc<double>(E * kk * std::max<int64_t>(r, 1));
+      S* dT = dalloc<S>(E * kk * std::max<int64_t>(r, 1));
+      L.dT = dT;
       L.q = r;
       if (r == 0) continue;
       if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
@@ -260,46 +313,46 @@ void compute_interpolation(hssc_tree* t, hipStream_t st0) {
       Ws& w = ws[l];
       const int64_t sA = M * n;
       const int64_t nsplit = (M + kchunk - 1) / kchunk;
-      w.A = dalloc<double>(E * sA);
-      w.V = dalloc<double>(E * M * nb);
-      w.W1 = dalloc<double>(E * nb * n);
-      w.W2 = dalloc<double>(E * nb * n);
-      w.Tf = dalloc<double>(E * nb * nb);
-      w.part = dalloc<double>(E * nsplit * nb * n);
-      double* A = w.A;
+      w.A = dalloc<S>(E * sA);
+      w.V = dalloc<S>(E * M * nb);
+      w.W1 = dalloc<S>(E * nb * n);
+      w.W2 = dalloc<S>(E * nb * n);
+      w.Tf = dalloc<S>(E * nb * nb);
+      w.part = dalloc<S>(E * nsplit * nb * n);
+      S* A = w.A;
       // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
-      k::gen_block<double>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M,
-                           sA, h);
+      k::gen_block<S>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
+                      ks);
       for (int64_t j0 = 0; j0 < kk; j0 += nb) {
         const int jb = int(std::min(nb, kk - j0));
-        k::qr_panel<double>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
+        k::qr_panel<S>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
         const int64_t nc = n - j0 - jb;
         if (nc <= 0) continue;
         const int64_t Mp = M - j0;
-        k::qr_copyV<double>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
-        double* C = A + j0 + (j0 + jb) * M;
-        // W1 = V^T C ; W2 = Tf^T W1 ; C -= V W2
-        k::gemm_tn_splitk<double>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
-                                  w.part, kchunk);
-        k::gemm<double>(st, true, false, jb, nc, jb, 1.0, w.Tf, nb, nb * nb, w.W1, jb, jb * nc, 0.0,
-                        w.W2, jb, jb * nc, E);
-        k::gemm<double>(st, false, false, Mp, nc, jb, -1.0, w.V, Mp, Mp * nb, w.W2, jb, jb * nc, 1.0,
-                        C, M, sA, E);
+        k::qr_copyV<S>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
+        S* C = A + j0 + (j0 + jb) * M;
+        // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
+        k::gemm_tn_splitk<S>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
+                             w.part, kchunk);
+        k::gemm_h<S>(st, jb, nc, jb, S(1), w.Tf, nb, nb * nb, w.W1, jb, jb * nc, S(0), w.W2, jb,
+                     jb * nc, E);
+        k::gemm<S>(st, false, false, Mp, nc, jb, S(-1), w.V, Mp, Mp * nb, w.W2, jb, jb * nc, S(1),
+                   C, M, sA, E);
       }
       // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
-      double* C1 = A + kk * M;
+      S* C1 = A + kk * M;
       const int64_t last = ((kk - 1) / nb) * nb;
       for (int64_t p = last; p >= 0; p -= nb) {
         const int bsz = int(std::min(nb, kk - p));
-        k::trinv_upper<double>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
+        k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
         // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
-        k::gemm<double>(st, false, false, bsz, r, bsz, 1.0, w.Tf, bsz, nb * nb, C1 + p, M, sA, 0.0,
-                        C1 + p, M, sA, E);
+        k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
+                   C1 + p, M, sA, E);
         if (p > 0)
-          k::gemm<double>(st, false, false, p, r, bsz, -1.0, A + p * M, M, sA, C1 + p, M, sA, 1.0,
-                          C1, M, sA, E);
+          k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1), C1,
+                     M, sA, E);
       }
-      k::copy2d<double, double>(st, kk, r, C1, M, sA, L.dT, kk, kk * r, E);
+      k::copy2d<S, S>(st, kk, r, C1, M, sA, dT, kk, kk * r, E);
     }
     for (auto& s : streams) HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
@@ -311,34 +364,36 @@ void compute_interpolation(hssc_tree* t, hipStream_t st0) {
   t->has_T = true;
 }
 
+void compute_interpolation(hssc_tree* t, hipStream_t st0) {
+  if (t->cx) compute_interpolation_impl<zdouble>(t, st0);
+  else compute_interpolation_impl<double>(t, st0);
+}
+
 // ---------------------------------------------------------------------------------------
 // Inversion (HSS2D/HSS2D_invert_fsym.m:73-195, dense F^-1 and E)
 // ---------------------------------------------------------------------------------------
 
+// T of a level in the precision of T (FP64 or the FP32 copy).
 template <typename T>
-const T* level_T(const Level& L);
-template <>
-const double* level_T<double>(const Level& L) {
-  return L.dT;
-}
-template <>
-const float* level_T<float>(const Level& L) {
-  return L.dT32;
+const T* level_T(const Level& L) {
+  return static_cast<const T*>(sizeof(real_t<T>) == 8 ? L.dT : L.dT32);
 }
 
 template <typename T>
 void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   const int D = t->depth;
-  const double h = t->p.h;
+  const k::KernelSpec ks = kernel_spec(t->p);
   const bool TI = t->p.transinv != 0;
   inv->FI.assign(D + 1, nullptr);
   inv->E.assign(D + 1, nullptr);
-  if (std::is_same<T, float>::value) {
+  if (sizeof(real_t<T>) == 4) {
     for (int l = 1; l <= D; ++l) {
       Level& L = t->levels[l];
       const int64_t r = L.n - L.k;
-      if (!L.dT32) L.dT32 = dalloc<float>(L.nkey * L.k * std::max<int64_t>(r, 1));
-      if (r > 0) k::axpby<double, float>(st, L.nkey * L.k * r, 1.0f, L.dT, 0.0f, L.dT32);
+      if (!L.dT32) L.dT32 = dalloc<T>(L.nkey * L.k * std::max<int64_t>(r, 1));
+      if (r > 0)
+        k::axpby<hi_t<T>, T>(st, L.nkey * L.k * r, T(1), static_cast<const hi_t<T>*>(L.dT), T(0),
+                             static_cast<T*>(L.dT32));
     }
   }
   int64_t wmax = 0;
@@ -353,13 +408,14 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
     T* FI = dalloc<T>(E * n * n);
     inv->FI[l] = FI;
     if (l == D) {
-      k::gen_block<T>(st, n, n, E, L.dPx, L.dPy, L.dPw, n, L.dPx, L.dPy, L.dPw, n, FI, n, n * n, h);
+      k::gen_block<T>(st, n, n, E, L.dPx, L.dPy, L.dPw, n, L.dPx, L.dPy, L.dPw, n, FI, n, n * n,
+                      ks);
     } else {
       const Level& C = t->levels[l + 1];
       const int64_t kc = C.k;
       const int64_t sEbox = TI ? 0 : kc * kc, sEpar = TI ? 0 : 2 * kc * kc;
       k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
-                       static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, h);
+                       static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, ks);
     }
     k::invert_batched<T>(st, n, FI, n, n * n, E);
     if (l > 0) {
@@ -417,9 +473,11 @@ void split_to_children(const hssc_tree* t, int l, const T* PHI, T* PDc, T* S, in
 }
 
 // HSS2D/HSS2D_applyinv_TI.m:63-225 and HSS2D_applyinv_fsym.m (Algorithm 6 of the paper).
+// X, Y: N x m in the FP64 type of the field (double or complex double).
 template <typename T>
-void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const double* X, double* Y,
+void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, hi_t<T>* Y,
                    int64_t m, hipStream_t st) {
+  using H = hi_t<T>;
   const hssc_tree* t = inv->tree;
   const int D = t->depth;
   const bool TI = t->p.transinv != 0;
@@ -433,7 +491,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const double* X, do
     const T* E = static_cast<const T*>(inv->E[l]);
     const T* Tm = level_T<T>(L);
     T* Q = w.Q[l];
-    if (l == D) k::gather<double, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, nb);
+    if (l == D) k::gather<H, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, nb);
     else form_parent_q<T>(t, l, w.PU[l + 1], Q, w.S1, m, st);
     // psi = FI q
     T* PSI = w.S2;
@@ -455,12 +513,12 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const double* X, do
     const Level& L = t->levels[0];
     const int64_t n = L.n;
     T* Q = w.Q[0];
-    if (D == 0) k::gather<double, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, 1);
+    if (D == 0) k::gather<H, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, 1);
     else form_parent_q<T>(t, 0, w.PU[1], Q, w.S1, m, st);
     T* PHI = w.S2;
     level_matvec<T>(st, false, n, n, T(1), static_cast<const T*>(inv->FI[0]), n, n * n, Q, n,
                     T(0), nullptr, n, PHI, n, 1, m, TI, w.work);
-    if (D == 0) k::scatter<T, double>(st, n, m, t->dLeafG, n, PHI, n, n * m, Y, N, 0, 1, false);
+    if (D == 0) k::scatter<T, H>(st, n, m, t->dLeafG, n, PHI, n, n * m, Y, N, 0, 1, false);
     else split_to_children<T>(t, 0, PHI, w.PD[1], w.S1, m, st);
   }
   // downward pass
@@ -484,38 +542,42 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const double* X, do
     // phi = FI v  (into Q, which is no longer needed)
     level_matvec<T>(st, false, n, n, T(1), FI, n, n * n, V, n, T(0), nullptr, n, Q, n, nb, m, TI,
                     w.work);
-    if (l == D) k::scatter<T, double>(st, n, m, t->dLeafG, n, Q, n, n * m, Y, N, 0, nb, false);
+    if (l == D) k::scatter<T, H>(st, n, m, t->dLeafG, n, Q, n, n * m, Y, N, 0, nb, false);
     else split_to_children<T>(t, l, Q, w.PD[l + 1], w.S1, m, st);
   }
 }
 
-// HSS2D/HSS2D_apply_TI.m:74-189 and HSS2D_apply_fsym.m: telescoping forward operator.
-void apply_impl(const hssc_operator* op, const double* X, double* Y, int64_t m, hipStream_t st) {
+// HSS2D/HSS2D_apply_TI.m:74-189 and HSS2D_apply_fsym.m: telescoping forward operator in
+// FP64 (S = double or complex double).
+template <typename S>
+void apply_impl(const hssc_operator* op, const S* X, S* Y, int64_t m, hipStream_t st) {
   const hssc_tree* t = op->tree;
-  SolveWork<double>& w = op->w;
+  SolveWork<S>& w = op->w.get<S>();
   const int D = t->depth;
   const bool TI = t->p.transinv != 0;
   const int64_t N = t->N;
   w.ensure(t, m);
+  auto Bl = [&](int l) { return static_cast<const S*>(op->B[l]); };
   // upward: phi_up = q_s + T q_r
   for (int l = D; l >= 1; --l) {
     const Level& L = t->levels[l];
     const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
-    double* Qd = w.Q[l];
-    if (l == D) k::gather<double, double>(st, n, m, t->dLeafG, n, X, N, 0, Qd, n, n * m, nb);
-    else form_parent_q<double>(t, l, w.PU[l + 1], Qd, w.S1, m, st);
+    const S* Tm = static_cast<const S*>(L.dT);
+    S* Qd = w.Q[l];
+    if (l == D) k::gather<S, S>(st, n, m, t->dLeafG, n, X, N, 0, Qd, n, n * m, nb);
+    else form_parent_q<S>(t, l, w.PU[l + 1], Qd, w.S1, m, st);
     if (r > 0)
-      level_matvec<double>(st, false, kk, r, 1.0, L.dT, kk, kk * r, Qd + kk, n, 1.0, Qd, n,
-                           w.PU[l], kk, nb, m, TI, w.work);
+      level_matvec<S>(st, false, kk, r, S(1), Tm, kk, kk * r, Qd + kk, n, S(1), Qd, n, w.PU[l],
+                      kk, nb, m, TI, w.work);
     else
-      k::copy2d<double, double>(st, kk, nb * m, Qd, n, 0, w.PU[l], kk, 0, 1);
+      k::copy2d<S, S>(st, kk, nb * m, Qd, n, 0, w.PU[l], kk, 0, 1);
   }
   if (D == 0) {
     const Level& L = t->levels[0];
-    k::gather<double, double>(st, L.n, m, t->dLeafG, L.n, X, N, 0, w.Q[0], L.n, L.n * m, 1);
-    level_matvec<double>(st, false, L.n, L.n, 1.0, op->B[0], L.n, L.n * L.n, w.Q[0], L.n, 0.0,
-                         nullptr, L.n, w.S1, L.n, 1, m, TI, w.work);
-    k::scatter<double, double>(st, L.n, m, t->dLeafG, L.n, w.S1, L.n, L.n * m, Y, N, 0, 1, false);
+    k::gather<S, S>(st, L.n, m, t->dLeafG, L.n, X, N, 0, w.Q[0], L.n, L.n * m, 1);
+    level_matvec<S>(st, false, L.n, L.n, S(1), Bl(0), L.n, L.n * L.n, w.Q[0], L.n, S(0), nullptr,
+                    L.n, w.S1, L.n, 1, m, TI, w.work);
+    k::scatter<S, S>(st, L.n, m, t->dLeafG, L.n, w.S1, L.n, L.n * m, Y, N, 0, 1, false);
     return;
   }
   // downward
@@ -523,36 +585,36 @@ void apply_impl(const hssc_operator* op, const double* X, double* Y, int64_t m, 
     const Level& L = t->levels[l];
     const Level& C = t->levels[l + 1];
     const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox, kc = C.k;
-    const double* B12 = op->B[l];
+    const S* B12 = Bl(l);
+    const S* Tm = static_cast<const S*>(L.dT);
     const int64_t sB = TI ? 0 : kc * kc;
-    double* S = w.S1;  // natural-order eta (n x m per box)
+    S* Sv = w.S1;  // natural-order eta (n x m per box)
     if (l > 0) {
       // eta_J = [phi_dn ; T^T phi_dn], then natural order
-      double* EJ = w.S2;
-      k::copy2d<double, double>(st, kk, nb * m, w.PD[l], kk, 0, EJ, n, 0, 1);
+      S* EJ = w.S2;
+      k::copy2d<S, S>(st, kk, nb * m, w.PD[l], kk, 0, EJ, n, 0, 1);
       if (r > 0)
-        level_matvec<double>(st, true, kk, r, 1.0, L.dT, kk, kk * r, w.PD[l], kk, 0.0, nullptr,
-                             n, EJ + kk, n, nb, m, TI, w.work);
-      k::scatter<double, double>(st, n, m, L.dJ, TI ? 0 : n, EJ, n, n * m, S, n, n * m, nb,
-                                 false);
+        level_matvec<S>(st, true, kk, r, S(1), Tm, kk, kk * r, w.PD[l], kk, S(0), nullptr, n,
+                        EJ + kk, n, nb, m, TI, w.work);
+      k::scatter<S, S>(st, n, m, L.dJ, TI ? 0 : n, EJ, n, n * m, Sv, n, n * m, nb, false);
     }
     // children: PD[2e] = B12 PU[2e+1] + eta[0:kc];  PD[2e+1] = B12^T PU[2e] + eta[kc:n]
-    double* PUc = w.PU[l + 1];
-    double* PDc = w.PD[l + 1];
+    S* PUc = w.PU[l + 1];
+    S* PDc = w.PD[l + 1];
     for (int s = 0; s < 2; ++s) {
-      const double* xin = PUc + (1 - s) * kc * m;
-      double* yout = PDc + s * kc * m;
-      const double* z = S + s * kc;
-      const double gamma = l > 0 ? 1.0 : 0.0;
+      const S* xin = PUc + (1 - s) * kc * m;
+      S* yout = PDc + s * kc * m;
+      const S* z = Sv + s * kc;
+      const S gamma = l > 0 ? S(1) : S(0);
       // per parent box: x block stride 2*kc*m, y block stride 2*kc*m, z stride n*m
       if (!TI && m == 1) {
-        k::gemv<double>(st, s == 1, kc, kc, 1.0, B12, kc, sB, xin, 2 * kc, gamma, z, n, yout,
-                        2 * kc, nb, w.work);
+        k::gemv<S>(st, s == 1, kc, kc, S(1), B12, kc, sB, xin, 2 * kc, gamma, z, n, yout, 2 * kc,
+                   nb, w.work);
       } else {
         if (l > 0)
-          k::add2d<double>(st, kc, m, 1.0, z, n, n * m, 0.0, z, n, n * m, yout, kc, 2 * kc * m, nb);
-        k::gemm<double>(st, s == 1, false, kc, m, kc, 1.0, B12, kc, sB, xin, kc, 2 * kc * m, gamma,
-                        yout, kc, 2 * kc * m, nb);
+          k::add2d<S>(st, kc, m, S(1), z, n, n * m, S(0), z, n, n * m, yout, kc, 2 * kc * m, nb);
+        k::gemm<S>(st, s == 1, false, kc, m, kc, S(1), B12, kc, sB, xin, kc, 2 * kc * m, gamma,
+                   yout, kc, 2 * kc * m, nb);
       }
     }
   }
@@ -560,20 +622,29 @@ void apply_impl(const hssc_operator* op, const double* X, double* Y, int64_t m, 
   {
     const Level& L = t->levels[D];
     const int64_t n = L.n, kk = L.k, r = n - kk, nb = L.nbox;
-    double* EJ = w.S2;
-    k::copy2d<double, double>(st, kk, nb * m, w.PD[D], kk, 0, EJ, n, 0, 1);
+    const S* Tm = static_cast<const S*>(L.dT);
+    S* EJ = w.S2;
+    k::copy2d<S, S>(st, kk, nb * m, w.PD[D], kk, 0, EJ, n, 0, 1);
     if (r > 0)
-      level_matvec<double>(st, true, kk, r, 1.0, L.dT, kk, kk * r, w.PD[D], kk, 0.0, nullptr, n,
-                           EJ + kk, n, nb, m, TI, w.work);
-    level_matvec<double>(st, false, n, n, 1.0, op->B[D], n, n * n, w.Q[D], n, 1.0, EJ, n, w.S1,
-                         n, nb, m, TI, w.work);
-    k::scatter<double, double>(st, n, m, t->dLeafG, n, w.S1, n, n * m, Y, N, 0, nb, false);
+      level_matvec<S>(st, true, kk, r, S(1), Tm, kk, kk * r, w.PD[D], kk, S(0), nullptr, n,
+                      EJ + kk, n, nb, m, TI, w.work);
+    level_matvec<S>(st, false, n, n, S(1), Bl(D), n, n * n, w.Q[D], n, S(1), EJ, n, w.S1, n, nb,
+                    m, TI, w.work);
+    k::scatter<S, S>(st, n, m, t->dLeafG, n, w.S1, n, n * m, Y, N, 0, nb, false);
   }
 }
 
-double device_norm(const double* x, int64_t len, hipStream_t st, double* part_d) {
+void apply_dispatch(const hssc_operator* op, const void* X, void* Y, int64_t m, hipStream_t st) {
+  if (op->tree->cx)
+    apply_impl<zdouble>(op, static_cast<const zdouble*>(X), static_cast<zdouble*>(Y), m, st);
+  else
+    apply_impl<double>(op, static_cast<const double*>(X), static_cast<double*>(Y), m, st);
+}
+
+template <typename S>
+double device_norm(const S* x, int64_t len, hipStream_t st, double* part_d) {
   const int nblk = 256;
-  k::sumsq<double>(st, len, x, part_d, nblk);
+  k::sumsq<S>(st, len, x, part_d, nblk);
   double part[256];
   HIPCHK(hipMemcpyAsync(part, part_d, sizeof(part), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -582,13 +653,48 @@ double device_norm(const double* x, int64_t len, hipStream_t st, double* part_d)
   return std::sqrt(s);
 }
 
-int applyinv_dispatch(const hssc_inverse* inv, const double* X, double* Y, int64_t m,
-                      hipStream_t st) {
-  if (inv->prec == HSSC_PREC_FP64)
-    applyinv_impl<double>(inv, inv->w64, X, Y, m, st);
-  else
-    applyinv_impl<float>(inv, inv->w32, X, Y, m, st);
-  return HSSC_OK;
+void applyinv_dispatch(const hssc_inverse* inv, const void* X, void* Y, int64_t m,
+                       hipStream_t st) {
+  const bool fp64 = inv->prec == HSSC_PREC_FP64;
+  if (inv->tree->cx) {
+    const auto* Xz = static_cast<const zdouble*>(X);
+    auto* Yz = static_cast<zdouble*>(Y);
+    if (fp64) applyinv_impl<zdouble>(inv, inv->w.get<zdouble>(), Xz, Yz, m, st);
+    else applyinv_impl<zfloat>(inv, inv->w.get<zfloat>(), Xz, Yz, m, st);
+  } else {
+    const auto* Xd = static_cast<const double*>(X);
+    auto* Yd = static_cast<double*>(Y);
+    if (fp64) applyinv_impl<double>(inv, inv->w.get<double>(), Xd, Yd, m, st);
+    else applyinv_impl<float>(inv, inv->w.get<float>(), Xd, Yd, m, st);
+  }
+}
+
+// Common/Iterative_Refinement.m:20-35 with A = the FP64 compressed operator.
+template <typename S>
+int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* Y, int64_t m,
+               double tol, int32_t maxit, int32_t* iters, double* relres, hipStream_t st) {
+  const int64_t len = inv->tree->N * m;
+  S* R = dalloc<S>(len);
+  S* D = dalloc<S>(len);
+  double* part = dalloc<double>(256);
+  applyinv_dispatch(inv, B, Y, m, st);
+  const double nb = device_norm<S>(B, len, st, part);
+  int it = 1;
+  double rel = 0;
+  for (;;) {
+    apply_dispatch(op, Y, R, m, st);
+    k::axpby<S, S>(st, len, S(1), B, S(-1), R);  // R = B - A Y
+    rel = device_norm<S>(R, len, st, part) / (nb > 0 ? nb : 1.0);
+    if (rel <= tol || it >= maxit) break;
+    applyinv_dispatch(inv, R, D, m, st);
+    k::axpby<S, S>(st, len, S(1), D, S(1), Y);
+    ++it;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(R), dfree(D), dfree(part);
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
 }
 
 }  // namespace
@@ -660,21 +766,28 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
                      int64_t n, double* A, int64_t lda) {
   if (!p || !X1 || !X2 || !A || m < 0 || n < 0 || lda < m)
     return fail(HSSC_ERR_INVALID_ARG, "invalid kernel_eval arguments");
-  if (p->flag_pot != HSSC_POT_SL_L_2D)
-    return fail(HSSC_ERR_UNSUPPORTED, "only SL_L_2D is implemented on the GPU");
+  if (p->flag_pot == HSSC_POT_SL_H_2D && !(p->kh > 0))
+    return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
   if (int s = check_device()) return s;
+  const bool cx = p->flag_pot == HSSC_POT_SL_H_2D;
+  const size_t es = cx ? sizeof(zdouble) : sizeof(double);
   return guarded([&]() {
     std::vector<double> w1(m), w2(n), x1(m), y1(m), x2(n), y2(n);
     for (int64_t i = 0; i < m; ++i) x1[i] = X1[i], y1[i] = X1[m + i], w1[i] = point_weight(*p, x1[i], y1[i]);
     for (int64_t j = 0; j < n; ++j) x2[j] = X2[j], y2[j] = X2[n + j], w2[j] = point_weight(*p, x2[j], y2[j]);
     double *dx1 = dupload(x1), *dy1 = dupload(y1), *dw1 = dupload(w1);
     double *dx2 = dupload(x2), *dy2 = dupload(y2), *dw2 = dupload(w2);
-    double* dA = dalloc<double>(std::max<int64_t>(1, m * n));
-    k::gen_block<double>(0, m, n, 1, dx1, dy1, dw1, 0, dx2, dy2, dw2, 0, dA, std::max<int64_t>(m, 1),
-                         0, p->h);
+    void* dA = dalloc<zdouble>(std::max<int64_t>(1, m * n));
+    const k::KernelSpec ks = kernel_spec(*p);
+    if (cx)
+      k::gen_block<zdouble>(0, m, n, 1, dx1, dy1, dw1, 0, dx2, dy2, dw2, 0,
+                            static_cast<zdouble*>(dA), std::max<int64_t>(m, 1), 0, ks);
+    else
+      k::gen_block<double>(0, m, n, 1, dx1, dy1, dw1, 0, dx2, dy2, dw2, 0,
+                           static_cast<double*>(dA), std::max<int64_t>(m, 1), 0, ks);
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy2D(A, lda * sizeof(double), dA, m * sizeof(double), m * sizeof(double), n,
-                       hipMemcpyDeviceToHost));
+    if (m > 0 && n > 0)
+      HIPCHK(hipMemcpy2D(A, lda * es, dA, m * es, m * es, n, hipMemcpyDeviceToHost));
     dfree(dx1), dfree(dy1), dfree(dw1), dfree(dx2), dfree(dy2), dfree(dw2), dfree(dA);
     return HSSC_OK;
   });
@@ -796,8 +909,10 @@ int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out) {
   if (lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no interpolation operator");
   const int64_t key = box_key(t, box, lev), cnt = L.k * (L.n - L.k);
   if (cnt == 0) return HSSC_OK;
+  const size_t es = t->cx ? sizeof(zdouble) : sizeof(double);
   return guarded([&]() {
-    HIPCHK(hipMemcpy(out, L.dT + key * cnt, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, static_cast<const char*>(L.dT) + key * cnt * es, cnt * es,
+                     hipMemcpyDeviceToHost));
     return HSSC_OK;
   });
 }
@@ -830,8 +945,13 @@ int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** out) {
     inv->tree = t;
     inv->prec = precision;
     try {
-      if (precision == HSSC_PREC_FP64) invert_impl<double>(t, inv, 0);
-      else invert_impl<float>(t, inv, 0);
+      if (t->cx) {
+        if (precision == HSSC_PREC_FP64) invert_impl<zdouble>(t, inv, 0);
+        else invert_impl<zfloat>(t, inv, 0);
+      } else {
+        if (precision == HSSC_PREC_FP64) invert_impl<double>(t, inv, 0);
+        else invert_impl<float>(t, inv, 0);
+      }
     } catch (...) {
       delete inv;
       throw;
@@ -850,17 +970,19 @@ static int inverse_block(const hssc_inverse* inv, int64_t box, double* out, bool
   if (wantE && lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no E");
   const int64_t key = box_key(t, box, lev);
   const int64_t n = wantE ? L.k : L.n;
+  // complex blocks come back as interleaved complex double (2 doubles per entry)
+  const int64_t nr = t->cx ? 2 : 1;
   return guarded([&]() {
-    std::vector<double> J(n * n);
+    std::vector<double> J(n * n * nr);
     const void* src = wantE ? inv->E[lev] : inv->FI[lev];
     if (inv->prec == HSSC_PREC_FP64) {
-      HIPCHK(hipMemcpy(J.data(), static_cast<const double*>(src) + key * n * n, n * n * 8,
-                       hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(J.data(), static_cast<const double*>(src) + key * n * n * nr,
+                       n * n * nr * 8, hipMemcpyDeviceToHost));
     } else {
-      std::vector<float> f(n * n);
-      HIPCHK(hipMemcpy(f.data(), static_cast<const float*>(src) + key * n * n, n * n * 4,
-                       hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < n * n; ++i) J[i] = f[i];
+      std::vector<float> f(n * n * nr);
+      HIPCHK(hipMemcpy(f.data(), static_cast<const float*>(src) + key * n * n * nr,
+                       n * n * nr * 4, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n * n * nr; ++i) J[i] = f[i];
     }
     if (wantE) {
       std::copy(J.begin(), J.end(), out);
@@ -868,7 +990,9 @@ static int inverse_block(const hssc_inverse* inv, int64_t box, double* out, bool
       // stored in J order: FI(J[a], J[c]) = stored(a, c)
       const int32_t* Jp = L.J.data() + key * L.n;
       for (int64_t c = 0; c < n; ++c)
-        for (int64_t a = 0; a < n; ++a) out[Jp[a] + int64_t(Jp[c]) * n] = J[a + c * n];
+        for (int64_t a = 0; a < n; ++a)
+          for (int64_t z = 0; z < nr; ++z)
+            out[(Jp[a] + int64_t(Jp[c]) * n) * nr + z] = J[(a + c * n) * nr + z];
     }
     return HSSC_OK;
   });
@@ -895,20 +1019,28 @@ int hssc_hss2d_compress(hssc_tree* t, hssc_operator** out) {
     op->tree = t;
     const int D = t->depth;
     op->B.assign(D + 1, nullptr);
-    try {
+    const k::KernelSpec ks = kernel_spec(t->p);
+    auto build = [&](auto tag) {
+      using S = decltype(tag);
       for (int l = 0; l <= D; ++l) {
         const Level& L = t->levels[l];
         if (l == D) {
-          op->B[l] = dalloc<double>(L.nkey * L.n * L.n);
-          k::gen_block<double>(0, L.n, L.n, L.nkey, L.dPx, L.dPy, L.dPw, L.n, L.dPx, L.dPy, L.dPw,
-                               L.n, op->B[l], L.n, L.n * L.n, t->p.h);
+          S* B = dalloc<S>(L.nkey * L.n * L.n);
+          op->B[l] = B;
+          k::gen_block<S>(0, L.n, L.n, L.nkey, L.dPx, L.dPy, L.dPw, L.n, L.dPx, L.dPy, L.dPw, L.n,
+                          B, L.n, L.n * L.n, ks);
         } else {
           const int64_t kc = t->levels[l + 1].k;
-          op->B[l] = dalloc<double>(L.nkey * kc * kc);
-          k::gen_block<double>(0, kc, kc, L.nkey, L.dPNx, L.dPNy, L.dPNw, L.n, L.dPNx + kc,
-                               L.dPNy + kc, L.dPNw + kc, L.n, op->B[l], kc, kc * kc, t->p.h);
+          S* B = dalloc<S>(L.nkey * kc * kc);
+          op->B[l] = B;
+          k::gen_block<S>(0, kc, kc, L.nkey, L.dPNx, L.dPNy, L.dPNw, L.n, L.dPNx + kc,
+                          L.dPNy + kc, L.dPNw + kc, L.n, B, kc, kc * kc, ks);
         }
       }
+    };
+    try {
+      if (t->cx) build(zdouble());
+      else build(double());
       HIPCHK(hipDeviceSynchronize());
       HIPCHK(hipGetLastError());
     } catch (...) {
@@ -927,7 +1059,7 @@ int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY
   if (!inv || !dX || !dY || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid applyinv arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
   return guarded([&]() {
-    applyinv_dispatch(inv, static_cast<const double*>(dX), static_cast<double*>(dY), m, st);
+    applyinv_dispatch(inv, dX, dY, m, st);
     HIPCHK(hipGetLastError());
     return HSSC_OK;
   });
@@ -938,22 +1070,24 @@ int hssc_hss2d_apply_device(const hssc_operator* op, const void* dX, void* dY, i
   if (!op || !dX || !dY || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid apply arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
   return guarded([&]() {
-    apply_impl(op, static_cast<const double*>(dX), static_cast<double*>(dY), m, st);
+    apply_dispatch(op, dX, dY, m, st);
     HIPCHK(hipGetLastError());
     return HSSC_OK;
   });
 }
 
-static int host_wrap(int64_t N, int64_t m, const double* X, double* Y,
-                     const std::function<int(const double*, double*)>& f) {
+// X, Y: N x m doubles, or N x m interleaved complex doubles for a complex kernel
+static int host_wrap(int64_t N, int64_t m, bool cx, const double* X, double* Y,
+                     const std::function<int(const void*, void*)>& f) {
+  const size_t bytes = size_t(N * m) * (cx ? sizeof(zdouble) : sizeof(double));
   return guarded([&]() {
-    double* dX = dalloc<double>(N * m);
-    double* dY = dalloc<double>(N * m);
-    HIPCHK(hipMemcpy(dX, X, N * m * 8, hipMemcpyHostToDevice));
+    void* dX = dalloc<char>(bytes);
+    void* dY = dalloc<char>(bytes);
+    HIPCHK(hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice));
     int s = f(dX, dY);
     if (s == HSSC_OK) {
       HIPCHK(hipDeviceSynchronize());
-      HIPCHK(hipMemcpy(Y, dY, N * m * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(Y, dY, bytes, hipMemcpyDeviceToHost));
     }
     dfree(dX), dfree(dY);
     return s;
@@ -962,14 +1096,14 @@ static int host_wrap(int64_t N, int64_t m, const double* X, double* Y,
 
 int hssc_hss2d_applyinv(const hssc_inverse* inv, const double* X, double* Y, int64_t m) {
   if (!inv || !X || !Y || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid applyinv arguments");
-  return host_wrap(inv->tree->N, m, X, Y, [&](const double* dX, double* dY) {
+  return host_wrap(inv->tree->N, m, inv->tree->cx, X, Y, [&](const void* dX, void* dY) {
     return hssc_hss2d_applyinv_device(inv, dX, dY, m, nullptr);
   });
 }
 
 int hssc_hss2d_apply(const hssc_operator* op, const double* X, double* Y, int64_t m) {
   if (!op || !X || !Y || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid apply arguments");
-  return host_wrap(op->tree->N, m, X, Y, [&](const double* dX, double* dY) {
+  return host_wrap(op->tree->N, m, op->tree->cx, X, Y, [&](const void* dX, void* dY) {
     return hssc_hss2d_apply_device(op, dX, dY, m, nullptr);
   });
 }
@@ -981,31 +1115,12 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
   if (!inv || !op || !dB || !dY || m < 1 || maxit < 1 || inv->tree != op->tree)
     return fail(HSSC_ERR_INVALID_ARG, "invalid solve arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int64_t len = inv->tree->N * m;
   return guarded([&]() {
-    const double* B = static_cast<const double*>(dB);
-    double* Y = static_cast<double*>(dY);
-    double* R = dalloc<double>(len);
-    double* D = dalloc<double>(len);
-    double* part = dalloc<double>(256);
-    applyinv_dispatch(inv, B, Y, m, st);
-    const double nb = device_norm(B, len, st, part);
-    int it = 1;
-    double rel = 0;
-    for (;;) {
-      apply_impl(op, Y, R, m, st);
-      k::axpby<double, double>(st, len, 1.0, B, -1.0, R);  // R = B - A Y
-      rel = device_norm(R, len, st, part) / (nb > 0 ? nb : 1.0);
-      if (rel <= tol || it >= maxit) break;
-      applyinv_dispatch(inv, R, D, m, st);
-      k::axpby<double, double>(st, len, 1.0, D, 1.0, Y);
-      ++it;
-    }
-    HIPCHK(hipStreamSynchronize(st));
-    dfree(R), dfree(D), dfree(part);
-    if (iters) *iters = it;
-    if (relres) *relres = rel;
-    return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
+    if (inv->tree->cx)
+      return solve_impl<zdouble>(inv, op, static_cast<const zdouble*>(dB), static_cast<zdouble*>(dY),
+                                 m, tol, maxit, iters, relres, st);
+    return solve_impl<double>(inv, op, static_cast<const double*>(dB), static_cast<double*>(dY), m,
+                              tol, maxit, iters, relres, st);
   });
 }
 

@@ -201,6 +201,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
   if (p.sym != 1)
     return fail(HSSC_ERR_UNSUPPORTED, "only symmetric kernels (HSS2D_*_fsym) are supported");
   t->p = p;
+  t->cx = p.flag_pot == HSSC_POT_SL_H_2D;
   const int depth = p.depth;
   t->depth = depth;
   const int64_t N = int64_t(p.np) * p.np * p.Ng * p.Ng;

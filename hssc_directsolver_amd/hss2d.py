@@ -11,7 +11,8 @@ Function names and argument meaning follow HSSC_DirectSolver (DOCUMENTATION_HSS2
     lHSS      = levHSS(INFO_TREE, n_cut, trinv)
 
 All numerical work runs on the GPU through the C ABI; X and Y are numpy arrays
-(N x m, X_source ordering) or torch tensors already on the GPU (the *_device helpers).
+(N x m, X_source ordering) or torch tensors already on the GPU (the *_device helpers):
+float64 for 'SL_L_2D', complex128 for the Helmholtz kernel 'SL_H_2D'.
 Errors raise HsscError (the reference raises MATLAB runtime errors instead).
 """
 from __future__ import annotations
@@ -40,6 +41,14 @@ class Params:
         return getattr(raw, name)
 
     @property
+    def is_complex(self) -> bool:
+        return self.raw.flag_pot == POT["SL_H_2D"]
+
+    @property
+    def dtype(self):
+        return np.complex128 if self.is_complex else np.float64
+
+    @property
     def N(self) -> int:
         return int(lib().hssc_params_npoints(C.byref(self.raw)))
 
@@ -51,8 +60,12 @@ class Params:
         return X.T
 
 
-def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None) -> Params:
-    """Common/HSS_tree_parameters.m:1."""
+def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
+                        proxy_layers=None) -> Params:
+    """Common/HSS_tree_parameters.m:1.
+
+    For kh > 0 the Duan-Rokhlin weight corr0 is computed by the library (``corr0``
+    overrides it); ``proxy_layers`` replaces the proxy-ring depth rule at lev > 0."""
     raw = HsscParams()
     check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
                                  float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
@@ -60,6 +73,8 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None) 
     if corr0 is not None:
         check(lib().hssc_params_set_dr_weights(C.byref(raw), float(np.real(corr0)),
                                                float(np.imag(corr0))))
+    if proxy_layers is not None:
+        raw.proxy_layers = int(proxy_layers)
     return Params(raw)
 
 
@@ -68,7 +83,7 @@ def Kernel_Eval(X1, X2, params: Params) -> np.ndarray:
     X1 = np.asfortranarray(X1, dtype=np.float64)
     X2 = np.asfortranarray(X2, dtype=np.float64)
     m, n = X1.shape[0], X2.shape[0]
-    A = np.empty((m, n), order="F")
+    A = np.empty((m, n), order="F", dtype=params.dtype)
     check(lib().hssc_kernel_eval(C.byref(params.raw), _dptr(X1), m, _dptr(X2), n, _dptr(A),
                                  max(m, 1)))
     return A
@@ -124,8 +139,8 @@ class Tree:
     def T(self, box):
         info = self.level_info(self._box_level(box))
         k, r = info["k"], info["n"] - info["k"]
-        out = np.empty((r, k)).T  # k x r column-major view
-        buf = np.empty(k * r)
+        out = np.empty((r, k), dtype=self.params.dtype).T  # k x r column-major view
+        buf = np.empty(k * r, dtype=self.params.dtype)
         check(lib().hssc_tree_get_T(self.handle, int(box), _dptr(buf)))
         return buf.reshape((r, k)).T.copy() if k * r else out
 
@@ -170,13 +185,13 @@ class Inverse:
 
     def FI(self, box):
         n = self.tree.level_info(self.tree._box_level(box))["n"]
-        buf = np.empty(n * n)
+        buf = np.empty(n * n, dtype=self.tree.params.dtype)
         check(lib().hssc_inverse_get_FI(self.handle, int(box), _dptr(buf)))
         return buf.reshape((n, n)).T.copy()
 
     def E(self, box):
         k = self.tree.level_info(self.tree._box_level(box))["k"]
-        buf = np.empty(k * k)
+        buf = np.empty(k * k, dtype=self.tree.params.dtype)
         check(lib().hssc_inverse_get_E(self.handle, int(box), _dptr(buf)))
         return buf.reshape((k, k)).T.copy()
 
@@ -209,8 +224,8 @@ def HSS2D_compress_fsym(tree: Tree, params: Params = None) -> Operator:
     return Operator(h, tree)
 
 
-def _host_call(fn, handle, X):
-    X = np.asarray(X, dtype=np.float64)
+def _host_call(fn, handle, X, dtype):
+    X = np.asarray(X, dtype=dtype)
     shape = X.shape
     Xf = np.asfortranarray(X.reshape(shape[0], -1))
     Y = np.empty_like(Xf, order="F")
@@ -220,7 +235,7 @@ def _host_call(fn, handle, X):
 
 def HSS2D_applyinv_TI(tree: Tree, A_HSS: Inverse, X, params: Params = None):
     """HSS2D/HSS2D_applyinv_TI.m:1."""
-    return _host_call(lib().hssc_hss2d_applyinv, A_HSS.handle, X)
+    return _host_call(lib().hssc_hss2d_applyinv, A_HSS.handle, X, A_HSS.tree.params.dtype)
 
 
 HSS2D_applyinv_fsym = HSS2D_applyinv_TI  # HSS2D/HSS2D_applyinv_fsym.m:1 (same C entry point)
@@ -228,20 +243,21 @@ HSS2D_applyinv_fsym = HSS2D_applyinv_TI  # HSS2D/HSS2D_applyinv_fsym.m:1 (same C
 
 def HSS2D_apply_TI(tree: Tree, M_HSS: Operator, X, params: Params = None):
     """HSS2D/HSS2D_apply_TI.m:1."""
-    return _host_call(lib().hssc_hss2d_apply, M_HSS.handle, X)
+    return _host_call(lib().hssc_hss2d_apply, M_HSS.handle, X, M_HSS.tree.params.dtype)
 
 
 HSS2D_apply_fsym = HSS2D_apply_TI  # HSS2D/HSS2D_apply_fsym.m:1
 
 
-# ---- device-resident variants (torch tensors on the GPU, float64, N x m column-major) ----
+# ---- device-resident variants (torch tensors on the GPU, float64 or complex128 for the
+# Helmholtz kernel, N x m column-major) ----
 
 def _stream_ptr(stream):
     return None if stream is None else C.c_void_p(stream.cuda_stream)
 
 
 def applyinv_device(A_HSS: Inverse, X, Y, stream=None):
-    """Y = A^-1 X with X, Y torch float64 tensors of shape (m, N) (each row one RHS)."""
+    """Y = A^-1 X with X, Y torch tensors of shape (m, N) (each row one RHS)."""
     m = X.shape[0] if X.dim() == 2 else 1
     check(lib().hssc_hss2d_applyinv_device(A_HSS.handle, C.c_void_p(X.data_ptr()),
                                            C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))

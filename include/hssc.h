@@ -87,8 +87,8 @@ int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_
  * replaces Common/DR_stencil.m:64-107 (LOCAL_compute_duan_rokhlin_weights, DVEC(1)), which
  * runs the Fortran side-car Common/FORTRAN_get_DR_weights.f:85-490 (corr_hank).  Computed
  * from the lattice-sum closed form corr0 = h^2 + i (2h^2/pi)[log(hk/2) + gamma + Z'(0) +
- * sum_m (-1)^m (hk/2)^2m / (m!)^2 Z'(-2m)], Z the Epstein zeta function of Z^2.  Like the
- * side-car, it accepts hk in [1e-4, 1.01] only (HSSC_ERR_INVALID_ARG otherwise). */
+ * sum_m (-1)^m (hk/2)^2m / (m!)^2 Z'(-2m)], Z the Epstein zeta function of Z^2.  Accepts
+ * 0 < hk <= 2 (the side-car's tables stop at 1.01); HSSC_ERR_INVALID_ARG otherwise. */
 int hssc_dr_corr0(double h, double k, double* corr0_re, double* corr0_im);
 
 /* Sets params.dr_weights = corr0 (Common/DR_stencil.m:10-25 output, order 4). */

@@ -104,8 +104,12 @@ def test_dr_corr0_known_answer_and_oracle():
         _lib.check(L.hssc_dr_corr0(h, k, ctypes.byref(cr), ctypes.byref(ci)))
         want = corr0(h, k)
         assert abs(complex(cr.value, ci.value) - want) <= 1e-15 * abs(want), (h, k)
-    # the side-car's accepted range of h*k
-    assert L.hssc_dr_corr0(1 / 32, 64.0, ctypes.byref(cr), ctypes.byref(ci)) != 0
+    # beyond the side-car's tables (hk <= 1.01) the series still holds: hk = 1.25, 2
+    for h, k in [(1 / 16, 20.0), (1 / 10, 20.0)]:
+        _lib.check(L.hssc_dr_corr0(h, k, ctypes.byref(cr), ctypes.byref(ci)))
+        want = corr0(h, k, terms=40)
+        assert abs(complex(cr.value, ci.value) - want) <= 1e-15 * abs(want), (h, k)
+    assert L.hssc_dr_corr0(1 / 32, 100.0, ctypes.byref(cr), ctypes.byref(ci)) != 0
     # params_init sets corr0 for kh > 0 (HSS_tree_parameters.m:167-175)
     P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 64, 1e-10, 3, 800, 0)
     assert abs(complex(P.dr_weights_re, P.dr_weights_im) - corr0(1 / 256, 20.0)) < 1e-18
