@@ -1,19 +1,22 @@
 #!/usr/bin/env python
 """HSS2D N = 512^2 build + solve on one MI355X.
 
-Workload: the reference's HSS2D volume integral equation (I + h^2 K), K = 1/2 log r^2
-(SL_L_2D, Common/Kernel_Eval.m:60-61) on the uniform 512 x 512 grid of
-HSS_tree_parameters('SL_L_2D', 0, 1, 8, 64, 1e-10, 2, 400, TI) (N = 262144, leaf 8x8,
-depth 12), non-translation-invariant (per-box operators, TI = 0), dense-block inverse,
-seeded unit-norm Gaussian right-hand side.
+Workload (default): the Lippmann-Schwinger Helmholtz volume integral equation of the
+reference, kernel w (den==0) + h^2 k^2 (i/4) b H0(k r) c (SL_H_2D, Common/Kernel_Eval.m:44-49,
+b = c = sqrt(bump), Duan-Rokhlin diagonal w) with k = 20 on the uniform 512 x 512 grid of
+HSS_tree_parameters('SL_H_2D', 20, 1, 8, 64, 1e-10, 3, 800, 0) (N = 262144, leaf 8x8,
+depth 12, per-box operators), 'bdry' skeletons at every level, 4-layer proxy rings,
+dense-block inverse, seeded unit-norm complex Gaussian right-hand side.
+`--kernel laplace` runs the SL_L_2D grid of HSS_tree_parameters('SL_L_2D', 0, 1, 8, 64,
+1e-10, 2, 400, 0) instead; it is also reported as a side case.
 
 One step = HSS2D_build_tree_fsym (host index sets + GPU interpolation operators)
          + HSS2D_compress_fsym (forward operator used by the refinement residual)
          + HSS2D_invert_fsym + solve to 1e-10 relative residual against the FP64
            compressed operator (Common/Iterative_Refinement.m).
 Two paths are timed in every step:
-  mixed: FP32 factorization (F^-1, E on MFMA), FP64 refinement  -> reported `value`
-  fp64 : FP64 factorization, FP64 refinement (plain path)
+  mixed: FP32 (complex64) factorization (F^-1, E on MFMA), FP64 refinement -> `value`
+  fp64 : FP64 (complex128) factorization, FP64 refinement (plain path)
 
 Prints one JSON line.  `value` is the mixed path's build + solve seconds for the whole
 job (time-like, lower is better).
@@ -29,7 +32,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}   # MI355X dense MFMA peaks
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "c128": 78.6, "c64": 157.3}  # MI355X dense MFMA
 PEAK_HBM_GBS = 8000.0
 
 
@@ -41,9 +44,14 @@ def parse():
     ap.add_argument("--Ng", type=int, default=8)
     ap.add_argument("--np", dest="np_", type=int, default=64)
     ap.add_argument("--TI", type=int, default=0)
+    ap.add_argument("--kernel", choices=("helmholtz", "laplace"), default="helmholtz")
+    ap.add_argument("--kh", type=float, default=20.0)
+    ap.add_argument("--proxy-layers", type=int, default=4)
+    ap.add_argument("--no-laplace", action="store_true", help="skip the Laplace side case")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-np", type=int, default=64, help="np of the CPU baseline sample")
+    ap.add_argument("--cpu-stop", type=int, default=3,
+                    help="CPU baseline: levels measured down to this one, above extrapolated")
     ap.add_argument("--no-hss1d", action="store_true", help="skip the HSS1D side case")
     return ap.parse_args()
 
@@ -52,9 +60,12 @@ def stats_snapshot(H):
     import ctypes as C
     L = H.lib()
     out = {}
-    for name in ("gemm_f64", "gemm_f32", "gemv_f64", "gemv_f32", "gemvt_f64", "gemvt_f32",
-                 "gj_small_f64", "gj_small_f32", "gj_diag_f64", "gj_diag_f32", "gen_block_f64",
-                 "gen_block_f32", "gen_parent_f64", "gen_parent_f32", "qr_panel_f64"):
+    names = [f"{k}_{t}" for k in ("gemv", "gemvt", "gj_small", "gj_diag", "gen_block",
+                                  "gen_parent", "qr_panel")
+             for t in ("f64", "f32", "c128", "c64")]
+    names += [f"gemm_{t}_{a}{b}" for t in ("f64", "f32", "c128", "c64") for a in "ntch"
+              for b in "nt"]
+    for name in names:
         n, fl, by, ms = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
         L.hssc_stats_get(name.encode(), C.byref(n), C.byref(fl), C.byref(by), C.byref(ms))
         if n.value:
@@ -127,35 +138,125 @@ def hss1d_case(H, torch, N=4096, reps=3):
                                 solve_ms=round(1e3 * (c3 - c2), 2), total_s=round(c3 - c0, 3)))
 
 
-def cpu_baseline(np_cpu):
+def level_flops(n, k, M, cx):
+    """Dense-block build flops of one box: T by Householder QR least squares
+    (2Mk^2 + 4Mk(n-k)) and F^-1, R F^-1 R^T, E^-1 (2n^3 + 2n^2 k + 2nk^2 + 2k^3); x4 for
+    complex arithmetic."""
+    f = 4 if cx else 1
+    return f * (2 * M * k * k + 4 * M * k * (n - k)), f * (2 * n**3 + 2 * n * n * k + 2 * n * k * k
+                                                          + 2 * k**3)
+
+
+def cpu_baseline(H, P, args):
     """The oracle (numpy/scipy FP64 restatement of the reference) on the TI tree of the same
-    grid: one box per level, per-level times scaled by the boxes per level give the NTI
-    build; the oracle's solve sweeps every box, as the NTI solve does."""
+    grid, one box per level: levels depth..cpu_stop are timed, each level's time is scaled by
+    its box count (the NTI build does every box), and the levels above cpu_stop are
+    extrapolated from the flop rates measured on level cpu_stop.  The solve (milliseconds on
+    the GPU, seconds on the CPU) is left out of the CPU figure."""
     from oracle import hss2d as o
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    p = o.tree_parameters("SL_L_2D", 0, 1, 8, np_cpu, 1e-10, 2, 400, 1)
+    cx = args.kernel == "helmholtz"
+    if cx:
+        from oracle.dr_weights import corr0
+        h = 2.0 / (args.Ng * args.np_)
+        p = o.tree_parameters("SL_H_2D", args.kh, 1, args.Ng, args.np_, 1e-10, 3, 800, 1,
+                              corr0=corr0(h, args.kh))
+        p["proxy_layers"] = args.proxy_layers
+    else:
+        p = o.tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, 1)
+    stop = args.cpu_stop
     tb, ti = {}, {}
     t0 = time.perf_counter()
-    tree = o.build_tree(p, interform="dense", timings=tb)
-    inv = o.invert(tree, timings=ti)
-    bvec = o.random_rhs(p["X_source"].shape[0])
+    tree = o.build_tree(p, interform="dense", timings=tb, stop_level=stop, preset_all=True)
+    o.invert(tree, timings=ti, stop_level=stop)
     t1 = time.perf_counter()
-    o.applyinv(tree, inv, bvec)
-    t2 = time.perf_counter()
-    build_nti = sum((tb[l] + ti[l]) * 2 ** l for l in tb)
-    return dict(value=build_nti + (t2 - t1), unit="s", cores=threads, kind="port",
-                sample=(f"oracle (numpy/scipy FP64) on the TI tree of the {8*np_cpu}^2 grid: "
-                        f"one box per level ({t1 - t0:.1f} s build+invert measured, per-level "
-                        f"time x boxes per level = {build_nti:.1f} s NTI build) + one full "
-                        f"solve sweep ({t2 - t1:.2f} s)"),
-                measured_s=round(t2 - t0, 2))
+    idx = H.HSS2D_build_index_sets(P)
+    shapes = {l: idx.level_info(l) for l in range(idx.depth + 1)}
+    fT, fI = level_flops(shapes[stop]["n"], shapes[stop]["k"], shapes[stop]["M"], cx)
+    rT, rI = fT / tb[stop], fI / ti[stop]
+    total = 0.0
+    for l, s in shapes.items():
+        if l >= stop:
+            total += (tb[l] + ti[l]) * s["nbox"]
+        else:
+            a, b = level_flops(s["n"], s["k"], s["M"], cx)
+            total += (a / rT * (l > 0) + b / rI) * s["nbox"]
+    return dict(value=round(total, 1), unit="s", cores=threads, kind="port",
+                sample=(f"oracle (numpy/scipy FP64) on the TI tree of the same grid: levels "
+                        f"{idx.depth}..{stop} timed on one box each ({t1 - t0:.1f} s), scaled by "
+                        f"boxes per level; levels {stop - 1}..0 extrapolated at the level-{stop} "
+                        f"rates ({rT / 1e9:.1f} GF/s least squares, {rI / 1e9:.1f} GF/s "
+                        f"inversion); build only"),
+                measured_s=round(t1 - t0, 2))
+
+
+def make_problem(H, torch, kernel, args):
+    """Params and the seeded unit-norm right-hand side of one workload."""
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    if kernel == "helmholtz":
+        P = H.HSS_tree_parameters("SL_H_2D", args.kh, 1, args.Ng, args.np_, 1e-10, 3, 800,
+                                  args.TI, proxy_layers=args.proxy_layers)
+        b = torch.complex(torch.randn(P.N, generator=gen, dtype=torch.float64),
+                          torch.randn(P.N, generator=gen, dtype=torch.float64))
+        wl = (f"HSS2D SL_H_2D Lippmann-Schwinger k={args.kh:g} {'TI' if args.TI else 'NTI'} "
+              f"Ng={args.Ng} np={args.np_} N={P.N} acc=1e-10 layers=3 proxy_layers="
+              f"{args.proxy_layers} dense blocks")
+    else:
+        P = H.HSS_tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, args.TI)
+        b = torch.randn(P.N, generator=gen, dtype=torch.float64)
+        wl = (f"HSS2D SL_L_2D {'TI' if args.TI else 'NTI'} Ng={args.Ng} np={args.np_} N={P.N} "
+              f"acc=1e-10 layers=2 dense blocks")
+    return P, (b / b.norm()).to("cuda:0"), wl
+
+
+def summarize(runs, N):
+    def avg(f):
+        return sum(f(r) for r in runs) / len(runs)
+
+    tree_s = avg(lambda r: r[0]["tree"])
+    mixed = {k: avg(lambda r, k=k: r[1]["fp32"][k]) for k in ("invert", "solve")}
+    fp64 = {k: avg(lambda r, k=k: r[1]["fp64"][k]) for k in ("invert", "solve")}
+    last = runs[-1][1]
+    tm = tree_s + mixed["invert"] + mixed["solve"]
+    t64 = tree_s + fp64["invert"] + fp64["solve"]
+    return {
+        "mixed": {"build_s": round(tree_s + mixed["invert"], 4), "tree_s": round(tree_s, 4),
+                  "invert_s": round(mixed["invert"], 4), "solve_ms": round(1e3 * mixed["solve"], 3),
+                  "iters": last["fp32"]["iters"], "relres": last["fp32"]["relres"],
+                  "total_s": round(tm, 4), "N_per_s": round(N / tm, 1)},
+        "fp64": {"build_s": round(tree_s + fp64["invert"], 4), "tree_s": round(tree_s, 4),
+                 "invert_s": round(fp64["invert"], 4), "solve_ms": round(1e3 * fp64["solve"], 3),
+                 "iters": last["fp64"]["iters"], "relres": last["fp64"]["relres"],
+                 "total_s": round(t64, 4), "N_per_s": round(N / t64, 1)},
+        "speedup_mixed_vs_fp64": round(t64 / tm, 3),
+    }
+
+
+def roofline(stats):
+    """Roofline object of the kernel class with the largest summed time in the
+    instrumented step: algorithmic flops (or bytes) per launch / average launch time."""
+    if not stats:
+        return None
+    name, s = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    prec = next(t for t in ("f64", "f32", "c128", "c64") if f"_{t}" in name)
+    common = dict(kernel=name, traffic=None, launches=s["launches"],
+                  avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2))
+    if name.startswith(("gemm", "gj_", "qr_panel")):
+        ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
+        return dict(bound="mfma", achieved=round(ach, 3), peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+                    frac=round(ach / PEAK_TFLOPS[prec], 4), flops_per_launch=s["flops"] / s["launches"],
+                    **common)
+    ach = s["bytes"] / (s["ms"] * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(ach / PEAK_HBM_GBS, 4), bytes_per_launch=s["bytes"] / s["launches"],
+                **common)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(see DESIGN.md: how the tree would shard)"}))
+                                   "(DESIGN.md describes how the tree would shard)"}))
         sys.exit(2)
     import torch
     import hssc_directsolver_amd as H
@@ -163,11 +264,8 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
     torch.cuda.set_device(0)
-    P = H.HSS_tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, args.TI)
+    P, b, workload = make_problem(H, torch, args.kernel, args)
     N = P.N
-    gen = torch.Generator(device="cpu").manual_seed(0)
-    b = torch.randn(N, generator=gen, dtype=torch.float64)
-    b = (b / b.norm()).to("cuda:0")
 
     for _ in range(args.warmup):
         one_step(H, P, torch, b, args.tol)
@@ -177,59 +275,33 @@ def main():
     T0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
-        runs.append(one_step(H, P, torch, b, args.tol))
+        r = one_step(H, P, torch, b, args.tol)
+        runs.append(r[:2])
+        del r
     torch.cuda.synchronize()
     T1 = time.perf_counter()
-    # One more step of the same workload with per-kernel HIP-event instrumentation (levels
-    # serialised, so event times are kernel durations comparable with rocprofv3's).
+    # One more step of the same workload with per-kernel HIP-event instrumentation (same launches
+    # and streams as the timed steps, so event times match a kernel trace of those steps).
     L.hssc_stats_reset()
     L.hssc_stats_enable(1)
     one_step(H, P, torch, b, args.tol)
     torch.cuda.synchronize()
     L.hssc_stats_enable(0)
     stats = stats_snapshot(H)
+    summ = summarize(runs, N)
 
-    def avg(f):
-        return sum(f(r) for r in runs) / len(runs)
-
-    tree_s = avg(lambda r: r[0]["tree"])
-    mixed = {k: avg(lambda r, k=k: r[1]["fp32"][k]) for k in ("invert", "solve")}
-    fp64 = {k: avg(lambda r, k=k: r[1]["fp64"][k]) for k in ("invert", "solve")}
-    last = runs[-1][1]
-    build_mixed = tree_s + mixed["invert"]
-    build_fp64 = tree_s + fp64["invert"]
-    total_mixed = build_mixed + mixed["solve"]
-    total_fp64 = build_fp64 + fp64["solve"]
-
-    # dominant kernel class over the instrumented step (both paths)
-    dom = max(stats.items(), key=lambda kv: kv[1]["ms"]) if stats else None
-    roof = None
-    if dom:
-        name, s = dom
-        prec = "f64" if name.endswith("f64") else "f32"
-        mfma = name.startswith(("gemm", "gj_"))
-        if mfma:
-            ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
-            roof = dict(kernel=name, bound="mfma", achieved=round(ach, 3),
-                        peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
-                        frac=round(ach / PEAK_TFLOPS[prec], 4), traffic=None,
-                        launches=s["launches"], avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2),
-                        flops_per_launch=s["flops"] / s["launches"])
-        else:
-            ach = s["bytes"] / (s["ms"] * 1e-3) / 1e9
-            roof = dict(kernel=name, bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS,
-                        unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 4), traffic=None,
-                        launches=s["launches"], avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2),
-                        bytes_per_launch=s["bytes"] / s["launches"])
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_np)
+    side = None
+    if args.kernel == "helmholtz" and not args.no_laplace:
+        Pl, bl, wl = make_problem(H, torch, "laplace", args)
+        one_step(H, Pl, torch, bl, args.tol)  # warm-up
+        side = dict(workload=wl, **summarize([one_step(H, Pl, torch, bl, args.tol)[:2]], Pl.N))
+    cpu = None if args.no_cpu else cpu_baseline(H, P, args)
     h1 = hss1d_case(H, torch) if not args.no_hss1d else None
+    cx = args.kernel == "helmholtz"
 
     out = {
         "metric": "HSS2D N=512² build s + solve ms @1e-10 resid: mixed+batched vs FP64 GPU",
-        "value": round(total_mixed, 4),
+        "value": summ["mixed"]["total_s"],
         "unit": "s (build + solve, mixed FP32 factorization + FP64 refinement)",
         "n_gpus": 1,
         "steps": args.steps,
@@ -238,26 +310,18 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factorization / f64 refinement",
+        "dtype": ("c64 factorization / c128 refinement" if cx
+                  else "f32 factorization / f64 refinement"),
         "data": "synthetic (seeded Gaussian RHS on the reference's uniform grid)",
-        "config": {"workload": f"HSS2D SL_L_2D {'TI' if args.TI else 'NTI'} Ng={args.Ng} "
-                               f"np={args.np_} N={N} acc=1e-10 layers=2 dense blocks",
-                   "N": N},
-        "mixed": {"build_s": round(build_mixed, 4), "tree_s": round(tree_s, 4),
-                  "invert_s": round(mixed["invert"], 4), "solve_ms": round(1e3 * mixed["solve"], 3),
-                  "iters": last["fp32"]["iters"], "relres": last["fp32"]["relres"],
-                  "N_per_s": round(N / total_mixed, 1)},
-        "fp64": {"build_s": round(build_fp64, 4), "invert_s": round(fp64["invert"], 4),
-                 "solve_ms": round(1e3 * fp64["solve"], 3), "iters": last["fp64"]["iters"],
-                 "relres": last["fp64"]["relres"], "total_s": round(total_fp64, 4),
-                 "N_per_s": round(N / total_fp64, 1)},
-        "speedup_mixed_vs_fp64": round(total_fp64 / total_mixed, 3),
-        "roofline": roof,
+        "config": {"workload": workload, "N": N},
+        **summ,
+        "roofline": roofline(stats),
         "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
                             tflops=round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else None,
                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
                     for k, v in stats.items()},
         "cpu_baseline": cpu,
+        "laplace": side,
         "hss1d": h1,
     }
     print(json.dumps(out), flush=True)

@@ -327,8 +327,11 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
   const double fl = (is_complex_v<T> ? 8.0 : 2.0) * rows * cols * double(kdim) * double(batch);
   const double by = (rows * kdim + kdim * cols + 2 * rows * cols) * sizeof(T) * double(batch);
   const bool conjA = is_complex_v<T> && ca;
+  // label per kernel variant: gemm_<type>_<op(A)><op(B)>, op = n, t, or h (conjugate transpose)
+  const std::string name = lbl<T>("gemm") + "_" + (ta ? (conjA ? "h" : "t") : (conjA ? "c" : "n")) +
+                           (tb ? "t" : "n");
 #define HSSC_GEMM_LAUNCH(TA_, CA_, TB_)                                                         \
-  HSSC_TIMED(lbl<T>("gemm").c_str(), st, fl, by,                                                \
+  HSSC_TIMED(name.c_str(), st, fl, by,                                                          \
              hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_>), grid, dim3(256), 0, st, m, n,  \
                                 kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, \
                                 cs0, csn, ksplit, kchunk))

@@ -287,9 +287,9 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
     S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
   };
   std::vector<Ws> ws(t->depth + 1);
-  // With instrumentation on, the levels run on one stream so that every kernel's event
-  // time is its own duration (no overlap with other levels).
-  const int nstreams = stats::enabled() ? 1 : std::max(1, std::min(t->depth, 8));
+  // (Instrumented runs keep the same streams, so their per-kernel event times match what a
+  // kernel trace of the uninstrumented run reports.)
+  const int nstreams = std::max(1, std::min(t->depth, 8));
   std::vector<hipStream_t> streams(nstreams, nullptr);
   HIPCHK(hipStreamSynchronize(st0));  // the tree upload is complete before the level streams start
   for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

@@ -488,7 +488,8 @@ def build_index_sets(params):
     return build_tree(params, with_T=False)
 
 
-def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False):
+def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False,
+               stop_level=0):
     """HSS2D_build_tree_fsym (uniform tree, preset 'bdry' skeletons and proxies).
 
     Restates :81-287 (tree), :292-623 (upward skeleton pass) and
@@ -496,7 +497,8 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
     INTERform ('lowrank' in the reference; 'dense' gives exact LS T at every level).
     For kh > 0 the reference selects the skeletons of the two bottom levels by an ID
     (:376-443); ``preset_all=True`` keeps the 'bdry' rule there as well (same operator
-    to tolerance, uniform level shapes).
+    to tolerance, uniform level shapes).  ``stop_level`` > 0 ends the upward pass there
+    (levels below the root only; used to time a bounded sample of a large tree).
     """
     if params["skel_rule"] != "bdry" or params["proxy_rule"] != "bdry":
         raise NotImplementedError("only the 'bdry' skeleton/proxy rules are restated")
@@ -512,7 +514,7 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
     q = np.zeros(depth + 5, dtype=int)  # ranks of low-rank T per level (:299)
     TI = params["transinv"]
     import time as _time
-    for lev in range(depth, -1, -1):
+    for lev in range(depth, stop_level - 1, -1):
         _t0 = _time.perf_counter()
         colrad, rowrad = level_radii(lev)
         lay_prox = proxy_layers(lev, params)
@@ -567,7 +569,7 @@ def R_matrix(J, k, T):
     return R
 
 
-def invert(tree, timings=None):
+def invert(tree, timings=None, stop_level=0):
     """HSS2D/HSS2D_invert_fsym.m:73-195 with Fopts='dense'.
 
     Leaf: FI = inv(K(I_src, I_src)) (:102-111).  Non-leaf: FI = inv([E1 K12; K12^T E2])
@@ -578,7 +580,7 @@ def invert(tree, timings=None):
     import time as _time
     out = Inverse()
     TI = params["transinv"]
-    for lev in range(tree.depth, -1, -1):
+    for lev in range(tree.depth, stop_level - 1, -1):
         _t0 = _time.perf_counter()
         boxes = [2 ** lev] if TI else range(2 ** lev, 2 ** (lev + 1))
         for b in boxes:
