@@ -108,9 +108,10 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
                      int64_t n, double* A, int64_t lda);
 
 /* Replaces HSS2D/HSS2D_build_tree_fsym.m:1 (INFO_TREE = HSS2D_build_tree_fsym(params)):
- * uniform tree, 'bdry' skeletons and proxies, interpolation operators T computed on the
- * GPU by least squares (rank-revealing randomized compression when interform is
- * HSSC_INTER_LOWRANK). */
+ * uniform tree, 'bdry' skeletons and proxies (at every level, also for kh > 0), and the
+ * interpolation operators T = K(Xp,X_sk) \ K(Xp,X_rs) computed on the GPU by Householder
+ * least squares and stored dense (the 'lowrank' INTERform of the reference stores the same
+ * operator to tolerance as T.U * T.V.').  Complex (SL_H_2D) trees hold complex T. */
 int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree);
 
 /* Geometry and index sets only (no interpolation operators, no GPU work).  Used by the
@@ -131,7 +132,8 @@ int hssc_tree_get_J(const hssc_tree* t, int64_t box, int64_t* out);
 /* Proxy points of box b (or its level), M x 2 column-major. */
 int hssc_tree_get_proxy(const hssc_tree* t, int64_t box, double* out);
 /* Interpolation operator of box b (or its level) as a dense k x (n-k) column-major
- * matrix (T = T.U * T.V.' when stored low-rank). */
+ * matrix (T = T.U * T.V.' in the reference's low-rank form); interleaved complex doubles
+ * for SL_H_2D. */
 int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out);
 void hssc_tree_destroy(hssc_tree* t);
 
@@ -141,7 +143,8 @@ int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss);
 /* Replaces HSS2D/HSS2D_invert_fsym.m:1 with params.Fopts = 'dense' (dense F^-1 and E per
  * box, or per level when transinv).  precision = HSSC_PREC_FP64 or HSSC_PREC_FP32. */
 int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** inv);
-/* F^-1 (n x n, rows/cols in I_src order) and E (k x k) of box b (or its level). */
+/* F^-1 (n x n, rows/cols in I_src order) and E (k x k) of box b (or its level), as doubles
+ * or interleaved complex doubles (FP32 factorizations are widened). */
 int hssc_inverse_get_FI(const hssc_inverse* inv, int64_t box, double* out);
 int hssc_inverse_get_E(const hssc_inverse* inv, int64_t box, double* out);
 void hssc_inverse_destroy(hssc_inverse* inv);
@@ -152,13 +155,14 @@ void hssc_operator_destroy(hssc_operator* op);
 
 /* Replaces HSS2D/HSS2D_applyinv_TI.m:1 and HSS2D/HSS2D_applyinv_fsym.m:1
  * (Y = HSS2D_applyinv_*(INFO_TREE,A_HSS,X,params)).  X, Y: N x m column-major host
- * arrays in the X_source ordering. */
+ * arrays in the X_source ordering; interleaved complex doubles (2 per entry) for SL_H_2D. */
 int hssc_hss2d_applyinv(const hssc_inverse* inv, const double* X, double* Y, int64_t m);
 
 /* Replaces HSS2D/HSS2D_apply_TI.m:1 and HSS2D/HSS2D_apply_fsym.m:1. */
 int hssc_hss2d_apply(const hssc_operator* op, const double* X, double* Y, int64_t m);
 
-/* Device-pointer variants (X, Y already in device memory; stream = hipStream_t or 0). */
+/* Device-pointer variants (X, Y already in device memory, double or interleaved complex
+ * double as above; stream = hipStream_t or 0). */
 int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY, int64_t m,
                                void* stream);
 int hssc_hss2d_apply_device(const hssc_operator* op, const void* dX, void* dY, int64_t m,
