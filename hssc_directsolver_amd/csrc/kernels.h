@@ -63,7 +63,7 @@ template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
 
 constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
-constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (QR_NT * QR_RS)
+constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (512 x 16)
 
 template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,

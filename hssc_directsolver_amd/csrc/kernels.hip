@@ -189,8 +189,10 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
   constexpr bool CX = is_complex_v<T>;
   constexpr int NP = CX ? 2 : 1;  // LDS planes per operand
   using v4 = typename Mfma<R>::v4;
-  __shared__ R As[NP][GB_K][GB_M + 1];  // As[p][kk][i] = plane p of op(A)(i0+i, k0+kk)
-  __shared__ R Bs[NP][GB_K][GB_N + 1];  // Bs[p][kk][j] = plane p of op(B)(k0+kk, j0+j)
+  // two buffers of As[p][kk][i] = plane p of op(A)(i0+i, k0+kk) and
+  // Bs[p][kk][j] = plane p of op(B)(k0+kk, j0+j)
+  __shared__ R As[2][NP][GB_K][GB_M + 1];
+  __shared__ R Bs[2][NP][GB_K][GB_N + 1];
   // blockIdx.z = e * ksplit + q: batch element e, K chunk q (rows [q*kchunk, ...) of op(B)
   // and columns of op(A)); chunk q writes its own output slice C + (e*ksplit + q)*sC.
   const int64_t z = blockIdx.z;
@@ -218,8 +220,10 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
       for (int y = 0; y < 2; ++y) acc[p][x][y] = v4{0, 0, 0, 0};
 
-  for (int64_t k0 = 0; k0 < kdim; k0 += GB_K) {
-    // Load op(A) tile (64 x 16): 1024 elements, 4 per thread.
+  // Software pipeline: the next K tile is loaded into registers while the MFMAs run on the
+  // current LDS buffer, then written to the other buffer (one barrier per tile).
+  T ra[4], rb[4];
+  auto gload = [&](int64_t k0) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int idx = tid + t * 256;
@@ -230,10 +234,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
         kk = idx & 15, ii = idx >> 4;  // contiguous along k
       }
       const int64_t gi = i0 + ii, gk = k0 + kk;
-      T v = T(0);
-      if (gi < m && gk < kdim) v = TA ? A[gk + gi * lda] : A[gi + gk * lda];
-      As[0][kk][ii] = re_s(v);
-      if constexpr (CX) As[NP - 1][kk][ii] = CA ? -im_s(v) : im_s(v);
+      ra[t] = (gi < m && gk < kdim) ? (TA ? A[gk + gi * lda] : A[gi + gk * lda]) : T(0);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -245,12 +246,32 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
         jj = idx & 63, kk = idx >> 6;  // contiguous along j
       }
       const int64_t gj = j0 + jj, gk = k0 + kk;
-      T v = T(0);
-      if (gj < n && gk < kdim) v = TB ? B[gj + gk * ldb] : B[gk + gj * ldb];
-      Bs[0][kk][jj] = re_s(v);
-      if constexpr (CX) Bs[NP - 1][kk][jj] = im_s(v);
+      rb[t] = (gj < n && gk < kdim) ? (TB ? B[gj + gk * ldb] : B[gk + gj * ldb]) : T(0);
     }
-    __syncthreads();
+  };
+  auto lstore = [&](int bf) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int idx = tid + t * 256;
+      const int ii = !TA ? (idx & 63) : (idx >> 4), kk = !TA ? (idx >> 6) : (idx & 15);
+      As[bf][0][kk][ii] = re_s(ra[t]);
+      if constexpr (CX) As[bf][NP - 1][kk][ii] = CA ? -im_s(ra[t]) : im_s(ra[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int idx = tid + t * 256;
+      const int jj = !TB ? (idx >> 4) : (idx & 63), kk = !TB ? (idx & 15) : (idx >> 6);
+      Bs[bf][0][kk][jj] = re_s(rb[t]);
+      if constexpr (CX) Bs[bf][NP - 1][kk][jj] = im_s(rb[t]);
+    }
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (int64_t k0 = 0; k0 < kdim; k0 += GB_K) {
+    const bool more = k0 + GB_K < kdim;
+    if (more) gload(k0 + GB_K);
 #pragma unroll
     for (int kk = 0; kk < GB_K; kk += 4) {
       const int kq = kk + (lane >> 4);
@@ -258,9 +279,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
 #pragma unroll
-        for (int x = 0; x < 2; ++x) av[p][x] = As[p][kq][wi + x * 16 + (lane & 15)];
+        for (int x = 0; x < 2; ++x) av[p][x] = As[cur][p][kq][wi + x * 16 + (lane & 15)];
 #pragma unroll
-        for (int y = 0; y < 2; ++y) bv[p][y] = Bs[p][kq][wj + y * 16 + (lane & 15)];
+        for (int y = 0; y < 2; ++y) bv[p][y] = Bs[cur][p][kq][wj + y * 16 + (lane & 15)];
       }
 #pragma unroll
       for (int x = 0; x < 2; ++x)
@@ -274,7 +295,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
           }
         }
     }
+    if (more) lstore(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
   // D(jrow, icol) = sum_k B(k, j) A(i, k): the D row of item r is the C column, the D
   // column lane%16 is the C row.
@@ -564,8 +587,7 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 // the identity and this is dgeqr2 / dlarft.
 
 constexpr int QR_NB = 32;  // outer panel width (the trailing update runs as GEMMs)
-constexpr int QR_RS = 16;  // row slots per thread: M - j0 <= QR_NT * QR_RS
-constexpr int QR_NT = 512; // threads per panel workgroup (8 waves)
+constexpr int QR_RS = 16;  // row slots per thread: M - j0 <= NT * QR_RS
 // inner block held in registers and reflectors accumulated per reduction pass: half as
 // many for complex data (twice the registers per entry)
 template <typename T>
@@ -583,52 +605,33 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Sums NV per-thread values over the whole (QR_NT-thread) workgroup: wave shuffles, then
-// one LDS pass.  out[v] is valid in every thread after the call.
-template <typename T, int NV>
-__device__ __forceinline__ void qr_block_reduce(T (&vals)[NV], T (*red)[NV], T* out, int tid) {
-  const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) vals[v] = wave_sum(vals[v]);
-  if (lane == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) red[wave][v] = vals[v];
-  }
-  __syncthreads();
-  if (tid < NV) {
-    T sum = T(0);
-    for (int w = 0; w < QR_NT / 64; ++w) sum += red[w][tid];
-    out[tid] = sum;
-  }
-  __syncthreads();
-}
-
 // Left-looking panel: for each inner block of IB columns, (1) apply the panel's earlier
 // reflectors through the compact-WY factor (x -= V Tf^H V^H x), (2) factor the block in
 // registers with one fused reduction per column (norm, dot products and the diagonal row
-// broadcast through double-buffered LDS, one barrier), (3) extend the Gram matrix V^H V and
-// the triangular factor Tf by the block's columns.  Row r = j0 + tid + QR_NT*s of the
-// panel lives in slot s of thread tid.
-template <typename T>
-__global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, int jb,
-                                                        T* __restrict__ A, int64_t lda, int64_t sA,
-                                                        T* __restrict__ Tfo, int64_t sT) {
+// broadcast through double-buffered LDS, one barrier), (3) extend the Gram matrix V^H V by
+// the block's columns and (4) the triangular factor Tf.  Row r = j0 + tid + NT*s of the
+// panel lives in slot s of thread tid.  The reductions of (1) and (3) run over all earlier
+// reflectors with wave shuffles into per-wave LDS partials and a single barrier per phase.
+// NT = 64 (one wave, M - j0 <= 64 RS = 1024) serves the many small boxes of the low levels;
+// NT = 512 (M - j0 <= 8192) the few large ones.
+template <typename T, int NT, int RS>
+__global__ void __launch_bounds__(NT) qr_panel_kernel(int64_t M, int64_t j0, int jb,
+                                                     T* __restrict__ A, int64_t lda, int64_t sA,
+                                                     T* __restrict__ Tfo, int64_t sT) {
   using R = real_t<T>;
-  constexpr int IB = qr_ib<T>(), AC = qr_ac<T>();
+  constexpr int IB = qr_ib<T>(), AC = qr_ac<T>(), NW = NT / 64;
   __shared__ T Tf[QR_NB][QR_NB + 1];
   __shared__ T G[QR_NB][QR_NB + 1];
   __shared__ T W[QR_NB][IB];
   __shared__ T Wp[QR_NB][IB];
-  __shared__ T red[2][QR_NT / 64][IB];
+  __shared__ T part[NW][QR_NB][IB];  // per-wave partial sums of (1) and (3)
+  __shared__ T red[2][NW][IB];
   __shared__ T rowv[2][IB];
   __shared__ T tau_s[QR_NB];
-  __shared__ T tmp[QR_NB];
-  __shared__ T redW[QR_NT / 64][AC * IB];
-  __shared__ T GW[AC][IB];
   A += int64_t(blockIdx.x) * sA;
   Tfo += int64_t(blockIdx.x) * sT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int idx = tid; idx < QR_NB * (QR_NB + 1); idx += QR_NT) {
+  for (int idx = tid; idx < QR_NB * (QR_NB + 1); idx += NT) {
     (&Tf[0][0])[idx] = T(0);
     (&G[0][0])[idx] = T(0);
   }
@@ -636,28 +639,28 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
   // local row index l = r - j0 (int); the panel block A(j0 + l, j0 + c) is P[l + c*lda]
   T* P = A + j0 + j0 * lda;
   const int Ml = int(M - j0);
-  T x[QR_RS][IB];
+  T x[RS][IB];
   int buf = 0;
   for (int c0 = 0; c0 < jb; c0 += IB) {
     const int ib = (jb - c0 < IB) ? jb - c0 : IB;
     T* Pb = P + int64_t(c0) * lda;  // first column of the block
     // load the raw block
 #pragma unroll
-    for (int s = 0; s < QR_RS; ++s) {
-      const int l = tid + QR_NT * s;
+    for (int s = 0; s < RS; ++s) {
+      const int l = tid + NT * s;
 #pragma unroll
       for (int jj = 0; jj < IB; ++jj)
         x[s][jj] = (l < Ml && jj < ib) ? Pb[l + int64_t(jj) * lda] : T(0);
     }
-    // (1) W = V_prev^H x (AC reflectors per pass), W' = Tf^H W, x -= V_prev W'
+    // (1) W = V_prev^H x, W' = Tf^H W, x -= V_prev W'
     if (c0 > 0) {
       for (int a0 = 0; a0 < c0; a0 += AC) {
         T acc[AC * IB];
 #pragma unroll
         for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
 #pragma unroll
-        for (int s = 0; s < QR_RS; ++s) {
-          const int l = tid + QR_NT * s;
+        for (int s = 0; s < RS; ++s) {
+          const int l = tid + NT * s;
           if (l >= Ml || l < a0) continue;
 #pragma unroll
           for (int u = 0; u < AC; ++u) {
@@ -669,18 +672,35 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
             }
           }
         }
-        qr_block_reduce<T, AC * IB>(acc, redW, &W[a0][0], tid);
+#pragma unroll
+        for (int v = 0; v < AC * IB; ++v) acc[v] = wave_sum(acc[v]);
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < AC; ++u)
+            if (a0 + u < c0) {
+#pragma unroll
+              for (int jj = 0; jj < IB; ++jj) part[wave][a0 + u][jj] = acc[u * IB + jj];
+            }
+        }
       }
-      if (tid < c0 * IB) {
-        const int a = tid / IB, jj = tid % IB;
+      __syncthreads();
+      for (int idx = tid; idx < c0 * IB; idx += NT) {
+        const int a = idx / IB, jj = idx % IB;
+        T sum = T(0);
+        for (int w = 0; w < NW; ++w) sum += part[w][a][jj];
+        W[a][jj] = sum;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < c0 * IB; idx += NT) {
+        const int a = idx / IB, jj = idx % IB;
         T sum = T(0);
         for (int b = 0; b <= a; ++b) sum += conj_s(Tf[b][a]) * W[b][jj];
         Wp[a][jj] = sum;
       }
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < QR_RS; ++s) {
-        const int l = tid + QR_NT * s;
+      for (int s = 0; s < RS; ++s) {
+        const int l = tid + NT * s;
         if (l >= Ml) continue;
         const int amax = (c0 < l + 1) ? c0 : l + 1;  // v_a(l) = 0 for l < a
         for (int a = 0; a < amax; ++a) {
@@ -695,14 +715,14 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
     for (int c = 0; c < IB; ++c) {
       if (c < ib) {
         const int ld = c0 + c;  // local row of the diagonal
-        const int own = ld % QR_NT, so = ld / QR_NT;
+        const int own = ld % NT, so = ld / NT;
         R sig = R(0);
         T d[IB];
 #pragma unroll
         for (int jj = 0; jj < IB; ++jj) d[jj] = T(0);
 #pragma unroll
-        for (int s = 0; s < QR_RS; ++s) {
-          const int l = tid + QR_NT * s;
+        for (int s = 0; s < RS; ++s) {
+          const int l = tid + NT * s;
           if (l >= ld && l < Ml) {
             const T xc = x[s][c];
             if (l > ld) sig += abs2(xc);
@@ -720,7 +740,7 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
         }
         if (tid == own) {
 #pragma unroll
-          for (int s = 0; s < QR_RS; ++s)
+          for (int s = 0; s < RS; ++s)
             if (s == so) {
               rowv[buf][0] = x[s][c];
 #pragma unroll
@@ -731,7 +751,7 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
         sig = R(0);
 #pragma unroll
         for (int jj = c + 1; jj < IB; ++jj) d[jj] = T(0);
-        for (int w = 0; w < QR_NT / 64; ++w) {
+        for (int w = 0; w < NW; ++w) {
           sig += re_s(red[buf][w][0]);
 #pragma unroll
           for (int jj = c + 1; jj < IB; ++jj) d[jj] += red[buf][w][jj];
@@ -758,8 +778,8 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
           tw[jj] = conj_s(tau) * (arow + conj_s(scal) * (d[jj] - conj_s(alpha) * arow));
         }
 #pragma unroll
-        for (int s = 0; s < QR_RS; ++s) {
-          const int l = tid + QR_NT * s;
+        for (int s = 0; s < RS; ++s) {
+          const int l = tid + NT * s;
           if (l >= ld && l < Ml) {
             const T v = (l == ld) ? T(1) : x[s][c] * scal;
 #pragma unroll
@@ -773,25 +793,24 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
     }
     // store the block
 #pragma unroll
-    for (int s = 0; s < QR_RS; ++s) {
-      const int l = tid + QR_NT * s;
+    for (int s = 0; s < RS; ++s) {
+      const int l = tid + NT * s;
       if (l < Ml) {
 #pragma unroll
         for (int jj = 0; jj < IB; ++jj)
           if (jj < ib) Pb[l + int64_t(jj) * lda] = x[s][jj];
       }
     }
-    __syncthreads();
     // (3) Gram columns G(a, c0+c) = v_a^H v_{c0+c} for a < c0+c, accumulated from the
     // block's registers (v_c: 0 above its diagonal, 1 on it, x below) against the earlier
-    // reflectors (AC per pass) and the block itself.
+    // reflectors and the block itself.
     for (int a0 = 0; a0 < c0 + IB; a0 += AC) {
       T acc[AC * IB];
 #pragma unroll
       for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
 #pragma unroll
-      for (int s = 0; s < QR_RS; ++s) {
-        const int l = tid + QR_NT * s;
+      for (int s = 0; s < RS; ++s) {
+        const int l = tid + NT * s;
         if (l >= Ml || l < c0) continue;
         T vb[IB];
 #pragma unroll
@@ -815,29 +834,46 @@ __global__ void __launch_bounds__(QR_NT) qr_panel_kernel(int64_t M, int64_t j0, 
           for (int c = 0; c < IB; ++c) acc[u * IB + c] += va * vb[c];
         }
       }
-      qr_block_reduce<T, AC * IB>(acc, redW, &GW[0][0], tid);
-      if (tid < AC * IB) {
-        const int u = tid / IB, c = tid % IB;
-        const int a = a0 + u, ci = c0 + c;
-        if (c < ib && a < ci) G[a][ci] = GW[u][c];
+#pragma unroll
+      for (int v = 0; v < AC * IB; ++v) acc[v] = wave_sum(acc[v]);
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < AC; ++u)
+          if (a0 + u < c0 + IB && a0 + u < QR_NB) {
+#pragma unroll
+            for (int c = 0; c < IB; ++c) part[wave][a0 + u][c] = acc[u * IB + c];
+          }
       }
-      __syncthreads();
     }
-    // (4) Tf(0:i, i) = -tau_i Tf(0:i, 0:i) G(0:i, i), Tf(i, i) = tau_i
-    for (int c = 0; c < ib; ++c) {
-      const int i = c0 + c;
-      if (tid < i) {
+    __syncthreads();
+    for (int idx = tid; idx < (c0 + IB) * IB; idx += NT) {
+      const int a = idx / IB, c = idx % IB;
+      const int ci = c0 + c;
+      if (c < ib && a < ci) {
         T sum = T(0);
-        for (int q = tid; q < i; ++q) sum += Tf[tid][q] * G[q][i];
-        tmp[tid] = sum;
+        for (int w = 0; w < NW; ++w) sum += part[w][a][c];
+        G[a][ci] = sum;
       }
-      __syncthreads();
-      if (tid < i) Tf[tid][i] = -(tau_s[i] * tmp[tid]);
-      if (tid == i) Tf[i][i] = tau_s[i];
-      __syncthreads();
     }
+    __syncthreads();
+    // (4) Tf(0:i, i) = -tau_i Tf(0:i, 0:i) G(0:i, i), Tf(i, i) = tau_i, one column after the
+    // other by wave 0 (its LDS accesses complete in program order)
+    if (wave == 0) {
+      for (int c = 0; c < ib; ++c) {
+        const int i = c0 + c;
+        if (lane < i) {
+          T sum = T(0);
+          for (int q = lane; q < i; ++q) sum += Tf[lane][q] * G[q][i];
+          Tf[lane][i] = -(tau_s[i] * sum);
+        } else if (lane == i) {
+          Tf[i][i] = tau_s[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __syncthreads();
   }
-  for (int idx = tid; idx < QR_NB * QR_NB; idx += QR_NT) {
+  for (int idx = tid; idx < QR_NB * QR_NB; idx += NT) {
     const int r = idx % QR_NB, i = idx / QR_NB;
     Tfo[r + i * QR_NB] = Tf[r][i];
   }
@@ -902,11 +938,17 @@ template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
               T* Tf, int64_t sT, int64_t batch) {
   const double rows = double(M - j0);
-  HSSC_TIMED(lbl<T>("qr_panel").c_str(), st,
-             (is_complex_v<T> ? 16.0 : 4.0) * rows * jb * jb * batch,
-             2.0 * rows * jb * sizeof(T) * batch,
-             hipLaunchKernelGGL(qr_panel_kernel<T>, dim3(batch), dim3(QR_NT), 0, st, M, j0, jb, A,
-                                lda, sA, Tf, sT));
+  const double fl = (is_complex_v<T> ? 16.0 : 4.0) * rows * jb * jb * batch;
+  const double by = 2.0 * rows * jb * sizeof(T) * batch;
+  if (M - j0 <= 64 * QR_RS) {
+    HSSC_TIMED(lbl<T>("qr_panel").c_str(), st, fl, by,
+               hipLaunchKernelGGL((qr_panel_kernel<T, 64, QR_RS>), dim3(batch), dim3(64), 0, st,
+                                  M, j0, jb, A, lda, sA, Tf, sT));
+  } else {
+    HSSC_TIMED(lbl<T>("qr_panel").c_str(), st, fl, by,
+               hipLaunchKernelGGL((qr_panel_kernel<T, 512, QR_RS>), dim3(batch), dim3(512), 0, st,
+                                  M, j0, jb, A, lda, sA, Tf, sT));
+  }
 }
 
 template <typename T>
