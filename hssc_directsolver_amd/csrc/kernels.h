@@ -58,9 +58,12 @@ void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T*
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                     int64_t batch, T* part, int64_t kchunk);
 
-// In-place inverse of a batch of n x n matrices.
+// In-place inverse of a batch of n x n matrices.  aux (optional) is a second stream on
+// which the blocked sweep inverts the next diagonal block while st finishes the current
+// step; all work is ordered back onto st before the call returns.
 template <typename T>
-void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
+void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
+                    hipStream_t aux = nullptr);
 
 constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
 constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (512 x 16)

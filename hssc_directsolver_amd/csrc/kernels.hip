@@ -426,10 +426,11 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
       v[s][t] = (i < n && j < n) ? A[i + int64_t(j) * lda] : T(0);
     }
   auto publish = [&](int c) {
-    // called by the wave owning column c
+    // called by the wave owning column c.  Candidates are ranked by a 64-bit key: the
+    // single-precision magnitude (non-negative floats order like their bit patterns) above
+    // the complemented row, so the largest magnitude wins and ties go to the lowest row.
     const int tc = c - wave * CB;
-    R best = R(-1);
-    int bi = n;
+    unsigned long long best = 0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       T x = T(0);
@@ -438,20 +439,24 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
         if (t == tc) x = v[s][t];
       const int i = lane + 64 * s;
       if (i < n) colbuf[c & 1][i] = x;
-      const R ax = abs1(x);
-      if (i >= c && i < n && ax > best) best = ax, bi = i;
+      if (i >= c && i < n) {
+        const unsigned long long key =
+            (static_cast<unsigned long long>(__float_as_uint(float(abs1(x)))) << 32) |
+            static_cast<unsigned>(0xffffffffu - unsigned(i));
+        if (key > best) best = key;
+      }
     }
     for (int o = 32; o > 0; o >>= 1) {
-      const R ob = shfl_xor_s(best, o);
-      const int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
+      const unsigned long long ob = __shfl_xor(best, o);
+      if (ob > best) best = ob;
     }
+    const int bi = int(0xffffffffu - unsigned(best & 0xffffffffu));
     if (lane == 0) spiv[c & 1] = bi, pivs[c] = bi;
   };
   if (wave == 0) publish(0);
   for (int c = 0; c < n; ++c) {
     __syncthreads();
-    const int p = spiv[c & 1];
+    const int p = __builtin_amdgcn_readfirstlane(spiv[c & 1]);
     const T* cb = colbuf[c & 1];
     const T piv = cb[p];
     const T inv = T(1) / piv;
@@ -466,8 +471,8 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
         if (s == sc) xc = v[s][t];
         if (s == sp) xp = v[s][t];
       }
-      rc[t] = shfl_s(xc, lc);
-      rp[t] = shfl_s(xp, lp);
+      rc[t] = readlane_s(xc, lc);
+      rp[t] = readlane_s(xp, lp);
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -536,17 +541,31 @@ void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, i
 constexpr int GJ_NB = 64;
 
 template <typename T>
-void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
+void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
+                int64_t batch) {
   // p advances in steps of 64 = the GEMM tile, so no 64-row tile straddles block B; the
   // tile-skip ranges of gemm_skip enumerate the tiles of [0,p) U [p+nb,n).  Steps (2) and
   // (4) write in place: each output tile is produced by one workgroup that reads all of
   // its inputs (K = nb <= 64 fits one tile) before it stores.
+  // Look-ahead (when an aux stream is given): step (3) first updates the row block B' of
+  // the next step; the inverse of A_B'B' then runs on the aux stream while the rest of (3)
+  // and (4) run on st.  Neither touches A_B'B'; step p+1 waits for the inverse.
   static_assert(GJ_NB == GB_M && GJ_NB == GB_N, "block size must equal the GEMM tile");
   const std::string diag = lbl<T>("gj_diag");
+  hipEvent_t upd = nullptr, inv = nullptr;
+  if (aux) {
+    if (hipEventCreateWithFlags(&upd, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&inv, hipEventDisableTiming) != hipSuccess) {
+      if (upd) (void)hipEventDestroy(upd);
+      upd = inv = nullptr;
+      aux = nullptr;
+    }
+  }
+  gj_reg<T>(st, int(n < GJ_NB ? n : GJ_NB), A, lda, sA, 0, batch, diag);
   for (int64_t p = 0; p < n; p += GJ_NB) {
     const int nb = int(p + GJ_NB <= n ? GJ_NB : n - p);
     const int64_t nout = n - nb;
-    gj_reg<T>(st, nb, A, lda, sA, p + p * lda, batch, diag);
+    if (p > 0 && aux) (void)hipStreamWaitEvent(st, inv, 0);  // inverse of A_BB (aux stream)
     if (nout <= 0) continue;
     const int64_t tiles = (nout + 63) / 64;
     T* Apb = A + p;             // rows B
@@ -555,24 +574,42 @@ void gj_blocked(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_
     // (2) A_XB = -A_XB * A_BB  (m = n rows with skip, n = nb cols)
     gemm_skip<T>(st, false, false, n, nb, nb, T(-1), A + p * lda, lda, sA, Abb, lda, sA, T(0),
                  Abp, lda, sA, batch, tiles, 1, p, nb, INT64_MAX, 0);
-    // (3) A_XY += A_XB * A_BY
-    gemm_skip<T>(st, false, false, n, n, nb, T(1), Abp, lda, sA, Apb, lda, sA, T(1), A, lda, sA,
-                 batch, tiles, tiles, p, nb, p, nb);
+    // (3) A_XY += A_XB * A_BY, the next row block B' = [p+nb, p+nb+nb2) first
+    const int64_t q = p + nb;
+    const int nb2 = int(q < n ? (q + GJ_NB <= n ? GJ_NB : n - q) : 0);
+    if (nb2 > 0) {
+      gemm_skip<T>(st, false, false, q + nb2, n, nb, T(1), Abp, lda, sA, Apb, lda, sA, T(1), A,
+                   lda, sA, batch, 1, tiles, 0, q, p, nb);
+      if (aux) {
+        (void)hipEventRecord(upd, st);
+        (void)hipStreamWaitEvent(aux, upd, 0);
+        gj_reg<T>(aux, nb2, A, lda, sA, q + q * lda, batch, diag);
+        (void)hipEventRecord(inv, aux);
+      }
+    }
+    const int64_t rest = nout - nb2;  // rows [0,p) U [q+nb2,n)
+    if (rest > 0)
+      gemm_skip<T>(st, false, false, n, n, nb, T(1), Abp, lda, sA, Apb, lda, sA, T(1), A, lda, sA,
+                   batch, (rest + 63) / 64, tiles, p, nb + nb2, p, nb);
     // (4) A_BY = A_BB * A_BY
     gemm_skip<T>(st, false, false, nb, n, nb, T(1), Abb, lda, sA, Apb, lda, sA, T(0), Apb, lda,
                  sA, batch, 1, tiles, INT64_MAX, 0, p, nb);
+    if (nb2 > 0 && !aux) gj_reg<T>(st, nb2, A, lda, sA, q + q * lda, batch, diag);
   }
+  if (upd) (void)hipEventDestroy(upd);
+  if (inv) (void)hipEventDestroy(inv);
 }
 
 template <typename T>
-void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
+void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
+                    hipStream_t aux) {
   if (n <= 0 || batch <= 0) return;
   // register-resident GJ: n <= 128 real, n <= 64 complex (twice the registers per entry)
   if (n <= (is_complex_v<T> ? 64 : 128)) {
     gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
     return;
   }
-  gj_blocked<T>(st, n, A, lda, sA, batch);
+  gj_blocked<T>(st, aux, n, A, lda, sA, batch);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1250,7 +1287,8 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void gemm_skip<T>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, T, const T*,    \
                              int64_t, int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, \
                              int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t);        \
-  template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t);            \
+  template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
+                                  hipStream_t);                                                    \
   template void qr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \
                             int64_t);                                                              \
   template void qr_copyV<T>(hipStream_t, int64_t, int64_t, int, const T*, int64_t, int64_t, T*,    \

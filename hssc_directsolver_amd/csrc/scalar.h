@@ -135,4 +135,19 @@ __device__ inline cplx<R> shfl_xor_s(const cplx<R>& v, int m) {
   return cplx<R>(__shfl_xor(v.re, m), __shfl_xor(v.im, m));
 }
 
+// Broadcast of lane `l` (wave-uniform) through the scalar unit: v_readlane, no LDS traffic.
+__device__ inline float readlane_s(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double readlane_s(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), l);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <typename R>
+__device__ inline cplx<R> readlane_s(const cplx<R>& v, int l) {
+  return cplx<R>(readlane_s(v.re, l), readlane_s(v.im, l));
+}
+
 }  // namespace hssc

@@ -402,6 +402,9 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
     wmax = std::max(wmax, L.nkey * L.k * (L.n - L.k));
   }
   T* Wk = dalloc<T>(wmax);
+  // second stream for the look-ahead of the blocked Gauss-Jordan sweeps
+  hipStream_t aux = nullptr;
+  if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) aux = nullptr;
   for (int l = D; l >= 0; --l) {
     const Level& L = t->levels[l];
     const int64_t n = L.n, kk = L.k, r = n - kk, E = L.nkey;
@@ -417,7 +420,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
                        static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, ks);
     }
-    k::invert_batched<T>(st, n, FI, n, n * n, E);
+    k::invert_batched<T>(st, n, FI, n, n * n, E, aux);
     if (l > 0) {
       // EI = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T ;  E = inv(EI)
       T* EI = dalloc<T>(E * kk * kk);
@@ -433,10 +436,11 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
         k::gemm<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI, kk,
                    kk * kk, E);
       }
-      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E);
+      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux);
     }
   }
   HIPCHK(hipStreamSynchronize(st));
+  if (aux) (void)hipStreamDestroy(aux);
   HIPCHK(hipGetLastError());
   dfree(Wk);
 }
