@@ -281,16 +281,17 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   const k::KernelSpec ks = kernel_spec(t->p);
   const int64_t nb = k::kQrNb;
   const int64_t kchunk = 512;
-  // Levels are independent: each level's QR sequence runs on its own non-blocking stream so
-  // the few large boxes of the top levels overlap with the many small boxes below.
+  // Levels are independent.  Each level's QR sequence is cut into steps (block generation,
+  // one step per 32-column panel, one per back-substitution block); the levels are spread
+  // over kLevelStreams streams by longest-processing-time on an estimated cost and their
+  // steps are submitted round-robin, so no level waits behind another's whole sequence when
+  // streams share a hardware queue.
+  constexpr int kLevelStreams = 8;
   struct Ws {
     S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
   };
   std::vector<Ws> ws(t->depth + 1);
-  // (Instrumented runs keep the same streams, so their per-kernel event times match what a
-  // kernel trace of the uninstrumented run reports.)
-  const int nstreams = std::max(1, std::min(t->depth, 8));
-  std::vector<hipStream_t> streams(nstreams, nullptr);
+  std::vector<hipStream_t> streams(kLevelStreams, nullptr);
   HIPCHK(hipStreamSynchronize(st0));  // the tree upload is complete before the level streams start
   for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   auto cleanup = [&]() {
@@ -299,6 +300,8 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
     for (auto& w : ws) dfree(w.A), dfree(w.V), dfree(w.W1), dfree(w.W2), dfree(w.Tf), dfree(w.part);
   };
   try {
+    std::vector<std::vector<std::function<void(hipStream_t)>>> steps(t->depth + 1);
+    std::vector<double> cost(t->depth + 1, 0.0);
     for (int l = t->depth; l >= 1; --l) {
       Level& L = t->levels[l];
       const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
@@ -309,7 +312,6 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
       if (M > k::kQrMaxRows)
         throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy larger than the QR panel kernel supports")};
-      hipStream_t st = streams[(l - 1) % nstreams];
       Ws& w = ws[l];
       const int64_t sA = M * n;
       const int64_t nsplit = (M + kchunk - 1) / kchunk;
@@ -320,40 +322,67 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       w.Tf = dalloc<S>(E * nb * nb);
       w.part = dalloc<S>(E * nsplit * nb * n);
       S* A = w.A;
+      auto& sl = steps[l];
       // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
-      k::gen_block<S>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
-                      ks);
+      sl.push_back([=, &L](hipStream_t st) {
+        k::gen_block<S>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
+                        ks);
+      });
       for (int64_t j0 = 0; j0 < kk; j0 += nb) {
-        const int jb = int(std::min(nb, kk - j0));
-        k::qr_panel<S>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
-        const int64_t nc = n - j0 - jb;
-        if (nc <= 0) continue;
-        const int64_t Mp = M - j0;
-        k::qr_copyV<S>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
-        S* C = A + j0 + (j0 + jb) * M;
-        // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
-        k::gemm_tn_splitk<S>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
-                             w.part, kchunk);
-        k::gemm_h<S>(st, jb, nc, jb, S(1), w.Tf, nb, nb * nb, w.W1, jb, jb * nc, S(0), w.W2, jb,
-                     jb * nc, E);
-        k::gemm<S>(st, false, false, Mp, nc, jb, S(-1), w.V, Mp, Mp * nb, w.W2, jb, jb * nc, S(1),
-                   C, M, sA, E);
+        sl.push_back([=](hipStream_t st) {
+          const int jb = int(std::min(nb, kk - j0));
+          k::qr_panel<S>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
+          const int64_t nc = n - j0 - jb;
+          if (nc <= 0) return;
+          const int64_t Mp = M - j0;
+          k::qr_copyV<S>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
+          S* C = A + j0 + (j0 + jb) * M;
+          // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
+          k::gemm_tn_splitk<S>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
+                               w.part, kchunk);
+          k::gemm_h<S>(st, jb, nc, jb, S(1), w.Tf, nb, nb * nb, w.W1, jb, jb * nc, S(0), w.W2, jb,
+                       jb * nc, E);
+          k::gemm<S>(st, false, false, Mp, nc, jb, S(-1), w.V, Mp, Mp * nb, w.W2, jb, jb * nc,
+                     S(1), C, M, sA, E);
+        });
       }
       // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
       S* C1 = A + kk * M;
       const int64_t last = ((kk - 1) / nb) * nb;
       for (int64_t p = last; p >= 0; p -= nb) {
-        const int bsz = int(std::min(nb, kk - p));
-        k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
-        // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
-        k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
-                   C1 + p, M, sA, E);
-        if (p > 0)
-          k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1), C1,
-                     M, sA, E);
+        sl.push_back([=](hipStream_t st) {
+          const int bsz = int(std::min(nb, kk - p));
+          k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
+          // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
+          k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
+                     C1 + p, M, sA, E);
+          if (p > 0)
+            k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1),
+                       C1, M, sA, E);
+        });
       }
-      k::copy2d<S, S>(st, kk, r, C1, M, sA, dT, kk, kk * r, E);
+      sl.push_back([=](hipStream_t st) { k::copy2d<S, S>(st, kk, r, C1, M, sA, dT, kk, kk * r, E); });
+      // estimated cost: least-squares flops plus a latency term per panel row
+      const double npan = double((kk + nb - 1) / nb);
+      cost[l] = double(E) * (2.0 * M * kk * kk + 4.0 * M * kk * r) + npan * double(M) * 2e6;
     }
+    // longest processing time first onto the least loaded stream
+    std::vector<int> order;
+    for (int l = 1; l <= t->depth; ++l)
+      if (!steps[l].empty()) order.push_back(l);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<double> load(kLevelStreams, 0.0);
+    std::vector<int> lst(t->depth + 1, 0);
+    for (int l : order) {
+      const int sidx = int(std::min_element(load.begin(), load.end()) - load.begin());
+      lst[l] = sidx;
+      load[sidx] += cost[l];
+    }
+    size_t maxsteps = 0;
+    for (int l : order) maxsteps = std::max(maxsteps, steps[l].size());
+    for (size_t i = 0; i < maxsteps; ++i)
+      for (int l : order)
+        if (i < steps[l].size()) steps[l][i](streams[lst[l]]);
     for (auto& s : streams) HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
   } catch (...) {
