@@ -68,6 +68,22 @@ SIGNATURES = [
                                  C.c_int64, C.c_uint64, _I64P, _I64P, _DP, _DP]),
     ("hssc_hss1d_compress", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32, C.c_double,
                                       C.c_int32, C.POINTER(_P)]),
+    ("hssc_hss1d_compress_ranked", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32,
+                                             C.c_double, C.c_int32, C.c_int32, _P, C.c_int32,
+                                             C.POINTER(_P)]),
+    ("hssc_hss1d_box_features", C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
+    ("hssc_hss1d_id_stats", C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
+    ("hssc_mlp_create", C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    ("hssc_mlp_destroy", None, [_P]),
+    ("hssc_mlp_num_params", C.c_int, []),
+    ("hssc_mlp_get_params", C.c_int, [_P, C.POINTER(C.c_float)]),
+    ("hssc_mlp_set_params", C.c_int, [_P, C.POINTER(C.c_float)]),
+    ("hssc_mlp_set_scaling", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("hssc_mlp_predict", C.c_int, [_P, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_float)]),
+    ("hssc_mlp_grads", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, _DP,
+                                 C.POINTER(C.c_float)]),
+    ("hssc_mlp_train", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
+                                 C.c_int32, C.c_float, _DP]),
     ("hssc_hss1d_invert", C.c_int, [_P]),
     ("hssc_hss1d_applyinv", C.c_int, [_P, _DP, _DP, C.c_int64]),
     ("hssc_hss1d_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -384,6 +385,10 @@ struct hssc_hss1d {
   // per-box vector offsets for the sweeps
   std::vector<int64_t> vUp, vDn;
   int64_t vlen = 0;
+  // rank-predicted sketching of the IDs: per-box features, counters
+  double Rcurve = 1;
+  std::vector<std::array<float, 8>> feat;
+  int64_t id_boxes = 0, id_fallbacks = 0, id_sketch_rows = 0, id_full_rows = 0;
   ~hssc_hss1d() {
     dfree1(S), dfree1(I), dfree1(dx), dfree1(dy), dfree1(dnx), dfree1(dny), dfree1(dw),
         dfree1(dkap);
@@ -605,6 +610,248 @@ IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* 
   return out;
 }
 
+// ---------------------------------------------------------------------------------------
+// Rank-predicted sketching (randomized ID with the predicted rank as the sample count)
+// ---------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Y_b (s_b x n_b) = G_b A_b with G_b a random sign matrix (s_b x m_b).
+__global__ void __launch_bounds__(256) sketch1d_kernel(
+    const int64_t* __restrict__ off, const int* __restrict__ mr, const int* __restrict__ nc,
+    const double* __restrict__ A, const int64_t* __restrict__ yoff, const int* __restrict__ sr,
+    double* __restrict__ Y, uint64_t seed) {
+  const int b = blockIdx.y;
+  const int m = mr[b], n = nc[b], s = sr[b];
+  const double* M = A + off[b];
+  const uint64_t key = seed ^ (uint64_t(b) << 24);
+  for (int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x; idx < int64_t(s) * n;
+       idx += int64_t(gridDim.x) * 256) {
+    const int i = int(idx % s), c = int(idx / s);
+    double acc = 0;
+    for (int r = 0; r < m; ++r) {
+      const uint64_t h = mix64(key + 0x9E3779B97F4A7C15ull * (uint64_t(i) * uint64_t(m) + r + 1));
+      acc += (h & 1ull) ? M[r + int64_t(c) * m] : -M[r + int64_t(c) * m];
+    }
+    Y[yoff[b] + idx] = acc;
+  }
+}
+
+// Acceptance of a sketched ID on the full block A_b (m x n): with a random unit x over the
+// residual columns and u = T x, e = |A_s u - A_r x|; with a random unit g over all columns,
+// a = |A g|.  Accepted when the sketch was not saturated (k < s, or s covers min(m,n)) and
+// e <= 10 acc a.  One workgroup per block; scratch: m doubles per block at wk + woff[b].
+__global__ void __launch_bounds__(256) verify1d_kernel(
+    const int64_t* __restrict__ off, const int* __restrict__ mr, const int* __restrict__ nc,
+    const int* __restrict__ sr, const double* __restrict__ A, const int* __restrict__ perm,
+    const int64_t* __restrict__ poff, const int* __restrict__ kk, const double* __restrict__ T,
+    const int64_t* __restrict__ toff, double* __restrict__ wk, const int64_t* __restrict__ woff,
+    double acc, uint64_t seed, int* __restrict__ ok) {
+  __shared__ double red[4];
+  __shared__ double xs[256], us[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int m = mr[b], n = nc[b], s = sr[b], k = kk[b], r_ = n - k;
+  const double* M = A + off[b];
+  const int* P = perm + poff[b];
+  const double* Tb = T + toff[b];
+  double* y = wk + woff[b];
+  auto bsum = [&](double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+  };
+  auto gauss = [&](uint64_t key, uint64_t i) {
+    const double u1 = fmax(double(mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 1)) >> 11) *
+                               (1.0 / 9007199254740992.0), 1e-300);
+    const double u2 = double(mix64(key + 0x9E3779B97F4A7C15ull * (2 * i + 2)) >> 11) *
+                      (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  };
+  const uint64_t key = seed ^ (uint64_t(b) << 24) ^ 0x7777ull;
+  // x over the residual columns (n <= 256 here), u = T x
+  double v = 0;
+  if (tid < r_) v = xs[tid] = gauss(key, tid);
+  const double xn = sqrt(bsum(v * v));
+  if (tid < r_) xs[tid] /= xn;
+  __syncthreads();
+  if (tid < k) {
+    double t = 0;
+    for (int c = 0; c < r_; ++c) t += Tb[tid + int64_t(c) * k] * xs[c];
+    us[tid] = t;
+  }
+  __syncthreads();
+  double e2 = 0;
+  for (int r = tid; r < m; r += 256) {
+    double t = 0;
+    for (int i = 0; i < k; ++i) t += M[r + int64_t(P[i]) * m] * us[i];
+    for (int c = 0; c < r_; ++c) t -= M[r + int64_t(P[k + c]) * m] * xs[c];
+    e2 += t * t;
+  }
+  e2 = bsum(e2);
+  // g over all columns, a = |A g|
+  v = 0;
+  if (tid < n) v = xs[tid] = gauss(key ^ 0x5555ull, tid);
+  const double gn = sqrt(bsum(v * v));
+  if (tid < n) xs[tid] /= gn;
+  __syncthreads();
+  double a2 = 0;
+  for (int r = tid; r < m; r += 256) {
+    double t = 0;
+    for (int c = 0; c < n; ++c) t += M[r + int64_t(c) * m] * xs[c];
+    a2 += t * t;
+  }
+  a2 = bsum(a2);
+  if (tid == 0) {
+    const int full = (m < n ? m : n);
+    const bool saturated = (k >= s) && (s < full);
+    ok[b] = (!saturated && sqrt(e2) <= 10.0 * acc * sqrt(a2)) ? 1 : 0;
+    (void)y;
+  }
+}
+
+// Randomized ID of every block of R with s[i] sketch rows: CPQR-ID (acc mode) of the sketch
+// Y = G A, then the acceptance test on A.  ok[i] = 0 marks blocks to redo with the full ID.
+IdResult run_id_sketched(const Ragged& R, double* A, double acc, const std::vector<int>& s,
+                         uint64_t seed, std::vector<int>* ok,
+                         const std::vector<int>* kfix = nullptr) {
+  const int nm = int(R.off.size());
+  IdResult out;
+  ok->assign(nm, 0);
+  if (nm == 0) return out;
+  Ragged Y;
+  int64_t ytot = 0;
+  std::vector<int64_t> yoff(nm);
+  for (int i = 0; i < nm; ++i) {
+    yoff[i] = ytot;
+    Y.off.push_back(ytot), Y.m.push_back(s[i]), Y.n.push_back(R.n[i]);
+    ytot += int64_t(s[i]) * R.n[i];
+  }
+  Y.total = ytot;
+  std::vector<void*> tmp;
+  double* dY = dmalloc<double>(ytot);
+  tmp.push_back(dY);
+  int64_t* doff = dput(R.off);
+  int* dm = dput(R.m);
+  int* dn = dput(R.n);
+  int* ds = dput(s);
+  int64_t* dyo = dput(yoff);
+  tmp.insert(tmp.end(), {doff, dm, dn, ds, dyo});
+  int64_t mx = 1;
+  for (int i = 0; i < nm; ++i) mx = std::max<int64_t>(mx, int64_t(s[i]) * R.n[i]);
+  dim3 grid(unsigned(std::min<int64_t>((mx + 255) / 256, 64)), nm);
+  HSSC_TIMED("sketch1d_f64", 0, 0.0, 0.0,
+             hipLaunchKernelGGL(sketch1d_kernel, grid, dim3(256), 0, 0, doff, dm, dn, A, dyo, ds,
+                                dY, seed));
+  out = run_id(Y, dY, acc, kfix, seed ^ 0x3141ull);
+  // acceptance on the full blocks
+  std::vector<int64_t> poff(nm), toff(nm), woff(nm);
+  std::vector<int> kk(nm), pflat;
+  std::vector<double> tflat;
+  int64_t wt = 0;
+  for (int i = 0; i < nm; ++i) {
+    poff[i] = int64_t(pflat.size());
+    pflat.insert(pflat.end(), out.perm[i].begin(), out.perm[i].end());
+    toff[i] = int64_t(tflat.size());
+    tflat.insert(tflat.end(), out.T[i].begin(), out.T[i].end());
+    kk[i] = out.k[i];
+    woff[i] = wt, wt += R.m[i];
+  }
+  int* dp = dput(pflat);
+  int64_t* dpo = dput(poff);
+  int* dk = dput(kk);
+  double* dT = dput(tflat);
+  int64_t* dto = dput(toff);
+  double* dw = dmalloc<double>(wt);
+  int64_t* dwo = dput(woff);
+  int* dok = dmalloc<int>(nm);
+  tmp.insert(tmp.end(), {dp, dpo, dk, dT, dto, dw, dwo, dok});
+  hipLaunchKernelGGL(verify1d_kernel, dim3(nm), dim3(256), 0, 0, doff, dm, dn, ds, A, dp, dpo,
+                     dk, dT, dto, dw, dwo, acc, seed, dok);
+  H1CHK(hipDeviceSynchronize());
+  H1CHK(hipGetLastError());
+  *ok = dget(dok, nm);
+  for (void* p : tmp) dfree1(p);
+  return out;
+}
+
+// A view of some blocks of R (same device storage).
+Ragged subset(const Ragged& R, const std::vector<int>& idx) {
+  Ragged S;
+  for (int i : idx) S.off.push_back(R.off[i]), S.m.push_back(R.m[i]), S.n.push_back(R.n[i]);
+  S.total = R.total;
+  return S;
+}
+
+// ID of every block: blocks with s[i] > 0 try the sketched ID first; the others and the
+// rejected sketches run the full CPQR ID.  Adds to the counters of h.
+IdResult run_id_mixed(hssc_hss1d* h, const Ragged& R, double* A, double acc,
+                      const std::vector<int>& s, uint64_t seed,
+                      const std::vector<int>* kfix = nullptr) {
+  const int nm = int(R.off.size());
+  IdResult out;
+  out.k.resize(nm), out.perm.resize(nm), out.T.resize(nm);
+  std::vector<int> sk, full;
+  for (int i = 0; i < nm; ++i) (s[i] > 0 ? sk : full).push_back(i);
+  if (!sk.empty()) {
+    std::vector<int> ss, ok, kf;
+    for (int i : sk) ss.push_back(s[i]);
+    if (kfix)
+      for (int i : sk) kf.push_back((*kfix)[i]);
+    IdResult r = run_id_sketched(subset(R, sk), A, acc, ss, seed, &ok, kfix ? &kf : nullptr);
+    for (size_t j = 0; j < sk.size(); ++j) {
+      h->id_sketch_rows += ss[j];
+      if (ok[j]) {
+        const int i = sk[j];
+        out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = std::move(r.T[j]);
+      } else {
+        full.push_back(sk[j]);
+        ++h->id_fallbacks;
+      }
+    }
+  }
+  if (!full.empty()) {
+    std::sort(full.begin(), full.end());
+    std::vector<int> kf;
+    if (kfix)
+      for (int i : full) kf.push_back((*kfix)[i]);
+    IdResult r = run_id(subset(R, full), A, acc, kfix ? &kf : nullptr, seed);
+    for (size_t j = 0; j < full.size(); ++j) {
+      const int i = full[j];
+      h->id_full_rows += R.m[i];
+      out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = std::move(r.T[j]);
+    }
+  }
+  if (!kfix) h->id_boxes += nm;
+  return out;
+}
+
+// Sketch rows of a box from its predicted rank: the rank plus oversampling, rounded up to
+// the 16-row MFMA granule, capped by the block's size.
+int sketch_rows(double pred, int oversample, int m, int n) {
+  const int want = int(std::ceil(std::max(pred, 1.0))) + oversample;
+  const int bucket = ((want + 15) / 16) * 16;
+  return std::min(bucket, std::min(m, n));
+}
+
+// HSS2D/predict_rank.m:5-23: rank ~ m_p log10(n) - b_p (r0_p below n = 150) for
+// p = ceil(-log10 acc) clamped to [7, 12].
+double predict_rank_table(int n, double acc) {
+  static const int mt[6] = {96, 138, 184, 275, 526, 487};
+  static const int bt[6] = {138, 215, 300, 496, 1074, 954};
+  static const int r0[6] = {86, 104, 116, 130, 137, 140};
+  int p = int(std::ceil(-std::log10(acc)));
+  p = std::min(12, std::max(7, p));
+  const int i = p - 7;
+  if (n < 150) return std::min(n, r0[i]);
+  return std::min<double>(n, std::ceil(mt[i] * std::log10(double(n)) - bt[i]));
+}
+
 }  // namespace
 }  // namespace hssc
 
@@ -612,11 +859,10 @@ IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* 
 // C ABI
 // ---------------------------------------------------------------------------------------
 
-extern "C" {
-
-int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, const double* kappa,
-                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
-                        hssc_hss1d** out) {
+namespace {
+int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, const double* kappa,
+                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy, int rank_mode,
+                        hssc_mlp* mlp, int oversample, hssc_hss1d** out) {
   if (!Xc || !nrm || !w || !kappa || !out || N < 4 || nbox_max < 2 || !(acc > 0 && acc < 1) ||
       nproxy < 4)
     return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d_compress arguments");
@@ -637,6 +883,11 @@ int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, co
     if (h->nboxes < 3) throw HipErr{fail(HSSC_ERR_INVALID_ARG, "tree needs at least 2 levels")};
     neighbours(h);
     const int nb = h->nboxes;
+    {
+      double cx, cy;
+      circum(h, range_of(h, 1), &cx, &cy, &h->Rcurve);
+    }
+    h->feat.assign(nb + 1, {});
     h->isk_up.assign(nb + 1, {}), h->isk_dn.assign(nb + 1, {});
     h->J_up.assign(nb + 1, {}), h->J_dn.assign(nb + 1, {});
     h->nskel.assign(nb + 1, 0), h->kskel.assign(nb + 1, 0);
@@ -652,6 +903,7 @@ int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, co
         if (h->level[b] == lev) boxes.push_back(b);
       Ragged Rout, Rin;
       std::vector<std::vector<int>> outs(boxes.size()), inns(boxes.size());
+      std::vector<double> Rbox(boxes.size());
       for (size_t q = 0; q < boxes.size(); ++q) {
         const int b = boxes[q];
         std::vector<int> out, inn;
@@ -699,33 +951,83 @@ int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, co
         Rout.add(iin, zx, zy, out);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
         Rin.add(oin, zx, zy, inn);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
         outs[q] = out, inns[q] = inn;
+        Rbox[q] = R;
+        // geometric features of the box (inputs of the rank predictor)
+        double kmax = 0, ksum = 0;
+        for (int i = h->start[b]; i < h->start[b] + h->count[b]; ++i)
+          kmax = std::max(kmax, std::fabs(h->KAP[i])), ksum += std::fabs(h->KAP[i]);
+        auto& f = h->feat[b];
+        f[0] = float(lev);
+        f[1] = float(std::log2(double(h->count[b])));
+        f[2] = float(std::log2(double(out.size())));
+        f[3] = float(std::log2(R / h->Rcurve));
+        f[4] = float(kmax * R);
+        f[5] = float(ksum / h->count[b] * R);
+        f[6] = float(std::log2(double(Rout.m.back())));
+        f[7] = float(-std::log10(acc));
       }
       std::vector<void*> tmp;
       double* Aout = gen_ragged(h, Rout, 0, &tmp);
       double* Ain = gen_ragged(h, Rin, 1, &tmp);
       tmp.push_back(Aout), tmp.push_back(Ain);
       const uint64_t seed = 0x1234ull + uint64_t(lev) * 7919ull;
-      IdResult io = run_id(Rout, Aout, acc, nullptr, seed);
-      IdResult ii = run_id(Rin, Ain, acc, nullptr, seed ^ 0xABCDull);
+      const int nq = int(boxes.size());
+      // sketch rows per box from the predicted rank (0 = full ID)
+      std::vector<int> s_out(nq, 0), s_in(nq, 0);
+      if (rank_mode != 0) {
+        std::vector<double> pred(nq);
+        if (rank_mode == 1) {
+          for (int q = 0; q < nq; ++q) pred[q] = predict_rank_table(int(outs[q].size()), acc);
+        } else {
+          std::vector<float> Xf(size_t(nq) * 8), y(nq);
+          for (int q = 0; q < nq; ++q)
+            for (int c = 0; c < 8; ++c) Xf[q + size_t(c) * nq] = h->feat[boxes[q]][c];
+          if (int st = hssc_mlp_predict(mlp, Xf.data(), nq, y.data())) throw HipErr{st};
+          for (int q = 0; q < nq; ++q) pred[q] = std::exp2(double(y[q]));
+        }
+        for (int q = 0; q < nq; ++q) {
+          if (Rout.n[q] <= 256) {
+            s_out[q] = sketch_rows(pred[q], oversample, Rout.m[q], Rout.n[q]);
+            if (s_out[q] >= std::min(Rout.m[q], Rout.n[q])) s_out[q] = 0;
+          }
+          if (Rin.n[q] <= 256) {
+            s_in[q] = sketch_rows(pred[q], oversample, Rin.m[q], Rin.n[q]);
+            if (s_in[q] >= std::min(Rin.m[q], Rin.n[q])) s_in[q] = 0;
+          }
+        }
+      }
+      IdResult io = run_id_mixed(h, Rout, Aout, acc, s_out, seed);
+      IdResult ii = run_id_mixed(h, Rin, Ain, acc, s_in, seed ^ 0xABCDull);
       // rank equalisation (HSS1D_compress_nsym.m:190-196): the smaller side is redone with
       // the fixed rank k = max(k_out, k_in)
-      std::vector<int> kf(boxes.size());
-      bool need_out = false, need_in = false;
-      for (size_t q = 0; q < boxes.size(); ++q) {
+      std::vector<int> kf(nq);
+      std::vector<int> redo_out, redo_in;
+      for (int q = 0; q < nq; ++q) {
         kf[q] = std::max(io.k[q], ii.k[q]);
-        need_out |= kf[q] != io.k[q];
-        need_in |= kf[q] != ii.k[q];
+        if (kf[q] != io.k[q]) redo_out.push_back(q);
+        if (kf[q] != ii.k[q]) redo_in.push_back(q);
       }
-      if (need_out) {
-        IdResult f = run_id(Rout, Aout, acc, &kf, seed);
-        for (size_t q = 0; q < boxes.size(); ++q)
-          if (io.k[q] != kf[q]) io.k[q] = f.k[q], io.perm[q] = f.perm[q], io.T[q] = f.T[q];
-      }
-      if (need_in) {
-        IdResult f = run_id(Rin, Ain, acc, &kf, seed ^ 0xABCDull);
-        for (size_t q = 0; q < boxes.size(); ++q)
-          if (ii.k[q] != kf[q]) ii.k[q] = f.k[q], ii.perm[q] = f.perm[q], ii.T[q] = f.T[q];
-      }
+      auto redo = [&](const Ragged& Rr, double* Ar, const std::vector<int>& sq,
+                      const std::vector<int>& idx, IdResult* res, uint64_t sd) {
+        if (idx.empty()) return;
+        std::vector<int> ks, ss;
+        for (int q : idx) {
+          ks.push_back(kf[q]);
+          int sr = 0;
+          if (sq[q] > 0) {
+            sr = sketch_rows(double(kf[q]), oversample, Rr.m[q], Rr.n[q]);
+            if (sr >= std::min(Rr.m[q], Rr.n[q]) || sr <= kf[q]) sr = 0;
+          }
+          ss.push_back(sr);
+        }
+        IdResult f = run_id_mixed(h, subset(Rr, idx), Ar, acc, ss, sd, &ks);
+        for (size_t j = 0; j < idx.size(); ++j) {
+          const int q = idx[j];
+          res->k[q] = f.k[j], res->perm[q] = std::move(f.perm[j]), res->T[q] = std::move(f.T[j]);
+        }
+      };
+      redo(Rout, Aout, s_out, redo_out, &io, seed);
+      redo(Rin, Ain, s_in, redo_in, &ii, seed ^ 0xABCDull);
       for (void* p : tmp) dfree1(p);
       for (size_t q = 0; q < boxes.size(); ++q) {
         const int b = boxes[q], k = kf[q];
@@ -807,6 +1109,41 @@ int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, co
     delete h;
     return fail(HSSC_ERR_OUT_OF_MEMORY, "host allocation failed");
   }
+}
+}  // namespace
+
+extern "C" {
+
+int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, const double* kappa,
+                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
+                        hssc_hss1d** out) {
+  return hss1d_compress_impl(Xc, nrm, w, kappa, N, nbox_max, acc, nproxy, 0, nullptr, 0, out);
+}
+
+int hssc_hss1d_compress_ranked(const double* Xc, const double* nrm, const double* w,
+                               const double* kappa, int64_t N, int32_t nbox_max, double acc,
+                               int32_t nproxy, int32_t rank_mode, hssc_mlp* mlp,
+                               int32_t oversample, hssc_hss1d** out) {
+  if (rank_mode < 0 || rank_mode > 2 || (rank_mode == 2 && !mlp) || oversample < 0)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid rank prediction arguments");
+  return hss1d_compress_impl(Xc, nrm, w, kappa, N, nbox_max, acc, nproxy, rank_mode, mlp,
+                             oversample, out);
+}
+
+int hssc_hss1d_box_features(const hssc_hss1d* h, int32_t box, float* f) {
+  if (!h || !f || box < 2 || box > h->nboxes) return fail(HSSC_ERR_INVALID_ARG, "bad box");
+  for (int c = 0; c < 8; ++c) f[c] = h->feat[box][c];
+  return HSSC_OK;
+}
+
+int hssc_hss1d_id_stats(const hssc_hss1d* h, int64_t* boxes, int64_t* fallbacks,
+                        int64_t* sketch_rows, int64_t* full_rows) {
+  if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
+  if (boxes) *boxes = h->id_boxes;
+  if (fallbacks) *fallbacks = h->id_fallbacks;
+  if (sketch_rows) *sketch_rows = h->id_sketch_rows;
+  if (full_rows) *full_rows = h->id_full_rows;
+  return HSSC_OK;
 }
 
 int hssc_hss1d_invert(hssc_hss1d* h) {

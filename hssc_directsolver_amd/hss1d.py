@@ -81,9 +81,30 @@ class HSS1D:
                                    dn.ctypes.data_as(C.POINTER(C.c_int64))))
         return dict(level=lev.value, nskel=n.value, kskel=k.value, isk_up=up, isk_dn=dn)
 
+    def features(self, b):
+        """Geometric features of box b fed to the rank network (hssc_hss1d_box_features)."""
+        f = np.empty(8, dtype=np.float32)
+        check(lib().hssc_hss1d_box_features(self.handle, int(b),
+                                            f.ctypes.data_as(C.POINTER(C.c_float))))
+        return f
 
-def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64) -> HSS1D:
-    """HSS1D/HSS1D_compress_nsym.m:1 with type 'green'."""
+    def id_stats(self):
+        """(blocks, fallbacks to the plain ID, sketch rows, rows factored by the plain ID)."""
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().hssc_hss1d_id_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("blocks", "fallbacks", "sketch_rows", "full_rows"), (x.value for x in v)))
+
+
+RANK_MODES = {"id": 0, "table": 1, "mlp": 2}
+
+
+def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64, rank="id", mlp=None,
+                        oversample=8) -> HSS1D:
+    """HSS1D/HSS1D_compress_nsym.m:1 with type 'green'.
+
+    ``rank`` picks how the IDs are sized: 'id' runs the plain ID on every block; 'table'
+    sketches each block with the closed-form rank guess of HSS2D/predict_rank.m; 'mlp' with
+    the prediction of ``mlp`` (a RankMLP).  Sketches that fail fall back to the plain ID."""
     if type_ != "green":
         raise ValueError("only the 'green' (proxy) compression is implemented")
     X = np.asfortranarray(curve.X, dtype=np.float64)
@@ -91,9 +112,10 @@ def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64) -> HSS1D:
     w = np.ascontiguousarray(curve.w, dtype=np.float64)
     k = np.ascontiguousarray(curve.kappa, dtype=np.float64)
     h = C.c_void_p()
-    check(lib().hssc_hss1d_compress(X.ctypes.data_as(_DP), Nm.ctypes.data_as(_DP),
-                                    w.ctypes.data_as(_DP), k.ctypes.data_as(_DP), X.shape[0],
-                                    int(nbox_max), float(acc), int(nproxy), C.byref(h)))
+    check(lib().hssc_hss1d_compress_ranked(
+        X.ctypes.data_as(_DP), Nm.ctypes.data_as(_DP), w.ctypes.data_as(_DP),
+        k.ctypes.data_as(_DP), X.shape[0], int(nbox_max), float(acc), int(nproxy),
+        RANK_MODES[rank], mlp.handle if mlp is not None else None, int(oversample), C.byref(h)))
     return HSS1D(h, X.shape[0])
 
 

@@ -206,6 +206,47 @@ typedef struct hssc_hss1d hssc_hss1d;
 int hssc_hss1d_compress(const double* X, const double* nrm, const double* w, const double* kappa,
                         int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
                         hssc_hss1d** out);
+/* ---- rank prediction (HSS2D/predict_rank.m:5-23, interpolation_operator.m:101-117) --- */
+/* A small network (8 -> 64 -> 64 -> 1, GELU, FP32) predicting log2 of a box's ID rank from
+ * geometric features; trained and evaluated by the library's own GPU kernels.  It takes
+ * the place of the closed-form rank guesses that size the randomized ID's sample count. */
+typedef struct hssc_mlp hssc_mlp;
+int hssc_mlp_create(uint64_t seed, hssc_mlp** out);
+void hssc_mlp_destroy(hssc_mlp* m);
+int hssc_mlp_num_params(void);
+/* Flat parameters: W1 (8x64, column-major), b1 (64), W2 (64x64), b2 (64), w3 (64), b3. */
+int hssc_mlp_get_params(const hssc_mlp* m, float* out);
+int hssc_mlp_set_params(hssc_mlp* m, const float* in);
+/* Standardisation: mu[0..7], sd[0..7] of the features, mu[8], sd[8] of the target. */
+int hssc_mlp_set_scaling(hssc_mlp* m, const float* mu, const float* sd);
+/* X: B x 8 column-major host array of raw features; y: B predicted targets. */
+int hssc_mlp_predict(hssc_mlp* m, const float* X, int32_t B, float* y);
+/* Mean squared error (standardised units) and its gradient w.r.t. the flat parameters. */
+int hssc_mlp_grads(hssc_mlp* m, const float* X, const float* t, int32_t B, double* loss,
+                   float* grads);
+/* Full-batch Adam for `epochs` steps; loss = MSE of the last step (standardised units). */
+int hssc_mlp_train(hssc_mlp* m, const float* X, const float* t, int32_t B, int32_t epochs,
+                   float lr, double* loss);
+
+/* HSS1D compression with rank-predicted randomized IDs.  rank_mode 0: the plain ID
+ * (hssc_hss1d_compress); 1: sketch rows from the closed-form rank guess of
+ * HSS2D/predict_rank.m:5-23; 2: from the network `mlp`.  Each block's sketch has the
+ * predicted rank + oversample rows rounded up to 16; a sketch whose ID is saturated or
+ * fails the acceptance test on the full block falls back to the plain ID, which remains
+ * the judge. */
+int hssc_hss1d_compress_ranked(const double* X, const double* nrm, const double* w,
+                               const double* kappa, int64_t N, int32_t nbox_max, double acc,
+                               int32_t nproxy, int32_t rank_mode, hssc_mlp* mlp,
+                               int32_t oversample, hssc_hss1d** out);
+/* Features of box b (2 <= b <= nboxes) as fed to the rank network: level, log2 points,
+ * log2 candidate skeleton size, log2(R_box / R_curve), max |kappa| R_box, mean |kappa|
+ * R_box, log2 rows of the outgoing block, -log10 acc. */
+int hssc_hss1d_box_features(const hssc_hss1d* h, int32_t box, float* f);
+/* Counters of the IDs done by the compression: blocks, fallbacks to the plain ID, sketch
+ * rows used and rows of the blocks factored by the plain ID. */
+int hssc_hss1d_id_stats(const hssc_hss1d* h, int64_t* boxes, int64_t* fallbacks,
+                        int64_t* sketch_rows, int64_t* full_rows);
+
 /* Replaces HSS1D/HSS1D_invert_nsym.m:1 ([HSSinv,FTinv] = HSS1D_invert_nsym(K_HSS,acc)). */
 int hssc_hss1d_invert(hssc_hss1d* h);
 /* Replaces HSS1D/HSS1D_applyinv_nsym.m:1 and HSS1D/HSS1D_apply_nsym.m:1 (one RHS, m = 1). */
