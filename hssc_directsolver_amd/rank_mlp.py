@@ -34,7 +34,10 @@ class RankMLP:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            lib().hssc_mlp_destroy(h)
+            try:
+                lib().hssc_mlp_destroy(h)
+            except TypeError:  # interpreter shutdown: module globals already cleared
+                pass
             self.handle = None
 
     @property
