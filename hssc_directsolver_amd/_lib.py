@@ -71,6 +71,9 @@ SIGNATURES = [
     ("hssc_hss1d_compress_ranked", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32,
                                              C.c_double, C.c_int32, C.c_int32, _P, C.c_int32,
                                              C.POINTER(_P)]),
+    ("hssc_hss1d_compress_batched", C.c_int, [C.c_int32, _I64P, _DP, _DP, _DP, _DP, C.c_int32,
+                                              C.c_double, C.c_int32, C.c_int32, _P, C.c_int32,
+                                              C.POINTER(_P)]),
     ("hssc_hss1d_box_features", C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
     ("hssc_hss1d_id_stats", C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
     ("hssc_mlp_create", C.c_int, [C.c_uint64, C.POINTER(_P)]),

@@ -284,12 +284,21 @@ __global__ void __launch_bounds__(256) hss1d_invert_kernel(const BoxInv* __restr
   }
 }
 
-__global__ void __launch_bounds__(256) hss1d_top_kernel(int64_t FT, int k2, int k3, int64_t E2,
-                                                        int64_t E3, int64_t S2, int64_t S3,
+// The top block of each problem: inv([E2 M_sib2; M_sib3 E3]) for the root's children
+// (HSS1D_invert_nsym.m:88-91), one workgroup per problem.
+struct TopOp {
+  int64_t FT, E2, E3, S2, S3;
+  int k2, k3;
+};
+
+__global__ void __launch_bounds__(256) hss1d_top_kernel(const TopOp* __restrict__ ops,
                                                         double* __restrict__ S) {
   __shared__ double f[256];
   __shared__ int piv[256];
   __shared__ int sp;
+  const TopOp O = ops[blockIdx.x];
+  const int64_t FT = O.FT, E2 = O.E2, E3 = O.E3, S2 = O.S2, S3 = O.S3;
+  const int k2 = O.k2, k3 = O.k3;
   const int n = k2 + k3, tid = threadIdx.x;
   double* F = S + FT;
   for (int idx = tid; idx < n * n; idx += blockDim.x) {
@@ -381,12 +390,15 @@ struct hssc_hss1d {
   // inverse
   bool inverted = false;
   std::vector<int64_t> oE, oLinv, oRinv, oDinv;
-  int64_t oFTinv = -1;
   // per-box vector offsets for the sweeps
   std::vector<int64_t> vUp, vDn;
   int64_t vlen = 0;
+  // forest of independent problems: root box of each problem (level 0), the root of every
+  // box, the circumradius of each problem's curve and each problem's top inverse block
+  std::vector<int> roots, rootof;
+  std::vector<double> Rroot;
+  std::vector<int64_t> oFTinvs;
   // rank-predicted sketching of the IDs: per-box features, counters
-  double Rcurve = 1;
   std::vector<std::array<float, 8>> feat;
   int64_t id_boxes = 0, id_fallbacks = 0, id_sketch_rows = 0, id_full_rows = 0;
   ~hssc_hss1d() {
@@ -394,14 +406,16 @@ struct hssc_hss1d {
         dfree1(dkap);
   }
   bool leaf(int b) const { return c1[b] <= 0 && c2[b] <= 0; }
+  bool root(int b) const { return parent[b] == 0; }
   CurveDev curve() const { return CurveDev{dx, dy, dnx, dny, dw, dkap}; }
 };
 
 namespace hssc {
 namespace {
 
-// LOCAL_get_tree (HSS1D_compress_nsym.m:277-334)
-void build_tree1d(hssc_hss1d* h, int nbox_max) {
+// LOCAL_get_tree (HSS1D_compress_nsym.m:277-334) for every problem of the batch: the roots
+// (boxes 1..P, problem p owning points [off_p, off_p + N_p)) are halved breadth first.
+void build_tree1d(hssc_hss1d* h, int nbox_max, const std::vector<int64_t>& Ns) {
   auto push = [&](int lev, int par, int st, int cnt) {
     h->level.push_back(lev), h->parent.push_back(par), h->c1.push_back(-1),
         h->c2.push_back(-1), h->start.push_back(st), h->count.push_back(cnt);
@@ -409,8 +423,14 @@ void build_tree1d(hssc_hss1d* h, int nbox_max) {
   h->level.clear(), h->parent.clear(), h->c1.clear(), h->c2.clear(), h->start.clear(),
       h->count.clear();
   push(0, 0, 0, 0);  // index 0 unused
-  push(0, 0, 0, int(h->N));
-  int first, last = 0, created = 1, lev = 0;
+  int64_t o = 0;
+  h->roots.clear();
+  for (int64_t n : Ns) {
+    push(0, 0, int(o), int(n));
+    h->roots.push_back(int(h->level.size()) - 1);
+    o += n;
+  }
+  int first, last = 0, created = int(Ns.size()), lev = 0;
   while (created > 0) {
     first = last + 1;
     last = last + created;
@@ -431,6 +451,8 @@ void build_tree1d(hssc_hss1d* h, int nbox_max) {
   h->nboxes = last;
   h->depth = 0;
   for (int b = 1; b <= last; ++b) h->depth = std::max(h->depth, h->level[b]);
+  h->rootof.assign(last + 1, 0);
+  for (int b = 1; b <= last; ++b) h->rootof[b] = h->parent[b] == 0 ? b : h->rootof[h->parent[b]];
 }
 
 // LOCAL_get_circum_circle (:339-359) on a list of point indices.
@@ -463,8 +485,9 @@ std::vector<int> range_of(const hssc_hss1d* h, int b) {
 void neighbours(hssc_hss1d* h) {
   const int nb = h->nboxes;
   h->nei.assign(nb + 1, {});
-  h->nei[1] = {1};
-  for (int b = 2; b <= nb; ++b) {
+  for (int r : h->roots) h->nei[r] = {r};
+  for (int b = 1; b <= nb; ++b) {
+    if (h->root(b)) continue;
     double xc, yc, R;
     circum(h, range_of(h, b), &xc, &yc, &R);
     std::vector<int> pot;
@@ -479,7 +502,8 @@ void neighbours(hssc_hss1d* h) {
       if (mn < kRadiusRel * kRadiusRel * R * R) h->nei[b].push_back(j);
     }
   }
-  for (int b = 2; b <= nb; ++b) {
+  for (int b = 1; b <= nb; ++b) {
+    if (h->root(b)) continue;
     auto& v = h->nei[b];
     v.erase(std::remove(v.begin(), v.end(), b), v.end());
     std::sort(v.begin(), v.end());
@@ -861,10 +885,16 @@ double predict_rank_table(int n, double acc) {
 
 namespace {
 int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, const double* kappa,
-                        int64_t N, int32_t nbox_max, double acc, int32_t nproxy, int rank_mode,
-                        hssc_mlp* mlp, int oversample, hssc_hss1d** out) {
-  if (!Xc || !nrm || !w || !kappa || !out || N < 4 || nbox_max < 2 || !(acc > 0 && acc < 1) ||
-      nproxy < 4)
+                        const std::vector<int64_t>& Ns, int32_t nbox_max, double acc,
+                        int32_t nproxy, int rank_mode, hssc_mlp* mlp, int oversample,
+                        hssc_hss1d** out) {
+  int64_t N = 0;
+  for (int64_t n : Ns) {
+    if (n < 4) return fail(HSSC_ERR_INVALID_ARG, "every curve needs at least 4 nodes");
+    N += n;
+  }
+  if (!Xc || !nrm || !w || !kappa || !out || Ns.empty() || nbox_max < 2 ||
+      !(acc > 0 && acc < 1) || nproxy < 4)
     return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d_compress arguments");
   *out = nullptr;
   int count = 0;
@@ -879,13 +909,15 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     h->W.assign(w, w + N), h->KAP.assign(kappa, kappa + N);
     h->dx = dput(h->X), h->dy = dput(h->Y), h->dnx = dput(h->NX), h->dny = dput(h->NY);
     h->dw = dput(h->W), h->dkap = dput(h->KAP);
-    build_tree1d(h, nbox_max);
-    if (h->nboxes < 3) throw HipErr{fail(HSSC_ERR_INVALID_ARG, "tree needs at least 2 levels")};
+    build_tree1d(h, nbox_max, Ns);
+    for (int r : h->roots)
+      if (h->leaf(r)) throw HipErr{fail(HSSC_ERR_INVALID_ARG, "tree needs at least 2 levels")};
     neighbours(h);
     const int nb = h->nboxes;
-    {
+    h->Rroot.assign(nb + 1, 1.0);
+    for (int r : h->roots) {
       double cx, cy;
-      circum(h, range_of(h, 1), &cx, &cy, &h->Rcurve);
+      circum(h, range_of(h, r), &cx, &cy, &h->Rroot[r]);
     }
     h->feat.assign(nb + 1, {});
     h->isk_up.assign(nb + 1, {}), h->isk_dn.assign(nb + 1, {});
@@ -899,7 +931,7 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     // compress level by level, deepest first (HSS1D_compress_nsym.m:84-216)
     for (int lev = h->depth; lev >= 1; --lev) {
       std::vector<int> boxes;
-      for (int b = 2; b <= nb; ++b)
+      for (int b = 1; b <= nb; ++b)
         if (h->level[b] == lev) boxes.push_back(b);
       Ragged Rout, Rin;
       std::vector<std::vector<int>> outs(boxes.size()), inns(boxes.size());
@@ -960,7 +992,7 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
         f[0] = float(lev);
         f[1] = float(std::log2(double(h->count[b])));
         f[2] = float(std::log2(double(out.size())));
-        f[3] = float(std::log2(R / h->Rcurve));
+        f[3] = float(std::log2(R / h->Rroot[h->rootof[b]]));
         f[4] = float(kmax * R);
         f[5] = float(ksum / h->count[b] * R);
         f[6] = float(std::log2(double(Rout.m.back())));
@@ -1095,8 +1127,10 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     h->I = dput(iarena);
     h->vUp.assign(nb + 1, 0), h->vDn.assign(nb + 1, 0);
     int64_t vv = 0;
-    for (int b = 2; b <= nb; ++b) h->vUp[b] = vv, vv += h->kskel[b];
-    for (int b = 2; b <= nb; ++b) h->vDn[b] = vv, vv += h->kskel[b];
+    for (int b = 1; b <= nb; ++b)
+      if (!h->root(b)) h->vUp[b] = vv, vv += h->kskel[b];
+    for (int b = 1; b <= nb; ++b)
+      if (!h->root(b)) h->vDn[b] = vv, vv += h->kskel[b];
     h->vlen = vv;
     H1CHK(hipDeviceSynchronize());
     for (void* p : tmp) dfree1(p);
@@ -1117,7 +1151,7 @@ extern "C" {
 int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, const double* kappa,
                         int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
                         hssc_hss1d** out) {
-  return hss1d_compress_impl(Xc, nrm, w, kappa, N, nbox_max, acc, nproxy, 0, nullptr, 0, out);
+  return hss1d_compress_impl(Xc, nrm, w, kappa, {N}, nbox_max, acc, nproxy, 0, nullptr, 0, out);
 }
 
 int hssc_hss1d_compress_ranked(const double* Xc, const double* nrm, const double* w,
@@ -1126,12 +1160,24 @@ int hssc_hss1d_compress_ranked(const double* Xc, const double* nrm, const double
                                int32_t oversample, hssc_hss1d** out) {
   if (rank_mode < 0 || rank_mode > 2 || (rank_mode == 2 && !mlp) || oversample < 0)
     return fail(HSSC_ERR_INVALID_ARG, "invalid rank prediction arguments");
-  return hss1d_compress_impl(Xc, nrm, w, kappa, N, nbox_max, acc, nproxy, rank_mode, mlp,
+  return hss1d_compress_impl(Xc, nrm, w, kappa, {N}, nbox_max, acc, nproxy, rank_mode, mlp,
                              oversample, out);
 }
 
+int hssc_hss1d_compress_batched(int32_t nprob, const int64_t* Ns, const double* Xc,
+                                const double* nrm, const double* w, const double* kappa,
+                                int32_t nbox_max, double acc, int32_t nproxy, int32_t rank_mode,
+                                hssc_mlp* mlp, int32_t oversample, hssc_hss1d** out) {
+  if (nprob < 1 || !Ns || rank_mode < 0 || rank_mode > 2 || (rank_mode == 2 && !mlp) ||
+      oversample < 0)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid batched hss1d_compress arguments");
+  return hss1d_compress_impl(Xc, nrm, w, kappa, std::vector<int64_t>(Ns, Ns + nprob), nbox_max,
+                             acc, nproxy, rank_mode, mlp, oversample, out);
+}
+
 int hssc_hss1d_box_features(const hssc_hss1d* h, int32_t box, float* f) {
-  if (!h || !f || box < 2 || box > h->nboxes) return fail(HSSC_ERR_INVALID_ARG, "bad box");
+  if (!h || !f || box < 1 || box > h->nboxes || h->root(box))
+    return fail(HSSC_ERR_INVALID_ARG, "bad box (roots have no features)");
   for (int c = 0; c < 8; ++c) f[c] = h->feat[box][c];
   return HSSC_OK;
 }
@@ -1151,14 +1197,15 @@ int hssc_hss1d_invert(hssc_hss1d* h) {
   if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
   try {
     const int nb = h->nboxes;
-    for (int b = 2; b <= nb; ++b)
-      if (h->nskel[b] > 256)
+    for (int b = 1; b <= nb; ++b)
+      if (!h->root(b) && h->nskel[b] > 256)
         throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D block larger than 256")};
     int64_t top = h->arena_n;
     h->oE.assign(nb + 1, -1), h->oLinv.assign(nb + 1, -1), h->oRinv.assign(nb + 1, -1),
         h->oDinv.assign(nb + 1, -1);
     std::vector<BoxInv> all(nb + 1);
-    for (int b = 2; b <= nb; ++b) {
+    for (int b = 1; b <= nb; ++b) {
+      if (h->root(b)) continue;
       const int n = h->nskel[b], k = h->kskel[b];
       BoxInv B{};
       B.n = n, B.k = k;
@@ -1174,18 +1221,32 @@ int hssc_hss1d_invert(hssc_hss1d* h) {
       B.Mself = h->oMself[b];
       all[b] = B;
     }
-    for (int b = 2; b <= nb; ++b)
-      if (!h->leaf(b)) {
+    for (int b = 1; b <= nb; ++b)
+      if (!h->root(b) && !h->leaf(b)) {
         all[b].E1 = h->oE[h->c1[b]], all[b].E2 = h->oE[h->c2[b]];
         all[b].S1 = h->oMsib[h->c1[b]], all[b].S2 = h->oMsib[h->c2[b]];
         all[b].k1 = h->kskel[h->c1[b]], all[b].k2 = h->kskel[h->c2[b]];
       }
-    const int k2 = h->kskel[2], k3 = h->kskel[3];
-    h->oFTinv = take(&top, int64_t(k2 + k3) * (k2 + k3));
+    std::vector<TopOp> tops;
+    h->oFTinvs.assign(nb + 1, -1);
+    for (int r : h->roots) {
+      const int a = h->c1[r], c = h->c2[r];
+      const int k2 = h->kskel[a], k3 = h->kskel[c];
+      if (k2 + k3 > 256) throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D top block larger than 256")};
+      TopOp O{};
+      O.FT = h->oFTinvs[r] = take(&top, int64_t(k2 + k3) * (k2 + k3));
+      O.k2 = k2, O.k3 = k3;
+      tops.push_back(O);
+    }
     grow(&h->S, &h->arena_n, top);
+    for (size_t i = 0; i < h->roots.size(); ++i) {
+      const int r = h->roots[i], a = h->c1[r], c = h->c2[r];
+      tops[i].E2 = h->oE[a], tops[i].E3 = h->oE[c], tops[i].S2 = h->oMsib[a],
+      tops[i].S3 = h->oMsib[c];
+    }
     for (int lev = h->depth; lev >= 1; --lev) {
       std::vector<BoxInv> ops;
-      for (int b = 2; b <= nb; ++b)
+      for (int b = 1; b <= nb; ++b)
         if (h->level[b] == lev) ops.push_back(all[b]);
       if (ops.empty()) continue;
       BoxInv* d = dput(ops);
@@ -1195,9 +1256,10 @@ int hssc_hss1d_invert(hssc_hss1d* h) {
       H1CHK(hipDeviceSynchronize());
       dfree1(d);
     }
-    hipLaunchKernelGGL(hss1d_top_kernel, dim3(1), dim3(256), 0, 0, h->oFTinv, k2, k3, h->oE[2],
-                       h->oE[3], h->oMsib[2], h->oMsib[3], h->S);
+    TopOp* dtops = dput(tops);
+    hipLaunchKernelGGL(hss1d_top_kernel, dim3(tops.size()), dim3(256), 0, 0, dtops, h->S);
     H1CHK(hipDeviceSynchronize());
+    dfree1(dtops);
     H1CHK(hipGetLastError());
     h->inverted = true;
     return HSSC_OK;
@@ -1240,20 +1302,24 @@ static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool in
   };
   for (int lev = h->depth; lev >= 1; --lev) {
     std::vector<SweepOp> ops;
-    for (int b = 2; b <= nb; ++b)
+    for (int b = 1; b <= nb; ++b)
       if (h->level[b] == lev) ops.push_back(up_op(b, inverse ? h->oRinv[b] : h->oR[b]));
     launch(ops);
   }
-  const int k2 = h->kskel[2], k3 = h->kskel[3];
   if (inverse) {
-    // [dn2; dn3] = FTinv [up2; up3]
-    SweepOp O{};
-    O.rows = k2 + k3, O.nterm = 1, O.out = h->vDn[2], O.out2 = h->vDn[3], O.split = k2;
-    O.t[0] = term(h->oFTinv, k2 + k3, k2 + k3, h->vUp[2], k2, h->vUp[3]);
-    launch({O});
+    // per problem: [dn2; dn3] = FTinv [up2; up3] for the root's children 2, 3
+    std::vector<SweepOp> tops;
+    for (int r : h->roots) {
+      const int a = h->c1[r], c = h->c2[r], k2 = h->kskel[a], k3 = h->kskel[c];
+      SweepOp O{};
+      O.rows = k2 + k3, O.nterm = 1, O.out = h->vDn[a], O.out2 = h->vDn[c], O.split = k2;
+      O.t[0] = term(h->oFTinvs[r], k2 + k3, k2 + k3, h->vUp[a], k2, h->vUp[c]);
+      tops.push_back(O);
+    }
+    launch(tops);
     for (int lev = 1; lev <= h->depth; ++lev) {
       std::vector<SweepOp> ops;
-      for (int b = 2; b <= nb; ++b) {
+      for (int b = 1; b <= nb; ++b) {
         if (h->level[b] != lev) continue;
         const int n = h->nskel[b], k = h->kskel[b];
         SweepOp O{};
@@ -1271,16 +1337,21 @@ static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool in
       launch(ops);
     }
   } else {
-    // dn2 = M_sib2 up3, dn3 = M_sib3 up2
-    SweepOp A2{}, A3{};
-    A2.rows = k2, A2.nterm = 1, A2.out = h->vDn[2], A2.out2 = -1;
-    A2.t[0] = term(h->oMsib[2], k2, k3, h->vUp[3], k3, 0);
-    A3.rows = k3, A3.nterm = 1, A3.out = h->vDn[3], A3.out2 = -1;
-    A3.t[0] = term(h->oMsib[3], k3, k2, h->vUp[2], k2, 0);
-    launch({A2, A3});
+    // per problem: dn2 = M_sib2 up3, dn3 = M_sib3 up2 for the root's children 2, 3
+    std::vector<SweepOp> tops;
+    for (int r : h->roots) {
+      const int a = h->c1[r], c = h->c2[r], k2 = h->kskel[a], k3 = h->kskel[c];
+      SweepOp A2{}, A3{};
+      A2.rows = k2, A2.nterm = 1, A2.out = h->vDn[a], A2.out2 = -1;
+      A2.t[0] = term(h->oMsib[a], k2, k3, h->vUp[c], k3, 0);
+      A3.rows = k3, A3.nterm = 1, A3.out = h->vDn[c], A3.out2 = -1;
+      A3.t[0] = term(h->oMsib[c], k3, k2, h->vUp[a], k2, 0);
+      tops.push_back(A2), tops.push_back(A3);
+    }
+    launch(tops);
     for (int lev = 1; lev <= h->depth; ++lev) {
       std::vector<SweepOp> ops;
-      for (int b = 2; b <= nb; ++b) {
+      for (int b = 1; b <= nb; ++b) {
         if (h->level[b] != lev) continue;
         const int n = h->nskel[b], k = h->kskel[b];
         if (h->leaf(b)) {

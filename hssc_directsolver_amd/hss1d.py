@@ -119,6 +119,32 @@ def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64, rank="id"
     return HSS1D(h, X.shape[0])
 
 
+def HSS1D_compress_batched(curves, nbox_max, acc, nproxy=64, rank="id", mlp=None,
+                           oversample=8) -> HSS1D:
+    """Many curves in one operator (one tree per curve, shared launches per level); vectors
+    are the curves' vectors concatenated in order (see split)."""
+    Ns = np.array([c.N for c in curves], dtype=np.int64)
+    X = np.asfortranarray(np.concatenate([c.X for c in curves]), dtype=np.float64)
+    Nm = np.asfortranarray(np.concatenate([c.nrm for c in curves]), dtype=np.float64)
+    w = np.ascontiguousarray(np.concatenate([c.w for c in curves]), dtype=np.float64)
+    k = np.ascontiguousarray(np.concatenate([c.kappa for c in curves]), dtype=np.float64)
+    h = C.c_void_p()
+    check(lib().hssc_hss1d_compress_batched(
+        len(curves), Ns.ctypes.data_as(C.POINTER(C.c_int64)), X.ctypes.data_as(_DP),
+        Nm.ctypes.data_as(_DP), w.ctypes.data_as(_DP), k.ctypes.data_as(_DP), int(nbox_max),
+        float(acc), int(nproxy), RANK_MODES[rank], mlp.handle if mlp is not None else None,
+        int(oversample), C.byref(h)))
+    H = HSS1D(h, int(Ns.sum()))
+    H.sizes = Ns
+    return H
+
+
+def split(K_HSS: HSS1D, v):
+    """The per-curve pieces of a concatenated vector of a batched operator."""
+    sizes = getattr(K_HSS, "sizes", np.array([K_HSS.N]))
+    return np.split(np.asarray(v), np.cumsum(sizes)[:-1])
+
+
 def HSS1D_invert_nsym(K_HSS: HSS1D, acc=None) -> HSS1D:
     """HSS1D/HSS1D_invert_nsym.m:1 (the inverse factors are stored in the same handle)."""
     check(lib().hssc_hss1d_invert(K_HSS.handle))

@@ -99,9 +99,12 @@ def random_contour(rng, N):
 def box_dataset(hss):
     """(features, log2 rank) of every non-root box of a compressed HSS1D operator."""
     X, t = [], []
-    for b in range(2, hss.nboxes + 1):
+    for b in range(1, hss.nboxes + 1):
+        info = hss.box(b)
+        if info["level"] == 0:  # roots have no ID
+            continue
         X.append(hss.features(b))
-        t.append(np.log2(max(hss.box(b)["kskel"], 1)))
+        t.append(np.log2(max(info["kskel"], 1)))
     return np.array(X, dtype=np.float32), np.array(t, dtype=np.float32)
 
 

@@ -238,7 +238,16 @@ int hssc_hss1d_compress_ranked(const double* X, const double* nrm, const double*
                                const double* kappa, int64_t N, int32_t nbox_max, double acc,
                                int32_t nproxy, int32_t rank_mode, hssc_mlp* mlp,
                                int32_t oversample, hssc_hss1d** out);
-/* Features of box b (2 <= b <= nboxes) as fed to the rank network: level, log2 points,
+/* Batched HSS1D (many curves in one operator): curve p has Ns[p] nodes, stored after the
+ * nodes of curves 0..p-1 in X (sum Ns x 2), nrm, w, kappa.  Every curve gets its own tree;
+ * the boxes of all curves at a level are compressed, inverted and swept by the same
+ * launches.  apply / applyinv act on the concatenated vectors (length sum Ns).  Box
+ * numbers: the roots are 1..nprob. */
+int hssc_hss1d_compress_batched(int32_t nprob, const int64_t* Ns, const double* X,
+                                const double* nrm, const double* w, const double* kappa,
+                                int32_t nbox_max, double acc, int32_t nproxy, int32_t rank_mode,
+                                hssc_mlp* mlp, int32_t oversample, hssc_hss1d** out);
+/* Features of a non-root box b as fed to the rank network: level, log2 points,
  * log2 candidate skeleton size, log2(R_box / R_curve), max |kappa| R_box, mean |kappa|
  * R_box, log2 rows of the outgoing block, -log10 acc. */
 int hssc_hss1d_box_features(const hssc_hss1d* h, int32_t box, float* f);

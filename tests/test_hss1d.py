@@ -68,3 +68,24 @@ def test_gpu_hss1d_vs_dense_and_oracle(N):
     assert rel(y, yo) < 1e-9
     for b in range(2, K.nboxes + 1):
         assert abs(K.box(b)["kskel"] - Ho.kskel[b]) <= 3, b
+
+
+@pytest.mark.gpu
+def test_gpu_hss1d_batched_matches_single_problems():
+    # configuration 3 in small: several star contours of different sizes in one operator
+    from hssc_directsolver_amd.rank_mlp import random_contour
+    rng = np.random.default_rng(5)
+    curves = [random_contour(rng, N) for N in (1024, 1500, 2048, 3000, 1111)]
+    K = H.HSS1D_compress_batched(curves, 64, 1e-10)
+    H.HSS1D_invert_nsym(K)
+    fs = [np.random.default_rng(10 + i).standard_normal(c.N) for i, c in enumerate(curves)]
+    f = np.concatenate(fs)
+    x = H.HSS1D_applyinv_nsym(K, f)
+    assert rel(H.HSS1D_apply_nsym(K, x), f) < 1e-12
+    for c, fi, xi in zip(curves, fs, H.split(K, x)):
+        Ki = H.HSS1D_compress_nsym("green", c, 64, 1e-10)
+        H.HSS1D_invert_nsym(Ki)
+        assert rel(xi, H.HSS1D_applyinv_nsym(Ki, fi)) < 1e-10
+    # the smallest against the dense solve
+    A = o1.dense_matrix(o1.Curve(curves[0].X, curves[0].nrm, curves[0].w, curves[0].kappa))
+    assert rel(H.split(K, x)[0], np.linalg.solve(A, fs[0])) < 1e-10
