@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--kernel", choices=("helmholtz", "laplace"), default="helmholtz")
     ap.add_argument("--kh", type=float, default=20.0)
     ap.add_argument("--proxy-layers", type=int, default=4)
+    ap.add_argument("--preset-all", action="store_true",
+                    help="kh > 0: 'bdry' skeletons at every level instead of the reference's "
+                         "ID-selected skeletons at the two bottom levels")
     ap.add_argument("--no-laplace", action="store_true", help="skip the Laplace side case")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -167,7 +170,8 @@ def cpu_baseline(H, P, args):
     stop = args.cpu_stop
     tb, ti = {}, {}
     t0 = time.perf_counter()
-    tree = o.build_tree(p, interform="dense", timings=tb, stop_level=stop, preset_all=True)
+    tree = o.build_tree(p, interform="dense", timings=tb, stop_level=stop,
+                        preset_all=args.preset_all)
     o.invert(tree, timings=ti, stop_level=stop)
     t1 = time.perf_counter()
     idx = H.HSS2D_build_index_sets(P)
@@ -195,12 +199,15 @@ def make_problem(H, torch, kernel, args):
     gen = torch.Generator(device="cpu").manual_seed(0)
     if kernel == "helmholtz":
         P = H.HSS_tree_parameters("SL_H_2D", args.kh, 1, args.Ng, args.np_, 1e-10, 3, 800,
-                                  args.TI, proxy_layers=args.proxy_layers)
+                                  args.TI, proxy_layers=args.proxy_layers,
+                                  preset_all=args.preset_all)
         b = torch.complex(torch.randn(P.N, generator=gen, dtype=torch.float64),
                           torch.randn(P.N, generator=gen, dtype=torch.float64))
         wl = (f"HSS2D SL_H_2D Lippmann-Schwinger k={args.kh:g} {'TI' if args.TI else 'NTI'} "
               f"Ng={args.Ng} np={args.np_} N={P.N} acc=1e-10 layers=3 proxy_layers="
-              f"{args.proxy_layers} dense blocks")
+              f"{args.proxy_layers} dense blocks, "
+              + ("'bdry' skeletons at every level" if args.preset_all else
+                 "ID skeletons at the two bottom levels"))
     else:
         P = H.HSS_tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, args.TI)
         b = torch.randn(P.N, generator=gen, dtype=torch.float64)

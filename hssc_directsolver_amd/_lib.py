@@ -21,7 +21,8 @@ class HsscParams(C.Structure):
         ("transinv", C.c_int32), ("depth", C.c_int32), ("h", C.c_double),
         ("max_particles", C.c_int32), ("interform", C.c_int32), ("k_cut", C.c_int32),
         ("dr_weights_re", C.c_double), ("dr_weights_im", C.c_double),
-        ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("preset_all", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -46,6 +47,7 @@ SIGNATURES = [
     ("hssc_tree_level_info", C.c_int, [_P, C.c_int32, _I64P, _I64P, _I64P, _I64P, _I64P, _I64P]),
     ("hssc_tree_get_isrc", C.c_int, [_P, C.c_int64, _I64P]),
     ("hssc_tree_get_isk", C.c_int, [_P, C.c_int64, _I64P]),
+    ("hssc_tree_box_ranks", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("hssc_tree_get_J", C.c_int, [_P, C.c_int64, _I64P]),
     ("hssc_tree_get_proxy", C.c_int, [_P, C.c_int64, _DP]),
     ("hssc_tree_get_T", C.c_int, [_P, C.c_int64, _DP]),

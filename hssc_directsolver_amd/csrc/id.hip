@@ -29,7 +29,7 @@ constexpr int ID_NT = 256;
 
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  for (int o = 32; o > 0; o >>= 1) v += shfl_xor_s(v, o);
   return v;
 }
 
@@ -58,6 +58,8 @@ __device__ __forceinline__ double urand(uint64_t seed, uint64_t i) {
 
 // mode: acc > 0 -> rank by |R_ii| > acc |R_11| plus the randomized check (ID.m:40-65);
 //       acc <= 0 -> fixed rank kfix[b] (ID.m:106-113, T by back substitution).
+// Complex data: Householder reflectors H = I - tau v v^H applied as H^H (xgeqp3
+// conventions, real R diagonal); the check uses conjugate transposes as ID.m's ' does.
 template <typename T>
 __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
     int nmat, const int64_t* __restrict__ off, const int* __restrict__ mr, const int* __restrict__ nc,
@@ -65,8 +67,9 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
     int* __restrict__ perm, const int64_t* __restrict__ poff, int* __restrict__ kout,
     T* __restrict__ Tout, const int64_t* __restrict__ toff, T* __restrict__ work,
     const int64_t* __restrict__ woff, uint64_t seed) {
-  __shared__ T red[ID_NT / 64];
-  __shared__ T sbest[ID_NT / 64];
+  using R = real_t<T>;
+  __shared__ R redr[ID_NT / 64];
+  __shared__ R sbest[ID_NT / 64];
   __shared__ int sidx[ID_NT / 64];
   __shared__ int s_k;
   const int b = blockIdx.x;
@@ -83,19 +86,19 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
   const int kmax = m < n ? m : n;
   for (int c = tid; c < n; c += ID_NT) {
     P[c] = c;
-    T s = T(0);
-    for (int r = 0; r < m; ++r) s += M[r + int64_t(c) * m] * M[r + int64_t(c) * m];
-    nrm[c] = s;
+    R s = R(0);
+    for (int r = 0; r < m; ++r) s += abs2(M[r + int64_t(c) * m]);
+    nrm[c] = T(s);
   }
   __syncthreads();
   for (int j = 0; j < kmax; ++j) {
     // pivot: largest remaining column norm (ties -> lowest index)
-    T best = T(-1);
+    R best = R(-1);
     int bi = j;
     for (int c = j + tid; c < n; c += ID_NT)
-      if (nrm[c] > best) best = nrm[c], bi = c;
+      if (re_s(nrm[c]) > best) best = re_s(nrm[c]), bi = c;
     for (int o = 32; o > 0; o >>= 1) {
-      const T ob = __shfl_xor(best, o);
+      const R ob = shfl_xor_s(best, o);
       const int oi = __shfl_xor(bi, o);
       if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
     }
@@ -122,37 +125,38 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
     __syncthreads();
     // Householder reflector for column j (rows j..m-1)
     T* a = M + int64_t(j) * m;
-    T sg = T(0);
-    for (int r = j + 1 + tid; r < m; r += ID_NT) sg += a[r] * a[r];
-    sg = bsum(sg, red);
+    R sg = R(0);
+    for (int r = j + 1 + tid; r < m; r += ID_NT) sg += abs2(a[r]);
+    sg = bsum(sg, redr);
     const T alpha = a[j];
-    T beta, tv, scal;
-    if (sg == T(0)) {
-      tv = T(0), beta = alpha, scal = T(0);
+    T tv, scal;
+    R beta;
+    if (sg == R(0) && im_s(alpha) == R(0)) {
+      tv = T(0), beta = re_s(alpha), scal = T(0);
     } else {
-      const T nr = sqrt(alpha * alpha + sg);
-      beta = alpha >= T(0) ? -nr : nr;
-      tv = (beta - alpha) / beta;
-      scal = T(1) / (alpha - beta);
+      const R nr = sqrt(abs2(alpha) + sg);
+      beta = re_s(alpha) >= R(0) ? -nr : nr;
+      tv = (T(beta) - alpha) / T(beta);
+      scal = T(1) / (alpha - T(beta));
     }
     __syncthreads();
     for (int r = j + 1 + tid; r < m; r += ID_NT) a[r] *= scal;
-    if (tid == 0) a[j] = beta, rdg[j] = fabs(beta), tau[j] = tv;
+    if (tid == 0) a[j] = T(beta), rdg[j] = T(fabs(beta)), tau[j] = tv;
     __syncthreads();
-    // apply to columns c > j (one wave per column) and recompute their trailing norms
+    // apply H^H to columns c > j (one wave per column) and recompute their trailing norms
     for (int c = j + 1 + wave; c < n; c += ID_NT / 64) {
       T* col = M + int64_t(c) * m;
       T s = T(0);
-      for (int r = j + lane; r < m; r += 64) s += (r == j ? T(1) : a[r]) * col[r];
-      s = wsum(s) * tv;
-      T q = T(0);
+      for (int r = j + lane; r < m; r += 64) s += conj_s(r == j ? T(1) : a[r]) * col[r];
+      s = wsum(s) * conj_s(tv);
+      R q = R(0);
       for (int r = j + lane; r < m; r += 64) {
         const T v = col[r] - (r == j ? T(1) : a[r]) * s;
         col[r] = v;
-        if (r > j) q += v * v;
+        if (r > j) q += abs2(v);
       }
       q = wsum(q);
-      if (lane == 0) nrm[c] = q;
+      if (lane == 0) nrm[c] = T(q);
     }
     __syncthreads();
   }
@@ -160,9 +164,9 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
   if (tid == 0) {
     int k;
     if (acc > 0) {
-      const T thr = kmax > 0 ? rdg[0] * T(acc) : T(0);
+      const R thr = kmax > 0 ? re_s(rdg[0]) * R(acc) : R(0);
       k = 0;
-      for (int i = 0; i < kmax; ++i) k += rdg[i] > thr ? 1 : 0;
+      for (int i = 0; i < kmax; ++i) k += re_s(rdg[i]) > thr ? 1 : 0;
     } else {
       k = kfix[b] < kmax ? kfix[b] : kmax;
     }
@@ -170,7 +174,7 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
   }
   __syncthreads();
   T* To = Tout + toff[b];
-  T thr = kmax > 0 ? rdg[0] * T(acc) : T(0);
+  R thr = kmax > 0 ? re_s(rdg[0]) * R(acc) : R(0);
   for (int round = 0;; ++round) {
     const int k = s_k;
     // T = R11^-1 R12 by back substitution, one thread per column of R12
@@ -186,21 +190,21 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
     }
     __syncthreads();
     if (acc <= 0 || k >= kmax || k == n) break;
-    // randomized check (ID.m:55-59,68-86): err = sqrt(|T^T A_s^T y - A_r^T y|) with
-    // y = A_s T x - A_r x for a random unit x; nrm = sqrt(|A_r^T A_r x|)
+    // randomized check (ID.m:55-59,68-86): err = sqrt(|T^H A_s^H y - A_r^H y|) with
+    // y = A_s T x - A_r x for a random unit x; nrm = sqrt(|A_r^H A_r x|)
     const int r_ = n - k;
     T* x = tau + n;        // r_ entries
     T* y = x + n;          // m entries
-    T* z = y + m;          // n entries (A_s^T y etc.)
-    T xs = T(0);
+    T* z = y + m;          // n entries (A_s^H y etc.)
+    R xs = R(0);
     for (int i = tid; i < r_; i += ID_NT) {
-      const T v = T(urand(seed ^ (uint64_t(b) << 20), uint64_t(round) * 4096 + i));
-      x[i] = v;
+      const R v = R(urand(seed ^ (uint64_t(b) << 20), uint64_t(round) * 4096 + i));
+      x[i] = T(v);
       xs += v * v;
     }
-    xs = bsum(xs, red);
-    const T xinv = T(1) / sqrt(xs);
-    for (int i = tid; i < r_; i += ID_NT) x[i] *= xinv;
+    xs = bsum(xs, redr);
+    const R xinv = R(1) / sqrt(xs);
+    for (int i = tid; i < r_; i += ID_NT) x[i] *= T(xinv);
     __syncthreads();
     // u = T x (k), stored in z
     for (int i = tid; i < k; i += ID_NT) {
@@ -217,33 +221,33 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
       y[rr] = s;
     }
     __syncthreads();
-    // e = T^T (A_s^T y) - A_r^T y ; |e|
+    // e = T^H (A_s^H y) - A_r^H y ; |e|
     for (int i = tid; i < k; i += ID_NT) {
       T s = T(0);
-      for (int rr = 0; rr < m; ++rr) s += Mc[rr + int64_t(P[i]) * m] * y[rr];
+      for (int rr = 0; rr < m; ++rr) s += conj_s(Mc[rr + int64_t(P[i]) * m]) * y[rr];
       z[i] = s;
     }
     __syncthreads();
-    T e2 = T(0);
+    R e2 = R(0);
     for (int c = tid; c < r_; c += ID_NT) {
       T s = T(0);
-      for (int i = 0; i < k; ++i) s += To[i + int64_t(c) * k] * z[i];
+      for (int i = 0; i < k; ++i) s += conj_s(To[i + int64_t(c) * k]) * z[i];
       T t = T(0);
-      for (int rr = 0; rr < m; ++rr) t += Mc[rr + int64_t(P[k + c]) * m] * y[rr];
+      for (int rr = 0; rr < m; ++rr) t += conj_s(Mc[rr + int64_t(P[k + c]) * m]) * y[rr];
       s -= t;
-      e2 += s * s;
+      e2 += abs2(s);
     }
-    e2 = bsum(e2, red);
-    // n2 = |A_r^T A_r x| with a fresh random x
-    T xs2 = T(0);
+    e2 = bsum(e2, redr);
+    // n2 = |A_r^H A_r x| with a fresh random x
+    R xs2 = R(0);
     for (int i = tid; i < r_; i += ID_NT) {
-      const T v = T(urand(seed ^ (uint64_t(b) << 20) ^ 0x5555ull, uint64_t(round) * 4096 + i));
-      x[i] = v;
+      const R v = R(urand(seed ^ (uint64_t(b) << 20) ^ 0x5555ull, uint64_t(round) * 4096 + i));
+      x[i] = T(v);
       xs2 += v * v;
     }
-    xs2 = bsum(xs2, red);
-    const T xinv2 = T(1) / sqrt(xs2);
-    for (int i = tid; i < r_; i += ID_NT) x[i] *= xinv2;
+    xs2 = bsum(xs2, redr);
+    const R xinv2 = R(1) / sqrt(xs2);
+    for (int i = tid; i < r_; i += ID_NT) x[i] *= T(xinv2);
     __syncthreads();
     for (int rr = tid; rr < m; rr += ID_NT) {
       T s = T(0);
@@ -251,24 +255,28 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
       y[rr] = s;
     }
     __syncthreads();
-    T n2 = T(0);
+    R n2 = R(0);
     for (int c = tid; c < r_; c += ID_NT) {
       T t = T(0);
-      for (int rr = 0; rr < m; ++rr) t += Mc[rr + int64_t(P[k + c]) * m] * y[rr];
-      n2 += t * t;
+      for (int rr = 0; rr < m; ++rr) t += conj_s(Mc[rr + int64_t(P[k + c]) * m]) * y[rr];
+      n2 += abs2(t);
     }
-    n2 = bsum(n2, red);
-    const T err = sqrt(sqrt(e2)), nrmr = sqrt(sqrt(n2));
-    if (!(err > nrmr * T(acc))) break;
-    // relax the threshold (Rmx = 0.75 Rmx) and recount
-    thr *= T(0.75);
-    __syncthreads();
-    if (tid == 0) {
-      int kk = 0;
-      for (int i = 0; i < kmax; ++i) kk += rdg[i] > thr ? 1 : 0;
-      s_k = kk;
+    n2 = bsum(n2, redr);
+    const R err = sqrt(sqrt(e2)), nrmr = sqrt(sqrt(n2));
+    if (!(err > nrmr * R(acc))) break;
+    // relax the threshold (Rmx = 0.75 Rmx) and recount; a count that does not grow would
+    // repeat the same check, so keep relaxing until it grows, and stop once it cannot
+    // (every thread runs the same loop on the same data)
+    int kk = k;
+    for (int g = 0; g < 4096 && kk == k && thr > R(0); ++g) {
+      thr *= R(0.75);
+      kk = 0;
+      for (int i = 0; i < kmax; ++i) kk += re_s(rdg[i]) > thr ? 1 : 0;
     }
     __syncthreads();
+    if (tid == 0) s_k = kk;
+    __syncthreads();
+    if (kk == k) break;
   }
   if (tid == 0) kout[b] = s_k;
 }
@@ -278,7 +286,7 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
              T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed) {
   if (nmat <= 0) return;
-  HSSC_TIMED(sizeof(T) == 8 ? "cpqr_id_f64" : "cpqr_id_f32", st, 0.0, 0.0,
+  HSSC_TIMED(is_complex_v<T> ? "cpqr_id_c128" : "cpqr_id_f64", st, 0.0, 0.0,
              hipLaunchKernelGGL(cpqr_id_kernel<T>, dim3(nmat), dim3(ID_NT), 0, st, nmat, off, mr,
                                 nc, A, Acopy, acc, kfix, perm, poff, kout, Tout, toff, work, woff,
                                 seed));
@@ -287,6 +295,9 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
 template void cpqr_id<double>(hipStream_t, int, const int64_t*, const int*, const int*, double*,
                               const double*, double, const int*, int*, const int64_t*, int*,
                               double*, const int64_t*, double*, const int64_t*, uint64_t);
+template void cpqr_id<zdouble>(hipStream_t, int, const int64_t*, const int*, const int*, zdouble*,
+                               const zdouble*, double, const int*, int*, const int64_t*, int*,
+                               zdouble*, const int64_t*, zdouble*, const int64_t*, uint64_t);
 
 // Standard normal numbers (Box-Muller on the counter-based uniforms): G[i], i < len,
 // element i drawn from (seed, base + i).

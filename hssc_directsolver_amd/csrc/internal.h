@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <utility>
@@ -46,6 +47,9 @@ struct Level {
   int64_t n = 0, k = 0, M = 0;  // |I_src|, |I_sk|, proxy size
   int64_t nbox = 0, nkey = 0;   // boxes on the level, distinct (J,T) sets
   int64_t q = 0;                // columns of the stored T (n-k for dense T)
+  bool id_skel = false;         // skeletons chosen by an ID (kh > 0, two bottom levels)
+  std::vector<int32_t> kbox;    // nkey: each box's own rank (ID) or 'bdry' count before
+                                // padding to the level's k
 
   // Host copies (index sets are built on the host).
   hvec<int32_t> J;      // nkey x n   local permutation, skeleton first
@@ -95,8 +99,21 @@ struct hssc_tree {
 
 namespace hssc {
 
-// Host builders (tree.cpp).
-int build_index_sets(const hssc_params& p, hssc_tree* t);
+// One level's batch of skeleton IDs: for every key box the matrix K(Q, S) of its M proxy
+// points Q against its n source points S (I_src order), with the per-point weights.
+struct IdJob {
+  int lev;
+  int64_t nbox, n, M;
+  const double *Sx, *Sy, *Sw;  // nbox x n
+  const double *Qx, *Qy, *Qw;  // nbox x M
+};
+// Fills order (nbox x n, the ID's column order, skeleton first) and rank (nbox adaptive
+// ranks); returns an HSSC_* status.
+using IdSelect = std::function<int(const hssc_params&, const IdJob&, int32_t* order,
+                                   int32_t* rank)>;
+
+// Host builders (tree.cpp).  sel runs the IDs of the levels whose skeletons come from one.
+int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel = nullptr);
 void source_points(int Ng, int np, double* X /* N x 2 col-major */);
 
 // Kernel weight of a point (Common/Kernel_Eval.m:111-159, params.proxy == 0).

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -226,6 +227,77 @@ k::KernelSpec kernel_spec(const hssc_params& p) {
     ks.scale = 0.5 * h2;
   }
   return ks;
+}
+
+// Device buffers released together when the owner goes out of scope.
+struct DevBufs {
+  std::vector<void*> v;
+  template <typename T>
+  T* get(size_t count) {
+    T* d = dalloc<T>(count);
+    v.push_back(d);
+    return d;
+  }
+  template <typename T>
+  T* upload(const T* h, size_t count) {
+    T* d = get<T>(count);
+    if (count) HIPCHK(hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  ~DevBufs() {
+    for (void* p : v) dfree(p);
+  }
+};
+
+// Skeleton selection of one ID level (LOCAL_Non_Preset_Skeleton_Info,
+// HSS2D/HSS2D_build_tree_fsym.m:686-699): A_b = K(X_proxy, X_src) of every key box is
+// generated on the GPU and factored by the batched column-pivoted QR ID with the adaptive
+// rank of Common/ID.m:40-65 (|R_ii| > acc |R_11|, then the randomized check).  Returns the
+// pivot order of every box (skeleton first) and its rank.
+int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank) {
+  if (int s = check_device()) return s;
+  if (p.flag_pot != HSSC_POT_SL_H_2D)
+    return fail(HSSC_ERR_UNSUPPORTED, "ID-selected skeletons are defined for kh > 0 only");
+  if (jb.n < 1 || jb.M < 1 || jb.n > INT32_MAX || jb.M > INT32_MAX)
+    return fail(HSSC_ERR_INVALID_ARG, "empty or oversized ID block");
+  return guarded([&]() {
+    const int64_t nb = jb.nbox, n = jb.n, M = jb.M;
+    const int64_t tsz = n * n / 4 + n, wsz = 6 * n + M;  // T is k x (n-k) <= n^2/4
+    DevBufs B;
+    const double* Sx = B.upload(jb.Sx, nb * n);
+    const double* Sy = B.upload(jb.Sy, nb * n);
+    const double* Sw = B.upload(jb.Sw, nb * n);
+    const double* Qx = B.upload(jb.Qx, nb * M);
+    const double* Qy = B.upload(jb.Qy, nb * M);
+    const double* Qw = B.upload(jb.Qw, nb * M);
+    std::vector<int64_t> off(nb), poff(nb), toff(nb), woff(nb);
+    std::vector<int32_t> mr(nb, int32_t(M)), nc(nb, int32_t(n)), kfix(nb, 0);
+    for (int64_t b = 0; b < nb; ++b)
+      off[b] = b * M * n, poff[b] = b * n, toff[b] = b * tsz, woff[b] = b * wsz;
+    const int64_t* doff = B.upload(off.data(), nb);
+    const int64_t* dpoff = B.upload(poff.data(), nb);
+    const int64_t* dtoff = B.upload(toff.data(), nb);
+    const int64_t* dwoff = B.upload(woff.data(), nb);
+    const int32_t* dmr = B.upload(mr.data(), nb);
+    const int32_t* dnc = B.upload(nc.data(), nb);
+    const int32_t* dkfix = B.upload(kfix.data(), nb);
+    zdouble* A = B.get<zdouble>(nb * M * n);
+    zdouble* Ac = B.get<zdouble>(nb * M * n);
+    zdouble* Tt = B.get<zdouble>(nb * tsz);
+    zdouble* W = B.get<zdouble>(nb * wsz);
+    int32_t* dperm = B.get<int32_t>(nb * n);
+    int32_t* dk = B.get<int32_t>(nb);
+    const hipStream_t st = 0;
+    k::gen_block<zdouble>(st, M, n, nb, Qx, Qy, Qw, M, Sx, Sy, Sw, n, A, M, M * n, kernel_spec(p));
+    HIPCHK(hipMemcpyAsync(Ac, A, size_t(nb * M * n) * sizeof(zdouble), hipMemcpyDeviceToDevice, st));
+    k::cpqr_id<zdouble>(st, int(nb), doff, dmr, dnc, A, Ac, p.acc, dkfix, dperm, dpoff, dk, Tt,
+                        dtoff, W, dwoff, 0x5EEDull ^ uint64_t(jb.lev));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(order, dperm, size_t(nb * n) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rank, dk, size_t(nb) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return HSSC_OK;
+  });
 }
 }  // namespace hssc
 
@@ -833,7 +905,7 @@ int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
     return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
   return guarded([&]() {
     auto* t = new hssc_tree();
-    const int s = build_index_sets(*p, t);
+    const int s = build_index_sets(*p, t, id_select_gpu);
     if (s != HSSC_OK) {
       delete t;
       return s;
@@ -899,6 +971,14 @@ int hssc_tree_get_isrc(const hssc_tree* t, int64_t box, int64_t* out) {
   if (int s = box_level(t, box, &lev)) return s;
   if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
   std::copy(t->Isrc[box].begin(), t->Isrc[box].end(), out);
+  return HSSC_OK;
+}
+
+int hssc_tree_box_ranks(const hssc_tree* t, int32_t lev, int32_t* ranks, int32_t* id_level) {
+  if (!t || lev < 0 || lev > t->depth) return fail(HSSC_ERR_INVALID_ARG, "level out of range");
+  const Level& L = t->levels[lev];
+  if (ranks) std::copy(L.kbox.begin(), L.kbox.end(), ranks);
+  if (id_level) *id_level = L.id_skel ? 1 : 0;
   return HSSC_OK;
 }
 

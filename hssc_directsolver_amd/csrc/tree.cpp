@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <numeric>
 #include <thread>
 
@@ -64,24 +65,15 @@ std::vector<int32_t> argsort_stable(const std::vector<double>& key) {
   return idx;
 }
 
-// LOCAL_Preset_Skeleton_Info, 'bdry' branch (:804-864) with LOCAL_orient_skeleton
-// (:866-874) and LOCAL_orient_residual (:877-883).
-void preset_skeleton(const std::vector<double>& xs, const std::vector<double>& ys, double cx,
-                     double cy, int lev, const hssc_params& p, std::vector<int32_t>* J,
-                     int64_t* k) {
-  double colrad, rowrad;
-  level_radii(lev, &colrad, &rowrad);
-  const double width = p.layers * p.h;
-  const int64_t n = int64_t(xs.size());
-  std::vector<int32_t> sk, rs;
-  for (int64_t i = 0; i < n; ++i) {
-    const bool bdry = (std::fabs(xs[i] - cx) > (colrad - width)) ||
-                      (std::fabs(ys[i] - cy) > (rowrad - width));
-    (bdry ? sk : rs).push_back(int32_t(i));
-  }
+// J = [skeleton in polar order ; residual in interface order]: LOCAL_orient_skeleton
+// (:866-874, stable sort by radius, then by angle from th0) and LOCAL_orient_residual
+// (:877-883, stable sort by the normal coordinate, then by minus the tangential one).
+// LOCAL_Non_Preset_Skeleton_Info (:705-724) closes with the same two sorts.
+void orient(const std::vector<int32_t>& sk, const std::vector<int32_t>& rs,
+            const std::vector<double>& xs, const std::vector<double>& ys, double cx, double cy,
+            int lev, std::vector<int32_t>* J) {
   const bool even = (lev % 2) == 0;
   const double th0 = even ? M_PI / 2 : 0.0;
-  // Skeleton: stable sort by radius, then stable sort by angle from th0.
   std::vector<double> th(sk.size()), rho(sk.size());
   for (size_t i = 0; i < sk.size(); ++i) {
     const double dx = xs[sk[i]] - cx, dy = ys[sk[i]] - cy;
@@ -94,7 +86,6 @@ void preset_skeleton(const std::vector<double>& xs, const std::vector<double>& y
   std::vector<double> thJ1(sk.size());
   for (size_t i = 0; i < sk.size(); ++i) thJ1[i] = th[J1[i]];
   std::vector<int32_t> J2 = argsort_stable(thJ1);
-  // Residual: stable sort by the normal coordinate, then by minus the tangential one.
   std::vector<double> nrm(rs.size()), tang(rs.size());
   for (size_t i = 0; i < rs.size(); ++i) {
     nrm[i] = even ? xs[rs[i]] : ys[rs[i]];
@@ -108,7 +99,52 @@ void preset_skeleton(const std::vector<double>& xs, const std::vector<double>& y
   J->clear();
   for (size_t i = 0; i < sk.size(); ++i) J->push_back(sk[J1[J2[i]]]);
   for (size_t i = 0; i < rs.size(); ++i) J->push_back(rs[R1[R2[i]]]);
+}
+
+// LOCAL_Preset_Skeleton_Info, 'bdry' branch (:804-864): the points within layers*h of the
+// box boundary.  pad_to > count adds the residual points nearest the boundary (distance
+// min(colrad - |x - cx|, rowrad - |y - cy|), ties by index) until there are pad_to: the
+// levels above ID-selected skeletons have ragged counts, padded to the level's maximum.
+void preset_skeleton(const std::vector<double>& xs, const std::vector<double>& ys, double cx,
+                     double cy, int lev, const hssc_params& p, int64_t pad_to,
+                     std::vector<int32_t>* J, int64_t* k) {
+  double colrad, rowrad;
+  level_radii(lev, &colrad, &rowrad);
+  const double width = p.layers * p.h;
+  const int64_t n = int64_t(xs.size());
+  std::vector<int32_t> sk, rs;
+  for (int64_t i = 0; i < n; ++i) {
+    const bool bdry = (std::fabs(xs[i] - cx) > (colrad - width)) ||
+                      (std::fabs(ys[i] - cy) > (rowrad - width));
+    (bdry ? sk : rs).push_back(int32_t(i));
+  }
+  if (pad_to > int64_t(sk.size()) && !rs.empty()) {
+    std::vector<double> d(rs.size());
+    for (size_t i = 0; i < rs.size(); ++i)
+      d[i] = std::min(colrad - std::fabs(xs[rs[i]] - cx), rowrad - std::fabs(ys[rs[i]] - cy));
+    const std::vector<int32_t> o = argsort_stable(d);
+    const size_t add = std::min<size_t>(size_t(pad_to) - sk.size(), rs.size());
+    std::vector<char> moved(rs.size(), 0);
+    for (size_t i = 0; i < add; ++i) sk.push_back(rs[o[i]]), moved[o[i]] = 1;
+    std::vector<int32_t> keep;
+    for (size_t i = 0; i < rs.size(); ++i)
+      if (!moved[i]) keep.push_back(rs[i]);
+    rs.swap(keep);
+  }
+  orient(sk, rs, xs, ys, cx, cy, lev, J);
   *k = int64_t(sk.size());
+}
+
+// Number of 'bdry' skeleton points of a box (the count preset_skeleton finds before padding).
+int64_t bdry_count(const std::vector<double>& xs, const std::vector<double>& ys, double cx,
+                   double cy, int lev, const hssc_params& p) {
+  double colrad, rowrad;
+  level_radii(lev, &colrad, &rowrad);
+  const double width = p.layers * p.h;
+  int64_t c = 0;
+  for (size_t i = 0; i < xs.size(); ++i)
+    c += (std::fabs(xs[i] - cx) > (colrad - width)) || (std::fabs(ys[i] - cy) > (rowrad - width));
+  return c;
 }
 
 // MATLAB a:d:b for spans that are integer multiples of d up to rounding.
@@ -194,7 +230,7 @@ int proxy_layers(int lev, const hssc_params& p) {
 
 }  // namespace
 
-int build_index_sets(const hssc_params& p, hssc_tree* t) {
+int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
   if (p.Ng < 1 || p.np < 1 || (p.np & (p.np - 1)) != 0)
     return fail(HSSC_ERR_INVALID_ARG, "np must be a power of two and Ng >= 1");
   if (p.np < 2) return fail(HSSC_ERR_INVALID_ARG, "np must be at least 2");
@@ -249,6 +285,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
 
   t->levels.assign(depth + 1, Level{});
   const bool TI = p.transinv != 0;
+  bool ragged_below = false;  // skeleton sets of the level below differ between boxes
   // Upward skeleton pass (:321-622).
   for (int lev = depth; lev >= 0; --lev) {
     Level& L = t->levels[lev];
@@ -271,6 +308,71 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
       if (bi == 0) L.n = n;
       if (n != L.n) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform box sizes on a level");
     }
+    // For kh > 0 the two bottom levels take their skeletons from an ID of K(X_proxy, X_src)
+    // (:374-395 selects, LOCAL_Non_Preset_Skeleton_Info :686-729 computes); the proxy is
+    // the level's 'bdry' ring (:408-420).  Every level of a batch has one rank: boxes keep
+    // the first k_max columns of their ID order, k_max the level's largest adaptive rank,
+    // and preset levels above pad their 'bdry' sets to the level's largest count.
+    const bool id_level = p.kh > 0 && !p.preset_all && lev >= depth - 1;
+    L.id_skel = id_level;
+    L.kbox.assign(L.nkey, -1);
+    std::vector<int32_t> id_order;
+    int64_t pad_to = 0;
+    const int64_t nthr_all = std::max<int64_t>(
+        1, std::min<int64_t>({L.nkey, int64_t(std::thread::hardware_concurrency()), 16}));
+    auto parallel_keys = [&](const std::function<bool(int64_t)>& f) {
+      std::vector<std::thread> pool;
+      std::vector<char> ok(nthr_all, 1);
+      for (int64_t w = 0; w < nthr_all; ++w)
+        pool.emplace_back([&, w]() {
+          for (int64_t bi = w; bi < L.nkey; bi += nthr_all)
+            if (!f(bi)) ok[w] = 0;
+        });
+      for (auto& th : pool) th.join();
+      for (char c : ok)
+        if (!c) return false;
+      return true;
+    };
+    if (id_level) {
+      if (!sel) return fail(HSSC_ERR_UNSUPPORTED, "ID skeletons (kh > 0) need the GPU");
+      const int64_t n = L.n;
+      std::vector<double> qx0, qy0;
+      box_proxy(t->cent[2 * b0], t->cent[2 * b0 + 1], colrad, rowrad, lay_prox, p.h, &qx0, &qy0);
+      const int64_t M = int64_t(qx0.size());
+      hvec<double> Sx(L.nkey * n), Sy(L.nkey * n), Sw(L.nkey * n);
+      hvec<double> Qx(L.nkey * M), Qy(L.nkey * M), Qw(L.nkey * M);
+      const bool same = parallel_keys([&](int64_t bi) {
+        const int64_t b = b0 + bi;
+        const auto& I = t->Isrc[b];
+        for (int64_t i = 0; i < n; ++i)
+          Sx[bi * n + i] = Xx[I[i]], Sy[bi * n + i] = Xy[I[i]], Sw[bi * n + i] = wgt[I[i]];
+        std::vector<double> qx, qy;
+        box_proxy(t->cent[2 * b], t->cent[2 * b + 1], colrad, rowrad, lay_prox, p.h, &qx, &qy);
+        if (int64_t(qx.size()) != M) return false;
+        for (int64_t r = 0; r < M; ++r)
+          Qx[bi * M + r] = qx[r], Qy[bi * M + r] = qy[r], Qw[bi * M + r] = point_weight(p, qx[r], qy[r]);
+        return true;
+      });
+      if (!same) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform proxy sizes");
+      IdJob job{lev, L.nkey, n, M, Sx.data(), Sy.data(), Sw.data(), Qx.data(), Qy.data(), Qw.data()};
+      id_order.resize(L.nkey * n);
+      std::vector<int32_t> rank(L.nkey);
+      if (int s = sel(p, job, id_order.data(), rank.data())) return s;
+      for (int64_t bi = 0; bi < L.nkey; ++bi) {
+        L.kbox[bi] = rank[bi];
+        pad_to = std::max<int64_t>(pad_to, rank[bi]);
+      }
+    } else if (ragged_below && !TI) {
+      parallel_keys([&](int64_t bi) {
+        const int64_t b = b0 + bi;
+        const auto& I = t->Isrc[b];
+        std::vector<double> xs(L.n), ys(L.n);
+        for (int64_t i = 0; i < L.n; ++i) xs[i] = Xx[I[i]], ys[i] = Xy[I[i]];
+        L.kbox[bi] = int32_t(bdry_count(xs, ys, t->cent[2 * b], t->cent[2 * b + 1], lev, p));
+        return true;
+      });
+      for (int64_t bi = 0; bi < L.nkey; ++bi) pad_to = std::max<int64_t>(pad_to, L.kbox[bi]);
+    }
     // skeleton, J order and proxy of one box (key slot = bi for NTI, 0 for TI)
     auto process = [&](int64_t bi) -> bool {
       const int64_t b = b0 + bi;
@@ -281,7 +383,14 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
       const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
       std::vector<int32_t> J;
       int64_t k;
-      preset_skeleton(xs, ys, cx, cy, lev, p, &J, &k);
+      if (id_level) {
+        const int32_t* o = id_order.data() + (TI ? 0 : bi) * n;
+        const std::vector<int32_t> sk(o, o + pad_to), rs(o + pad_to, o + n);
+        orient(sk, rs, xs, ys, cx, cy, lev, &J);
+        k = pad_to;
+      } else {
+        preset_skeleton(xs, ys, cx, cy, lev, p, pad_to, &J, &k);
+      }
       std::vector<double> qx, qy;
       box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, &qx, &qy);
       if (bi == 0) {
@@ -313,7 +422,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
       return true;
     };
     if (!process(0)) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
-    if (!TI && L.nbox > 1 && dyadic) {
+    if (!TI && L.nbox > 1 && dyadic && !id_level && !ragged_below) {
       // All coordinates are exact dyadic rationals, so every box of the level is an exact
       // translate of box 0: the same J (same comparisons on bit-identical relative
       // coordinates) and the same proxy ring shifted by the centre difference.
@@ -371,6 +480,9 @@ int build_index_sets(const hssc_params& p, hssc_tree* t) {
       S.resize(L.k);
       for (int64_t a = 0; a < L.k; ++a) S[a] = t->Isrc[b][L.J[key * L.n + a]];
     }
+    for (auto& kb : L.kbox)
+      if (kb < 0) kb = int32_t(L.k);
+    if (id_level && !TI) ragged_below = true;
     L.q = L.n - L.k;
   }
   // Leaf gather map: global index of each J-ordered leaf point.

@@ -61,11 +61,13 @@ class Params:
 
 
 def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
-                        proxy_layers=None) -> Params:
+                        proxy_layers=None, preset_all=None) -> Params:
     """Common/HSS_tree_parameters.m:1.
 
     For kh > 0 the Duan-Rokhlin weight corr0 is computed by the library (``corr0``
-    overrides it); ``proxy_layers`` replaces the proxy-ring depth rule at lev > 0."""
+    overrides it); ``proxy_layers`` replaces the proxy-ring depth rule at lev > 0;
+    ``preset_all=True`` keeps the 'bdry' skeleton rule at the two bottom levels of a kh > 0
+    tree, where the reference selects skeletons by an ID."""
     raw = HsscParams()
     check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
                                  float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
@@ -75,6 +77,8 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
                                                float(np.imag(corr0))))
     if proxy_layers is not None:
         raw.proxy_layers = int(proxy_layers)
+    if preset_all is not None:
+        raw.preset_all = int(bool(preset_all))
     return Params(raw)
 
 
@@ -112,6 +116,16 @@ class Tree:
         vals = [C.c_int64() for _ in range(6)]
         check(lib().hssc_tree_level_info(self.handle, lev, *[C.byref(v) for v in vals]))
         return dict(zip(("n", "k", "M", "nbox", "nkey", "q"), (v.value for v in vals)))
+
+    def box_ranks(self, lev):
+        """(ranks, id_level): each key box's own skeleton count before the level's padding
+        (the adaptive ID rank on ID levels, the 'bdry' count on preset levels)."""
+        nkey = self.level_info(lev)["nkey"]
+        out = np.empty(nkey, dtype=np.int32)
+        idl = C.c_int32()
+        check(lib().hssc_tree_box_ranks(self.handle, int(lev),
+                                        out.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(idl)))
+        return out, bool(idl.value)
 
     def _i64(self, fn, box, size):
         out = np.empty(size, dtype=np.int64)

@@ -64,7 +64,10 @@ typedef struct hssc_params {
   int32_t factor_precision; /* HSSC_PREC_* (no reference counterpart: GPU policy)   */
   int32_t proxy_layers; /* > 0: proxy ring depth at lev > 0 instead of lay_prox
                            (HSS2D_build_tree_fsym.m:339-347); 0 keeps the reference rule */
-  int32_t reserved[6];
+  int32_t preset_all;  /* kh > 0: 0 = skeletons of the two bottom levels chosen by an ID of
+                          K(X_proxy, X_src) (HSS2D_build_tree_fsym.m:376, :686-729), as the
+                          reference; 1 = the 'bdry' rule at every level (uniform shapes) */
+  int32_t reserved[5];
 } hssc_params;
 
 typedef struct hssc_tree hssc_tree;         /* INFO_TREE + params.X_source            */
@@ -108,14 +111,20 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
                      int64_t n, double* A, int64_t lda);
 
 /* Replaces HSS2D/HSS2D_build_tree_fsym.m:1 (INFO_TREE = HSS2D_build_tree_fsym(params)):
- * uniform tree, 'bdry' skeletons and proxies (at every level, also for kh > 0), and the
- * interpolation operators T = K(Xp,X_sk) \ K(Xp,X_rs) computed on the GPU by Householder
- * least squares and stored dense (the 'lowrank' INTERform of the reference stores the same
+ * uniform tree, 'bdry' skeletons and proxies, and for kh > 0 (unless preset_all) skeletons
+ * of the two bottom levels chosen by a batched GPU ID of K(X_proxy, X_src)
+ * (LOCAL_Non_Preset_Skeleton_Info, :686-729).  Every box of a level keeps the level's
+ * largest rank: a box of adaptive rank k_b < k takes the next k - k_b columns of its ID
+ * order, and the preset levels above pad their 'bdry' sets to the level's largest count
+ * with the points nearest the box boundary.  Then the interpolation operators
+ * T = K(Xp,X_sk) \ K(Xp,X_rs), computed on the GPU by Householder least squares (at the
+ * ID levels the same T as the ID's R11^-1 R12, :726) and stored dense (the 'lowrank' INTERform of the reference stores the same
  * operator to tolerance as T.U * T.V.').  Complex (SL_H_2D) trees hold complex T. */
 int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree);
 
-/* Geometry and index sets only (no interpolation operators, no GPU work).  Used by the
- * same builder; exposed so index-set parity can be checked without a device. */
+/* Geometry and index sets only (no interpolation operators).  Used by the same builder;
+ * exposed so index-set parity can be checked without a device.  Needs the GPU only for the
+ * ID-selected skeletons of kh > 0 trees (preset_all = 0). */
 int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree);
 
 /* Tree queries (the fields of INFO_TREE, HSS2D/HSS2D_build_tree_fsym.m:655-680). */
@@ -126,6 +135,10 @@ int hssc_tree_level_info(const hssc_tree* t, int32_t lev, int64_t* n, int64_t* k
                          int64_t* M, int64_t* nbox, int64_t* nkey, int64_t* q);
 /* Heap box number b (1-based, as in the reference) -> I_src / I_sk (0-based global). */
 int hssc_tree_get_isrc(const hssc_tree* t, int64_t box, int64_t* out);
+/* Per key box of a level (nkey entries): its own skeleton count before the level's
+ * padding, i.e. the adaptive ID rank (ID levels) or the 'bdry' count (preset levels);
+ * *id_level = 1 if the level's skeletons come from the ID. */
+int hssc_tree_box_ranks(const hssc_tree* t, int32_t lev, int32_t* ranks, int32_t* id_level);
 int hssc_tree_get_isk(const hssc_tree* t, int64_t box, int64_t* out);
 /* J of box b (NTI) or of b's level (TI), 0-based local indices, skeleton first. */
 int hssc_tree_get_J(const hssc_tree* t, int64_t box, int64_t* out);

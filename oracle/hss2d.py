@@ -228,23 +228,62 @@ def level_orientation(lev):
     return 1, 0, 0.0
 
 
-def preset_skeleton(Xsrc, cB, lev, params):
+def preset_skeleton(Xsrc, cB, lev, params, pad_to=0):
     """'bdry' rule of LOCAL_Preset_Skeleton_Info, HSS2D/HSS2D_build_tree_fsym.m:804-864.
 
     Returns (J, k): J = [skeleton (polar order) ; residual (interface order)] as local
-    0-based indices into Xsrc.
+    0-based indices into Xsrc.  ``pad_to`` > count moves the residual points nearest the
+    box boundary (distance min(colrad - |x - cx|, rowrad - |y - cy|), ties by index) into
+    the skeleton until it has pad_to points (the library's uniform-level padding above
+    ID-selected levels; the reference keeps ragged counts).
     """
     colrad, rowrad = level_radii(lev)
     width = params["layers"] * params["h"]
     bdry = (np.abs(Xsrc[:, 0] - cB[0]) > (colrad - width)) | (
         np.abs(Xsrc[:, 1] - cB[1]) > (rowrad - width))
-    nrm, tan, th0 = level_orientation(lev)
     idx = np.arange(Xsrc.shape[0])
-    I_rs = idx[~bdry]
-    I_rs = I_rs[orient_residual(Xsrc[~bdry], nrm, tan)]
-    I_sk = idx[bdry]
-    I_sk = I_sk[orient_skeleton(Xsrc[bdry], cB, th0)]
+    I_sk, I_rs = idx[bdry], idx[~bdry]
+    if pad_to > I_sk.size and I_rs.size:
+        d = np.minimum(colrad - np.abs(Xsrc[I_rs, 0] - cB[0]),
+                       rowrad - np.abs(Xsrc[I_rs, 1] - cB[1]))
+        o = np.argsort(d, kind="stable")[:pad_to - I_sk.size]
+        I_sk = np.concatenate([I_sk, I_rs[o]])
+        I_rs = np.delete(I_rs, o)
+    nrm, tan, th0 = level_orientation(lev)
+    I_rs = I_rs[orient_residual(Xsrc[I_rs], nrm, tan)]
+    I_sk = I_sk[orient_skeleton(Xsrc[I_sk], cB, th0)]
     return np.concatenate([I_sk, I_rs]), I_sk.size
+
+
+def bdry_count(Xsrc, cB, lev, params):
+    colrad, rowrad = level_radii(lev)
+    width = params["layers"] * params["h"]
+    return int(np.sum((np.abs(Xsrc[:, 0] - cB[0]) > (colrad - width)) | (
+        np.abs(Xsrc[:, 1] - cB[1]) > (rowrad - width))))
+
+
+def non_preset_skeleton(Xp, Xsrc, cB, lev, params, rng, k=None):
+    """LOCAL_Non_Preset_Skeleton_Info, HSS2D/HSS2D_build_tree_fsym.m:686-729.
+
+    ID of A21 = K(X_ext, X_src) with the adaptive rank of Common/ID.m (``k`` fixes the
+    rank instead, the library's padding to the level's largest rank), then the skeleton in
+    polar order around cB and the residual along the interface (:705-724), T.U =
+    Ttmp(Js, Jr) and T.V = I (:726-727; stored dense here).  Returns (J, k, T, rank): rank
+    is the adaptive rank of the box.
+    """
+    A21 = kernel_eval(Xp, Xsrc, params)
+    Ttmp, I = id_acc(A21, params["acc"], rng)
+    rank = Ttmp.shape[0]
+    if k is None:
+        k = rank
+    elif k != rank:
+        Ttmp, I = id_fixed(A21, k)
+    nrm, tan, th0 = level_orientation(lev)
+    I_sk, I_rs = I[:k], I[k:]
+    Js = orient_skeleton(Xsrc[I_sk], cB, th0)
+    Jr = orient_residual(Xsrc[I_rs], nrm, tan)
+    J = np.concatenate([I_sk[Js], I_rs[Jr]])
+    return J, k, Ttmp[np.ix_(Js, Jr)], rank
 
 
 def _colon(a, d, b):
@@ -351,6 +390,12 @@ def id_acc(A, acc, rng):
         k = int(np.sum(Rdg > Rmx))
         T = _tsolve(R[:k, :k], R[:k, k:])
     return T, I
+
+
+def id_fixed(A, k):
+    """Common/ID.m:106-113 (fixed rank): T = R11^-1 R12 of the column-pivoted QR."""
+    Q, R, I = sla.qr(A, mode="economic", pivoting=True)
+    return _tsolve(R[:k, :k], R[:k, k:]), I
 
 
 def _rand_norm(A, I, k, rng):
@@ -467,6 +512,7 @@ class Tree:
     k: dict = field(default_factory=dict)       # key -> skeleton size
     T: dict = field(default_factory=dict)       # key -> (U, V) with T = U if V is None else U V^T
     Xp: dict = field(default_factory=dict)      # key -> proxy points
+    box_rank: dict = field(default_factory=dict)  # key -> own skeleton count (before padding)
 
     def key(self, b):
         """TI trees store (J, k, T) per level, NTI trees per box (:308-312)."""
@@ -489,21 +535,22 @@ def build_index_sets(params):
 
 
 def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False,
-               stop_level=0):
-    """HSS2D_build_tree_fsym (uniform tree, preset 'bdry' skeletons and proxies).
+               stop_level=0, pad=False):
+    """HSS2D_build_tree_fsym (uniform tree, 'bdry' proxies; 'bdry' or ID skeletons).
 
     Restates :81-287 (tree), :292-623 (upward skeleton pass) and
     LOCAL_Interpolation_Operators :1010-1064.  ``interform`` defaults to params
     INTERform ('lowrank' in the reference; 'dense' gives exact LS T at every level).
-    For kh > 0 the reference selects the skeletons of the two bottom levels by an ID
-    (:376-443); ``preset_all=True`` keeps the 'bdry' rule there as well (same operator
-    to tolerance, uniform level shapes).  ``stop_level`` > 0 ends the upward pass there
+    For kh > 0 the skeletons of the two bottom levels come from an ID of K(X_ext, X_src)
+    (:374-395 selects, LOCAL_Non_Preset_Skeleton_Info :686-729), with ragged per-box ranks
+    as in the reference; ``preset_all=True`` keeps the 'bdry' rule there instead.
+    ``pad=True`` restates the library's uniform levels: every box of an ID level keeps the
+    level's largest adaptive rank (fixed-rank ID), and preset levels above pad their 'bdry'
+    sets to the level's largest count.  ``stop_level`` > 0 ends the upward pass there
     (levels below the root only; used to time a bounded sample of a large tree).
     """
     if params["skel_rule"] != "bdry" or params["proxy_rule"] != "bdry":
         raise NotImplementedError("only the 'bdry' skeleton/proxy rules are restated")
-    if params["kh"] > 0 and not preset_all:
-        raise NotImplementedError("kh>0 with ID skeletons at the bottom levels: pass preset_all")
     interform = interform or params["INTERform"]
     rng = np.random.default_rng(seed)
     depth = params["depth"]
@@ -513,38 +560,60 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
     tree.I_src.update(leaf_slabs(params, cent))
     q = np.zeros(depth + 5, dtype=int)  # ranks of low-rank T per level (:299)
     TI = params["transinv"]
+    ragged_below = False
     import time as _time
     for lev in range(depth, stop_level - 1, -1):
         _t0 = _time.perf_counter()
         colrad, rowrad = level_radii(lev)
         lay_prox = proxy_layers(lev, params)
-        for b in range(2 ** lev, 2 ** (lev + 1)):
+        id_level = params["kh"] > 0 and not preset_all and lev >= depth - 1
+        boxes = range(2 ** lev, 2 ** (lev + 1))
+        for b in boxes:
             if lev < depth:
                 tree.I_src[b] = np.concatenate([tree.I_sk[2 * b], tree.I_sk[2 * b + 1]])
+        keyboxes = [2 ** lev] if TI else list(boxes)
+        pad_to = None
+        if pad and id_level:
+            ranks = [id_acc(kernel_eval(box_proxy(cent[b], colrad, rowrad, lay_prox, params["h"]),
+                                        X[tree.I_src[b]], params), params["acc"],
+                            np.random.default_rng([seed, b]))[0].shape[0]
+                     for b in keyboxes]
+            pad_to = max(ranks)
+        elif pad and ragged_below and not TI:
+            pad_to = max(bdry_count(X[tree.I_src[b]], cent[b], lev, params) for b in keyboxes)
+        for b in boxes:
             Xsrc = X[tree.I_src[b]]
             key = tree.key(b)
             if not TI or b == 2 ** lev:
                 cB = cent[b]
-                J, k = preset_skeleton(Xsrc, cB, lev, params)
                 Xp = box_proxy(cB, colrad, rowrad, lay_prox, params["h"])
-                n = J.size
-                if not with_T:
-                    U, V = None, None
-                elif n > k:
-                    lowrank = interform == "lowrank" and lev < depth - 1
-                    if lowrank and k >= params["k_cut"]:
-                        # interpolation_operator_HSS: same T to tolerance; restated by the
-                        # dense least-squares branch (HSS1D LS is out of scope)
-                        lowrank = True
-                    U, V = interpolation_operator(params, Xp, Xsrc[J[:k]], Xsrc[J[k:]],
-                                                  lowrank, q, lev, rng)
+                n = Xsrc.shape[0]
+                if id_level:
+                    J, k, Tid, rank = non_preset_skeleton(Xp, Xsrc, cB, lev, params,
+                                                          np.random.default_rng([seed, b]), pad_to)
+                    tree.box_rank[key] = rank
+                    U, V = (Tid, None) if with_T else (None, None)
                 else:
-                    U, V = np.zeros((k, 0)), None
+                    J, k = preset_skeleton(Xsrc, cB, lev, params, pad_to or 0)
+                    tree.box_rank[key] = bdry_count(Xsrc, cB, lev, params)
+                    if not with_T:
+                        U, V = None, None
+                    elif n > k:
+                        lowrank = interform == "lowrank" and lev < depth - 1
+                        # k >= k_cut selects interpolation_operator_HSS in the reference:
+                        # the same T to tolerance, restated by the dense least squares
+                        # (the HSS1D least squares is out of scope)
+                        U, V = interpolation_operator(params, Xp, Xsrc[J[:k]], Xsrc[J[k:]],
+                                                      lowrank, q, lev, rng)
+                    else:
+                        U, V = np.zeros((k, 0)), None
                 tree.J[key], tree.k[key], tree.T[key], tree.Xp[key] = J, k, (U, V), Xp
                 if with_T and V is not None and lev > 0:
                     q[lev] = max(q[lev], U.shape[1]) if not TI else U.shape[1]
             J, k = tree.J[key], tree.k[key]
             tree.I_sk[b] = tree.I_src[b][J[:k]]
+        if id_level and not TI:
+            ragged_below = True
         if timings is not None:
             timings[lev] = timings.get(lev, 0.0) + _time.perf_counter() - _t0
     return tree

@@ -83,8 +83,8 @@ def test_helmholtz_level_table_512():
             8: (504, 348, 576), 7: (696, 540, 832), 6: (1080, 732, 1088), 5: (1464, 1116, 1600),
             4: (2232, 1500, 2112), 3: (3000, 2268, 3136), 2: (4536, 3036, 4160),
             1: (6072, 4572, 6208), 0: (9144, 6108, 6180)}
-    P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 64, 1e-10, 3, 800, 0)
-    P.raw.proxy_layers = 4
+    P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 64, 1e-10, 3, 800, 0, proxy_layers=4,
+                              preset_all=True)
     t = H.HSS2D_build_index_sets(P)
     for lev, (n, k, M) in want.items():
         info = t.level_info(lev)
@@ -122,3 +122,9 @@ def test_gpu_entry_points_fail_loudly_without_device():
     P = H.HSS_tree_parameters("SL_L_2D", 0, 1, 8, 4, 1e-10, 2, 400, 1)
     with pytest.raises(H.HsscError):
         H.HSS2D_build_tree_fsym(P)
+    # kh > 0 index sets need the GPU for the ID-selected bottom levels, and not otherwise
+    Ph = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 4, 1e-10, 3, 800, 0)
+    with pytest.raises(H.HsscError):
+        H.HSS2D_build_index_sets(Ph)
+    Ph = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 4, 1e-10, 3, 800, 0, preset_all=True)
+    assert H.HSS2D_build_index_sets(Ph).level_info(4)["k"] == 60

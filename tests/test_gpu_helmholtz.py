@@ -2,9 +2,10 @@
 
 Kernel (Common/Kernel_Eval.m:44-49): w (den==0) + h^2 k^2 (i/4) b H0(k r) c with the
 Lippmann-Schwinger weights b = c = sqrt(bump) and the Duan-Rokhlin diagonal w.  k = 20,
-layers = 3, acc = 1e-10, 4-layer proxy rings.  The oracle keeps the 'bdry' skeletons at every
-level (preset_all) like the GPU tree; the reference's ID skeletons at the two bottom levels
-give the same operator to tolerance.  Complex symmetric: plain transposes throughout.
+layers = 3, acc = 1e-10, 4-layer proxy rings.  These cases keep the 'bdry' skeletons at every
+level (preset_all) in both the GPU tree and the oracle, so every index set agrees exactly; the
+reference's ID skeletons at the two bottom levels are covered by test_gpu_id_skeletons.py.
+Complex symmetric: plain transposes throughout.
 """
 import numpy as np
 import pytest
@@ -19,7 +20,8 @@ _cache = {}
 
 
 def params(Ng, np_, TI):
-    P = H.HSS_tree_parameters("SL_H_2D", KH, 1, Ng, np_, 1e-10, 3, 800, TI, proxy_layers=4)
+    P = H.HSS_tree_parameters("SL_H_2D", KH, 1, Ng, np_, 1e-10, 3, 800, TI, proxy_layers=4,
+                              preset_all=True)
     h = 2.0 / (Ng * np_)
     p = o.tree_parameters("SL_H_2D", KH, 1, Ng, np_, 1e-10, 3, 800, TI, corr0=corr0(h, KH))
     p["proxy_layers"] = 4

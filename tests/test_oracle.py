@@ -90,3 +90,88 @@ def test_lowrank_interpolation_matches_dense():
     yd = o.applyinv(t := o.build_tree(p, interform="dense"), o.invert(t), b).ravel()
     yl = o.applyinv(t2 := o.build_tree(p, interform="lowrank"), o.invert(t2), b).ravel()
     assert rel(yl, yd) < 1e-8
+
+
+def _helm(Ng, np_, TI=0):
+    h = 2 / (Ng * np_)
+    p = o.tree_parameters("SL_H_2D", 20.0, 1, Ng, np_, 1e-10, 3, 800, TI, corr0=corr0(h, 20.0))
+    p["proxy_layers"] = 4
+    return p
+
+
+def test_id_skeletons_of_a_box():
+    # LOCAL_Non_Preset_Skeleton_Info (HSS2D_build_tree_fsym.m:686-729) on one leaf of a
+    # 32^2 Helmholtz grid: the ID reconstructs K(X_ext, X_src) to the tolerance, the rank is
+    # the adaptive one, J is a permutation with the skeleton in polar order around the box
+    # centre and the residual along the interface, and T = R11^-1 R12 is the least-squares
+    # operator of the same skeleton (the library computes it that way)
+    p = _helm(8, 4)
+    X = p["X_source"]
+    cent = o.box_centers(p["depth"])
+    lev, b = p["depth"], 2 ** p["depth"] + 5
+    Xsrc = X[o.leaf_slabs(p, cent)[b]]
+    colrad, rowrad = o.level_radii(lev)
+    Xp = o.box_proxy(cent[b], colrad, rowrad, o.proxy_layers(lev, p), p["h"])
+    J, k, T, rank = o.non_preset_skeleton(Xp, Xsrc, cent[b], lev, p, np.random.default_rng(0))
+    assert k == rank and 0 < k < Xsrc.shape[0]
+    assert np.array_equal(np.sort(J), np.arange(Xsrc.shape[0]))
+    A = o.kernel_eval(Xp, Xsrc, p)
+    err = np.linalg.norm(A[:, J[:k]] @ T - A[:, J[k:]])
+    assert err <= 10 * p["acc"] * np.linalg.norm(A)
+    # one column fewer breaks the tolerance (the rank is not padded)
+    T2, I2 = o.id_fixed(A, k - 8)
+    assert np.linalg.norm(A[:, I2[:k - 8]] @ T2 - A[:, I2[k - 8:]]) > 10 * err
+    nrm, tan, th0 = o.level_orientation(lev)
+    Xs, Xr = Xsrc[J[:k]], Xsrc[J[k:]]
+    assert np.array_equal(o.orient_skeleton(Xs, cent[b], th0), np.arange(k))
+    assert np.array_equal(o.orient_residual(Xr, nrm, tan), np.arange(Xr.shape[0]))
+    th = np.arctan2(Xs[:, 1] - cent[b][1], Xs[:, 0] - cent[b][0])
+    th = np.where(th < th0, th + 2 * np.pi, th)
+    assert np.all(np.diff(th) >= 0)
+    assert rel(o.ls_solve(A[:, J[:k]], A[:, J[k:]]), T) < 1e-6
+    # a fixed rank above the adaptive one is the padded form: same leading pivots
+    Tp, Ip = o.id_fixed(A, k + 2)
+    _, I0 = o.id_fixed(A, k)
+    assert np.array_equal(Ip[:k], I0[:k])
+
+
+def test_preset_padding_adds_the_points_nearest_the_boundary():
+    p = _helm(8, 4)
+    rng = np.random.default_rng(5)
+    cB = np.array([0.25, -0.5])
+    lev = 3
+    colrad, rowrad = o.level_radii(lev)
+    Xsrc = np.column_stack([cB[0] + rng.uniform(-colrad, colrad, 90),
+                            cB[1] + rng.uniform(-rowrad, rowrad, 90)])
+    J0, k0 = o.preset_skeleton(Xsrc, cB, lev, p)
+    assert k0 == o.bdry_count(Xsrc, cB, lev, p)
+    J, k = o.preset_skeleton(Xsrc, cB, lev, p, pad_to=k0 + 7)
+    assert k == k0 + 7
+    assert set(J0[:k0]) <= set(J[:k])
+    d = np.minimum(colrad - np.abs(Xsrc[:, 0] - cB[0]), rowrad - np.abs(Xsrc[:, 1] - cB[1]))
+    added = np.array(sorted(set(J[:k]) - set(J0[:k0])))
+    rest = J[k:]
+    assert added.size == 7 and d[added].max() <= d[rest].min()
+
+
+def test_id_skeleton_tree_vs_dense():
+    # kh > 0 with the reference's ID skeletons at the two bottom levels (ragged ranks) and
+    # the library's padded form: both solve the dense system to the tolerance and agree
+    # (the adaptive ID ranks of Common/ID.m leave the ragged tree at ~1e-9 here, the few
+    # extra skeletons of the padded one at ~1e-11)
+    p = _helm(8, 4)
+    A = o.dense_matrix(p)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(A.shape[0]) + 1j * rng.standard_normal(A.shape[0])
+    ytrue = np.linalg.solve(A, b)
+    t = o.build_tree(p, interform="dense")
+    tp = o.build_tree(p, interform="dense", pad=True)
+    d = t.depth
+    ks = {t.k[t.key(bb)] for bb in range(2 ** (d - 1), 2 ** d)}
+    assert len(ks) > 1 or len({t.k[t.key(bb)] for bb in range(2 ** d, 2 ** (d + 1))}) > 1
+    for lev in range(d + 1):
+        assert len({tp.k[tp.key(bb)] for bb in range(2 ** lev, 2 ** (lev + 1))}) == 1
+    y = o.applyinv(t, o.invert(t), b).ravel()
+    yp = o.applyinv(tp, o.invert(tp), b).ravel()
+    assert rel(y, ytrue) < 1e-8 and rel(yp, ytrue) < 1e-9
+    assert rel(yp, y) < 1e-8
