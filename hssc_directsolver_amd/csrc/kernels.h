@@ -71,6 +71,24 @@ constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (51
 template <typename T>
 void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
               T* Tf, int64_t sT, int64_t batch);
+// Tall panels (M - j0 > kCqrMinRows): shifted CholeskyQR3 with Householder reconstruction.
+// Writes the panel's R rows into A, V (rows j0..M-1, ld ldv) and Tf as qr_panel + qr_copyV
+// do.  ws: batch * cqr_ws_elems(M) elements.
+constexpr int kCqrRows = 256;
+constexpr int64_t kCqrMinRows = 1024;
+constexpr int64_t cqr_ws_elems(int64_t M) {
+  return int64_t(2 + 2 * (kQrMaxRows / kCqrRows)) * kQrNb * kQrNb + M * kQrNb;
+}
+template <typename T>
+void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
+               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, T* ws, int64_t batch);
+// T (n x n, upper) of H_0 ... H_{ns-1} from the 32 x 32 factors Ts (stride 32*32 per block,
+// sTs per matrix) and the Gram matrix GV = V^H V (n x n); X: n * 32 scratch per matrix.
+template <typename T>
+void t_assemble(hipStream_t st, int n, const T* Ts, int64_t sTs, const T* GV, int64_t ldg,
+                int64_t sG, T* To, int64_t ldt, int64_t sT, T* X, int64_t sX, int64_t batch);
+template <typename T>
+void zero2d(hipStream_t st, int64_t m, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
 template <typename T>
 void qr_copyV(hipStream_t st, int64_t M, int64_t j0, int jb, const T* A, int64_t lda, int64_t sA,
               T* V, int64_t ldv, int64_t sV, int64_t batch);

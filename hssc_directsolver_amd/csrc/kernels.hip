@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <limits>
 #include <cstdint>
 #include <string>
 
@@ -1004,6 +1005,375 @@ void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int
 }
 
 // ------------------------------------------------------------------------------------
+// Tall panels: shifted CholeskyQR3 and Householder reconstruction
+// ------------------------------------------------------------------------------------
+// A panel of Mp = M - j0 > 1024 rows keeps the one-workgroup Householder kernel streaming it
+// through L2 once per column, on one CU.  Tall panels are factored instead by shifted
+// CholeskyQR3 (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42,
+// 2020): Q_1 = A R_1^-1, Q_2 = Q_1 R_2^-1, Q = Q_2 R_3^-1 with R_p the Cholesky factor of
+// the Gram matrix of the previous Q (the first one shifted by 11 (Mp jb + jb (jb + 1)) u
+// ||A||_F^2), R = R_3 R_2 R_1.  The orthonormal Q is turned back into Householder form
+// (Ballard, Demmel, Grigori, Jacquelin, Knight, Nguyen, IPDPS 2014): the LU factorisation
+// Q_top - S = L U, no pivoting, with the unit-modulus diagonal S chosen during the
+// elimination (s_k = -c / |c| for the pivot candidate c, so |pivot| >= 1), gives
+// V = [L; Q_below U^-1], Tf = -U S^H L^-H and the panel's R rows S R, with
+// I - V Tf V^H = H_0 ... H_{jb-1} unitary and its first jb columns Q S^H: what the
+// Householder panel produces up to unit-modulus row scalings of R, so the trailing update
+// and the back substitution are shared.
+//
+// Four launches of one kernel, 256 rows per workgroup.  Pass p > 0 first forms the
+// Cholesky factor of the summed partial Gram matrices of pass p - 1 (every workgroup
+// redundantly: 32 x 32, one barrier per elimination step), then applies it to its rows by
+// forward substitution; passes 0-2 leave the partial Gram matrices of their rows for the
+// next pass (two alternating buffers).  Workgroup 0 accumulates R and, in pass 3, writes Tf,
+// the R rows of A and the top rows of V.
+
+constexpr int CQR_ROWS = kCqrRows;                     // rows per workgroup
+constexpr int CQR_PMAX = int(kQrMaxRows / CQR_ROWS);   // partial Gram blocks per matrix
+constexpr int CQR_G = QR_NB * QR_NB;                    // one NB x NB block
+constexpr int CQR_NE = CQR_G / CQR_ROWS;                // NB x NB entries per thread
+static_assert(QR_NB == kQrNb, "panel width");
+static_assert(CQR_ROWS == 256 && CQR_G % CQR_ROWS == 0, "entry split");
+
+// workspace offsets (per matrix): Racc | Qtop | partial Gram A | partial Gram B | Q_1
+constexpr int64_t CQR_RACC = 0, CQR_QTOP = CQR_G, CQR_PART0 = 2 * CQR_G,
+                  CQR_PART1 = CQR_PART0 + int64_t(CQR_PMAX) * CQR_G,
+                  CQR_Q1 = CQR_PART1 + int64_t(CQR_PMAX) * CQR_G;
+static_assert(CQR_Q1 == int64_t(2 + 2 * (kQrMaxRows / kCqrRows)) * kQrNb * kQrNb,
+              "workspace layout");
+
+template <typename T>
+__global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
+    int pass, int64_t Mp, int jb, int P, const T* X, int64_t ldx, int64_t sX, T* Qo,
+    int64_t ldq, int64_t sQ, T* __restrict__ ws, int64_t sW, T* V, int64_t ldv, int64_t sV,
+    T* __restrict__ A, int64_t lda, int64_t sA, int64_t j0, T* __restrict__ Tfo, int64_t sT) {
+  using R = real_t<T>;
+  __shared__ T Rm[QR_NB][QR_NB + 1];       // Cholesky factor of this pass
+  __shared__ T Wb[2 * QR_NB][QR_NB + 1];   // Gram staging (64 rows); pass 3: LU | scratch
+  __shared__ T sdiag[QR_NB];               // S of the reconstruction
+  __shared__ R rdi[QR_NB];                 // 1 / R_cc
+  __shared__ T udi[QR_NB];                 // 1 / U_cc
+  const int64_t e = blockIdx.y;
+  const int blk = blockIdx.x, tid = threadIdx.x;
+  const bool wg0 = blk == 0;
+  ws += e * sW;
+  X += e * sX;
+  const T* pin = ws + ((pass & 1) ? CQR_PART0 : CQR_PART1);   // partial Grams of pass - 1
+  T* pout = ws + ((pass & 1) ? CQR_PART1 : CQR_PART0);        // partial Grams of this pass
+  if (pass > 0) {
+    // R = chol(sum of the partial Gram matrices (+ shift))
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) {
+      const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+      T sum = T(0);
+      for (int q = 0; q < P; ++q) sum += pin[int64_t(q) * CQR_G + ent];
+      Rm[i][j] = sum;
+    }
+    __syncthreads();
+    if (pass == 1) {
+      R tr = R(0);
+      for (int i = 0; i < jb; ++i) tr += re_s(Rm[i][i]);
+      const R u = std::numeric_limits<R>::epsilon() / 2;
+      const R shift = R(11) * (R(Mp) * R(jb) + R(jb) * R(jb + 1)) * u * tr;
+      __syncthreads();
+      if (tid < jb) Rm[tid][tid] += T(shift);
+      __syncthreads();
+    }
+    for (int k = 0; k < jb; ++k) {
+      const R d2 = re_s(Rm[k][k]);
+      const R d = d2 > R(0) ? sqrt(d2) : std::numeric_limits<R>::min();
+      const T inv2 = T(R(1) / (d * d));
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        if (i > k && i <= j && j < jb) Rm[i][j] -= conj_s(Rm[k][i]) * Rm[k][j] * inv2;
+      }
+      __syncthreads();
+      // row k of R (the next step reads rows > k only)
+      if (tid >= k && tid < jb) Rm[k][tid] = tid == k ? T(d) : Rm[k][tid] * T(R(1) / d);
+    }
+    __syncthreads();
+    if (tid < jb) rdi[tid] = R(1) / re_s(Rm[tid][tid]);
+    if (wg0) {
+      // Racc <- R Racc (Racc = R_1 after pass 1)
+      T* Racc = ws + CQR_RACC;
+      T* Old = &Wb[QR_NB][0];
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        Wb[QR_NB + i][j] = pass == 1 ? (i == j ? T(1) : T(0)) : Racc[ent];
+      }
+      (void)Old;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        T sum = T(0);
+        if (i <= j && j < jb)
+          for (int t = i; t <= j; ++t) sum += Rm[i][t] * Wb[QR_NB + t][j];
+        Racc[ent] = sum;
+      }
+    }
+    __syncthreads();
+  }
+  if (pass == 3) {
+    // Q_top = Q2_top R^-1 (rows by forward substitution), then LU of Q_top - S
+    const T* Qtop = ws + CQR_QTOP;
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) {
+      const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+      Wb[i][j] = (i < jb && j < jb) ? Qtop[ent] : T(0);
+    }
+    __syncthreads();
+    if (tid < jb) {
+      const int i = tid;
+      for (int c = 0; c < jb; ++c) {
+        T sum = Wb[i][c];
+        for (int t = 0; t < c; ++t) sum -= Wb[i][t] * Rm[t][c];
+        Wb[i][c] = sum * T(rdi[c]);
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < jb; ++k) {
+      const T c = Wb[k][k];
+      const R a = sqrt(abs2(c));
+      const T sk = a > R(0) ? -(c * T(R(1) / a)) : T(-1);
+      const T piv = c - sk;
+      const T pinv = T(1) / piv;
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        if (i > k && j > k && i < jb && j < jb) Wb[i][j] -= Wb[i][k] * pinv * Wb[k][j];
+      }
+      __syncthreads();
+      // column k of L and the pivot (the next step reads column k + 1 and row k + 1 only)
+      if (tid > k && tid < jb) Wb[tid][k] = Wb[tid][k] * pinv;
+      if (tid == k) Wb[k][k] = piv, sdiag[k] = sk;
+    }
+    __syncthreads();
+    if (tid < jb) udi[tid] = T(1) / Wb[tid][tid];
+    __syncthreads();
+  }
+  // rows of this workgroup: x <- x R^-1 (pass 3: then x <- x U^-1)
+  const int64_t r = int64_t(blk) * CQR_ROWS + tid;
+  const bool live = r < Mp && !(pass == 3 && r < jb);
+  T x[QR_NB];
+#pragma unroll
+  for (int c = 0; c < QR_NB; ++c) x[c] = (live && c < jb) ? X[r + int64_t(c) * ldx] : T(0);
+  if (pass > 0) {
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) {
+      if (c < jb) {
+        T sum = x[c];
+#pragma unroll
+        for (int i = 0; i < c; ++i) sum -= x[i] * Rm[i][c];
+        x[c] = sum * T(rdi[c]);
+      }
+    }
+  }
+  if (pass == 3) {
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) {
+      if (c < jb) {
+        T sum = x[c];
+#pragma unroll
+        for (int i = 0; i < c; ++i) sum -= x[i] * Wb[i][c];
+        x[c] = sum * udi[c];
+      }
+    }
+  }
+  if (Qo && live) {
+    Qo += e * sQ;
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c)
+      if (c < jb) Qo[r + int64_t(c) * ldq] = x[c];
+  }
+  if (pass == 2 && wg0 && tid < jb) {
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) ws[CQR_QTOP + tid + c * QR_NB] = c < jb ? x[c] : T(0);
+  }
+  if (pass < 3) {
+    // partial Gram matrix of this workgroup's rows, staged through LDS 64 rows at a time
+    T acc[CQR_NE];
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) acc[m] = T(0);
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int w = 0; w < CQR_ROWS / 64; ++w) {
+      __syncthreads();
+      if (wave == w) {
+#pragma unroll
+        for (int c = 0; c < QR_NB; ++c) Wb[lane][c] = x[c];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        T sum = T(0);
+        for (int t = 0; t < 64; ++t) sum += conj_s(Wb[t][i]) * Wb[t][j];
+        acc[m] += sum;
+      }
+    }
+    T* pp = pout + int64_t(blk) * CQR_G;
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) pp[tid + m * CQR_ROWS] = acc[m];
+    return;
+  }
+  if (!wg0) return;
+  // pass 3, workgroup 0: Tf = -U S^H L^-H, the panel's R rows S Racc, the top rows of V = L
+  T* Racc = ws + CQR_RACC;
+  Tfo += e * sT;
+  A += e * sA;
+  V += e * sV;
+#pragma unroll
+  for (int m = 0; m < CQR_NE; ++m) {
+    const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+    if (i < jb && j < jb) {
+      A[(j0 + i) + (j0 + j) * lda] = i <= j ? sdiag[i] * Racc[ent] : T(0);
+      V[i + int64_t(j) * ldv] = i > j ? Wb[i][j] : (i == j ? T(1) : T(0));
+    }
+  }
+  if (tid < QR_NB) {
+    // row i of Tf: Tf L^H = B with B = -U S^H (upper); unit upper L^H, ascending columns
+    const int i = tid;
+    T* Tr = &Wb[QR_NB + i][0];
+    for (int j = 0; j < QR_NB; ++j) {
+      T b = (i <= j && j < jb) ? -(Wb[i][j] * conj_s(sdiag[j])) : T(0);
+      if (j < jb)
+        for (int t = 0; t < j; ++t) b -= Tr[t] * conj_s(Wb[j][t]);
+      Tr[j] = (i < jb && j < jb) ? b : T(0);
+      Tfo[i + j * QR_NB] = Tr[j];
+    }
+  }
+}
+
+template <typename T>
+void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
+               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, T* ws, int64_t batch) {
+  const int64_t Mp = M - j0, sW = cqr_ws_elems(M);
+  if (Mp <= 0 || jb <= 0 || batch <= 0) return;
+  const int P = int((Mp + CQR_ROWS - 1) / CQR_ROWS);
+  T* Q1 = ws + CQR_Q1;
+  const T* Ap = A + j0 + j0 * lda;
+  const dim3 g(P, batch), b(CQR_ROWS);
+  const double fl = (is_complex_v<T> ? 4.0 : 1.0) * 2.0 * 5.0 * double(Mp) * jb * jb * batch;
+  const double by = 7.0 * double(Mp) * jb * sizeof(T) * batch;
+  HSSC_TIMED(lbl<T>("cqr_panel").c_str(), st, fl, by, {
+    // pass 0: Gram of the panel; 1: Q_1 -> workspace; 2: Q_2 -> V; 3: V <- Householder form
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 0, Mp, jb, P, Ap, lda, sA, (T*)nullptr,
+                       int64_t(0), int64_t(0), ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 1, Mp, jb, P, Ap, lda, sA, Q1, Mp, sW, ws,
+                       sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 2, Mp, jb, P, (const T*)Q1, Mp, sW, V,
+                       ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 3, Mp, jb, P, (const T*)V, ldv, sV, V,
+                       ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+  });
+}
+
+// Triangular factor of a product of block reflectors: H = H_0 H_1 ... H_{ns-1}, H_s =
+// I - V_s Ts_s V_s^H with V = [V_0 ... V_{ns-1}] (n columns, blocks of 32) gives
+// H = I - V T V^H with T upper block triangular, diagonal blocks Ts_s and
+// T(0:c0, c0:c0+w) = -T(0:c0, 0:c0) (V^H V)(0:c0, c0:c0+w) Ts_s for the block at c0 = 32 s
+// (LAPACK xlarft's recurrence, one block at a time).  One workgroup per matrix; X is scratch
+// of n * 32 elements per matrix.
+template <typename T>
+__global__ void __launch_bounds__(256) t_assemble_kernel(int n, const T* __restrict__ Ts,
+                                                         int64_t sTs, const T* __restrict__ GV,
+                                                         int64_t ldg, int64_t sG,
+                                                         T* __restrict__ To, int64_t ldt,
+                                                         int64_t sT, T* __restrict__ X,
+                                                         int64_t sX) {
+  __shared__ T Ta[QR_NB][QR_NB + 1];
+  __shared__ T Tb[QR_NB][QR_NB + 1];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x, ti = tid % QR_NB, tj = tid / QR_NB;  // entries (ti, tj + 8 m)
+  Ts += e * sTs;
+  GV += e * sG;
+  To += e * sT;
+  X += e * sX;
+  const int ns = (n + QR_NB - 1) / QR_NB;
+  // diagonal blocks Ts_s, zeros below
+  for (int idx = tid; idx < n * n; idx += 256) {
+    const int i = idx % n, j = idx / n;
+    To[i + int64_t(j) * ldt] =
+        (i / QR_NB == j / QR_NB) ? Ts[(i / QR_NB) * CQR_G + (i % QR_NB) + (j % QR_NB) * QR_NB]
+                                 : T(0);
+  }
+  for (int sb = 1; sb < ns; ++sb) {
+    const int c0 = sb * QR_NB, w = n - c0 < QR_NB ? n - c0 : QR_NB;
+    // Y_q = GV(q block, c0 block) Ts_sb for q < sb, into X (32 x 32 per q)
+    __syncthreads();
+    for (int idx = tid; idx < CQR_G; idx += 256) {
+      const int i = idx % QR_NB, j = idx / QR_NB;
+      Ta[i][j] = (i < w && j < w) ? Ts[sb * CQR_G + idx] : T(0);
+    }
+    for (int q = 0; q < sb; ++q) {
+      for (int idx = tid; idx < CQR_G; idx += 256) {
+        const int i = idx % QR_NB, j = idx / QR_NB;
+        Tb[i][j] = j < w ? GV[(q * QR_NB + i) + int64_t(c0 + j) * ldg] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int j = tj + 8 * m;
+        T sum = T(0);
+        for (int t = 0; t < QR_NB; ++t) sum += Tb[ti][t] * Ta[t][j];
+        X[q * CQR_G + ti + j * QR_NB] = sum;
+      }
+      __syncthreads();
+    }
+    // T(r block, c0 block) = -sum_{q = r}^{sb - 1} T(r, q) Y_q
+    for (int r = 0; r < sb; ++r) {
+      T acc[CQR_NE];
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) acc[m] = T(0);
+      for (int q = r; q < sb; ++q) {
+        for (int idx = tid; idx < CQR_G; idx += 256) {
+          const int i = idx % QR_NB, j = idx / QR_NB;
+          Tb[i][j] = To[(r * QR_NB + i) + int64_t(q * QR_NB + j) * ldt];
+          Ta[i][j] = X[q * CQR_G + idx];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < CQR_NE; ++m) {
+          const int j = tj + 8 * m;
+          for (int t = 0; t < QR_NB; ++t) acc[m] += Tb[ti][t] * Ta[t][j];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int j = tj + 8 * m;
+        if (j < w) To[(r * QR_NB + ti) + int64_t(c0 + j) * ldt] = -acc[m];
+      }
+    }
+  }
+}
+
+template <typename T>
+void t_assemble(hipStream_t st, int n, const T* Ts, int64_t sTs, const T* GV, int64_t ldg,
+                int64_t sG, T* To, int64_t ldt, int64_t sT, T* X, int64_t sX, int64_t batch) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(t_assemble_kernel<T>, dim3(batch), dim3(256), 0, st, n, Ts, sTs, GV, ldg, sG,
+                     To, ldt, sT, X, sX);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) zero2d_kernel(int64_t m, int64_t n, T* A, int64_t lda,
+                                                     int64_t sA) {
+  const int64_t e = blockIdx.z;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x, j = blockIdx.y;
+  if (i < m) A[e * sA + i + j * lda] = T(0);
+}
+
+template <typename T>
+void zero2d(hipStream_t st, int64_t m, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(zero2d_kernel<T>, dim3((m + 255) / 256, n, batch), dim3(256), 0, st, m, n, A,
+                     lda, sA);
+}
+
+// ------------------------------------------------------------------------------------
 // Batched GEMV (solve sweeps of HSS2D_applyinv_*.m / HSS2D_apply_*.m, one RHS)
 // ------------------------------------------------------------------------------------
 // y = alpha * op(A) x + gamma * z   (z may alias nothing; gamma = 0 ignores it)
@@ -1295,6 +1665,11 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
                             int64_t, int64_t, int64_t);                                            \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
+  template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \
+                             int64_t, T*, int64_t, T*, int64_t);                                   \
+  template void t_assemble<T>(hipStream_t, int, const T*, int64_t, const T*, int64_t, int64_t, T*,  \
+                              int64_t, int64_t, T*, int64_t, int64_t);                             \
+  template void zero2d<T>(hipStream_t, int64_t, int64_t, T*, int64_t, int64_t, int64_t);            \
   template void gemv<T>(hipStream_t, bool, int64_t, int64_t, T, const T*, int64_t, int64_t,        \
                         const T*, int64_t, T, const T*, int64_t, T*, int64_t, int64_t, T*);        \
   template void sumsq<T>(hipStream_t, int64_t, const T*, double*, int);                           \

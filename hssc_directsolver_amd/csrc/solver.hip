@@ -12,6 +12,7 @@
 #include <cmath>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <type_traits>
@@ -211,6 +212,13 @@ SolveWork<zfloat>& SolveWorks::get<zfloat>() {
   return c;
 }
 
+// Hardware queues of this process: HIP's GPU_MAX_HW_QUEUES (default 4), at most 8.
+int hw_queues() {
+  const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+  const int q = v ? std::atoi(v) : 4;
+  return std::min(8, std::max(1, q > 0 ? q : 4));
+}
+
 // The GPU form of Common/Kernel_Eval.m for the tree's kernel.
 k::KernelSpec kernel_spec(const hssc_params& p) {
   k::KernelSpec ks;
@@ -358,9 +366,14 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   // over kLevelStreams streams by longest-processing-time on an estimated cost and their
   // steps are submitted round-robin, so no level waits behind another's whole sequence when
   // streams share a hardware queue.
-  constexpr int kLevelStreams = 8;
+  // Streams are mapped onto the process's few hardware queues by the runtime (kernels of
+  // streams sharing a queue run one after the other); twice as many streams as queues keeps
+  // every queue fed whatever the mapping.
+  const int kLevelStreams = 2 * hw_queues();
+  constexpr int64_t kOuterNb = 128;  // outer panel width of the tall levels
   struct Ws {
-    S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr;
+    S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
+      *cq = nullptr, *Ts = nullptr, *GV = nullptr;
   };
   std::vector<Ws> ws(t->depth + 1);
   std::vector<hipStream_t> streams(kLevelStreams, nullptr);
@@ -368,8 +381,13 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   auto cleanup = [&]() {
     for (auto& s : streams)
-      if (s) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
-    for (auto& w : ws) dfree(w.A), dfree(w.V), dfree(w.W1), dfree(w.W2), dfree(w.Tf), dfree(w.part);
+      if (s) (void)hipStreamSynchronize(s);
+    stats::flush();
+    for (auto& s : streams)
+      if (s) (void)hipStreamDestroy(s);
+    for (auto& w : ws)
+      dfree(w.A), dfree(w.V), dfree(w.W1), dfree(w.W2), dfree(w.Tf), dfree(w.part), dfree(w.cq),
+          dfree(w.Ts), dfree(w.GV);
   };
   try {
     std::vector<std::vector<std::function<void(hipStream_t)>>> steps(t->depth + 1);
@@ -387,12 +405,22 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       Ws& w = ws[l];
       const int64_t sA = M * n;
       const int64_t nsplit = (M + kchunk - 1) / kchunk;
+      // Tall levels: 128-column outer panels made of 32-column CholeskyQR3 sub-panels, so
+      // the trailing matrix (the HBM-bound part) is streamed once per 128 columns.  Other
+      // levels: 32-column one-workgroup Householder panels.
+      const bool tall = M > k::kCqrMinRows;
+      const int64_t nbo = tall ? kOuterNb : nb;
       w.A = dalloc<S>(E * sA);
-      w.V = dalloc<S>(E * M * nb);
-      w.W1 = dalloc<S>(E * nb * n);
-      w.W2 = dalloc<S>(E * nb * n);
-      w.Tf = dalloc<S>(E * nb * nb);
-      w.part = dalloc<S>(E * nsplit * nb * n);
+      w.V = dalloc<S>(E * M * nbo);
+      w.W1 = dalloc<S>(E * nbo * n);
+      w.W2 = dalloc<S>(E * nbo * n);
+      w.Tf = dalloc<S>(E * nbo * nbo);
+      w.part = dalloc<S>(E * nsplit * nbo * n);
+      if (tall) {
+        w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
+        w.Ts = dalloc<S>(E * kOuterNb * nb);
+        w.GV = dalloc<S>(E * kOuterNb * kOuterNb);
+      }
       S* A = w.A;
       auto& sl = steps[l];
       // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
@@ -400,21 +428,50 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
         k::gen_block<S>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
                         ks);
       });
-      for (int64_t j0 = 0; j0 < kk; j0 += nb) {
+      for (int64_t j0 = 0; j0 < kk; j0 += nbo) {
         sl.push_back([=](hipStream_t st) {
-          const int jb = int(std::min(nb, kk - j0));
-          k::qr_panel<S>(st, M, j0, jb, A, M, sA, w.Tf, nb * nb, E);
-          const int64_t nc = n - j0 - jb;
-          if (nc <= 0) return;
+          const int jbo = int(std::min(nbo, kk - j0));
+          const int64_t nc = n - j0 - jbo;
           const int64_t Mp = M - j0;
-          k::qr_copyV<S>(st, M, j0, jb, A, M, sA, w.V, Mp, Mp * nb, E);
-          S* C = A + j0 + (j0 + jb) * M;
+          const int64_t sV = Mp * nbo;
+          if (tall) {
+            k::zero2d<S>(st, Mp, jbo, w.V, Mp, sV, E);
+            for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
+              const int64_t j1 = j0 + s0, Mps = M - j1;
+              const int jb = int(std::min<int64_t>(nb, jbo - s0));
+              S* Vs = w.V + s0 + s0 * Mp;
+              S* Tss = w.Ts + (s0 / nb) * nb * nb;
+              k::cqr_panel<S>(st, M, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq, E);
+              // this sub-panel's reflectors on the rest of the outer panel
+              const int64_t nin = jbo - s0 - jb;
+              if (nin <= 0) continue;
+              S* Cin = A + j1 + (j1 + jb) * M;
+              k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin,
+                                   E, w.part, kchunk);
+              k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
+                           w.W2, jb, jb * nin, E);
+              k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
+                         S(1), Cin, M, sA, E);
+            }
+            if (nc <= 0) return;
+            // T of the outer panel from the sub-panel factors and V^H V
+            k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo,
+                                 jbo * jbo, E, w.part, kchunk);
+            k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
+                             nbo * nbo, w.W2, nbo * n, E);
+          } else {
+            k::qr_panel<S>(st, M, j0, jbo, A, M, sA, w.Tf, nb * nb, E);
+            if (nc <= 0) return;
+            k::qr_copyV<S>(st, M, j0, jbo, A, M, sA, w.V, Mp, sV, E);
+          }
+          const int64_t ldt = tall ? jbo : nb;
+          S* C = A + j0 + (j0 + jbo) * M;
           // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
-          k::gemm_tn_splitk<S>(st, jb, nc, Mp, w.V, Mp, Mp * nb, C, M, sA, w.W1, jb, jb * nc, E,
+          k::gemm_tn_splitk<S>(st, jbo, nc, Mp, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
                                w.part, kchunk);
-          k::gemm_h<S>(st, jb, nc, jb, S(1), w.Tf, nb, nb * nb, w.W1, jb, jb * nc, S(0), w.W2, jb,
-                       jb * nc, E);
-          k::gemm<S>(st, false, false, Mp, nc, jb, S(-1), w.V, Mp, Mp * nb, w.W2, jb, jb * nc,
+          k::gemm_h<S>(st, jbo, nc, jbo, S(1), w.Tf, ldt, nbo * nbo, w.W1, jbo, jbo * nc, S(0),
+                       w.W2, jbo, jbo * nc, E);
+          k::gemm<S>(st, false, false, Mp, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
                      S(1), C, M, sA, E);
         });
       }
@@ -505,7 +562,11 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   T* Wk = dalloc<T>(wmax);
   // second stream for the look-ahead of the blocked Gauss-Jordan sweeps
   hipStream_t aux = nullptr;
-  if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) aux = nullptr;
+  HIPCHK(hipGetLastError());
+  if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) {
+    aux = nullptr;  // the sweeps run without look-ahead
+    (void)hipGetLastError();
+  }
   for (int l = D; l >= 0; --l) {
     const Level& L = t->levels[l];
     const int64_t n = L.n, kk = L.k, r = n - kk, E = L.nkey;
@@ -541,7 +602,11 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
     }
   }
   HIPCHK(hipStreamSynchronize(st));
-  if (aux) (void)hipStreamDestroy(aux);
+  if (aux) {
+    (void)hipStreamSynchronize(aux);
+    stats::flush();
+    (void)hipStreamDestroy(aux);
+  }
   HIPCHK(hipGetLastError());
   dfree(Wk);
 }

@@ -29,7 +29,13 @@ std::vector<Slot*> pending;
 std::vector<Slot*> pool;
 std::map<std::string, Totals> totals;
 
+// Reads the timings of the recorded launches.  The event calls can fail (an event recorded
+// on a stream that has since been destroyed); such a launch is not counted, and the error
+// it leaves is cleared so that it does not surface as the library's own last error (an
+// error that was already pending is left in place).
 void drain_locked() {
+  const bool clean = hipPeekAtLastError() == hipSuccess;
+  bool failed = false;
   for (Slot* s : pending) {
     float ms = 0;
     if (hipEventSynchronize(s->stop) == hipSuccess &&
@@ -39,10 +45,13 @@ void drain_locked() {
       t.flops += s->flops;
       t.bytes += s->bytes;
       t.ms += ms;
+    } else {
+      failed = true;
     }
     pool.push_back(s);
   }
   pending.clear();
+  if (failed && clean) (void)hipGetLastError();
 }
 }  // namespace
 
@@ -66,6 +75,11 @@ Slot* begin(const char* name, hipStream_t st, double flops, double bytes) {
   s->bytes = bytes;
   (void)hipEventRecord(s->start, st);
   return s;
+}
+
+void flush() {
+  std::lock_guard<std::mutex> g(mu);
+  if (!pending.empty()) drain_locked();
 }
 
 void end(Slot* s, hipStream_t st) {

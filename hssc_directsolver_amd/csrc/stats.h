@@ -15,6 +15,8 @@ bool enabled();
 struct Slot;
 Slot* begin(const char* name, hipStream_t st, double flops, double bytes);
 void end(Slot* s, hipStream_t st);
+// Reads the recorded launches now (call before destroying a stream they were recorded on).
+void flush();
 
 }  // namespace stats
 }  // namespace hssc
