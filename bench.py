@@ -3,10 +3,12 @@
 
 Workload (default): the Lippmann-Schwinger Helmholtz volume integral equation of the
 reference, kernel w (den==0) + h^2 k^2 (i/4) b H0(k r) c (SL_H_2D, Common/Kernel_Eval.m:44-49,
-b = c = sqrt(bump), Duan-Rokhlin diagonal w) with k = 20 on the uniform 512 x 512 grid of
+Duan-Rokhlin diagonal w) with k = 20 on the uniform 512 x 512 grid of
 HSS_tree_parameters('SL_H_2D', 20, 1, 8, 64, 1e-10, 3, 800, 0) (N = 262144, leaf 8x8,
-depth 12, per-box operators), 'bdry' skeletons at every level, 4-layer proxy rings,
-dense-block inverse, seeded unit-norm complex Gaussian right-hand side.
+depth 12, per-box operators), b = c = sqrt(q) for the reference's tanh-plateau contrast q
+(Bump_function option 4, Common/Kernel_Eval.m:133,143), ID-selected skeletons at the two
+bottom levels and 'bdry' skeletons above, 4-layer proxy rings, dense-block inverse, seeded
+unit-norm complex Gaussian right-hand side.
 `--kernel laplace` runs the SL_L_2D grid of HSS_tree_parameters('SL_L_2D', 0, 1, 8, 64,
 1e-10, 2, 400, 0) instead; it is also reported as a side case.
 
@@ -47,6 +49,8 @@ def parse():
     ap.add_argument("--kernel", choices=("helmholtz", "laplace"), default="helmholtz")
     ap.add_argument("--kh", type=float, default=20.0)
     ap.add_argument("--proxy-layers", type=int, default=4)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched HSS1D side case (256 contours)")
     ap.add_argument("--preset-all", action="store_true",
                     help="kh > 0: 'bdry' skeletons at every level instead of the reference's "
                          "ID-selected skeletons at the two bottom levels")
@@ -141,6 +145,36 @@ def hss1d_case(H, torch, N=4096, reps=3):
                                 solve_ms=round(1e3 * (c3 - c2), 2), total_s=round(c3 - c0, 3)))
 
 
+def batched_case(H, torch, P=256, seed=2024):
+    """Batched HSS1D: P star contours r(t) = 1 + a cos(m t), m in {3..8}, a in [0.1, 0.4], N
+    log-uniform in [1024, 16384] (seeded), Laplace double layer, leaf 64, ID tol 1e-10, 64
+    proxy points; one forest compressed, inverted and solved by shared per-level launches."""
+    import numpy as np
+    from hssc_directsolver_amd.rank_mlp import random_contour
+    rng = np.random.default_rng(seed)
+    Ns = [int(np.exp(rng.uniform(np.log(1024), np.log(16384)))) for _ in range(P)]
+    curves = [random_contour(rng, N) for N in Ns]
+    f = np.random.default_rng(0).standard_normal(sum(Ns))
+    best = None
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K = H.HSS1D_compress_batched(curves, 64, 1e-10, nproxy=64)
+        t1 = time.perf_counter()
+        H.HSS1D_invert_nsym(K)
+        t2 = time.perf_counter()
+        x = H.HSS1D_applyinv_nsym(K, f)
+        t3 = time.perf_counter()
+        r = (t1 - t0, t2 - t1, t3 - t2)
+        best = r if best is None or sum(r) < sum(best) else best
+    resid = float(np.linalg.norm(H.HSS1D_apply_nsym(K, x) - f) / np.linalg.norm(f))
+    return dict(workload=f"batched HSS1D DL, {P} random star contours, total N={sum(Ns)}, "
+                         f"leaf 64 tol 1e-10 nproxy 64, FP64",
+                compress_s=round(best[0], 4), invert_s=round(best[1], 4),
+                solve_ms=round(1e3 * best[2], 3), total_s=round(sum(best), 4),
+                N_per_s=round(sum(Ns) / sum(best), 1), relres_vs_hss=resid)
+
+
 def level_flops(n, k, M, cx):
     """Dense-block build flops of one box: T by Householder QR least squares
     (2Mk^2 + 4Mk(n-k)) and F^-1, R F^-1 R^T, E^-1 (2n^3 + 2n^2 k + 2nk^2 + 2k^3); x4 for
@@ -194,6 +228,12 @@ def cpu_baseline(H, P, args):
                 measured_s=round(t1 - t0, 2))
 
 
+def level_shapes(H, P):
+    """(n, k, M) of every level of the tree the workload builds."""
+    t = H.HSS2D_build_index_sets(P)
+    return {lev: t.level_info(lev) for lev in range(t.depth + 1)}
+
+
 def make_problem(H, torch, kernel, args):
     """Params and the seeded unit-norm right-hand side of one workload."""
     gen = torch.Generator(device="cpu").manual_seed(0)
@@ -205,7 +245,8 @@ def make_problem(H, torch, kernel, args):
                           torch.randn(P.N, generator=gen, dtype=torch.float64))
         wl = (f"HSS2D SL_H_2D Lippmann-Schwinger k={args.kh:g} {'TI' if args.TI else 'NTI'} "
               f"Ng={args.Ng} np={args.np_} N={P.N} acc=1e-10 layers=3 proxy_layers="
-              f"{args.proxy_layers} dense blocks, "
+              f"{args.proxy_layers} dense blocks, tanh-plateau contrast (Bump_function "
+              "option 4), "
               + ("'bdry' skeletons at every level" if args.preset_all else
                  "ID skeletons at the two bottom levels"))
     else:
@@ -304,6 +345,7 @@ def main():
         side = dict(workload=wl, **summarize([one_step(H, Pl, torch, bl, args.tol)[:2]], Pl.N))
     cpu = None if args.no_cpu else cpu_baseline(H, P, args)
     h1 = hss1d_case(H, torch) if not args.no_hss1d else None
+    hb = batched_case(H, torch) if not args.no_batched else None
     cx = args.kernel == "helmholtz"
 
     out = {
@@ -328,8 +370,10 @@ def main():
                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
                     for k, v in stats.items()},
         "cpu_baseline": cpu,
+        "levels": {lev: [v["n"], v["k"], v["M"]] for lev, v in level_shapes(H, P).items()},
         "laplace": side,
         "hss1d": h1,
+        "hss1d_batched": hb,
     }
     print(json.dumps(out), flush=True)
 

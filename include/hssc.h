@@ -118,8 +118,9 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
  * order, and the preset levels above pad their 'bdry' sets to the level's largest count
  * with the points nearest the box boundary.  Then the interpolation operators
  * T = K(Xp,X_sk) \ K(Xp,X_rs), computed on the GPU by Householder least squares (at the
- * ID levels the same T as the ID's R11^-1 R12, :726) and stored dense (the 'lowrank' INTERform of the reference stores the same
- * operator to tolerance as T.U * T.V.').  Complex (SL_H_2D) trees hold complex T. */
+ * ID levels the same T as the ID's R11^-1 R12, :726) and stored dense (the 'lowrank'
+ * INTERform of the reference stores the same operator to tolerance as T.U * T.V.').
+ * Complex (SL_H_2D) trees hold complex T. */
 int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree);
 
 /* Geometry and index sets only (no interpolation operators).  Used by the same builder;
