@@ -149,8 +149,8 @@ void gen_parent(hipStream_t st, int64_t n, int64_t kc, int64_t batch, const int3
 // Each wave owns a 32x32 quarter = 2x2 MFMA 16x16 tiles.  The MFMA is issued as
 // D = B^T A^T so that a lane's 4 accumulators run down one column of C with lane%16
 // indexing rows: column-major stores of 16 consecutive rows coalesce.
-// Complex operands are split into real and imaginary LDS planes and each k-step issues
-// four real MFMAs per tile (re*re - im*im, re*im + im*re).
+// Complex operands are split into real and imaginary LDS planes; each k-step issues four
+// real MFMAs per tile for complex double and three (Gauss's product) for complex float.
 
 // Operand layout (both types): lane l supplies A(l%16, l/16) and B(l/16, l%16).
 // Result layout, measured on gfx950: item r of lane l holds D(row(l,r), l%16) with
@@ -213,9 +213,15 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
   if (i0 >= m || j0 >= n) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
-  v4 acc[NP][2][2];
+  // complex float: three real products per tile (Gauss): P0 = re re, P1 = im im,
+  // P2 = (re + im)(re + im); Re = P0 - P1, Im = P2 - P0 - P1.  Complex double keeps the four
+  // products (re re - im im, re im + im re): faster there, and the FP64 path's rounding stays
+  // that of the plain product.
+  constexpr bool G3 = CX && sizeof(R) == 4;
+  constexpr int NACC = CX ? (G3 ? 3 : 2) : 1;
+  v4 acc[NACC][2][2];
 #pragma unroll
-  for (int p = 0; p < NP; ++p)
+  for (int p = 0; p < NACC; ++p)
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -287,14 +293,29 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          acc[0][x][y] = Mfma<R>::mma(bv[0][y], av[0][x], acc[0][x][y]);
-          if constexpr (CX) {
+        for (int y = 0; y < 2; ++y) acc[0][x][y] = Mfma<R>::mma(bv[0][y], av[0][x], acc[0][x][y]);
+      if constexpr (G3) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y)
+            acc[1][x][y] = Mfma<R>::mma(bv[NP - 1][y], av[NP - 1][x], acc[1][x][y]);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y)
+            acc[NACC - 1][x][y] = Mfma<R>::mma(bv[0][y] + bv[NP - 1][y], av[0][x] + av[NP - 1][x],
+                                               acc[NACC - 1][x][y]);
+      } else if constexpr (CX) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
             acc[0][x][y] = Mfma<R>::mma(-bv[NP - 1][y], av[NP - 1][x], acc[0][x][y]);
-            acc[NP - 1][x][y] = Mfma<R>::mma(bv[NP - 1][y], av[0][x], acc[NP - 1][x][y]);
-            acc[NP - 1][x][y] = Mfma<R>::mma(bv[0][y], av[NP - 1][x], acc[NP - 1][x][y]);
+            acc[1][x][y] = Mfma<R>::mma(bv[NP - 1][y], av[0][x], acc[1][x][y]);
+            acc[1][x][y] = Mfma<R>::mma(bv[0][y], av[NP - 1][x], acc[1][x][y]);
           }
-        }
+      }
     }
     if (more) lstore(cur ^ 1);
     __syncthreads();
@@ -313,8 +334,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
         if (gi < m && gj < n) {
           T* c = C + gi + gj * ldc;
           T a;
-          if constexpr (CX) a = T(acc[0][x][y][r], acc[NP - 1][x][y][r]);
-          else a = acc[0][x][y][r];
+          if constexpr (G3)
+            a = T(acc[0][x][y][r] - acc[1][x][y][r],
+                  acc[NACC - 1][x][y][r] - acc[0][x][y][r] - acc[1][x][y][r]);
+          else if constexpr (CX)
+            a = T(acc[0][x][y][r], acc[1][x][y][r]);
+          else
+            a = acc[0][x][y][r];
           const T v = alpha * a;
           *c = (beta == T(0)) ? v : v + beta * *c;
         }
