@@ -14,5 +14,5 @@ from .hss2d import (  # noqa: F401
 from .interp import ID, LRID_rand  # noqa: F401
 from .hss1d import (  # noqa: F401
     Curve, HSS1D_apply_nsym, HSS1D_applyinv_nsym, HSS1D_compress_batched, HSS1D_compress_nsym,
-    HSS1D_invert_nsym, split, star_contour,
+    HSS1D_invert_nsym, HSS1D_solve, split, star_contour,
 )

@@ -90,6 +90,11 @@ SIGNATURES = [
     ("hssc_mlp_train", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
                                  C.c_int32, C.c_float, _DP]),
     ("hssc_hss1d_invert", C.c_int, [_P]),
+    ("hssc_hss1d_invert_prec", C.c_int, [_P, C.c_int32]),
+    ("hssc_hss1d_solve", C.c_int, [_P, _DP, _DP, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]),
+    ("hssc_hss1d_solve_device", C.c_int, [_P, _P, _P, C.c_double, C.c_int32,
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     ("hssc_hss1d_applyinv", C.c_int, [_P, _DP, _DP, C.c_int64]),
     ("hssc_hss1d_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),
     ("hssc_hss1d_applyinv_device", C.c_int, [_P, _P, _P, C.c_int64]),

@@ -145,20 +145,21 @@ struct BoxInv {
   int k1, k2;
 };
 
-__device__ void gj_inplace(double* A, int n, int* piv, double* f, int* sp) {
+template <typename T>
+__device__ void gj_inplace(T* A, int n, int* piv, T* f, int* sp) {
   // Gauss-Jordan with partial pivoting on an n x n column-major matrix in global memory
   // (one workgroup); swaps undone by columns at the end.
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int c = 0; c < n; ++c) {
     if (tid < 64) {
-      double best = -1;
+      T best = -1;
       int bi = c;
       for (int r = c + tid; r < n; r += 64) {
-        const double v = fabs(A[r + c * n]);
+        const T v = fabs(A[r + c * n]);
         if (v > best) best = v, bi = r;
       }
       for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o);
+        const T ob = __shfl_xor(best, o);
         const int oi = __shfl_xor(bi, o);
         if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
       }
@@ -168,14 +169,14 @@ __device__ void gj_inplace(double* A, int n, int* piv, double* f, int* sp) {
     const int p = *sp;
     if (p != c)
       for (int j = tid; j < n; j += nt) {
-        const double t = A[c + j * n];
+        const T t = A[c + j * n];
         A[c + j * n] = A[p + j * n];
         A[p + j * n] = t;
       }
     __syncthreads();
     for (int i = tid; i < n; i += nt) f[i] = A[i + c * n];
     __syncthreads();
-    const double inv = 1.0 / f[c];
+    const T inv = T(1) / f[c];
     for (int j = tid; j < n; j += nt) A[c + j * n] = (j == c) ? inv : A[c + j * n] * inv;
     __syncthreads();
     for (int idx = tid; idx < n * n; idx += nt) {
@@ -190,7 +191,7 @@ __device__ void gj_inplace(double* A, int n, int* piv, double* f, int* sp) {
     const int p = piv[c];
     if (p != c) {
       for (int i = tid; i < n; i += nt) {
-        const double t = A[i + c * n];
+        const T t = A[i + c * n];
         A[i + c * n] = A[i + p * n];
         A[i + p * n] = t;
       }
@@ -199,86 +200,92 @@ __device__ void gj_inplace(double* A, int n, int* piv, double* f, int* sp) {
   }
 }
 
+// Blocks of the inverse (F, E, R Finv, Finv L, Linv, Rinv, Dinv) live in W (element type T:
+// double for the FP64 inverse, float for the FP32 one), at their arena offsets minus wofs;
+// the compressed operator (M_self, M_sib, T, permutations) is read from S (double) and I.
+template <typename T>
 __global__ void __launch_bounds__(256) hss1d_invert_kernel(const BoxInv* __restrict__ boxes,
-                                                           double* __restrict__ S,
+                                                           const double* __restrict__ S,
+                                                           T* __restrict__ W, int64_t wofs,
                                                            const int* __restrict__ I) {
-  __shared__ double f[256];
+  __shared__ T f[256];
   __shared__ int piv[256];
   __shared__ int sp;
   const BoxInv B = boxes[blockIdx.x];
   const int n = B.n, k = B.k, tid = threadIdx.x, nt = blockDim.x;
-  double* F = S + B.F;
+  W -= wofs;  // W[o] for arena offsets o >= wofs
+  T* F = W + B.F;
   // assemble F
   for (int idx = tid; idx < n * n; idx += nt) {
     const int i = idx % n, j = idx / n;
-    double v;
+    T v;
     if (B.leaf) {
-      v = S[B.Mself + idx];
+      v = T(S[B.Mself + idx]);
     } else if (i < B.k1 && j < B.k1) {
-      v = S[B.E1 + i + j * B.k1];
+      v = W[B.E1 + i + j * B.k1];
     } else if (i >= B.k1 && j >= B.k1) {
-      v = S[B.E2 + (i - B.k1) + (j - B.k1) * B.k2];
+      v = W[B.E2 + (i - B.k1) + (j - B.k1) * B.k2];
     } else if (i < B.k1) {
-      v = S[B.S1 + i + (j - B.k1) * B.k1];  // M_sib(c1): k1 x k2
+      v = T(S[B.S1 + i + (j - B.k1) * B.k1]);  // M_sib(c1): k1 x k2
     } else {
-      v = S[B.S2 + (i - B.k1) + j * B.k2];  // M_sib(c2): k2 x k1
+      v = T(S[B.S2 + (i - B.k1) + j * B.k2]);  // M_sib(c2): k2 x k1
     }
     F[idx] = v;
   }
   __syncthreads();
-  gj_inplace(F, n, piv, f, &sp);
+  gj_inplace<T>(F, n, piv, f, &sp);
   __syncthreads();
   const int* Ju = I + B.Jup;
   const int* Jd = I + B.Jdn;
   const double* Tu = S + B.Tup;  // k x (n-k)
   const double* Td = S + B.Tdn;
-  double* RF = S + B.RF;
-  double* FL = S + B.FL;
+  T* RF = W + B.RF;
+  T* FL = W + B.FL;
   // RF = R Finv: row i = Finv(Ju[i], :) + sum_c Tu(i, c) Finv(Ju[k+c], :)
   for (int idx = tid; idx < k * n; idx += nt) {
     const int i = idx % k, j = idx / k;
-    double s = F[Ju[i] + j * n];
-    for (int c = 0; c < n - k; ++c) s += Tu[i + c * k] * F[Ju[k + c] + j * n];
+    T s = F[Ju[i] + j * n];
+    for (int c = 0; c < n - k; ++c) s += T(Tu[i + c * k]) * F[Ju[k + c] + j * n];
     RF[idx] = s;
   }
   // FL = Finv L: column j = Finv(:, Jd[j]) + sum_c Finv(:, Jd[k+c]) Td(j, c)
   for (int idx = tid; idx < n * k; idx += nt) {
     const int i = idx % n, j = idx / n;
-    double s = F[i + Jd[j] * n];
-    for (int c = 0; c < n - k; ++c) s += F[i + Jd[k + c] * n] * Td[j + c * k];
+    T s = F[i + Jd[j] * n];
+    for (int c = 0; c < n - k; ++c) s += F[i + Jd[k + c] * n] * T(Td[j + c * k]);
     FL[idx] = s;
   }
   __syncthreads();
   // E^-1 = RF L = R Finv L
-  double* E = S + B.E;
+  T* E = W + B.E;
   for (int idx = tid; idx < k * k; idx += nt) {
     const int i = idx % k, j = idx / k;
-    double s = RF[i + Jd[j] * k];
-    for (int c = 0; c < n - k; ++c) s += RF[i + Jd[k + c] * k] * Td[j + c * k];
+    T s = RF[i + Jd[j] * k];
+    for (int c = 0; c < n - k; ++c) s += RF[i + Jd[k + c] * k] * T(Td[j + c * k]);
     E[idx] = s;
   }
   __syncthreads();
-  gj_inplace(E, k, piv, f, &sp);
+  gj_inplace<T>(E, k, piv, f, &sp);
   __syncthreads();
-  double* Li = S + B.Linv;
-  double* Ri = S + B.Rinv;
-  double* Di = S + B.Dinv;
+  T* Li = W + B.Linv;
+  T* Ri = W + B.Rinv;
+  T* Di = W + B.Dinv;
   for (int idx = tid; idx < n * k; idx += nt) {
     const int i = idx % n, j = idx / n;
-    double s = 0;
+    T s = 0;
     for (int q = 0; q < k; ++q) s += FL[i + q * n] * E[q + j * k];
     Li[idx] = s;  // Linv = Finv L E
   }
   for (int idx = tid; idx < k * n; idx += nt) {
     const int i = idx % k, j = idx / k;
-    double s = 0;
+    T s = 0;
     for (int q = 0; q < k; ++q) s += E[i + q * k] * RF[q + j * k];
     Ri[idx] = s;  // Rinv = E R Finv
   }
   __syncthreads();
   for (int idx = tid; idx < n * n; idx += nt) {
     const int i = idx % n, j = idx / n;
-    double s = F[idx];
+    T s = F[idx];
     for (int q = 0; q < k; ++q) s -= Li[i + q * n] * RF[q + j * k];
     Di[idx] = s;  // Dinv = Finv - Linv R Finv
   }
@@ -291,27 +298,30 @@ struct TopOp {
   int k2, k3;
 };
 
+template <typename T>
 __global__ void __launch_bounds__(256) hss1d_top_kernel(const TopOp* __restrict__ ops,
-                                                        double* __restrict__ S) {
-  __shared__ double f[256];
+                                                        const double* __restrict__ S,
+                                                        T* __restrict__ W, int64_t wofs) {
+  __shared__ T f[256];
   __shared__ int piv[256];
   __shared__ int sp;
   const TopOp O = ops[blockIdx.x];
   const int64_t FT = O.FT, E2 = O.E2, E3 = O.E3, S2 = O.S2, S3 = O.S3;
   const int k2 = O.k2, k3 = O.k3;
   const int n = k2 + k3, tid = threadIdx.x;
-  double* F = S + FT;
+  W -= wofs;
+  T* F = W + FT;
   for (int idx = tid; idx < n * n; idx += blockDim.x) {
     const int i = idx % n, j = idx / n;
-    double v;
-    if (i < k2 && j < k2) v = S[E2 + i + j * k2];
-    else if (i >= k2 && j >= k2) v = S[E3 + (i - k2) + (j - k2) * k3];
-    else if (i < k2) v = S[S2 + i + (j - k2) * k2];
-    else v = S[S3 + (i - k2) + j * k3];
+    T v;
+    if (i < k2 && j < k2) v = W[E2 + i + j * k2];
+    else if (i >= k2 && j >= k2) v = W[E3 + (i - k2) + (j - k2) * k3];
+    else if (i < k2) v = T(S[S2 + i + (j - k2) * k2]);
+    else v = T(S[S3 + (i - k2) + j * k3]);
     F[idx] = v;
   }
   __syncthreads();
-  gj_inplace(F, n, piv, f, &sp);
+  gj_inplace<T>(F, n, piv, f, &sp);
 }
 
 // One level of a sweep: out_box = sum_t M_t x_t (up to three terms), one workgroup per
@@ -332,23 +342,27 @@ struct SweepOp {
   int64_t gofs;
 };
 
+// The matrices come from one arena (Tm: double, or float for the FP32 inverse) at offset -
+// sofs; vectors and accumulation stay in double.
+template <typename Tm>
 __global__ void __launch_bounds__(256) hss1d_sweep_kernel(const SweepOp* __restrict__ ops,
-                                                          const double* __restrict__ S,
+                                                          const Tm* __restrict__ S, int64_t sofs,
                                                           double* __restrict__ V,
                                                           const double* __restrict__ gin,
                                                           double* __restrict__ gout) {
   const SweepOp O = ops[blockIdx.x];
+  S -= sofs;
   for (int i = threadIdx.x; i < O.rows; i += blockDim.x) {
     double s = 0;
     for (int t = 0; t < O.nterm; ++t) {
       const SweepTerm& T = O.t[t];
-      const double* M = S + T.mat + T.rofs + i;
+      const Tm* M = S + T.mat + T.rofs + i;
       for (int c = 0; c < T.cols; ++c) {
         double x;
         if (T.src_global) x = gin[O.gofs + c];
         else if (c < T.len1) x = V[T.in + c];
         else x = V[T.in2 + c - T.len1];
-        s += M[int64_t(c) * T.ld] * x;
+        s += double(M[int64_t(c) * T.ld]) * x;
       }
     }
     if (O.scatter) gout[O.gofs + i] = s;
@@ -387,9 +401,14 @@ struct hssc_hss1d {
   int* I = nullptr;
   double *dx = nullptr, *dy = nullptr, *dnx = nullptr, *dny = nullptr, *dw = nullptr,
          *dkap = nullptr;
-  // inverse
-  bool inverted = false;
+  // inverse: block offsets (laid out once, beyond the compressed operator in the arena
+  // offsets [inv_lo, inv_hi)); the FP64 blocks live in S, the FP32 ones in W32
+  bool inverted = false, inverted32 = false, laid_out = false;
   std::vector<int64_t> oE, oLinv, oRinv, oDinv;
+  std::vector<BoxInv> inv_boxes;
+  std::vector<TopOp> inv_tops;
+  int64_t inv_lo = 0, inv_hi = 0;
+  float* W32 = nullptr;
   // per-box vector offsets for the sweeps
   std::vector<int64_t> vUp, vDn;
   int64_t vlen = 0;
@@ -403,7 +422,7 @@ struct hssc_hss1d {
   int64_t id_boxes = 0, id_fallbacks = 0, id_sketch_rows = 0, id_full_rows = 0;
   ~hssc_hss1d() {
     dfree1(S), dfree1(I), dfree1(dx), dfree1(dy), dfree1(dnx), dfree1(dny), dfree1(dw),
-        dfree1(dkap);
+        dfree1(dkap), dfree1(W32);
   }
   bool leaf(int b) const { return c1[b] <= 0 && c2[b] <= 0; }
   bool root(int b) const { return parent[b] == 0; }
@@ -1192,81 +1211,114 @@ int hssc_hss1d_id_stats(const hssc_hss1d* h, int64_t* boxes, int64_t* fallbacks,
   return HSSC_OK;
 }
 
-int hssc_hss1d_invert(hssc_hss1d* h) {
-  // HSS1D/HSS1D_invert_nsym.m:44-91, one launch per level (deepest first) + the top block.
-  if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
-  try {
-    const int nb = h->nboxes;
+}  // extern "C"
+
+namespace {
+
+// Offsets of every inverse block (HSS1D_invert_nsym.m:44-91), computed once per operator.
+void inverse_layout(hssc_hss1d* h) {
+  if (h->laid_out) return;
+  const int nb = h->nboxes;
+  for (int b = 1; b <= nb; ++b)
+    if (!h->root(b) && h->nskel[b] > 256)
+      throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D block larger than 256")};
+  int64_t top = h->arena_n;
+  h->inv_lo = top;
+  h->oE.assign(nb + 1, -1), h->oLinv.assign(nb + 1, -1), h->oRinv.assign(nb + 1, -1),
+      h->oDinv.assign(nb + 1, -1);
+  std::vector<BoxInv> all(nb + 1);
+  for (int b = 1; b <= nb; ++b) {
+    if (h->root(b)) continue;
+    const int n = h->nskel[b], k = h->kskel[b];
+    BoxInv B{};
+    B.n = n, B.k = k;
+    B.F = take(&top, int64_t(n) * n);
+    B.Jup = h->oJup[b], B.Jdn = h->oJdn[b], B.Tup = h->oTup[b], B.Tdn = h->oTdn[b];
+    B.E = h->oE[b] = take(&top, int64_t(k) * k);
+    B.RF = take(&top, int64_t(k) * n);
+    B.FL = take(&top, int64_t(n) * k);
+    B.Linv = h->oLinv[b] = take(&top, int64_t(n) * k);
+    B.Rinv = h->oRinv[b] = take(&top, int64_t(k) * n);
+    B.Dinv = h->oDinv[b] = take(&top, int64_t(n) * n);
+    B.leaf = h->leaf(b) ? 1 : 0;
+    B.Mself = h->oMself[b];
+    all[b] = B;
+  }
+  for (int b = 1; b <= nb; ++b)
+    if (!h->root(b) && !h->leaf(b)) {
+      all[b].E1 = h->oE[h->c1[b]], all[b].E2 = h->oE[h->c2[b]];
+      all[b].S1 = h->oMsib[h->c1[b]], all[b].S2 = h->oMsib[h->c2[b]];
+      all[b].k1 = h->kskel[h->c1[b]], all[b].k2 = h->kskel[h->c2[b]];
+    }
+  std::vector<TopOp> tops;
+  h->oFTinvs.assign(nb + 1, -1);
+  for (int r : h->roots) {
+    const int a = h->c1[r], c = h->c2[r];
+    const int k2 = h->kskel[a], k3 = h->kskel[c];
+    if (k2 + k3 > 256) throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D top block larger than 256")};
+    TopOp O{};
+    O.FT = h->oFTinvs[r] = take(&top, int64_t(k2 + k3) * (k2 + k3));
+    O.k2 = k2, O.k3 = k3;
+    O.E2 = h->oE[a], O.E3 = h->oE[c], O.S2 = h->oMsib[a], O.S3 = h->oMsib[c];
+    tops.push_back(O);
+  }
+  h->inv_boxes.swap(all);
+  h->inv_tops.swap(tops);
+  h->inv_hi = top;
+  h->laid_out = true;
+}
+
+// The inverse in precision T into W (arena offsets minus wofs): one launch per level
+// (deepest first), then the top blocks.
+template <typename T>
+void invert_run(hssc_hss1d* h, T* W, int64_t wofs) {
+  const int nb = h->nboxes;
+  for (int lev = h->depth; lev >= 1; --lev) {
+    std::vector<BoxInv> ops;
     for (int b = 1; b <= nb; ++b)
-      if (!h->root(b) && h->nskel[b] > 256)
-        throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D block larger than 256")};
-    int64_t top = h->arena_n;
-    h->oE.assign(nb + 1, -1), h->oLinv.assign(nb + 1, -1), h->oRinv.assign(nb + 1, -1),
-        h->oDinv.assign(nb + 1, -1);
-    std::vector<BoxInv> all(nb + 1);
-    for (int b = 1; b <= nb; ++b) {
-      if (h->root(b)) continue;
-      const int n = h->nskel[b], k = h->kskel[b];
-      BoxInv B{};
-      B.n = n, B.k = k;
-      B.F = take(&top, int64_t(n) * n);
-      B.Jup = h->oJup[b], B.Jdn = h->oJdn[b], B.Tup = h->oTup[b], B.Tdn = h->oTdn[b];
-      B.E = h->oE[b] = take(&top, int64_t(k) * k);
-      B.RF = take(&top, int64_t(k) * n);
-      B.FL = take(&top, int64_t(n) * k);
-      B.Linv = h->oLinv[b] = take(&top, int64_t(n) * k);
-      B.Rinv = h->oRinv[b] = take(&top, int64_t(k) * n);
-      B.Dinv = h->oDinv[b] = take(&top, int64_t(n) * n);
-      B.leaf = h->leaf(b) ? 1 : 0;
-      B.Mself = h->oMself[b];
-      all[b] = B;
-    }
-    for (int b = 1; b <= nb; ++b)
-      if (!h->root(b) && !h->leaf(b)) {
-        all[b].E1 = h->oE[h->c1[b]], all[b].E2 = h->oE[h->c2[b]];
-        all[b].S1 = h->oMsib[h->c1[b]], all[b].S2 = h->oMsib[h->c2[b]];
-        all[b].k1 = h->kskel[h->c1[b]], all[b].k2 = h->kskel[h->c2[b]];
-      }
-    std::vector<TopOp> tops;
-    h->oFTinvs.assign(nb + 1, -1);
-    for (int r : h->roots) {
-      const int a = h->c1[r], c = h->c2[r];
-      const int k2 = h->kskel[a], k3 = h->kskel[c];
-      if (k2 + k3 > 256) throw HipErr{fail(HSSC_ERR_RANK_OVERFLOW, "HSS1D top block larger than 256")};
-      TopOp O{};
-      O.FT = h->oFTinvs[r] = take(&top, int64_t(k2 + k3) * (k2 + k3));
-      O.k2 = k2, O.k3 = k3;
-      tops.push_back(O);
-    }
-    grow(&h->S, &h->arena_n, top);
-    for (size_t i = 0; i < h->roots.size(); ++i) {
-      const int r = h->roots[i], a = h->c1[r], c = h->c2[r];
-      tops[i].E2 = h->oE[a], tops[i].E3 = h->oE[c], tops[i].S2 = h->oMsib[a],
-      tops[i].S3 = h->oMsib[c];
-    }
-    for (int lev = h->depth; lev >= 1; --lev) {
-      std::vector<BoxInv> ops;
-      for (int b = 1; b <= nb; ++b)
-        if (h->level[b] == lev) ops.push_back(all[b]);
-      if (ops.empty()) continue;
-      BoxInv* d = dput(ops);
-      HSSC_TIMED("hss1d_invert_f64", 0, 0.0, 0.0,
-                 hipLaunchKernelGGL(hss1d_invert_kernel, dim3(ops.size()), dim3(256), 0, 0, d,
-                                    h->S, h->I));
-      H1CHK(hipDeviceSynchronize());
-      dfree1(d);
-    }
-    TopOp* dtops = dput(tops);
-    hipLaunchKernelGGL(hss1d_top_kernel, dim3(tops.size()), dim3(256), 0, 0, dtops, h->S);
+      if (h->level[b] == lev) ops.push_back(h->inv_boxes[b]);
+    if (ops.empty()) continue;
+    BoxInv* d = dput(ops);
+    HSSC_TIMED(sizeof(T) == 8 ? "hss1d_invert_f64" : "hss1d_invert_f32", 0, 0.0, 0.0,
+               hipLaunchKernelGGL(hss1d_invert_kernel<T>, dim3(ops.size()), dim3(256), 0, 0, d,
+                                  h->S, W, wofs, h->I));
     H1CHK(hipDeviceSynchronize());
-    dfree1(dtops);
-    H1CHK(hipGetLastError());
-    h->inverted = true;
+    dfree1(d);
+  }
+  TopOp* dtops = dput(h->inv_tops);
+  hipLaunchKernelGGL(hss1d_top_kernel<T>, dim3(h->inv_tops.size()), dim3(256), 0, 0, dtops, h->S,
+                     W, wofs);
+  H1CHK(hipDeviceSynchronize());
+  dfree1(dtops);
+  H1CHK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int hssc_hss1d_invert_prec(hssc_hss1d* h, int32_t precision) {
+  if (!h) return fail(HSSC_ERR_INVALID_ARG, "null handle");
+  if (precision != HSSC_PREC_FP64 && precision != HSSC_PREC_FP32)
+    return fail(HSSC_ERR_INVALID_ARG, "precision must be HSSC_PREC_FP64 or HSSC_PREC_FP32");
+  try {
+    inverse_layout(h);
+    if (precision == HSSC_PREC_FP64) {
+      grow(&h->S, &h->arena_n, h->inv_hi);
+      invert_run<double>(h, h->S, 0);
+      h->inverted = true;
+    } else {
+      if (!h->W32) h->W32 = dmalloc<float>(h->inv_hi - h->inv_lo);
+      invert_run<float>(h, h->W32, h->inv_lo);
+      h->inverted32 = true;
+    }
     return HSSC_OK;
   } catch (const HipErr& e) {
     return e.status;
   }
 }
+
+int hssc_hss1d_invert(hssc_hss1d* h) { return hssc_hss1d_invert_prec(h, HSSC_PREC_FP64); }
 
 static SweepTerm term(int64_t mat, int ld, int cols, int64_t in, int len1, int64_t in2,
                       int rofs = 0, int src_global = 0) {
@@ -1276,9 +1328,10 @@ static SweepTerm term(int64_t mat, int ld, int cols, int64_t in, int len1, int64
   return T;
 }
 
-static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool inverse) {
+static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool inverse,
+                   bool fp32 = false) {
   // inverse: HSS1D/HSS1D_applyinv_nsym.m:43-86; forward: HSS1D/HSS1D_apply_nsym.m:38-101.
-  // One launch per level and direction.
+  // One launch per level and direction.  fp32: the inverse blocks of the FP32 inverse.
   const int nb = h->nboxes;
   double* V = dmalloc<double>(h->vlen + 1);
   std::vector<void*> tmp{V};
@@ -1286,9 +1339,15 @@ static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool in
     if (ops.empty()) return;
     SweepOp* d = dput(ops);
     tmp.push_back(d);
-    HSSC_TIMED("hss1d_sweep_f64", 0, 0.0, 0.0,
-               hipLaunchKernelGGL(hss1d_sweep_kernel, dim3(ops.size()), dim3(256), 0, 0, d, h->S,
-                                  V, dIn, dOut));
+    if (fp32) {
+      HSSC_TIMED("hss1d_sweep_f32", 0, 0.0, 0.0,
+                 hipLaunchKernelGGL(hss1d_sweep_kernel<float>, dim3(ops.size()), dim3(256), 0, 0,
+                                    d, (const float*)h->W32, h->inv_lo, V, dIn, dOut));
+    } else {
+      HSSC_TIMED("hss1d_sweep_f64", 0, 0.0, 0.0,
+                 hipLaunchKernelGGL(hss1d_sweep_kernel<double>, dim3(ops.size()), dim3(256), 0,
+                                    0, d, (const double*)h->S, int64_t(0), V, dIn, dOut));
+    }
   };
   auto up_op = [&](int b, int64_t mat) {
     // up[b] = M [f_b or (up c1; up c2)]  with M = Rinv (inverse) or R (forward)
@@ -1388,6 +1447,72 @@ int hssc_hss1d_applyinv_device(const hssc_hss1d* h, const void* dF, void* dX, in
   if (!h->inverted) return fail(HSSC_ERR_INVALID_ARG, "hss1d operator is not inverted");
   try {
     return sweep1d(h, static_cast<const double*>(dF), static_cast<double*>(dX), true);
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_solve_device(const hssc_hss1d* h, const void* dF, void* dX, double tol,
+                            int32_t maxit, int32_t* iters, double* relres) {
+  // Common/Iterative_Refinement.m:20-35 with the FP32 inverse (the FP64 one when no FP32
+  // inverse exists) as the approximate solver and the FP64 compressed operator as A.
+  if (!h || !dF || !dX || maxit < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d solve arguments");
+  const bool fp32 = h->inverted32;
+  if (!fp32 && !h->inverted) return fail(HSSC_ERR_INVALID_ARG, "hss1d operator is not inverted");
+  try {
+    const int64_t N = h->N;
+    const double* F = static_cast<const double*>(dF);
+    double* X = static_cast<double*>(dX);
+    double* Rv = dmalloc<double>(N);
+    double* D = dmalloc<double>(N);
+    double* part = dmalloc<double>(256);
+    std::vector<void*> tmp{Rv, D, part};
+    auto norm = [&](const double* v) {
+      k::sumsq<double>(0, N, v, part, 256);
+      double ph[256];
+      H1CHK(hipMemcpy(ph, part, sizeof(ph), hipMemcpyDeviceToHost));
+      double ss = 0;
+      for (double x : ph) ss += x;
+      return std::sqrt(ss);
+    };
+    const double nf = norm(F);
+    int s = sweep1d(h, F, X, true, fp32);
+    int it = 1;
+    double rel = 0;
+    while (s == HSSC_OK) {
+      s = sweep1d(h, X, Rv, false);  // R = A X
+      if (s != HSSC_OK) break;
+      k::axpby<double, double>(0, N, 1.0, F, -1.0, Rv);  // R = F - A X
+      rel = norm(Rv) / (nf > 0 ? nf : 1.0);
+      if (rel <= tol || it >= maxit) break;
+      s = sweep1d(h, Rv, D, true, fp32);
+      if (s != HSSC_OK) break;
+      k::axpby<double, double>(0, N, 1.0, D, 1.0, X);  // X += A^-1 R
+      ++it;
+    }
+    H1CHK(hipDeviceSynchronize());
+    for (void* p : tmp) dfree1(p);
+    if (s != HSSC_OK) return s;
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "hss1d refinement did not reach tol");
+  } catch (const HipErr& e) {
+    return e.status;
+  }
+}
+
+int hssc_hss1d_solve(const hssc_hss1d* h, const double* F, double* X, double tol, int32_t maxit,
+                     int32_t* iters, double* relres) {
+  if (!h || !F || !X) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d solve arguments");
+  try {
+    double* dF = dmalloc<double>(h->N);
+    double* dX = dmalloc<double>(h->N);
+    H1CHK(hipMemcpy(dF, F, h->N * 8, hipMemcpyHostToDevice));
+    const int s = hssc_hss1d_solve_device(h, dF, dX, tol, maxit, iters, relres);
+    if (s == HSSC_OK || s == HSSC_ERR_NOT_CONVERGED)
+      H1CHK(hipMemcpy(X, dX, h->N * 8, hipMemcpyDeviceToHost));
+    dfree1(dF), dfree1(dX);
+    return s;
   } catch (const HipErr& e) {
     return e.status;
   }

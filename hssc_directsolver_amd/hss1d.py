@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import PREC, check, lib
 
 _DP = C.POINTER(C.c_double)
 
@@ -145,10 +145,23 @@ def split(K_HSS: HSS1D, v):
     return np.split(np.asarray(v), np.cumsum(sizes)[:-1])
 
 
-def HSS1D_invert_nsym(K_HSS: HSS1D, acc=None) -> HSS1D:
-    """HSS1D/HSS1D_invert_nsym.m:1 (the inverse factors are stored in the same handle)."""
-    check(lib().hssc_hss1d_invert(K_HSS.handle))
+def HSS1D_invert_nsym(K_HSS: HSS1D, acc=None, precision="fp64") -> HSS1D:
+    """HSS1D/HSS1D_invert_nsym.m:1 (the inverse factors are stored in the same handle).
+
+    ``precision="fp32"`` builds FP32 factors beside the FP64 operator for HSS1D_solve."""
+    check(lib().hssc_hss1d_invert_prec(K_HSS.handle, PREC[precision]))
     return K_HSS
+
+
+def HSS1D_solve(K_HSS: HSS1D, f, tol=1e-10, maxit=10):
+    """Iterative refinement (Common/Iterative_Refinement.m:20-35): FP32 factors (FP64 if
+    only those exist) against the FP64 compressed operator.  Returns (x, solves, relres)."""
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    x = np.empty_like(f)
+    it, rel = C.c_int32(), C.c_double()
+    check(lib().hssc_hss1d_solve(K_HSS.handle, f.ctypes.data_as(_DP), x.ctypes.data_as(_DP),
+                                 float(tol), int(maxit), C.byref(it), C.byref(rel)))
+    return x, it.value, rel.value
 
 
 def _vec(fn, K_HSS, v):

@@ -272,6 +272,18 @@ int hssc_hss1d_id_stats(const hssc_hss1d* h, int64_t* boxes, int64_t* fallbacks,
 
 /* Replaces HSS1D/HSS1D_invert_nsym.m:1 ([HSSinv,FTinv] = HSS1D_invert_nsym(K_HSS,acc)). */
 int hssc_hss1d_invert(hssc_hss1d* h);
+/* The same inverse factors in HSSC_PREC_FP64 (as hssc_hss1d_invert) or HSSC_PREC_FP32 (kept
+ * beside the FP64 operator, no reference counterpart: the factorization of the mixed
+ * precision solve). */
+int hssc_hss1d_invert_prec(hssc_hss1d* h, int32_t precision);
+/* Common/Iterative_Refinement.m:20-35 on an HSS1D operator: X = A^-1 F by the FP32 inverse
+ * (the FP64 one if no FP32 inverse exists), refined against the FP64 compressed operator
+ * until ||F - A X|| <= tol ||F|| or maxit solves.  Returns the solves done and the relative
+ * residual; HSSC_ERR_NOT_CONVERGED if tol is not reached (X holds the last iterate). */
+int hssc_hss1d_solve(const hssc_hss1d* h, const double* F, double* X, double tol,
+                     int32_t maxit, int32_t* iters, double* relres);
+int hssc_hss1d_solve_device(const hssc_hss1d* h, const void* dF, void* dX, double tol,
+                            int32_t maxit, int32_t* iters, double* relres);
 /* Replaces HSS1D/HSS1D_applyinv_nsym.m:1 and HSS1D/HSS1D_apply_nsym.m:1 (one RHS, m = 1). */
 int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t m);
 int hssc_hss1d_apply(const hssc_hss1d* h, const double* Q, double* U, int64_t m);

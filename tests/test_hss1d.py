@@ -89,3 +89,32 @@ def test_gpu_hss1d_batched_matches_single_problems():
     # the smallest against the dense solve
     A = o1.dense_matrix(o1.Curve(curves[0].X, curves[0].nrm, curves[0].w, curves[0].kappa))
     assert rel(H.split(K, x)[0], np.linalg.solve(A, fs[0])) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_hss1d_fp32_factors_with_refinement():
+    # configuration 3's mixed path: FP32 inverse factors, FP64 refinement against the FP64
+    # compressed operator (Common/Iterative_Refinement.m:20-35), single and batched
+    from hssc_directsolver_amd.rank_mlp import random_contour
+    N = 4096
+    C = H.star_contour(N)
+    A = o1.dense_matrix(o1.star_contour(N))
+    f = np.random.default_rng(3).standard_normal(N)
+    K = H.HSS1D_compress_nsym("green", C, 64, 1e-10)
+    H.HSS1D_invert_nsym(K)
+    y64 = H.HSS1D_applyinv_nsym(K, f)
+    H.HSS1D_invert_nsym(K, precision="fp32")
+    x, it, relres = H.HSS1D_solve(K, f, tol=1e-10, maxit=10)
+    assert relres <= 1e-10 and 2 <= it <= 6
+    assert rel(x, y64) < 1e-9
+    assert rel(x, np.linalg.solve(A, f)) < 1e-9
+    rng = np.random.default_rng(8)
+    curves = [random_contour(rng, n) for n in (1024, 2500, 1800)]
+    Kb = H.HSS1D_compress_batched(curves, 64, 1e-10)
+    H.HSS1D_invert_nsym(Kb, precision="fp32")
+    fb = np.random.default_rng(9).standard_normal(sum(c.N for c in curves))
+    xb, itb, relb = H.HSS1D_solve(Kb, fb, tol=1e-10, maxit=10)
+    assert relb <= 1e-10 and itb <= 6
+    assert rel(H.HSS1D_apply_nsym(Kb, xb), fb) <= 1e-10
+    H.HSS1D_invert_nsym(Kb)
+    assert rel(xb, H.HSS1D_applyinv_nsym(Kb, fb)) < 1e-9
