@@ -18,8 +18,10 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -66,6 +68,22 @@ std::vector<T> dget(const T* d, size_t n) {
 
 void dfree1(void* p) {
   if (p) (void)hipFree(p);
+}
+
+// Runs f(i) for i in [0, n) on up to 16 host threads (independent iterations).
+template <class F>
+void par_for(int n, F&& f) {
+  const int nt = std::max(1, std::min({n / 64 + 1, int(std::thread::hardware_concurrency()), 16}));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t]() {
+      for (int i = t; i < n; i += nt) f(i);
+    });
+  for (auto& th : pool) th.join();
 }
 
 constexpr double kPi = 3.14159265358979323846;
@@ -371,6 +389,37 @@ __global__ void __launch_bounds__(256) hss1d_sweep_kernel(const SweepOp* __restr
   }
 }
 
+// One box of the compressed operator laid out in a level's block arena: T_up and T_dn
+// (k x (n-k)), R = [I T_up] with columns permuted by J_up (k x n) and L = [I; T_dn^T] with
+// rows permuted by J_dn (n x k).  J_up and J_dn are permutations, so every entry of R and L
+// is written once.
+struct PackOp {
+  const double *tu, *td;
+  int k, n;
+  int64_t otu, otd, orr, ol;
+  int64_t pu, pd;  // offsets of J_up, J_dn in the permutation array
+};
+
+__global__ void __launch_bounds__(256) hss1d_pack_kernel(const PackOp* __restrict__ ops,
+                                                         const int* __restrict__ J,
+                                                         double* __restrict__ S) {
+  const PackOp O = ops[blockIdx.x];
+  const int k = O.k, n = O.n, r = n - k, tid = threadIdx.x;
+  const int* Ju = J + O.pu;
+  const int* Jd = J + O.pd;
+  for (int idx = tid; idx < k * r; idx += 256) {
+    S[O.otu + idx] = O.tu[idx];
+    S[O.otd + idx] = O.td[idx];
+  }
+  for (int idx = tid; idx < k * n; idx += 256) {
+    const int i = idx % k, c = idx / k;  // R(i, Ju[c]) and L(Jd[c], i)
+    const double ru = c < k ? (c == i ? 1.0 : 0.0) : O.tu[i + int64_t(c - k) * k];
+    const double ld = c < k ? (c == i ? 1.0 : 0.0) : O.td[i + int64_t(c - k) * k];
+    S[O.orr + i + int64_t(Ju[c]) * k] = ru;
+    S[O.ol + Jd[c] + int64_t(i) * n] = ld;
+  }
+}
+
 }  // namespace
 }  // namespace hssc
 
@@ -395,7 +444,6 @@ struct hssc_hss1d {
   // device scratch: all matrices (T, M_self, M_sib, inverse blocks) in one arena, all ints
   // (permutations) in another; offsets per box
   std::vector<int64_t> oTup, oTdn, oJup, oJdn, oMself, oMsib, oR, oL;
-  std::vector<double> arena_h;  // unused on host; sizes tracked
   int64_t arena_n = 0, iarena_n = 0;
   double* S = nullptr;
   int* I = nullptr;
@@ -597,13 +645,16 @@ double* gen_ragged(const hssc_hss1d* h, const Ragged& R, int mode, std::vector<v
   return A;
 }
 
-// Runs the acc-mode ID on a ragged batch, then the fixed-rank ID with k = kfix (per matrix)
-// on a fresh copy.  Returns perms (host) and T (device arena offsets appended).
+// Result of a batched ID: ranks and permutations on the host, each block's interpolation
+// matrix T on the device (in buffers the result keeps alive).
 struct IdResult {
   std::vector<int> k;
   std::vector<std::vector<int>> perm;
-  std::vector<std::vector<double>> T;  // k x (n-k) column-major
+  std::vector<const double*> T;  // device: k x (n-k) column-major (leading dimension k)
+  std::vector<std::shared_ptr<void>> bufs;  // device buffers holding T
 };
+
+std::shared_ptr<void> dev_owner(void* p) { return std::shared_ptr<void>(p, dfree1); }
 
 IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* kfix,
                 uint64_t seed) {
@@ -615,7 +666,7 @@ IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* 
   int64_t pt = 0, tt = 0, wt = 0;
   for (int i = 0; i < nm; ++i) {
     poff[i] = pt, pt += R.n[i];
-    toff[i] = tt, tt += int64_t(R.n[i]) * R.n[i];
+    toff[i] = tt, tt += int64_t(R.n[i]) * R.n[i] / 4 + R.n[i];  // k (n - k) <= n^2 / 4
     woff[i] = wt, wt += 6 * int64_t(R.n[i]) + R.m[i];
   }
   std::vector<void*> tmp;
@@ -640,13 +691,13 @@ IdResult run_id(const Ragged& R, double* A, double acc, const std::vector<int>* 
   H1CHK(hipGetLastError());
   std::vector<int> hk = dget(dk, nm);
   std::vector<int> hp = dget(dperm, pt);
-  std::vector<double> hT = dget(dT, tt);
   for (int i = 0; i < nm; ++i) {
     out.k[i] = hk[i];
     out.perm[i].assign(hp.begin() + poff[i], hp.begin() + poff[i] + R.n[i]);
-    const int64_t cnt = int64_t(hk[i]) * (R.n[i] - hk[i]);
-    out.T[i].assign(hT.begin() + toff[i], hT.begin() + toff[i] + cnt);
+    out.T[i] = dT + toff[i];
   }
+  tmp.erase(std::find(tmp.begin(), tmp.end(), static_cast<void*>(dT)));
+  out.bufs.push_back(dev_owner(dT));
   // restore A from the copy so a second pass can reuse it
   H1CHK(hipMemcpy(A, Ac, R.total * 8, hipMemcpyDeviceToDevice));
   for (void* p : tmp) dfree1(p);
@@ -691,8 +742,8 @@ __global__ void __launch_bounds__(256) sketch1d_kernel(
 __global__ void __launch_bounds__(256) verify1d_kernel(
     const int64_t* __restrict__ off, const int* __restrict__ mr, const int* __restrict__ nc,
     const int* __restrict__ sr, const double* __restrict__ A, const int* __restrict__ perm,
-    const int64_t* __restrict__ poff, const int* __restrict__ kk, const double* __restrict__ T,
-    const int64_t* __restrict__ toff, double* __restrict__ wk, const int64_t* __restrict__ woff,
+    const int64_t* __restrict__ poff, const int* __restrict__ kk,
+    const double* const* __restrict__ Tp, double* __restrict__ wk, const int64_t* __restrict__ woff,
     double acc, uint64_t seed, int* __restrict__ ok) {
   __shared__ double red[4];
   __shared__ double xs[256], us[256];
@@ -700,7 +751,7 @@ __global__ void __launch_bounds__(256) verify1d_kernel(
   const int m = mr[b], n = nc[b], s = sr[b], k = kk[b], r_ = n - k;
   const double* M = A + off[b];
   const int* P = perm + poff[b];
-  const double* Tb = T + toff[b];
+  const double* Tb = Tp[b];
   double* y = wk + woff[b];
   auto bsum = [&](double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -793,29 +844,25 @@ IdResult run_id_sketched(const Ragged& R, double* A, double acc, const std::vect
                                 dY, seed));
   out = run_id(Y, dY, acc, kfix, seed ^ 0x3141ull);
   // acceptance on the full blocks
-  std::vector<int64_t> poff(nm), toff(nm), woff(nm);
+  std::vector<int64_t> poff(nm), woff(nm);
   std::vector<int> kk(nm), pflat;
-  std::vector<double> tflat;
   int64_t wt = 0;
   for (int i = 0; i < nm; ++i) {
     poff[i] = int64_t(pflat.size());
     pflat.insert(pflat.end(), out.perm[i].begin(), out.perm[i].end());
-    toff[i] = int64_t(tflat.size());
-    tflat.insert(tflat.end(), out.T[i].begin(), out.T[i].end());
     kk[i] = out.k[i];
     woff[i] = wt, wt += R.m[i];
   }
   int* dp = dput(pflat);
   int64_t* dpo = dput(poff);
   int* dk = dput(kk);
-  double* dT = dput(tflat);
-  int64_t* dto = dput(toff);
+  const double** dTp = dput(out.T);
   double* dw = dmalloc<double>(wt);
   int64_t* dwo = dput(woff);
   int* dok = dmalloc<int>(nm);
-  tmp.insert(tmp.end(), {dp, dpo, dk, dT, dto, dw, dwo, dok});
+  tmp.insert(tmp.end(), {dp, dpo, dk, (void*)dTp, dw, dwo, dok});
   hipLaunchKernelGGL(verify1d_kernel, dim3(nm), dim3(256), 0, 0, doff, dm, dn, ds, A, dp, dpo,
-                     dk, dT, dto, dw, dwo, acc, seed, dok);
+                     dk, dTp, dw, dwo, acc, seed, dok);
   H1CHK(hipDeviceSynchronize());
   H1CHK(hipGetLastError());
   *ok = dget(dok, nm);
@@ -847,11 +894,12 @@ IdResult run_id_mixed(hssc_hss1d* h, const Ragged& R, double* A, double acc,
     if (kfix)
       for (int i : sk) kf.push_back((*kfix)[i]);
     IdResult r = run_id_sketched(subset(R, sk), A, acc, ss, seed, &ok, kfix ? &kf : nullptr);
+    out.bufs.insert(out.bufs.end(), r.bufs.begin(), r.bufs.end());
     for (size_t j = 0; j < sk.size(); ++j) {
       h->id_sketch_rows += ss[j];
       if (ok[j]) {
         const int i = sk[j];
-        out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = std::move(r.T[j]);
+        out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = r.T[j];
       } else {
         full.push_back(sk[j]);
         ++h->id_fallbacks;
@@ -867,8 +915,9 @@ IdResult run_id_mixed(hssc_hss1d* h, const Ragged& R, double* A, double acc,
     for (size_t j = 0; j < full.size(); ++j) {
       const int i = full[j];
       h->id_full_rows += R.m[i];
-      out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = std::move(r.T[j]);
+      out.k[i] = r.k[j], out.perm[i] = std::move(r.perm[j]), out.T[i] = r.T[j];
     }
+    out.bufs.insert(out.bufs.end(), r.bufs.begin(), r.bufs.end());
   }
   if (!kfix) h->id_boxes += nm;
   return out;
@@ -945,17 +994,29 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     h->oTup.assign(nb + 1, -1), h->oTdn.assign(nb + 1, -1), h->oJup.assign(nb + 1, -1),
         h->oJdn.assign(nb + 1, -1), h->oMself.assign(nb + 1, -1), h->oMsib.assign(nb + 1, -1);
     h->oR.assign(nb + 1, -1), h->oL.assign(nb + 1, -1);
-    std::vector<double> arena;  // host staging of T blocks (uploaded with the M blocks)
-    std::vector<int> iarena;
+    std::vector<int> iarena;            // permutations of every box (host, uploaded at the end)
+    std::vector<double*> level_blocks;  // per level: T, R, L blocks packed on the device
+    std::vector<int64_t> level_sizes;
+    std::vector<int> box_block_level(nb + 1, -1);
+    struct BlocksFree {
+      std::vector<double*>* v;
+      ~BlocksFree() {
+        for (double* p : *v) dfree1(p);
+      }
+    } blocks_free{&level_blocks};
     // compress level by level, deepest first (HSS1D_compress_nsym.m:84-216)
     for (int lev = h->depth; lev >= 1; --lev) {
       std::vector<int> boxes;
       for (int b = 1; b <= nb; ++b)
         if (h->level[b] == lev) boxes.push_back(b);
       Ragged Rout, Rin;
-      std::vector<std::vector<int>> outs(boxes.size()), inns(boxes.size());
-      std::vector<double> Rbox(boxes.size());
-      for (size_t q = 0; q < boxes.size(); ++q) {
+      const int nq0 = int(boxes.size());
+      std::vector<std::vector<int>> outs(nq0), inns(nq0), oins(nq0), iins(nq0);
+      std::vector<std::vector<double>> zxs(nq0), zys(nq0);
+      std::vector<double> Rbox(nq0);
+      // per box (independent, threaded): the skeleton candidates, the proxy circle and the
+      // near points inside it, and the box's geometric features
+      par_for(nq0, [&](int q) {
         const int b = boxes[q];
         std::vector<int> out, inn;
         if (h->leaf(b)) {
@@ -973,7 +1034,9 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
         double xc, yc, R;
         circum(h, uni, &xc, &yc, &R);
         const double rp = kRadiusRel * R;
-        std::vector<double> zx(nproxy), zy(nproxy);
+        std::vector<double>& zx = zxs[q];
+        std::vector<double>& zy = zys[q];
+        zx.resize(nproxy), zy.resize(nproxy);
         for (int t = 0; t < nproxy; ++t) {
           // linspace(0, 2pi(1-1/n), n)
           const double th = (nproxy > 1) ? (2 * kPi * (1 - 1.0 / nproxy)) * t / (nproxy - 1) : 0;
@@ -986,23 +1049,20 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
             omay.insert(omay.end(), r.begin(), r.end());
             imay.insert(imay.end(), r.begin(), r.end());
           } else {
-            for (int s : {h->c1[j], h->c2[j]}) {
-              omay.insert(omay.end(), h->isk_up[s].begin(), h->isk_up[s].end());
-              imay.insert(imay.end(), h->isk_dn[s].begin(), h->isk_dn[s].end());
+            for (int sb : {h->c1[j], h->c2[j]}) {
+              omay.insert(omay.end(), h->isk_up[sb].begin(), h->isk_up[sb].end());
+              imay.insert(imay.end(), h->isk_dn[sb].begin(), h->isk_dn[sb].end());
             }
           }
         }
-        std::vector<int> oin, iin;
+        std::vector<int>& oin = oins[q];
+        std::vector<int>& iin = iins[q];
         for (int i : omay)
           if ((h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc) < rp * rp)
             oin.push_back(i);
         for (int i : imay)
           if ((h->X[i] - xc) * (h->X[i] - xc) + (h->Y[i] - yc) * (h->Y[i] - yc) < rp * rp)
             iin.push_back(i);
-        Rout.add(iin, zx, zy, out);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
-        Rin.add(oin, zx, zy, inn);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
-        outs[q] = out, inns[q] = inn;
-        Rbox[q] = R;
         // geometric features of the box (inputs of the rank predictor)
         double kmax = 0, ksum = 0;
         for (int i = h->start[b]; i < h->start[b] + h->count[b]; ++i)
@@ -1014,8 +1074,14 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
         f[3] = float(std::log2(R / h->Rroot[h->rootof[b]]));
         f[4] = float(kmax * R);
         f[5] = float(ksum / h->count[b] * R);
-        f[6] = float(std::log2(double(Rout.m.back())));
+        f[6] = float(std::log2(double(iin.size() + size_t(nproxy))));  // rows of A_out
         f[7] = float(-std::log10(acc));
+        outs[q].swap(out), inns[q].swap(inn);
+        Rbox[q] = R;
+      });
+      for (int q = 0; q < nq0; ++q) {
+        Rout.add(iins[q], zxs[q], zys[q], outs[q]);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
+        Rin.add(oins[q], zxs[q], zys[q], inns[q]);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
       }
       std::vector<void*> tmp;
       double* Aout = gen_ragged(h, Rout, 0, &tmp);
@@ -1072,46 +1138,63 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
           ss.push_back(sr);
         }
         IdResult f = run_id_mixed(h, subset(Rr, idx), Ar, acc, ss, sd, &ks);
+        res->bufs.insert(res->bufs.end(), f.bufs.begin(), f.bufs.end());
         for (size_t j = 0; j < idx.size(); ++j) {
           const int q = idx[j];
-          res->k[q] = f.k[j], res->perm[q] = std::move(f.perm[j]), res->T[q] = std::move(f.T[j]);
+          res->k[q] = f.k[j], res->perm[q] = std::move(f.perm[j]), res->T[q] = f.T[j];
         }
       };
       redo(Rout, Aout, s_out, redo_out, &io, seed);
       redo(Rin, Ain, s_in, redo_in, &ii, seed ^ 0xABCDull);
       for (void* p : tmp) dfree1(p);
-      for (size_t q = 0; q < boxes.size(); ++q) {
+      // per box: skeleton index sets and permutations (host); T blocks and the explicit R, L
+      // packed on the device into this level's block arena
+      std::vector<int64_t> aoff(nq + 1, 0), ioff(nq + 1, int64_t(iarena.size()));
+      for (int q = 0; q < nq; ++q) {
+        const int64_t n = int64_t(outs[q].size()), k = kf[q];
+        aoff[q + 1] = aoff[q] + 2 * k * (n - k) + 2 * k * n;
+        ioff[q + 1] = ioff[q] + int64_t(io.perm[q].size()) + int64_t(ii.perm[q].size());
+      }
+      iarena.resize(ioff[nq]);
+      std::vector<PackOp> pk(nq);
+      const int lvl = int(level_blocks.size());
+      par_for(nq, [&](int q) {
         const int b = boxes[q], k = kf[q];
-        h->nskel[b] = int(outs[q].size());
+        const int n = int(outs[q].size());
+        h->nskel[b] = n;
         h->kskel[b] = k;
         h->J_up[b] = io.perm[q];
         h->J_dn[b] = ii.perm[q];
         h->isk_up[b].resize(k), h->isk_dn[b].resize(k);
         for (int i = 0; i < k; ++i)
           h->isk_up[b][i] = outs[q][io.perm[q][i]], h->isk_dn[b][i] = inns[q][ii.perm[q][i]];
-        h->oTup[b] = int64_t(arena.size());
-        arena.insert(arena.end(), io.T[q].begin(), io.T[q].end());
-        h->oTdn[b] = int64_t(arena.size());
-        arena.insert(arena.end(), ii.T[q].begin(), ii.T[q].end());
-        h->oJup[b] = int64_t(iarena.size());
-        iarena.insert(iarena.end(), io.perm[q].begin(), io.perm[q].end());
-        h->oJdn[b] = int64_t(iarena.size());
-        iarena.insert(iarena.end(), ii.perm[q].begin(), ii.perm[q].end());
-        // explicit R = [I T_up] (columns permuted by J_up) and L = [I; T_dn^T] (rows by J_dn)
-        const int n = h->nskel[b];
-        std::vector<double> Rm(int64_t(k) * n, 0.0), Lm(int64_t(n) * k, 0.0);
-        for (int i = 0; i < k; ++i) {
-          Rm[i + int64_t(io.perm[q][i]) * k] = 1.0;
-          Lm[ii.perm[q][i] + int64_t(i) * n] = 1.0;
-          for (int c = 0; c < n - k; ++c) {
-            Rm[i + int64_t(io.perm[q][k + c]) * k] = io.T[q][i + int64_t(c) * k];
-            Lm[ii.perm[q][k + c] + int64_t(i) * n] = ii.T[q][i + int64_t(c) * k];
-          }
-        }
-        h->oR[b] = int64_t(arena.size());
-        arena.insert(arena.end(), Rm.begin(), Rm.end());
-        h->oL[b] = int64_t(arena.size());
-        arena.insert(arena.end(), Lm.begin(), Lm.end());
+        int64_t io_ = ioff[q];
+        h->oJup[b] = io_;
+        std::copy(io.perm[q].begin(), io.perm[q].end(), iarena.begin() + io_);
+        io_ += int64_t(io.perm[q].size());
+        h->oJdn[b] = io_;
+        std::copy(ii.perm[q].begin(), ii.perm[q].end(), iarena.begin() + io_);
+        PackOp& P = pk[q];
+        P.tu = io.T[q], P.td = ii.T[q], P.k = k, P.n = n;
+        P.otu = aoff[q], P.otd = P.otu + int64_t(k) * (n - k);
+        P.orr = P.otd + int64_t(k) * (n - k), P.ol = P.orr + int64_t(k) * n;
+        P.pu = h->oJup[b], P.pd = h->oJdn[b];
+        // level-local offsets, shifted to the arena once every level is laid out
+        h->oTup[b] = P.otu, h->oTdn[b] = P.otd, h->oR[b] = P.orr, h->oL[b] = P.ol;
+        box_block_level[b] = lvl;
+      });
+      {
+        // this level's permutations are the tail of iarena: upload them with the ops
+        int* dJ = dput(std::vector<int>(iarena.begin(), iarena.end()));
+        double* blk = dmalloc<double>(std::max<int64_t>(aoff[nq], 1));
+        PackOp* dpk = dput(pk);
+        if (nq > 0)
+          hipLaunchKernelGGL(hss1d_pack_kernel, dim3(nq), dim3(256), 0, 0, dpk, dJ, blk);
+        H1CHK(hipDeviceSynchronize());
+        H1CHK(hipGetLastError());
+        dfree1(dJ), dfree1(dpk);
+        level_blocks.push_back(blk);
+        level_sizes.push_back(aoff[nq]);
       }
     }
     // M_self at leaves, M_sib at children of every parent (HSS1D_compress_nsym.m:220-271)
@@ -1132,15 +1215,25 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     std::vector<void*> tmp;
     double* Mblk = gen_ragged(h, Rm, 0, &tmp);
     tmp.push_back(Mblk);
-    const int64_t base = int64_t(arena.size());
+    // arena: the levels' packed blocks, then the M blocks
+    std::vector<int64_t> lbase(level_blocks.size());
+    int64_t base = 0;
+    for (size_t l = 0; l < level_blocks.size(); ++l) lbase[l] = base, base += level_sizes[l];
+    for (int b = 1; b <= nb; ++b) {
+      const int l = box_block_level[b];
+      if (l < 0) continue;
+      h->oTup[b] += lbase[l], h->oTdn[b] += lbase[l], h->oR[b] += lbase[l], h->oL[b] += lbase[l];
+    }
     for (size_t q = 0; q < which.size(); ++q) {
       const int b = which[q].first;
       (which[q].second == 0 ? h->oMself[b] : h->oMsib[b]) = base + Rm.off[q];
     }
-    arena.resize(base + Rm.total);
-    h->arena_n = int64_t(arena.size());
+    h->arena_n = base + Rm.total;
     h->S = dmalloc<double>(h->arena_n);
-    H1CHK(hipMemcpy(h->S, arena.data(), base * 8, hipMemcpyHostToDevice));
+    for (size_t l = 0; l < level_blocks.size(); ++l)
+      if (level_sizes[l] > 0)
+        H1CHK(hipMemcpy(h->S + lbase[l], level_blocks[l], level_sizes[l] * 8,
+                        hipMemcpyDeviceToDevice));
     H1CHK(hipMemcpy(h->S + base, Mblk, Rm.total * 8, hipMemcpyDeviceToDevice));
     h->iarena_n = int64_t(iarena.size());
     h->I = dput(iarena);
