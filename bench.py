@@ -80,21 +80,23 @@ def stats_snapshot(H):
     return out
 
 
-def one_step(H, P, torch, b, tol):
-    """Runs both paths once; returns timings (s) and residual info."""
+def one_step(H, P, torch, b, tol, split=False):
+    """Runs each path end to end (tree with its interpolation operators, compressed operator,
+    inverse, refined solve); returns timings (s) and residual info per path.  Without `split`
+    the tree, compression and inversion are called back to back with no host synchronisation,
+    as a caller would: the inversion waits on the device, level by level, for the interpolation
+    operators it needs, so the two overlap.  With `split` every phase is synchronised on its
+    own to report the breakdown."""
     sync = torch.cuda.synchronize
-    t = {}
-    sync()
-    t0 = time.perf_counter()
-    tree = H.HSS2D_build_tree_fsym(P)
-    op = H.HSS2D_compress_fsym(tree)
-    sync()
-    t1 = time.perf_counter()
-    t["tree"] = t1 - t0
     res = {}
     for prec in ("fp32", "fp64"):
         sync()
         a = time.perf_counter()
+        tree = H.HSS2D_build_tree_fsym(P)
+        op = H.HSS2D_compress_fsym(tree)
+        if split:
+            sync()
+        m = time.perf_counter()
         inv = H.HSS2D_invert_fsym(tree, precision=prec)
         sync()
         c = time.perf_counter()
@@ -102,9 +104,11 @@ def one_step(H, P, torch, b, tol):
         it, rel = H.solve_device(inv, op, b, y, tol=tol, maxit=12)
         sync()
         d = time.perf_counter()
-        res[prec] = dict(invert=c - a, solve=d - c, iters=it, relres=rel)
-        del inv
-    return t, res, tree, op
+        res[prec] = dict(build=c - a, solve=d - c, iters=it, relres=rel)
+        if split:
+            res[prec].update(tree=m - a, invert=c - m)
+        del inv, op, tree
+    return res
 
 
 def hss1d_case(H, torch, N=4096, reps=3):
@@ -270,27 +274,25 @@ def make_problem(H, torch, kernel, args):
     return P, (b / b.norm()).to("cuda:0"), wl
 
 
-def summarize(runs, N):
-    def avg(f):
-        return sum(f(r) for r in runs) / len(runs)
+def summarize(runs, N, split=None):
+    """Averages of the overlapped steps; `split` (one synchronised step) gives the breakdown of
+    the build into the tree (with the compressed operator) and the inversion."""
+    def path(prec):
+        build = sum(r[prec]["build"] for r in runs) / len(runs)
+        solve = sum(r[prec]["solve"] for r in runs) / len(runs)
+        last = runs[-1][prec]
+        out = {"build_s": round(build, 4), "solve_ms": round(1e3 * solve, 3),
+               "iters": last["iters"], "relres": last["relres"],
+               "total_s": round(build + solve, 4), "N_per_s": round(N / (build + solve), 1)}
+        if split:
+            out["synchronised"] = {"tree_s": round(split[prec]["tree"], 4),
+                                   "invert_s": round(split[prec]["invert"], 4),
+                                   "solve_ms": round(1e3 * split[prec]["solve"], 3)}
+        return out
 
-    tree_s = avg(lambda r: r[0]["tree"])
-    mixed = {k: avg(lambda r, k=k: r[1]["fp32"][k]) for k in ("invert", "solve")}
-    fp64 = {k: avg(lambda r, k=k: r[1]["fp64"][k]) for k in ("invert", "solve")}
-    last = runs[-1][1]
-    tm = tree_s + mixed["invert"] + mixed["solve"]
-    t64 = tree_s + fp64["invert"] + fp64["solve"]
-    return {
-        "mixed": {"build_s": round(tree_s + mixed["invert"], 4), "tree_s": round(tree_s, 4),
-                  "invert_s": round(mixed["invert"], 4), "solve_ms": round(1e3 * mixed["solve"], 3),
-                  "iters": last["fp32"]["iters"], "relres": last["fp32"]["relres"],
-                  "total_s": round(tm, 4), "N_per_s": round(N / tm, 1)},
-        "fp64": {"build_s": round(tree_s + fp64["invert"], 4), "tree_s": round(tree_s, 4),
-                 "invert_s": round(fp64["invert"], 4), "solve_ms": round(1e3 * fp64["solve"], 3),
-                 "iters": last["fp64"]["iters"], "relres": last["fp64"]["relres"],
-                 "total_s": round(t64, 4), "N_per_s": round(N / t64, 1)},
-        "speedup_mixed_vs_fp64": round(t64 / tm, 3),
-    }
+    mixed, fp64 = path("fp32"), path("fp64")
+    return {"mixed": mixed, "fp64": fp64,
+            "speedup_mixed_vs_fp64": round(fp64["total_s"] / mixed["total_s"], 3)}
 
 
 def roofline(stats):
@@ -336,11 +338,10 @@ def main():
     T0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
-        r = one_step(H, P, torch, b, args.tol)
-        runs.append(r[:2])
-        del r
+        runs.append(one_step(H, P, torch, b, args.tol))
     torch.cuda.synchronize()
     T1 = time.perf_counter()
+    split = one_step(H, P, torch, b, args.tol, split=True)
     # One more step of the same workload with per-kernel HIP-event instrumentation (same launches
     # and streams as the timed steps, so event times match a kernel trace of those steps).
     L.hssc_stats_reset()
@@ -349,13 +350,13 @@ def main():
     torch.cuda.synchronize()
     L.hssc_stats_enable(0)
     stats = stats_snapshot(H)
-    summ = summarize(runs, N)
+    summ = summarize(runs, N, split)
 
     side = None
     if args.kernel == "helmholtz" and not args.no_laplace:
         Pl, bl, wl = make_problem(H, torch, "laplace", args)
         one_step(H, Pl, torch, bl, args.tol)  # warm-up
-        side = dict(workload=wl, **summarize([one_step(H, Pl, torch, bl, args.tol)[:2]], Pl.N))
+        side = dict(workload=wl, **summarize([one_step(H, Pl, torch, bl, args.tol)], Pl.N))
     cpu = None if args.no_cpu else cpu_baseline(H, P, args)
     h1 = hss1d_case(H, torch) if not args.no_hss1d else None
     hb = batched_case(H, torch) if not args.no_batched else None

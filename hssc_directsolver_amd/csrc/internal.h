@@ -1,6 +1,8 @@
 // Internal declarations shared by the host builder, the GPU orchestration and the kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -75,6 +77,9 @@ struct Level {
   // Helmholtz kernel), plus the FP32 copy (float / complex float) of the FP32 factorization
   void* dT = nullptr;
   void* dT32 = nullptr;
+  // recorded once T of this level is complete (the build runs on its own streams and
+  // returns before the GPU has finished; readers of T wait on this)
+  hipEvent_t ready = nullptr;
 };
 
 }  // namespace hssc
@@ -92,6 +97,10 @@ struct hssc_tree {
   int32_t* dLeafG = nullptr;
   bool has_T = false;
   bool on_device = false;
+  // interpolation operators still being computed: the build's streams and workspaces,
+  // released by hssc::tree_sync once the device work is done
+  std::vector<hipStream_t> bstreams;
+  std::vector<void*> bpending;
   bool cx = false;  // complex (Helmholtz) kernel: every matrix and vector is complex
   int device = 0;
   ~hssc_tree();
@@ -115,6 +124,10 @@ using IdSelect = std::function<int(const hssc_params&, const IdJob&, int32_t* or
 // Host builders (tree.cpp).  sel runs the IDs of the levels whose skeletons come from one.
 int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel = nullptr);
 void source_points(int Ng, int np, double* X /* N x 2 col-major */);
+
+// Waits for the tree's asynchronous build (interpolation operators) and releases its
+// streams and workspaces.  Idempotent.
+void tree_sync(hssc_tree* t);
 
 // Kernel weight of a point (Common/Kernel_Eval.m:111-159, params.proxy == 0).
 double point_weight(const hssc_params& p, double x, double y);

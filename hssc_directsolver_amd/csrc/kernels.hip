@@ -1087,13 +1087,27 @@ __global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
   const T* pin = ws + ((pass & 1) ? CQR_PART0 : CQR_PART1);   // partial Grams of pass - 1
   T* pout = ws + ((pass & 1) ? CQR_PART1 : CQR_PART0);        // partial Grams of this pass
   if (pass > 0) {
-    // R = chol(sum of the partial Gram matrices (+ shift))
+    // R = chol(sum of the partial Gram matrices (+ shift)); the partials are loaded eight at
+    // a time and summed in block order
+    T sum[CQR_NE];
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) sum[m] = T(0);
+    for (int q0 = 0; q0 < P; q0 += 8) {
+      T v[8][CQR_NE];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int m = 0; m < CQR_NE; ++m)
+          v[u][m] = q0 + u < P ? pin[int64_t(q0 + u) * CQR_G + tid + m * CQR_ROWS] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int m = 0; m < CQR_NE; ++m) sum[m] += v[u][m];
+    }
 #pragma unroll
     for (int m = 0; m < CQR_NE; ++m) {
-      const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-      T sum = T(0);
-      for (int q = 0; q < P; ++q) sum += pin[int64_t(q) * CQR_G + ent];
-      Rm[i][j] = sum;
+      const int ent = tid + m * CQR_ROWS;
+      Rm[ent % QR_NB][ent / QR_NB] = sum[m];
     }
     __syncthreads();
     if (pass == 1) {
@@ -1219,29 +1233,49 @@ __global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
     for (int c = 0; c < QR_NB; ++c) ws[CQR_QTOP + tid + c * QR_NB] = c < jb ? x[c] : T(0);
   }
   if (pass < 3) {
-    // partial Gram matrix of this workgroup's rows, staged through LDS 64 rows at a time
-    T acc[CQR_NE];
-#pragma unroll
-    for (int m = 0; m < CQR_NE; ++m) acc[m] = T(0);
+    // Partial Gram matrix X^H X of this workgroup's rows on MFMA.  The rows are staged
+    // through LDS 64 at a time as real and imaginary planes (in Wb's storage); wave w
+    // accumulates the 16 x 16 tile (I0, J0) = 16 (w & 1, w >> 1).  Operand a of lane l is
+    // conj X(t, I0 + l%16), operand b is X(t, J0 + l%16), t = 4 kq + l/16; item r of lane l
+    // is G(I0 + row(l, r), J0 + l%16).
+    using v4 = typename Mfma<R>::v4;
+    constexpr bool CX = is_complex_v<T>;
+    constexpr int LDP = QR_NB + 1;
+    R* Sr = reinterpret_cast<R*>(&Wb[0][0]);
+    R* Si = Sr + 64 * LDP;
     const int wave = tid >> 6, lane = tid & 63;
+    const int I0 = (wave & 1) * 16, J0 = (wave >> 1) * 16, lc = lane & 15, lk = lane >> 4;
+    v4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
     for (int w = 0; w < CQR_ROWS / 64; ++w) {
       __syncthreads();
       if (wave == w) {
 #pragma unroll
-        for (int c = 0; c < QR_NB; ++c) Wb[lane][c] = x[c];
+        for (int c = 0; c < QR_NB; ++c) {
+          Sr[lane * LDP + c] = re_s(x[c]);
+          if constexpr (CX) Si[lane * LDP + c] = im_s(x[c]);
+        }
       }
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < CQR_NE; ++m) {
-        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-        T sum = T(0);
-        for (int t = 0; t < 64; ++t) sum += conj_s(Wb[t][i]) * Wb[t][j];
-        acc[m] += sum;
+      for (int kq = 0; kq < 16; ++kq) {
+        const int t = 4 * kq + lk;
+        const R ar = Sr[t * LDP + I0 + lc], br = Sr[t * LDP + J0 + lc];
+        gr = Mfma<R>::mma(ar, br, gr);
+        if constexpr (CX) {
+          const R ai = Si[t * LDP + I0 + lc], bi = Si[t * LDP + J0 + lc];
+          gr = Mfma<R>::mma(ai, bi, gr);
+          gi = Mfma<R>::mma(ar, bi, gi);
+          gi = Mfma<R>::mma(-ai, br, gi);
+        }
       }
     }
     T* pp = pout + int64_t(blk) * CQR_G;
 #pragma unroll
-    for (int m = 0; m < CQR_NE; ++m) pp[tid + m * CQR_ROWS] = acc[m];
+    for (int r = 0; r < 4; ++r) {
+      const int i = I0 + Mfma<R>::row(lane, r), j = J0 + lc;
+      if constexpr (CX) pp[i + j * QR_NB] = T(gr[r], gi[r]);
+      else pp[i + j * QR_NB] = T(gr[r]);
+    }
     return;
   }
   if (!wg0) return;
@@ -1620,8 +1654,17 @@ __global__ void __launch_bounds__(256) sum_split_kernel(int64_t len, int64_t nsp
   const int64_t e = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= len) return;
+  const T* pe = part + e * nsplit * len + i;
   T s = T(0);
-  for (int64_t q = 0; q < nsplit; ++q) s += part[(e * nsplit + q) * len + i];
+  int64_t q = 0;
+  for (; q + 8 <= nsplit; q += 8) {  // eight loads in flight, summed in chunk order
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pe[(q + u) * len];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; q < nsplit; ++q) s += pe[q * len];
   out[e * sout + i] = s;
 }
 

@@ -131,13 +131,27 @@ using namespace hssc;
 // ---------------------------------------------------------------------------------------
 
 hssc_tree::~hssc_tree() {
+  tree_sync(this);
   for (auto& L : levels) {
     dfree(L.dJ), dfree(L.dJinv), dfree(L.dPx), dfree(L.dPy), dfree(L.dPw);
     dfree(L.dPNx), dfree(L.dPNy), dfree(L.dPNw);
     dfree(L.dXpx), dfree(L.dXpy), dfree(L.dXpw), dfree(L.dT), dfree(L.dT32);
+    if (L.ready) (void)hipEventDestroy(L.ready);
   }
   dfree(dLeafG);
 }
+
+namespace hssc {
+void tree_sync(hssc_tree* t) {
+  if (!t || t->bstreams.empty()) return;
+  for (hipStream_t s : t->bstreams) (void)hipStreamSynchronize(s);
+  stats::flush();
+  for (hipStream_t s : t->bstreams) (void)hipStreamDestroy(s);
+  for (void* p : t->bpending) dfree(p);
+  t->bstreams.clear();
+  t->bpending.clear();
+}
+}  // namespace hssc
 
 namespace hssc {
 // FP64 type of the same field as T: double for real T, complex double for complex T.
@@ -257,6 +271,21 @@ struct DevBufs {
   }
 };
 
+// A non-blocking stream of one call, drained and released when the owner goes out of scope.
+// Work that must not wait for the tree's level streams runs on one: commands on the null stream
+// are ordered after the work of every other stream of the device.
+struct OwnStream {
+  hipStream_t s = nullptr;
+  OwnStream() { HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~OwnStream() {
+    (void)hipStreamSynchronize(s);
+    stats::flush();
+    (void)hipStreamDestroy(s);
+  }
+  OwnStream(const OwnStream&) = delete;
+  OwnStream& operator=(const OwnStream&) = delete;
+};
+
 // Skeleton selection of one ID level (LOCAL_Non_Preset_Skeleton_Info,
 // HSS2D/HSS2D_build_tree_fsym.m:686-699): A_b = K(X_proxy, X_src) of every key box is
 // generated on the GPU and factored by the batched column-pivoted QR ID with the adaptive
@@ -361,6 +390,8 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   const k::KernelSpec ks = kernel_spec(t->p);
   const int64_t nb = k::kQrNb;
   const int64_t kchunk = 512;
+  // K chunk of the thin products (at most 128 x 128 outputs): more, shorter chunks
+  const int64_t kchunk_thin = 128;
   // Levels are independent.  Each level's QR sequence is cut into steps (block generation,
   // one step per 32-column panel, one per back-substitution block); the levels are spread
   // over kLevelStreams streams by longest-processing-time on an estimated cost and their
@@ -377,6 +408,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   };
   std::vector<Ws> ws(t->depth + 1);
   std::vector<hipStream_t> streams(kLevelStreams, nullptr);
+  tree_sync(t);
   HIPCHK(hipStreamSynchronize(st0));  // the tree upload is complete before the level streams start
   for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   auto cleanup = [&]() {
@@ -415,7 +447,8 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       w.W1 = dalloc<S>(E * nbo * n);
       w.W2 = dalloc<S>(E * nbo * n);
       w.Tf = dalloc<S>(E * nbo * nbo);
-      w.part = dalloc<S>(E * nsplit * nbo * n);
+      const int64_t nsplit_thin = (M + kchunk_thin - 1) / kchunk_thin;
+      w.part = dalloc<S>(E * std::max(nsplit * nbo * n, nsplit_thin * nbo * nbo));
       if (tall) {
         w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
         w.Ts = dalloc<S>(E * kOuterNb * nb);
@@ -447,7 +480,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
               if (nin <= 0) continue;
               S* Cin = A + j1 + (j1 + jb) * M;
               k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin,
-                                   E, w.part, kchunk);
+                                   E, w.part, kchunk_thin);
               k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
                            w.W2, jb, jb * nin, E);
               k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
@@ -456,7 +489,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
             if (nc <= 0) return;
             // T of the outer panel from the sub-panel factors and V^H V
             k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo,
-                                 jbo * jbo, E, w.part, kchunk);
+                                 jbo * jbo, E, w.part, kchunk_thin);
             k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
                              nbo * nbo, w.W2, nbo * n, E);
           } else {
@@ -509,16 +542,28 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
     }
     size_t maxsteps = 0;
     for (int l : order) maxsteps = std::max(maxsteps, steps[l].size());
+    for (int l : order) {
+      Level& L = t->levels[l];
+      if (!L.ready) HIPCHK(hipEventCreateWithFlags(&L.ready, hipEventDisableTiming));
+    }
     for (size_t i = 0; i < maxsteps; ++i)
       for (int l : order)
-        if (i < steps[l].size()) steps[l][i](streams[lst[l]]);
-    for (auto& s : streams) HIPCHK(hipStreamSynchronize(s));
+        if (i < steps[l].size()) {
+          steps[l][i](streams[lst[l]]);
+          if (i + 1 == steps[l].size())
+            HIPCHK(hipEventRecord(t->levels[l].ready, streams[lst[l]]));
+        }
     HIPCHK(hipGetLastError());
   } catch (...) {
     cleanup();
     throw;
   }
-  cleanup();
+  // the work runs on: the tree keeps the streams and workspaces until tree_sync
+  for (auto& s : streams) t->bstreams.push_back(s);
+  for (auto& w : ws)
+    for (void* p : {(void*)w.A, (void*)w.V, (void*)w.W1, (void*)w.W2, (void*)w.Tf,
+                    (void*)w.part, (void*)w.cq, (void*)w.Ts, (void*)w.GV})
+      if (p) t->bpending.push_back(p);
   t->has_T = true;
 }
 
@@ -544,16 +589,6 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   const bool TI = t->p.transinv != 0;
   inv->FI.assign(D + 1, nullptr);
   inv->E.assign(D + 1, nullptr);
-  if (sizeof(real_t<T>) == 4) {
-    for (int l = 1; l <= D; ++l) {
-      Level& L = t->levels[l];
-      const int64_t r = L.n - L.k;
-      if (!L.dT32) L.dT32 = dalloc<T>(L.nkey * L.k * std::max<int64_t>(r, 1));
-      if (r > 0)
-        k::axpby<hi_t<T>, T>(st, L.nkey * L.k * r, T(1), static_cast<const hi_t<T>*>(L.dT), T(0),
-                             static_cast<T*>(L.dT32));
-    }
-  }
   int64_t wmax = 0;
   for (int l = 1; l <= D; ++l) {
     const Level& L = t->levels[l];
@@ -589,6 +624,14 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       inv->E[l] = EI;
       k::copy2d<T, T>(st, kk, kk, FI, n, n * n, EI, kk, kk * kk, E);
       if (r > 0) {
+        // T of this level may still be in the making on the build's streams
+        if (L.ready) HIPCHK(hipStreamWaitEvent(st, L.ready, 0));
+        if (sizeof(real_t<T>) == 4) {
+          Level& Lm = t->levels[l];
+          if (!Lm.dT32) Lm.dT32 = dalloc<T>(Lm.nkey * Lm.k * r);
+          k::axpby<hi_t<T>, T>(st, Lm.nkey * Lm.k * r, T(1), static_cast<const hi_t<T>*>(Lm.dT),
+                               T(0), static_cast<T*>(Lm.dT32));
+        }
         const T* Tm = level_T<T>(L);
         k::copy2d<T, T>(st, kk, r, FI + kk * n, n, n * n, Wk, kk, kk * r, E);
         k::gemm<T>(st, false, false, kk, r, r, T(1), Tm, kk, kk * r, FI + kk + kk * n, n, n * n,
@@ -609,6 +652,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   }
   HIPCHK(hipGetLastError());
   dfree(Wk);
+  tree_sync(t);  // every level's T has been waited for: release the build's resources
 }
 
 // ---------------------------------------------------------------------------------------
@@ -805,6 +849,7 @@ void apply_impl(const hssc_operator* op, const S* X, S* Y, int64_t m, hipStream_
 }
 
 void apply_dispatch(const hssc_operator* op, const void* X, void* Y, int64_t m, hipStream_t st) {
+  tree_sync(op->tree);  // the sweeps read T
   if (op->tree->cx)
     apply_impl<zdouble>(op, static_cast<const zdouble*>(X), static_cast<zdouble*>(Y), m, st);
   else
@@ -825,6 +870,7 @@ double device_norm(const S* x, int64_t len, hipStream_t st, double* part_d) {
 
 void applyinv_dispatch(const hssc_inverse* inv, const void* X, void* Y, int64_t m,
                        hipStream_t st) {
+  tree_sync(inv->tree);
   const bool fp64 = inv->prec == HSSC_PREC_FP64;
   if (inv->tree->cx) {
     const auto* Xz = static_cast<const zdouble*>(X);
@@ -1083,6 +1129,7 @@ int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out) {
   if (int s = box_level(t, box, &lev)) return s;
   if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
   if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
+  tree_sync(const_cast<hssc_tree*>(t));
   const Level& L = t->levels[lev];
   if (lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no interpolation operator");
   const int64_t key = box_key(t, box, lev), cnt = L.k * (L.n - L.k);
@@ -1123,12 +1170,13 @@ int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** out) {
     inv->tree = t;
     inv->prec = precision;
     try {
+      OwnStream os;
       if (t->cx) {
-        if (precision == HSSC_PREC_FP64) invert_impl<zdouble>(t, inv, 0);
-        else invert_impl<zfloat>(t, inv, 0);
+        if (precision == HSSC_PREC_FP64) invert_impl<zdouble>(t, inv, os.s);
+        else invert_impl<zfloat>(t, inv, os.s);
       } else {
-        if (precision == HSSC_PREC_FP64) invert_impl<double>(t, inv, 0);
-        else invert_impl<float>(t, inv, 0);
+        if (precision == HSSC_PREC_FP64) invert_impl<double>(t, inv, os.s);
+        else invert_impl<float>(t, inv, os.s);
       }
     } catch (...) {
       delete inv;
@@ -1198,6 +1246,7 @@ int hssc_hss2d_compress(hssc_tree* t, hssc_operator** out) {
     const int D = t->depth;
     op->B.assign(D + 1, nullptr);
     const k::KernelSpec ks = kernel_spec(t->p);
+    hipStream_t st = nullptr;
     auto build = [&](auto tag) {
       using S = decltype(tag);
       for (int l = 0; l <= D; ++l) {
@@ -1205,21 +1254,25 @@ int hssc_hss2d_compress(hssc_tree* t, hssc_operator** out) {
         if (l == D) {
           S* B = dalloc<S>(L.nkey * L.n * L.n);
           op->B[l] = B;
-          k::gen_block<S>(0, L.n, L.n, L.nkey, L.dPx, L.dPy, L.dPw, L.n, L.dPx, L.dPy, L.dPw, L.n,
+          k::gen_block<S>(st, L.n, L.n, L.nkey, L.dPx, L.dPy, L.dPw, L.n, L.dPx, L.dPy, L.dPw, L.n,
                           B, L.n, L.n * L.n, ks);
         } else {
           const int64_t kc = t->levels[l + 1].k;
           S* B = dalloc<S>(L.nkey * kc * kc);
           op->B[l] = B;
-          k::gen_block<S>(0, kc, kc, L.nkey, L.dPNx, L.dPNy, L.dPNw, L.n, L.dPNx + kc,
+          k::gen_block<S>(st, kc, kc, L.nkey, L.dPNx, L.dPNy, L.dPNw, L.n, L.dPNx + kc,
                           L.dPNy + kc, L.dPNw + kc, L.n, B, kc, kc * kc, ks);
         }
       }
     };
     try {
+      // a stream of its own: the tree's interpolation operators may still be in the making on
+      // the tree's streams, and the blocks made here do not need them
+      OwnStream os;
+      st = os.s;
       if (t->cx) build(zdouble());
       else build(double());
-      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipStreamSynchronize(st));
       HIPCHK(hipGetLastError());
     } catch (...) {
       delete op;
