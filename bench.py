@@ -68,7 +68,7 @@ def stats_snapshot(H):
     L = H.lib()
     out = {}
     names = [f"{k}_{t}" for k in ("gemv", "gemvt", "gj_small", "gj_diag", "gen_block",
-                                  "gen_parent", "qr_panel")
+                                  "gen_parent", "cqr_panel")
              for t in ("f64", "f32", "c128", "c64")]
     names += [f"gemm_{t}_{a}{b}" for t in ("f64", "f32", "c128", "c64") for a in "ntch"
               for b in "nt"]
@@ -304,7 +304,7 @@ def roofline(stats):
     prec = next(t for t in ("f64", "f32", "c128", "c64") if f"_{t}" in name)
     common = dict(kernel=name, traffic=None, launches=s["launches"],
                   avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2))
-    if name.startswith(("gemm", "gj_", "qr_panel")):
+    if name.startswith(("gemm", "gj_", "cqr_panel")):
         ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
         return dict(bound="mfma", achieved=round(ach, 3), peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
                     frac=round(ach / PEAK_TFLOPS[prec], 4), flops_per_launch=s["flops"] / s["launches"],

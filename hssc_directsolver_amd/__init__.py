@@ -5,7 +5,7 @@ include/hssc.h).  This package only binds it and mirrors the reference's MATLAB 
 points (see hss2d.py).  Importing it does not require a GPU; calling a GPU entry point
 without a device raises HsscError.
 """
-from ._lib import HsscError, lib  # noqa: F401
+from ._lib import HsscError, cached_memory, lib, release_cached_memory  # noqa: F401
 from .hss2d import (  # noqa: F401
     HSS2D_apply_fsym, HSS2D_apply_TI, HSS2D_applyinv_fsym, HSS2D_applyinv_TI,
     HSS2D_build_index_sets, HSS2D_build_tree_fsym, HSS2D_compress_fsym, HSS2D_invert_fsym,

@@ -103,6 +103,8 @@ SIGNATURES = [
     ("hssc_hss1d_box", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), _I64P, _I64P]),
     ("hssc_hss1d_destroy", None, [_P]),
+    ("hssc_release_cached_memory", C.c_int, []),
+    ("hssc_cached_memory", C.c_int, [_I64P]),
     ("hssc_stats_enable", C.c_int, [C.c_int32]),
     ("hssc_stats_reset", C.c_int, []),
     ("hssc_stats_get", C.c_int, [C.c_char_p, _I64P, _DP, _DP, _DP]),
@@ -136,3 +138,15 @@ def check(status):
     if status != HSSC_OK:
         raise HsscError(status, lib().hssc_last_error().decode())
     return status
+
+
+def release_cached_memory():
+    """Synchronises the device and frees the device blocks the library keeps for reuse."""
+    check(lib().hssc_release_cached_memory())
+
+
+def cached_memory() -> int:
+    """Bytes of released device blocks the library keeps for reuse on the current device."""
+    n = C.c_int64()
+    check(lib().hssc_cached_memory(C.byref(n)))
+    return n.value

@@ -24,6 +24,7 @@
 #include <thread>
 #include <vector>
 
+#include "pool.h"
 #include "internal.h"
 #include "kernels.h"
 #include "stats.h"
@@ -48,7 +49,7 @@ struct HipErr {
 template <typename T>
 T* dmalloc(size_t n) {
   void* p = nullptr;
-  H1CHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  H1CHK(pool::alloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
   return static_cast<T*>(p);
 }
 
@@ -66,9 +67,7 @@ std::vector<T> dget(const T* d, size_t n) {
   return h;
 }
 
-void dfree1(void* p) {
-  if (p) (void)hipFree(p);
-}
+void dfree1(void* p) { pool::release(p); }
 
 // Runs f(i) for i in [0, n) on up to 16 host threads (independent iterations).
 template <class F>

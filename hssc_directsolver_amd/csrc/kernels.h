@@ -66,18 +66,16 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
                     hipStream_t aux = nullptr);
 
 constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
-constexpr int64_t kQrMaxRows = 8192;  // panel rows handled by one workgroup (512 x 16)
 
-template <typename T>
-void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
-              T* Tf, int64_t sT, int64_t batch);
-// Tall panels (M - j0 > kCqrMinRows): shifted CholeskyQR3 with Householder reconstruction.
-// Writes the panel's R rows into A, V (rows j0..M-1, ld ldv) and Tf as qr_panel + qr_copyV
-// do.  ws: batch * cqr_ws_elems(M) elements.
+// One 32-column panel of the blocked Householder QR: shifted CholeskyQR3 with Householder
+// reconstruction.  Writes the panel's R rows into A, the unit lower trapezoidal V (rows
+// j0..M-1, ld ldv) and the compact-WY factor Tf (32 x 32).  ws: batch * cqr_ws_elems(M)
+// elements, laid out per matrix as Racc | Qtop | two sets of ceil(M / kCqrRows) partial Gram
+// blocks | Q_1 (M x 32).
 constexpr int kCqrRows = 256;
-constexpr int64_t kCqrMinRows = 1024;
+constexpr int64_t cqr_row_blocks(int64_t M) { return (M + kCqrRows - 1) / kCqrRows; }
 constexpr int64_t cqr_ws_elems(int64_t M) {
-  return int64_t(2 + 2 * (kQrMaxRows / kCqrRows)) * kQrNb * kQrNb + M * kQrNb;
+  return (2 + 2 * cqr_row_blocks(M)) * kQrNb * kQrNb + M * kQrNb;
 }
 template <typename T>
 void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
@@ -89,9 +87,6 @@ void t_assemble(hipStream_t st, int n, const T* Ts, int64_t sTs, const T* GV, in
                 int64_t sG, T* To, int64_t ldt, int64_t sT, T* X, int64_t sX, int64_t batch);
 template <typename T>
 void zero2d(hipStream_t st, int64_t m, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
-template <typename T>
-void qr_copyV(hipStream_t st, int64_t M, int64_t j0, int jb, const T* A, int64_t lda, int64_t sA,
-              T* V, int64_t ldv, int64_t sV, int64_t batch);
 template <typename T>
 void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
                  int64_t sW, int64_t batch);

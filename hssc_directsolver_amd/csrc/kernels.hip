@@ -642,328 +642,13 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 // ------------------------------------------------------------------------------------
 // Householder QR least squares (MATLAB K11\K21 in HSS2D/interpolation_operator.m:66)
 // ------------------------------------------------------------------------------------
-// Panel kernel: one workgroup factors columns [j0, j0+jb) of A (rows j0..M-1), streaming
-// the panel through L2.  Reflector i: H_i = I - tau_i v_i v_i^H with v_i(0) = 1 stored
-// below the diagonal and the real beta on the diagonal (LAPACK xgeqr2 / xlarfg
-// conventions: beta = -sign(Re a) * norm, tau = (beta - a) / beta; the columns are updated
-// by H_i^H).  It then forms the compact-WY triangular factor Tf (jb x jb, upper) with
-// H_0 ... H_{jb-1} = I - V Tf V^H (xlarft, forward, columnwise).  For real data conj is
-// the identity and this is dgeqr2 / dlarft.
+// The interpolation least squares run as blocked Householder QR: each 32-column panel is
+// factored by the row-parallel CholeskyQR3 kernel below and brought back to Householder form
+// (V unit lower trapezoidal, the compact-WY factor Tf with H_0 ... H_{jb-1} = I - V Tf V^H,
+// as LAPACK's xgeqrf / xlarft produce it, up to unit-modulus scalings of the rows of R); the
+// trailing updates are GEMMs.
 
-constexpr int QR_NB = 32;  // outer panel width (the trailing update runs as GEMMs)
-constexpr int QR_RS = 16;  // row slots per thread: M - j0 <= NT * QR_RS
-// inner block held in registers and reflectors accumulated per reduction pass: half as
-// many for complex data (twice the registers per entry)
-template <typename T>
-constexpr int qr_ib() {
-  return is_complex_v<T> ? 2 : 4;
-}
-template <typename T>
-constexpr int qr_ac() {
-  return is_complex_v<T> ? 2 : 4;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += shfl_xor_s(v, o);
-  return v;
-}
-
-// Left-looking panel: for each inner block of IB columns, (1) apply the panel's earlier
-// reflectors through the compact-WY factor (x -= V Tf^H V^H x), (2) factor the block in
-// registers with one fused reduction per column (norm, dot products and the diagonal row
-// broadcast through double-buffered LDS, one barrier), (3) extend the Gram matrix V^H V by
-// the block's columns and (4) the triangular factor Tf.  Row r = j0 + tid + NT*s of the
-// panel lives in slot s of thread tid.  The reductions of (1) and (3) run over all earlier
-// reflectors with wave shuffles into per-wave LDS partials and a single barrier per phase.
-// NT = 64 (one wave, M - j0 <= 64 RS = 1024) serves the many small boxes of the low levels;
-// NT = 512 (M - j0 <= 8192) the few large ones.
-template <typename T, int NT, int RS>
-__global__ void __launch_bounds__(NT) qr_panel_kernel(int64_t M, int64_t j0, int jb,
-                                                     T* __restrict__ A, int64_t lda, int64_t sA,
-                                                     T* __restrict__ Tfo, int64_t sT) {
-  using R = real_t<T>;
-  constexpr int IB = qr_ib<T>(), AC = qr_ac<T>(), NW = NT / 64;
-  __shared__ T Tf[QR_NB][QR_NB + 1];
-  __shared__ T G[QR_NB][QR_NB + 1];
-  __shared__ T W[QR_NB][IB];
-  __shared__ T Wp[QR_NB][IB];
-  __shared__ T part[NW][QR_NB][IB];  // per-wave partial sums of (1) and (3)
-  __shared__ T red[2][NW][IB];
-  __shared__ T rowv[2][IB];
-  __shared__ T tau_s[QR_NB];
-  A += int64_t(blockIdx.x) * sA;
-  Tfo += int64_t(blockIdx.x) * sT;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int idx = tid; idx < QR_NB * (QR_NB + 1); idx += NT) {
-    (&Tf[0][0])[idx] = T(0);
-    (&G[0][0])[idx] = T(0);
-  }
-  __syncthreads();
-  // local row index l = r - j0 (int); the panel block A(j0 + l, j0 + c) is P[l + c*lda]
-  T* P = A + j0 + j0 * lda;
-  const int Ml = int(M - j0);
-  T x[RS][IB];
-  int buf = 0;
-  for (int c0 = 0; c0 < jb; c0 += IB) {
-    const int ib = (jb - c0 < IB) ? jb - c0 : IB;
-    T* Pb = P + int64_t(c0) * lda;  // first column of the block
-    // load the raw block
-#pragma unroll
-    for (int s = 0; s < RS; ++s) {
-      const int l = tid + NT * s;
-#pragma unroll
-      for (int jj = 0; jj < IB; ++jj)
-        x[s][jj] = (l < Ml && jj < ib) ? Pb[l + int64_t(jj) * lda] : T(0);
-    }
-    // (1) W = V_prev^H x, W' = Tf^H W, x -= V_prev W'
-    if (c0 > 0) {
-      for (int a0 = 0; a0 < c0; a0 += AC) {
-        T acc[AC * IB];
-#pragma unroll
-        for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-          const int l = tid + NT * s;
-          if (l >= Ml || l < a0) continue;
-#pragma unroll
-          for (int u = 0; u < AC; ++u) {
-            const int a = a0 + u;
-            if (a < c0 && l >= a) {
-              const T v = conj_s((l == a) ? T(1) : P[l + int64_t(a) * lda]);
-#pragma unroll
-              for (int jj = 0; jj < IB; ++jj) acc[u * IB + jj] += v * x[s][jj];
-            }
-          }
-        }
-#pragma unroll
-        for (int v = 0; v < AC * IB; ++v) acc[v] = wave_sum(acc[v]);
-        if (lane == 0) {
-#pragma unroll
-          for (int u = 0; u < AC; ++u)
-            if (a0 + u < c0) {
-#pragma unroll
-              for (int jj = 0; jj < IB; ++jj) part[wave][a0 + u][jj] = acc[u * IB + jj];
-            }
-        }
-      }
-      __syncthreads();
-      for (int idx = tid; idx < c0 * IB; idx += NT) {
-        const int a = idx / IB, jj = idx % IB;
-        T sum = T(0);
-        for (int w = 0; w < NW; ++w) sum += part[w][a][jj];
-        W[a][jj] = sum;
-      }
-      __syncthreads();
-      for (int idx = tid; idx < c0 * IB; idx += NT) {
-        const int a = idx / IB, jj = idx % IB;
-        T sum = T(0);
-        for (int b = 0; b <= a; ++b) sum += conj_s(Tf[b][a]) * W[b][jj];
-        Wp[a][jj] = sum;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < RS; ++s) {
-        const int l = tid + NT * s;
-        if (l >= Ml) continue;
-        const int amax = (c0 < l + 1) ? c0 : l + 1;  // v_a(l) = 0 for l < a
-        for (int a = 0; a < amax; ++a) {
-          const T v = (l == a) ? T(1) : P[l + int64_t(a) * lda];
-#pragma unroll
-          for (int jj = 0; jj < IB; ++jj) x[s][jj] -= v * Wp[a][jj];
-        }
-      }
-    }
-    // (2) factor the block
-#pragma unroll
-    for (int c = 0; c < IB; ++c) {
-      if (c < ib) {
-        const int ld = c0 + c;  // local row of the diagonal
-        const int own = ld % NT, so = ld / NT;
-        R sig = R(0);
-        T d[IB];
-#pragma unroll
-        for (int jj = 0; jj < IB; ++jj) d[jj] = T(0);
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-          const int l = tid + NT * s;
-          if (l >= ld && l < Ml) {
-            const T xc = x[s][c];
-            if (l > ld) sig += abs2(xc);
-#pragma unroll
-            for (int jj = c + 1; jj < IB; ++jj) d[jj] += conj_s(xc) * x[s][jj];
-          }
-        }
-        sig = wave_sum(sig);
-#pragma unroll
-        for (int jj = c + 1; jj < IB; ++jj) d[jj] = wave_sum(d[jj]);
-        if (lane == 0) {
-          red[buf][wave][0] = T(sig);
-#pragma unroll
-          for (int jj = c + 1; jj < IB; ++jj) red[buf][wave][jj] = d[jj];
-        }
-        if (tid == own) {
-#pragma unroll
-          for (int s = 0; s < RS; ++s)
-            if (s == so) {
-              rowv[buf][0] = x[s][c];
-#pragma unroll
-              for (int jj = c + 1; jj < IB; ++jj) rowv[buf][jj] = x[s][jj];
-            }
-        }
-        __syncthreads();
-        sig = R(0);
-#pragma unroll
-        for (int jj = c + 1; jj < IB; ++jj) d[jj] = T(0);
-        for (int w = 0; w < NW; ++w) {
-          sig += re_s(red[buf][w][0]);
-#pragma unroll
-          for (int jj = c + 1; jj < IB; ++jj) d[jj] += red[buf][w][jj];
-        }
-        const T alpha = rowv[buf][0];
-        T tau, scal;
-        R beta;
-        if (sig == R(0) && im_s(alpha) == R(0)) {
-          tau = T(0), beta = re_s(alpha), scal = T(0);
-        } else {
-          const R nrm = sqrt(abs2(alpha) + sig);
-          beta = re_s(alpha) >= R(0) ? -nrm : nrm;
-          tau = (T(beta) - alpha) * T(R(1) / beta);
-          scal = T(1) / (alpha - T(beta));
-        }
-        // w_j = v^H a_j = a_j(ld) + conj(scal) (sum_{l>=ld} conj(x_l) a_j(l) - conj(alpha)
-        // a_j(ld)); column j is updated by H^H: a_j -= v conj(tau) w_j
-        T tw[IB];
-#pragma unroll
-        for (int jj = 0; jj < IB; ++jj) tw[jj] = T(0);
-#pragma unroll
-        for (int jj = c + 1; jj < IB; ++jj) {
-          const T arow = rowv[buf][jj];
-          tw[jj] = conj_s(tau) * (arow + conj_s(scal) * (d[jj] - conj_s(alpha) * arow));
-        }
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-          const int l = tid + NT * s;
-          if (l >= ld && l < Ml) {
-            const T v = (l == ld) ? T(1) : x[s][c] * scal;
-#pragma unroll
-            for (int jj = c + 1; jj < IB; ++jj) x[s][jj] -= v * tw[jj];
-            x[s][c] = (l == ld) ? T(beta) : v;
-          }
-        }
-        if (tid == 0) tau_s[c0 + c] = tau;
-        buf ^= 1;
-      }
-    }
-    // store the block
-#pragma unroll
-    for (int s = 0; s < RS; ++s) {
-      const int l = tid + NT * s;
-      if (l < Ml) {
-#pragma unroll
-        for (int jj = 0; jj < IB; ++jj)
-          if (jj < ib) Pb[l + int64_t(jj) * lda] = x[s][jj];
-      }
-    }
-    // (3) Gram columns G(a, c0+c) = v_a^H v_{c0+c} for a < c0+c, accumulated from the
-    // block's registers (v_c: 0 above its diagonal, 1 on it, x below) against the earlier
-    // reflectors and the block itself.
-    for (int a0 = 0; a0 < c0 + IB; a0 += AC) {
-      T acc[AC * IB];
-#pragma unroll
-      for (int v = 0; v < AC * IB; ++v) acc[v] = T(0);
-#pragma unroll
-      for (int s = 0; s < RS; ++s) {
-        const int l = tid + NT * s;
-        if (l >= Ml || l < c0) continue;
-        T vb[IB];
-#pragma unroll
-        for (int c = 0; c < IB; ++c)
-          vb[c] = (c >= ib || l < c0 + c) ? T(0) : (l == c0 + c ? T(1) : x[s][c]);
-#pragma unroll
-        for (int u = 0; u < AC; ++u) {
-          const int a = a0 + u;
-          T va;
-          if (a < c0) {
-            va = (l < a) ? T(0) : (l == a ? T(1) : P[l + int64_t(a) * lda]);
-          } else {
-            T t = T(0);
-#pragma unroll
-            for (int c = 0; c < IB; ++c)
-              if (a - c0 == c) t = vb[c];
-            va = t;
-          }
-          va = conj_s(va);
-#pragma unroll
-          for (int c = 0; c < IB; ++c) acc[u * IB + c] += va * vb[c];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < AC * IB; ++v) acc[v] = wave_sum(acc[v]);
-      if (lane == 0) {
-#pragma unroll
-        for (int u = 0; u < AC; ++u)
-          if (a0 + u < c0 + IB && a0 + u < QR_NB) {
-#pragma unroll
-            for (int c = 0; c < IB; ++c) part[wave][a0 + u][c] = acc[u * IB + c];
-          }
-      }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < (c0 + IB) * IB; idx += NT) {
-      const int a = idx / IB, c = idx % IB;
-      const int ci = c0 + c;
-      if (c < ib && a < ci) {
-        T sum = T(0);
-        for (int w = 0; w < NW; ++w) sum += part[w][a][c];
-        G[a][ci] = sum;
-      }
-    }
-    __syncthreads();
-    // (4) Tf(0:i, i) = -tau_i Tf(0:i, 0:i) G(0:i, i), Tf(i, i) = tau_i, one column after the
-    // other by wave 0 (its LDS accesses complete in program order)
-    if (wave == 0) {
-      for (int c = 0; c < ib; ++c) {
-        const int i = c0 + c;
-        if (lane < i) {
-          T sum = T(0);
-          for (int q = lane; q < i; ++q) sum += Tf[lane][q] * G[q][i];
-          Tf[lane][i] = -(tau_s[i] * sum);
-        } else if (lane == i) {
-          Tf[i][i] = tau_s[i];
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < QR_NB * QR_NB; idx += NT) {
-    const int r = idx % QR_NB, i = idx / QR_NB;
-    Tfo[r + i * QR_NB] = Tf[r][i];
-  }
-}
-
-// Copies V (unit lower trapezoidal, rows j0..M-1, jb columns) into a dense workspace
-// so the trailing update can use the plain GEMM kernel.
-template <typename T>
-__global__ void __launch_bounds__(256) qr_copyV_kernel(int64_t M, int64_t j0, int jb,
-                                                       const T* __restrict__ A, int64_t lda,
-                                                       int64_t sA, T* __restrict__ V, int64_t ldv,
-                                                       int64_t sV) {
-  const int64_t e = blockIdx.z;
-  A += e * sA;
-  V += e * sV;
-  const int64_t r = j0 + int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (r >= M) return;
-  for (int c = 0; c < jb; ++c) {
-    const int64_t rr = r - j0;
-    T v;
-    if (rr < c) v = T(0);
-    else if (rr == c) v = T(1);
-    else v = A[r + (j0 + c) * lda];
-    V[rr + c * ldv] = v;
-  }
-}
+constexpr int QR_NB = 32;  // panel width (the trailing update runs as GEMMs)
 
 // Triangular solve helper: in-place inverse of an upper-triangular nb x nb diagonal block
 // (used by the blocked back substitution for T = R11^-1 C).
@@ -999,31 +684,6 @@ __global__ void __launch_bounds__(256) trinv_upper_kernel(int64_t p, int nb, con
 }
 
 template <typename T>
-void qr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
-              T* Tf, int64_t sT, int64_t batch) {
-  const double rows = double(M - j0);
-  const double fl = (is_complex_v<T> ? 16.0 : 4.0) * rows * jb * jb * batch;
-  const double by = 2.0 * rows * jb * sizeof(T) * batch;
-  if (M - j0 <= 64 * QR_RS) {
-    HSSC_TIMED(lbl<T>("qr_panel").c_str(), st, fl, by,
-               hipLaunchKernelGGL((qr_panel_kernel<T, 64, QR_RS>), dim3(batch), dim3(64), 0, st,
-                                  M, j0, jb, A, lda, sA, Tf, sT));
-  } else {
-    HSSC_TIMED(lbl<T>("qr_panel").c_str(), st, fl, by,
-               hipLaunchKernelGGL((qr_panel_kernel<T, 512, QR_RS>), dim3(batch), dim3(512), 0, st,
-                                  M, j0, jb, A, lda, sA, Tf, sT));
-  }
-}
-
-template <typename T>
-void qr_copyV(hipStream_t st, int64_t M, int64_t j0, int jb, const T* A, int64_t lda, int64_t sA,
-              T* V, int64_t ldv, int64_t sV, int64_t batch) {
-  dim3 grid((M - j0 + 255) / 256, 1, batch);
-  hipLaunchKernelGGL(qr_copyV_kernel<T>, grid, dim3(256), 0, st, M, j0, jb, A, lda, sA, V, ldv,
-                     sV);
-}
-
-template <typename T>
 void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
                  int64_t sW, int64_t batch) {
   hipLaunchKernelGGL(trinv_upper_kernel<T>, dim3(batch), dim3(256), 0, st, p, nb, R, ldr, sR, W,
@@ -1031,11 +691,11 @@ void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int
 }
 
 // ------------------------------------------------------------------------------------
-// Tall panels: shifted CholeskyQR3 and Householder reconstruction
+// Panels: shifted CholeskyQR3 and Householder reconstruction
 // ------------------------------------------------------------------------------------
-// A panel of Mp = M - j0 > 1024 rows keeps the one-workgroup Householder kernel streaming it
-// through L2 once per column, on one CU.  Tall panels are factored instead by shifted
-// CholeskyQR3 (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42,
+// A 32-column panel (Mp = M - j0 rows) is factored by shifted CholeskyQR3, which keeps every
+// step row-parallel (a one-workgroup Householder panel streams the panel through L2 once per
+// column on one CU, a latency chain of 32 reductions): (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42,
 // 2020): Q_1 = A R_1^-1, Q_2 = Q_1 R_2^-1, Q = Q_2 R_3^-1 with R_p the Cholesky factor of
 // the Gram matrix of the previous Q (the first one shifted by 11 (Mp jb + jb (jb + 1)) u
 // ||A||_F^2), R = R_3 R_2 R_1.  The orthonormal Q is turned back into Householder form
@@ -1055,24 +715,21 @@ void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int
 // the R rows of A and the top rows of V.
 
 constexpr int CQR_ROWS = kCqrRows;                     // rows per workgroup
-constexpr int CQR_PMAX = int(kQrMaxRows / CQR_ROWS);   // partial Gram blocks per matrix
 constexpr int CQR_G = QR_NB * QR_NB;                    // one NB x NB block
 constexpr int CQR_NE = CQR_G / CQR_ROWS;                // NB x NB entries per thread
 static_assert(QR_NB == kQrNb, "panel width");
 static_assert(CQR_ROWS == 256 && CQR_G % CQR_ROWS == 0, "entry split");
 
-// workspace offsets (per matrix): Racc | Qtop | partial Gram A | partial Gram B | Q_1
-constexpr int64_t CQR_RACC = 0, CQR_QTOP = CQR_G, CQR_PART0 = 2 * CQR_G,
-                  CQR_PART1 = CQR_PART0 + int64_t(CQR_PMAX) * CQR_G,
-                  CQR_Q1 = CQR_PART1 + int64_t(CQR_PMAX) * CQR_G;
-static_assert(CQR_Q1 == int64_t(2 + 2 * (kQrMaxRows / kCqrRows)) * kQrNb * kQrNb,
-              "workspace layout");
+// workspace offsets (per matrix): Racc | Qtop | partial Gram A | partial Gram B | Q_1; the
+// second partial set starts at part1 = CQR_PART0 + cqr_row_blocks(M) * CQR_G
+constexpr int64_t CQR_RACC = 0, CQR_QTOP = CQR_G, CQR_PART0 = 2 * CQR_G;
 
 template <typename T>
 __global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
-    int pass, int64_t Mp, int jb, int P, const T* X, int64_t ldx, int64_t sX, T* Qo,
-    int64_t ldq, int64_t sQ, T* __restrict__ ws, int64_t sW, T* V, int64_t ldv, int64_t sV,
-    T* __restrict__ A, int64_t lda, int64_t sA, int64_t j0, T* __restrict__ Tfo, int64_t sT) {
+    int pass, int64_t Mp, int jb, int P, int64_t part1, const T* X, int64_t ldx, int64_t sX,
+    T* Qo, int64_t ldq, int64_t sQ, T* __restrict__ ws, int64_t sW, T* V, int64_t ldv,
+    int64_t sV, T* __restrict__ A, int64_t lda, int64_t sA, int64_t j0, T* __restrict__ Tfo,
+    int64_t sT) {
   using R = real_t<T>;
   __shared__ T Rm[QR_NB][QR_NB + 1];       // Cholesky factor of this pass
   __shared__ T Wb[2 * QR_NB][QR_NB + 1];   // Gram staging (64 rows); pass 3: LU | scratch
@@ -1084,8 +741,8 @@ __global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
   const bool wg0 = blk == 0;
   ws += e * sW;
   X += e * sX;
-  const T* pin = ws + ((pass & 1) ? CQR_PART0 : CQR_PART1);   // partial Grams of pass - 1
-  T* pout = ws + ((pass & 1) ? CQR_PART1 : CQR_PART0);        // partial Grams of this pass
+  const T* pin = ws + ((pass & 1) ? CQR_PART0 : part1);   // partial Grams of pass - 1
+  T* pout = ws + ((pass & 1) ? part1 : CQR_PART0);        // partial Grams of this pass
   if (pass > 0) {
     // R = chol(sum of the partial Gram matrices (+ shift)); the partials are loaded eight at
     // a time and summed in block order
@@ -1309,23 +966,24 @@ __global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
 template <typename T>
 void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
                T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, T* ws, int64_t batch) {
-  const int64_t Mp = M - j0, sW = cqr_ws_elems(M);
+  const int64_t Mp = M - j0, sW = cqr_ws_elems(M), Pm = cqr_row_blocks(M);
   if (Mp <= 0 || jb <= 0 || batch <= 0) return;
   const int P = int((Mp + CQR_ROWS - 1) / CQR_ROWS);
-  T* Q1 = ws + CQR_Q1;
+  const int64_t part1 = CQR_PART0 + Pm * CQR_G;
+  T* Q1 = ws + part1 + Pm * CQR_G;
   const T* Ap = A + j0 + j0 * lda;
   const dim3 g(P, batch), b(CQR_ROWS);
   const double fl = (is_complex_v<T> ? 4.0 : 1.0) * 2.0 * 5.0 * double(Mp) * jb * jb * batch;
   const double by = 7.0 * double(Mp) * jb * sizeof(T) * batch;
   HSSC_TIMED(lbl<T>("cqr_panel").c_str(), st, fl, by, {
     // pass 0: Gram of the panel; 1: Q_1 -> workspace; 2: Q_2 -> V; 3: V <- Householder form
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 0, Mp, jb, P, Ap, lda, sA, (T*)nullptr,
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 0, Mp, jb, P, part1, Ap, lda, sA, (T*)nullptr,
                        int64_t(0), int64_t(0), ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 1, Mp, jb, P, Ap, lda, sA, Q1, Mp, sW, ws,
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 1, Mp, jb, P, part1, Ap, lda, sA, Q1, Mp, sW, ws,
                        sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 2, Mp, jb, P, (const T*)Q1, Mp, sW, V,
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 2, Mp, jb, P, part1, (const T*)Q1, Mp, sW, V,
                        ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 3, Mp, jb, P, (const T*)V, ldv, sV, V,
+    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 3, Mp, jb, P, part1, (const T*)V, ldv, sV, V,
                        ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
   });
 }
@@ -1728,10 +1386,6 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
                              int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t);        \
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
                                   hipStream_t);                                                    \
-  template void qr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \
-                            int64_t);                                                              \
-  template void qr_copyV<T>(hipStream_t, int64_t, int64_t, int, const T*, int64_t, int64_t, T*,    \
-                            int64_t, int64_t, int64_t);                                            \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
   template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \

@@ -1,0 +1,150 @@
+// Device memory cache (see pool.h).
+#include "pool.h"
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace hssc {
+namespace pool {
+namespace {
+
+struct Device {
+  std::unordered_map<void*, size_t> live;   // handed out: size of each block
+  std::multimap<size_t, void*> ready;       // reusable now
+  std::vector<std::pair<void*, size_t>> released;  // reusable after the next device sync
+  size_t cached = 0;                        // bytes in ready + released
+};
+
+std::mutex mu;
+// never destroyed: blocks outlive static destruction order (the runtime may be gone by then)
+std::map<int, Device>& devices() {
+  static auto* d = new std::map<int, Device>();
+  return *d;
+}
+
+size_t cap_bytes() {
+  static const size_t cap = [] {
+    const char* v = std::getenv("HSSC_POOL_CAP_GB");
+    const double gb = v ? std::atof(v) : 96.0;
+    return size_t(gb > 0 ? gb * (1ull << 30) : 0);
+  }();
+  return cap;
+}
+
+// sizes are rounded so that requests of nearly the same size share blocks
+size_t round_size(size_t b) {
+  if (b == 0) b = 1;
+  const size_t g = b >= (size_t(1) << 20) ? (size_t(2) << 20) : 4096;
+  return (b + g - 1) / g * g;
+}
+
+Device& current() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return devices()[dev];
+}
+
+void make_ready_locked(Device& d) {
+  if (d.released.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& b : d.released) d.ready.emplace(b.second, b.first);
+  d.released.clear();
+}
+
+void free_all_locked(Device& d) {
+  make_ready_locked(d);
+  for (auto& b : d.ready) (void)hipFree(b.second);
+  d.ready.clear();
+  d.cached = 0;
+}
+
+bool take_locked(Device& d, size_t s, void** p) {
+  auto it = d.ready.find(s);
+  if (it == d.ready.end()) return false;
+  *p = it->second;
+  d.ready.erase(it);
+  d.cached -= s;
+  d.live[*p] = s;
+  return true;
+}
+
+}  // namespace
+
+hipError_t alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  const size_t s = round_size(bytes);
+  std::lock_guard<std::mutex> g(mu);
+  Device& d = current();
+  if (take_locked(d, s, p)) return hipSuccess;
+  for (auto& b : d.released)
+    if (b.second == s) {
+      make_ready_locked(d);
+      if (take_locked(d, s, p)) return hipSuccess;
+      break;
+    }
+  hipError_t e = hipMalloc(p, s);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    free_all_locked(d);
+    e = hipMalloc(p, s);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return e;
+  }
+  d.live[*p] = s;
+  return hipSuccess;
+}
+
+void release(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu);
+  Device& d = current();
+  auto it = d.live.find(p);
+  if (it == d.live.end()) {
+    // not from this device's cache (another device current): free it directly
+    for (auto& kv : devices())
+      if (kv.second.live.count(p)) {
+        kv.second.released.emplace_back(p, kv.second.live[p]);
+        kv.second.cached += kv.second.live[p];
+        kv.second.live.erase(p);
+        return;
+      }
+    (void)hipFree(p);
+    return;
+  }
+  d.released.emplace_back(p, it->second);
+  d.cached += it->second;
+  d.live.erase(it);
+  if (d.cached > cap_bytes()) {
+    // over the cap: free the largest reusable blocks first
+    make_ready_locked(d);
+    while (d.cached > cap_bytes() && !d.ready.empty()) {
+      auto last = std::prev(d.ready.end());
+      (void)hipFree(last->second);
+      d.cached -= last->first;
+      d.ready.erase(last);
+    }
+  }
+}
+
+void reclaim() {
+  std::lock_guard<std::mutex> g(mu);
+  make_ready_locked(current());
+}
+
+void trim() {
+  std::lock_guard<std::mutex> g(mu);
+  free_all_locked(current());
+}
+
+size_t cached_bytes() {
+  std::lock_guard<std::mutex> g(mu);
+  return current().cached;
+}
+
+}  // namespace pool
+}  // namespace hssc

@@ -22,6 +22,7 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "pool.h"
 #include "stats.h"
 
 namespace hssc {
@@ -52,7 +53,7 @@ template <typename T>
 T* dalloc(size_t count) {
   if (count == 0) count = 1;
   void* p = nullptr;
-  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  HIPCHK(pool::alloc(&p, count * sizeof(T)));
   return static_cast<T*>(p);
 }
 
@@ -63,9 +64,7 @@ T* dupload(const std::vector<T, A>& h) {
   return d;
 }
 
-void dfree(void* p) {
-  if (p) (void)hipFree(p);
-}
+void dfree(void* p) { pool::release(p); }
 
 // Runs f and maps exceptions to status codes.
 template <typename F>
@@ -401,7 +400,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   // streams sharing a queue run one after the other); twice as many streams as queues keeps
   // every queue fed whatever the mapping.
   const int kLevelStreams = 2 * hw_queues();
-  constexpr int64_t kOuterNb = 128;  // outer panel width of the tall levels
+  constexpr int64_t kOuterNb = 128;  // outer panel width
   struct Ws {
     S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
       *cq = nullptr, *Ts = nullptr, *GV = nullptr;
@@ -432,16 +431,12 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       L.q = r;
       if (r == 0) continue;
       if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
-      if (M > k::kQrMaxRows)
-        throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy larger than the QR panel kernel supports")};
       Ws& w = ws[l];
       const int64_t sA = M * n;
       const int64_t nsplit = (M + kchunk - 1) / kchunk;
-      // Tall levels: 128-column outer panels made of 32-column CholeskyQR3 sub-panels, so
-      // the trailing matrix (the HBM-bound part) is streamed once per 128 columns.  Other
-      // levels: 32-column one-workgroup Householder panels.
-      const bool tall = M > k::kCqrMinRows;
-      const int64_t nbo = tall ? kOuterNb : nb;
+      // Outer panels of up to 128 columns made of 32-column CholeskyQR3 sub-panels, so the
+      // trailing matrix (the HBM-bound part) is streamed once per outer panel.
+      const int64_t nbo = std::min<int64_t>(kOuterNb, (kk + nb - 1) / nb * nb);
       w.A = dalloc<S>(E * sA);
       w.V = dalloc<S>(E * M * nbo);
       w.W1 = dalloc<S>(E * nbo * n);
@@ -449,11 +444,9 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       w.Tf = dalloc<S>(E * nbo * nbo);
       const int64_t nsplit_thin = (M + kchunk_thin - 1) / kchunk_thin;
       w.part = dalloc<S>(E * std::max(nsplit * nbo * n, nsplit_thin * nbo * nbo));
-      if (tall) {
-        w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
-        w.Ts = dalloc<S>(E * kOuterNb * nb);
-        w.GV = dalloc<S>(E * kOuterNb * kOuterNb);
-      }
+      w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
+      w.Ts = dalloc<S>(E * kOuterNb * nb);
+      w.GV = dalloc<S>(E * nbo * nbo);
       S* A = w.A;
       auto& sl = steps[l];
       // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
@@ -467,37 +460,31 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
           const int64_t nc = n - j0 - jbo;
           const int64_t Mp = M - j0;
           const int64_t sV = Mp * nbo;
-          if (tall) {
-            k::zero2d<S>(st, Mp, jbo, w.V, Mp, sV, E);
-            for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
-              const int64_t j1 = j0 + s0, Mps = M - j1;
-              const int jb = int(std::min<int64_t>(nb, jbo - s0));
-              S* Vs = w.V + s0 + s0 * Mp;
-              S* Tss = w.Ts + (s0 / nb) * nb * nb;
-              k::cqr_panel<S>(st, M, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq, E);
-              // this sub-panel's reflectors on the rest of the outer panel
-              const int64_t nin = jbo - s0 - jb;
-              if (nin <= 0) continue;
-              S* Cin = A + j1 + (j1 + jb) * M;
-              k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin,
-                                   E, w.part, kchunk_thin);
-              k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
-                           w.W2, jb, jb * nin, E);
-              k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
-                         S(1), Cin, M, sA, E);
-            }
-            if (nc <= 0) return;
-            // T of the outer panel from the sub-panel factors and V^H V
-            k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo,
-                                 jbo * jbo, E, w.part, kchunk_thin);
-            k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
-                             nbo * nbo, w.W2, nbo * n, E);
-          } else {
-            k::qr_panel<S>(st, M, j0, jbo, A, M, sA, w.Tf, nb * nb, E);
-            if (nc <= 0) return;
-            k::qr_copyV<S>(st, M, j0, jbo, A, M, sA, w.V, Mp, sV, E);
+          k::zero2d<S>(st, Mp, jbo, w.V, Mp, sV, E);
+          for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
+            const int64_t j1 = j0 + s0, Mps = M - j1;
+            const int jb = int(std::min<int64_t>(nb, jbo - s0));
+            S* Vs = w.V + s0 + s0 * Mp;
+            S* Tss = w.Ts + (s0 / nb) * nb * nb;
+            k::cqr_panel<S>(st, M, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq, E);
+            // this sub-panel's reflectors on the rest of the outer panel
+            const int64_t nin = jbo - s0 - jb;
+            if (nin <= 0) continue;
+            S* Cin = A + j1 + (j1 + jb) * M;
+            k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
+                                 w.part, kchunk_thin);
+            k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
+                         w.W2, jb, jb * nin, E);
+            k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
+                       S(1), Cin, M, sA, E);
           }
-          const int64_t ldt = tall ? jbo : nb;
+          if (nc <= 0) return;
+          // T of the outer panel from the sub-panel factors and V^H V
+          k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
+                               E, w.part, kchunk_thin);
+          k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
+                           nbo * nbo, w.W2, nbo * n, E);
+          const int64_t ldt = jbo;
           S* C = A + j0 + (j0 + jbo) * M;
           // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
           k::gemm_tn_splitk<S>(st, jbo, nc, Mp, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
@@ -926,6 +913,18 @@ const char* hssc_last_error(void) { return hssc::g_last_error.c_str(); }
 
 const char* hssc_version(void) { return "hssc-mi355x 0.1 (gfx950)"; }
 
+int hssc_release_cached_memory(void) {
+  if (int s = check_device()) return s;
+  pool::trim();
+  return HSSC_OK;
+}
+
+int hssc_cached_memory(int64_t* bytes) {
+  if (!bytes) return fail(HSSC_ERR_INVALID_ARG, "null argument");
+  *bytes = int64_t(pool::cached_bytes());
+  return HSSC_OK;
+}
+
 int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_t Ng,
                      int32_t np, double acc, int32_t lay, int32_t n_cut, int32_t TI) {
   if (!p) return fail(HSSC_ERR_INVALID_ARG, "null params");
@@ -1030,6 +1029,7 @@ int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
   if (int s = check_device()) return s;
+  pool::reclaim();  // the device is expected idle here: released blocks become reusable
   int s = hssc_hss2d_build_index_sets(p, tree);
   if (s != HSSC_OK) return s;
   hssc_tree* t = *tree;

@@ -296,10 +296,21 @@ int hssc_hss1d_box(const hssc_hss1d* h, int32_t box, int32_t* level, int32_t* ns
                    int32_t* kskel, int64_t* isk_up, int64_t* isk_dn);
 void hssc_hss1d_destroy(hssc_hss1d* h);
 
+/* ---- device memory cache -----------------------------------------------------------
+ * No counterpart in the reference (MATLAB owns its memory): device blocks released by the
+ * library (tree workspaces, inverses, operators) are kept per device and reused by later
+ * calls of the same sizes, up to HSSC_POOL_CAP_GB (environment, default 96) GiB.  A released
+ * block is reused only after a device-wide synchronisation that follows its release.
+ * hssc_release_cached_memory synchronises the current device and frees every cached block
+ * (for callers that share the GPU with another allocator); hssc_cached_memory reports the
+ * bytes held. */
+int hssc_release_cached_memory(void);
+int hssc_cached_memory(int64_t* bytes);
+
 /* ---- instrumentation -------------------------------------------------------------
  * No counterpart in the reference (which times stages with tic/toc,
  * HSS2D/Test/Test_HSS2D_InverseCompression.m:44-50): when enabled, every launch of a
- * kernel class ("gemm_f64", "gemv_f64", "gj_small_f64", "gen_block_f64", "qr_panel_f64",
+ * kernel class ("gemm_f64", "gemv_f64", "gj_small_f64", "gen_block_f64", "cqr_panel_f64",
  * ... and the _f32 variants) is bracketed by HIP events on its own stream, and its
  * algorithmic flops and bytes are accumulated.  hssc_stats_get synchronises the pending
  * events and returns the totals since the last reset. */

@@ -139,3 +139,32 @@ def test_fp32_factorization_with_refinement(Ng, np_, TI):
     inv = H.HSS2D_invert_fsym(tree)
     y64 = H.HSS2D_applyinv_TI(tree, inv, b)
     assert rel(y, y64) < 1e-9
+
+
+@pytest.mark.gpu
+def test_memory_cache_reuse_is_bit_identical():
+    """Released device blocks are reused by the next build of the same sizes (they hold the
+    previous tree's data, unlike fresh allocations): the rebuilt tree, its inverse and the
+    solution are bit-identical, and releasing the cache empties it."""
+    import gc
+    P = H.HSS_tree_parameters("SL_H_2D", 20.0, 1, 8, 8, 1e-10, 3, 800, 0, proxy_layers=4)
+    rng = np.random.default_rng(3)
+    N = 64 * 64
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+
+    def run():
+        tree = H.HSS2D_build_tree_fsym(P)
+        inv = H.HSS2D_invert_fsym(tree)
+        y = H.HSS2D_applyinv_TI(tree, inv, b)
+        Ts = [tree.T(2 ** lev) for lev in range(1, tree.depth + 1)]
+        del inv, tree
+        gc.collect()
+        return y, Ts
+
+    y1, T1 = run()
+    assert H.cached_memory() > 0
+    y2, T2 = run()
+    assert np.array_equal(y1, y2)
+    assert all(np.array_equal(a, c) for a, c in zip(T1, T2))
+    H.release_cached_memory()
+    assert H.cached_memory() == 0

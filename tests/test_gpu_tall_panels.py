@@ -1,6 +1,6 @@
-"""Least squares with tall proxies (M - j0 > 1024 rows per panel): the interpolation
-operators of these levels are factored by the row-parallel CholeskyQR3 panel with Householder
-reconstruction instead of the one-workgroup Householder panel.  Pinned as every other level:
+"""Least squares with tall proxies (more than 1024 rows per panel), the longest CholeskyQR3
+panels (row-parallel, with Householder reconstruction) of the interpolation operators.  Pinned
+as every other level:
 the least-squares residual of T on the proxy (HSS2D/Test/Test_HSS2D_skeletons.m:52-70) against
 the oracle's LAPACK least squares (interpolation_operator.m:57-66), and the solution against the
 oracle's.  TI trees at 128^2 (one box per level, so the oracle stays quick): Laplace (level 1:
