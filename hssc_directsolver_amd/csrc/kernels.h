@@ -61,9 +61,12 @@ void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T*
 // In-place inverse of a batch of n x n matrices.  aux (optional) is a second stream on
 // which the blocked sweep inverts the next diagonal block while st finishes the current
 // step; all work is ordered back onto st before the call returns.
+// work: invert_ws_elems<T>(n, batch) elements (none when one kernel inverts the matrices).
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux = nullptr);
+                    hipStream_t aux, T* work);
+template <typename T>
+int64_t invert_ws_elems(int64_t n, int64_t batch);
 
 constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
 

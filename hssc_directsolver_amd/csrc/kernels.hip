@@ -557,28 +557,43 @@ void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, i
   }
 }
 
-// Blocked GJ without pivoting across diagonal blocks (pivoting inside each nb x nb block).
-// Step p with block rows/cols B = [p, p+nb):
-//   (1) A_BB <- inv(A_BB)                                   gj_reg on the block
-//   (2) A_XB <- -A_XB * A_BB          (rows X outside B)    gemm_skip
-//   (3) A_XY <- A_XY + A_XB * A_BY    (X, Y outside B)      gemm_skip
-//   (4) A_BY <- A_BB * A_BY           (cols Y outside B)    gemm_skip
-// which is the block sweep A11' = A11^-1, A12' = A11^-1 A12, A21' = -A21 A11^-1,
-// A22' = A22 - A21 A11^-1 A12.
-constexpr int GJ_NB = 64;
+// Largest n one register-resident kernel inverts: 128 for real entries, 64 for complex ones.
+// The blocked sweep uses blocks of that size.  (Complex float would fit 128 in registers, but
+// on the Helmholtz F blocks the sweep with 128 blocks, pivoting inside each block only, left
+// the FP32 inverse visibly less accurate: eight refinement steps instead of four.)
+template <typename T>
+constexpr int gj_reg_max() {
+  return is_complex_v<T> ? 64 : 128;
+}
 
+// Blocked GJ without pivoting across diagonal blocks (pivoting inside each nb x nb block).
+// Step p with block rows/cols B = [p, p+nb), nb = gj_reg_max (128, or 64 for complex double):
+//   (1) A_BB <- inv(A_BB)                                   gj_reg on the block
+//   (2) W_X  <- -A_XB * A_BB          (rows X outside B)    gemm_skip into workspace
+//   (3) A_XY <- A_XY + W_X * A_BY     (X, Y outside B)      gemm_skip
+//       A_XB <- W_X
+//   (4) W_Y  <- A_BB * A_BY           (cols Y outside B)    gemm_skip into workspace
+//       A_BY <- W_Y
+// which is the block sweep A11' = A11^-1, A12' = A11^-1 A12, A21' = -A21 A11^-1,
+// A22' = A22 - A21 A11^-1 A12.  With 128 blocks, steps (2) and (4) go through the workspace
+// because an output tile (64 wide) covers only part of the block its inputs span; 64 blocks
+// work in place.  Rank-128 updates keep the trailing GEMM, which streams all of A once per
+// step, at half the traffic per flop of rank-64 ones.
 template <typename T>
 void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
-                int64_t batch) {
-  // p advances in steps of 64 = the GEMM tile, so no 64-row tile straddles block B; the
-  // tile-skip ranges of gemm_skip enumerate the tiles of [0,p) U [p+nb,n).  Steps (2) and
-  // (4) write in place: each output tile is produced by one workgroup that reads all of
-  // its inputs (K = nb <= 64 fits one tile) before it stores.
+                int64_t batch, T* work) {
+  // p advances in multiples of 64 = the GEMM tile, so no 64-row tile straddles block B; the
+  // tile-skip ranges of gemm_skip enumerate the tiles of [0,p) U [p+nb,n).
   // Look-ahead (when an aux stream is given): step (3) first updates the row block B' of
   // the next step; the inverse of A_B'B' then runs on the aux stream while the rest of (3)
   // and (4) run on st.  Neither touches A_B'B'; step p+1 waits for the inverse.
-  static_assert(GJ_NB == GB_M && GJ_NB == GB_N, "block size must equal the GEMM tile");
+  constexpr int NB = gj_reg_max<T>();
+  static_assert(NB % GB_M == 0 && GB_M == GB_N, "blocks of whole GEMM tiles");
   const std::string diag = lbl<T>("gj_diag");
+  constexpr bool in_place = NB == GB_M;  // one-tile blocks need no workspace
+  const int64_t sW = 2 * n * NB;
+  T* Wx = work;            // n x nb, ld n (rows X)
+  T* Wy = work + n * NB;   // nb x n, ld NB (cols Y)
   hipEvent_t upd = nullptr, inv = nullptr;
   if (aux) {
     if (hipEventCreateWithFlags(&upd, hipEventDisableTiming) != hipSuccess ||
@@ -588,25 +603,29 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
       aux = nullptr;
     }
   }
-  gj_reg<T>(st, int(n < GJ_NB ? n : GJ_NB), A, lda, sA, 0, batch, diag);
-  for (int64_t p = 0; p < n; p += GJ_NB) {
-    const int nb = int(p + GJ_NB <= n ? GJ_NB : n - p);
+  const int64_t big = INT64_MAX;
+  gj_reg<T>(st, int(n < NB ? n : NB), A, lda, sA, 0, batch, diag);
+  for (int64_t p = 0; p < n; p += NB) {
+    const int nb = int(p + NB <= n ? NB : n - p);
     const int64_t nout = n - nb;
     if (p > 0 && aux) (void)hipStreamWaitEvent(st, inv, 0);  // inverse of A_BB (aux stream)
     if (nout <= 0) continue;
-    const int64_t tiles = (nout + 63) / 64;
+    const int64_t tiles = (nout + GB_M - 1) / GB_M, btiles = (nb + GB_M - 1) / GB_M;
     T* Apb = A + p;             // rows B
     T* Abp = A + p * lda;       // cols B
     T* Abb = A + p + p * lda;   // block B
-    // (2) A_XB = -A_XB * A_BB  (m = n rows with skip, n = nb cols)
-    gemm_skip<T>(st, false, false, n, nb, nb, T(-1), A + p * lda, lda, sA, Abb, lda, sA, T(0),
-                 Abp, lda, sA, batch, tiles, 1, p, nb, INT64_MAX, 0);
-    // (3) A_XY += A_XB * A_BY, the next row block B' = [p+nb, p+nb+nb2) first
+    // (2) W_X = -A_XB * A_BB (in place when the block is one tile wide: each output tile is
+    // then produced by one workgroup that reads all of its inputs before it stores)
+    T* X = in_place ? Abp : Wx;
+    const int64_t ldx = in_place ? lda : n, sX = in_place ? sA : sW;
+    gemm_skip<T>(st, false, false, n, nb, nb, T(-1), Abp, lda, sA, Abb, lda, sA, T(0), X, ldx, sX,
+                 batch, tiles, btiles, p, nb, big, 0);
+    // (3) A_XY += W_X * A_BY, the next row block B' = [p+nb, p+nb+nb2) first
     const int64_t q = p + nb;
-    const int nb2 = int(q < n ? (q + GJ_NB <= n ? GJ_NB : n - q) : 0);
+    const int nb2 = int(q < n ? (q + NB <= n ? NB : n - q) : 0);
     if (nb2 > 0) {
-      gemm_skip<T>(st, false, false, q + nb2, n, nb, T(1), Abp, lda, sA, Apb, lda, sA, T(1), A,
-                   lda, sA, batch, 1, tiles, 0, q, p, nb);
+      gemm_skip<T>(st, false, false, q + nb2, n, nb, T(1), X, ldx, sX, Apb, lda, sA, T(1), A, lda,
+                   sA, batch, (nb2 + GB_M - 1) / GB_M, tiles, 0, q, p, nb);
       if (aux) {
         (void)hipEventRecord(upd, st);
         (void)hipStreamWaitEvent(aux, upd, 0);
@@ -616,11 +635,21 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
     }
     const int64_t rest = nout - nb2;  // rows [0,p) U [q+nb2,n)
     if (rest > 0)
-      gemm_skip<T>(st, false, false, n, n, nb, T(1), Abp, lda, sA, Apb, lda, sA, T(1), A, lda, sA,
-                   batch, (rest + 63) / 64, tiles, p, nb + nb2, p, nb);
-    // (4) A_BY = A_BB * A_BY
-    gemm_skip<T>(st, false, false, nb, n, nb, T(1), Abb, lda, sA, Apb, lda, sA, T(0), Apb, lda,
-                 sA, batch, 1, tiles, INT64_MAX, 0, p, nb);
+      gemm_skip<T>(st, false, false, n, n, nb, T(1), X, ldx, sX, Apb, lda, sA, T(1), A, lda, sA,
+                   batch, (rest + GB_M - 1) / GB_M, tiles, p, nb + nb2, p, nb);
+    if (in_place) {
+      // (4) A_BY = A_BB * A_BY
+      gemm_skip<T>(st, false, false, nb, n, nb, T(1), Abb, lda, sA, Apb, lda, sA, T(0), Apb, lda,
+                   sA, batch, btiles, tiles, big, 0, p, nb);
+    } else {
+      copy2d<T, T>(st, p, nb, Wx, n, sW, Abp, lda, sA, batch);
+      copy2d<T, T>(st, n - q, nb, Wx + q, n, sW, Abp + q, lda, sA, batch);
+      // (4) W_Y = A_BB * A_BY
+      gemm_skip<T>(st, false, false, nb, n, nb, T(1), Abb, lda, sA, Apb, lda, sA, T(0), Wy, NB,
+                   sW, batch, btiles, tiles, big, 0, p, nb);
+      copy2d<T, T>(st, nb, p, Wy, NB, sW, Apb, lda, sA, batch);
+      copy2d<T, T>(st, nb, n - q, Wy + q * NB, NB, sW, Apb + q * lda, lda, sA, batch);
+    }
     if (nb2 > 0 && !aux) gj_reg<T>(st, nb2, A, lda, sA, q + q * lda, batch, diag);
   }
   if (upd) (void)hipEventDestroy(upd);
@@ -628,15 +657,19 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
 }
 
 template <typename T>
+int64_t invert_ws_elems(int64_t n, int64_t batch) {
+  return (n > gj_reg_max<T>() && gj_reg_max<T>() != GB_M) ? 2 * n * gj_reg_max<T>() * batch : 0;
+}
+
+template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux) {
+                    hipStream_t aux, T* work) {
   if (n <= 0 || batch <= 0) return;
-  // register-resident GJ: n <= 128 real, n <= 64 complex (twice the registers per entry)
-  if (n <= (is_complex_v<T> ? 64 : 128)) {
+  if (n <= gj_reg_max<T>()) {
     gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
     return;
   }
-  gj_blocked<T>(st, aux, n, A, lda, sA, batch);
+  gj_blocked<T>(st, aux, n, A, lda, sA, batch, work);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1385,7 +1418,8 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
                              int64_t, int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, \
                              int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t);        \
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
-                                  hipStream_t);                                                    \
+                                  hipStream_t, T*);                                                \
+  template int64_t invert_ws_elems<T>(int64_t, int64_t);                                           \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
   template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \

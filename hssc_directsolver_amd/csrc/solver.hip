@@ -576,12 +576,14 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   const bool TI = t->p.transinv != 0;
   inv->FI.assign(D + 1, nullptr);
   inv->E.assign(D + 1, nullptr);
-  int64_t wmax = 0;
-  for (int l = 1; l <= D; ++l) {
+  int64_t wmax = 0, gmax = 0;
+  for (int l = 0; l <= D; ++l) {
     const Level& L = t->levels[l];
-    wmax = std::max(wmax, L.nkey * L.k * (L.n - L.k));
+    if (l > 0) wmax = std::max(wmax, L.nkey * L.k * (L.n - L.k));
+    gmax = std::max({gmax, k::invert_ws_elems<T>(L.n, L.nkey), k::invert_ws_elems<T>(L.k, L.nkey)});
   }
   T* Wk = dalloc<T>(wmax);
+  T* Wg = gmax > 0 ? dalloc<T>(gmax) : nullptr;  // blocked Gauss-Jordan workspace
   // second stream for the look-ahead of the blocked Gauss-Jordan sweeps
   hipStream_t aux = nullptr;
   HIPCHK(hipGetLastError());
@@ -604,7 +606,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
                        static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, ks);
     }
-    k::invert_batched<T>(st, n, FI, n, n * n, E, aux);
+    k::invert_batched<T>(st, n, FI, n, n * n, E, aux, Wg);
     if (l > 0) {
       // EI = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T ;  E = inv(EI)
       T* EI = dalloc<T>(E * kk * kk);
@@ -628,7 +630,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
         k::gemm<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI, kk,
                    kk * kk, E);
       }
-      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux);
+      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg);
     }
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -639,6 +641,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   }
   HIPCHK(hipGetLastError());
   dfree(Wk);
+  dfree(Wg);
   tree_sync(t);  // every level's T has been waited for: release the build's resources
 }
 
