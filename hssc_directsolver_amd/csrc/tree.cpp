@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -32,22 +34,56 @@ void source_points(int Ng, int np, double* X) {
   }
 }
 
+namespace {
+// one factor of the Helmholtz weight: the tanh plateau of Bump_function option 4
+double bump_factor(double h, double x) {
+  const double c = 4 * h, D = 10.0;
+  return 0.5 * std::tanh(D * (-std::fabs(x) + 1 - c)) + 0.5;
+}
+}  // namespace
+
 double point_weight(const hssc_params& p, double x, double y) {
   // Common/Kernel_Eval.m:111-159 for params.proxy == 0 and a symmetric kernel:
   // TI -> 1; NTI Laplace -> b_func(.,0.3,0.6,0.25) (:135,:146); NTI Helmholtz ->
   // sqrt(Bump_function(.,4,h)) (:133,:143).
   if (p.transinv) return 1.0;
-  if (p.kh > 0) {
-    const double c = 4 * p.h, D = 10.0;
-    const double f1 = 0.5 * std::tanh(D * (-std::fabs(x) + 1 - c)) + 0.5;
-    const double f2 = 0.5 * std::tanh(D * (-std::fabs(y) + 1 - c)) + 0.5;
-    return std::sqrt(f1 * f2);
-  }
+  if (p.kh > 0) return std::sqrt(bump_factor(p.h, x) * bump_factor(p.h, y));
   const double d1 = x - 0.3, d2 = y - 0.6;
   return 1.0 + 0.25 * std::exp(-d1 * d1 - d2 * d2);
 }
 
 namespace {
+
+// point_weight of one tree build.  The Helmholtz weight sqrt(f(x) f(y)) is a product of two
+// one-variable factors, and with dyadic coordinates every grid and proxy point lies on the
+// half-h lattice: the factors are tabulated there once (the same expression, so the values
+// are bit-identical) and looked up; other points and kernels evaluate point_weight.
+class WeightTable {
+ public:
+  explicit WeightTable(const hssc_params& p) : p_(p) {
+    if (p.transinv || !(p.kh > 0) || !(p.h > 0)) return;
+    step_ = 0.5 * p.h;
+    K_ = int64_t(std::ceil(4.0 / step_));
+    if (K_ > (int64_t(1) << 22)) return;
+    f_.resize(2 * K_ + 1);
+    for (int64_t k = -K_; k <= K_; ++k) f_[k + K_] = bump_factor(p.h, double(k) * step_);
+  }
+  double operator()(double x, double y) const {
+    if (f_.empty()) return point_weight(p_, x, y);
+    const double kx = x / step_, ky = y / step_;
+    if (kx != std::floor(kx) || ky != std::floor(ky) || std::fabs(kx) > double(K_) ||
+        std::fabs(ky) > double(K_) || double(int64_t(kx)) * step_ != x ||
+        double(int64_t(ky)) * step_ != y)
+      return point_weight(p_, x, y);
+    return std::sqrt(f_[int64_t(kx) + K_] * f_[int64_t(ky) + K_]);
+  }
+
+ private:
+  const hssc_params& p_;
+  double step_ = 0;
+  int64_t K_ = 0;
+  std::vector<double> f_;
+};
 
 void level_radii(int lev, double* colrad, double* rowrad) {
   // HSS2D/HSS2D_build_tree_fsym.m:322-326 (colrad = x half-width, rowrad = y).
@@ -228,6 +264,71 @@ int proxy_layers(int lev, const hssc_params& p) {
   return p.layers;
 }
 
+// Worker threads kept for one tree build: the per-level loops below are short (a few ms), so
+// starting threads for each of them would cost about as much as the work.
+class WorkPool {
+ public:
+  explicit WorkPool(int n) : n_(std::max(1, n)) {
+    for (int w = 1; w < n_; ++w) th_.emplace_back([this, w]() { loop(w); });
+  }
+  ~WorkPool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  int size() const { return n_; }
+  // f(w) for w in [0, nthr), nthr <= size(), on the workers and the caller; returns when all
+  // are done
+  void run(int nthr, const std::function<void(int)>& f) {
+    nthr = std::max(1, std::min(nthr, n_));
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      f_ = &f;
+      nthr_ = nthr;
+      pending_ = nthr - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    f(0);
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [&] { return pending_ == 0; });
+    f_ = nullptr;
+  }
+
+ private:
+  void loop(int w) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        if (w >= nthr_) continue;
+        f = f_;
+      }
+      (*f)(w);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        --pending_;
+      }
+      done_.notify_one();
+    }
+  }
+  int n_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* f_ = nullptr;
+  int nthr_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
 }  // namespace
 
 int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
@@ -250,8 +351,17 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
   // rationals with short mantissas: every coordinate and every difference is exact.
   const bool dyadic = (p.Ng & (p.Ng - 1)) == 0 && (p.np & (p.np - 1)) == 0;
   // per-point kernel weights b(x) = c(x), evaluated once per grid point
+  WorkPool pool(int(std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u)));
+  // contiguous chunks of [0, count) over the pool
+  auto parallel_range = [&](int64_t count, const std::function<void(int64_t, int64_t)>& f) {
+    const int nw = int(std::max<int64_t>(1, std::min<int64_t>(pool.size(), count / 256)));
+    pool.run(nw, [&](int w) { f(count * w / nw, count * (w + 1) / nw); });
+  };
+  const WeightTable weight(p);
   std::vector<double> wgt(N);
-  for (int64_t g = 0; g < N; ++g) wgt[g] = point_weight(p, Xx[g], Xy[g]);
+  parallel_range(N, [&](int64_t g0, int64_t g1) {
+    for (int64_t g = g0; g < g1; ++g) wgt[g] = weight(Xx[g], Xy[g]);
+  });
 
   // Heap-numbered centres (bisect_level, :1289-1344).
   const int64_t nboxes = (int64_t(1) << (depth + 1)) - 1;
@@ -297,17 +407,19 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
     const int lay_prox = proxy_layers(lev, p);
     const int64_t b0 = int64_t(1) << lev;
     // I_src of every box of the level (children's skeletons, :358-363)
-    for (int64_t bi = 0; bi < L.nbox; ++bi) {
-      const int64_t b = b0 + bi;
-      if (lev < depth) {
-        auto& I = t->Isrc[b];
-        I = t->Isk[2 * b];
-        I.insert(I.end(), t->Isk[2 * b + 1].begin(), t->Isk[2 * b + 1].end());
-      }
-      const int64_t n = int64_t(t->Isrc[b].size());
-      if (bi == 0) L.n = n;
-      if (n != L.n) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform box sizes on a level");
-    }
+    if (lev < depth)
+      parallel_range(L.nbox, [&](int64_t i0, int64_t i1) {
+        for (int64_t bi = i0; bi < i1; ++bi) {
+          const int64_t b = b0 + bi;
+          auto& I = t->Isrc[b];
+          I = t->Isk[2 * b];
+          I.insert(I.end(), t->Isk[2 * b + 1].begin(), t->Isk[2 * b + 1].end());
+        }
+      });
+    L.n = int64_t(t->Isrc[b0].size());
+    for (int64_t bi = 0; bi < L.nbox; ++bi)
+      if (int64_t(t->Isrc[b0 + bi].size()) != L.n)
+        return fail(HSSC_ERR_UNSUPPORTED, "non-uniform box sizes on a level");
     // For kh > 0 the two bottom levels take their skeletons from an ID of K(X_proxy, X_src)
     // (:374-395 selects, LOCAL_Non_Preset_Skeleton_Info :686-729 computes); the proxy is
     // the level's 'bdry' ring (:408-420).  Every level of a batch has one rank: boxes keep
@@ -318,17 +430,13 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
     L.kbox.assign(L.nkey, -1);
     std::vector<int32_t> id_order;
     int64_t pad_to = 0;
-    const int64_t nthr_all = std::max<int64_t>(
-        1, std::min<int64_t>({L.nkey, int64_t(std::thread::hardware_concurrency()), 16}));
+    const int64_t nthr_all = std::max<int64_t>(1, std::min<int64_t>(L.nkey, pool.size()));
     auto parallel_keys = [&](const std::function<bool(int64_t)>& f) {
-      std::vector<std::thread> pool;
       std::vector<char> ok(nthr_all, 1);
-      for (int64_t w = 0; w < nthr_all; ++w)
-        pool.emplace_back([&, w]() {
-          for (int64_t bi = w; bi < L.nkey; bi += nthr_all)
-            if (!f(bi)) ok[w] = 0;
-        });
-      for (auto& th : pool) th.join();
+      pool.run(int(nthr_all), [&](int w) {
+        for (int64_t bi = w; bi < L.nkey; bi += nthr_all)
+          if (!f(bi)) ok[w] = 0;
+      });
       for (char c : ok)
         if (!c) return false;
       return true;
@@ -350,7 +458,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
         box_proxy(t->cent[2 * b], t->cent[2 * b + 1], colrad, rowrad, lay_prox, p.h, &qx, &qy);
         if (int64_t(qx.size()) != M) return false;
         for (int64_t r = 0; r < M; ++r)
-          Qx[bi * M + r] = qx[r], Qy[bi * M + r] = qy[r], Qw[bi * M + r] = point_weight(p, qx[r], qy[r]);
+          Qx[bi * M + r] = qx[r], Qy[bi * M + r] = qy[r], Qw[bi * M + r] = weight(qx[r], qy[r]);
         return true;
       });
       if (!same) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform proxy sizes");
@@ -417,7 +525,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       for (int64_t r = 0; r < L.M; ++r) {
         L.Xpx[key * L.M + r] = qx[r];
         L.Xpy[key * L.M + r] = qy[r];
-        L.Xpw[key * L.M + r] = point_weight(p, qx[r], qy[r]);
+        L.Xpw[key * L.M + r] = weight(qx[r], qy[r]);
       }
       return true;
     };
@@ -447,39 +555,33 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
           const double qx = L.Xpx[r] + dx, qy = L.Xpy[r] + dy;
           L.Xpx[key * M + r] = qx;
           L.Xpy[key * M + r] = qy;
-          L.Xpw[key * M + r] = point_weight(p, qx, qy);
+          L.Xpw[key * M + r] = weight(qx, qy);
         }
       };
-      const int64_t nthr = std::max<int64_t>(
-          1, std::min<int64_t>({(L.nbox - 1 + 63) / 64, int64_t(std::thread::hardware_concurrency()), 16}));
-      std::vector<std::thread> pool;
-      for (int64_t w = 0; w < nthr; ++w)
-        pool.emplace_back([&, w]() {
-          for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr) copy_box(bi);
-        });
-      for (auto& th : pool) th.join();
+      const int64_t nthr = std::max<int64_t>(1, std::min<int64_t>(L.nbox - 1, pool.size()));
+      pool.run(int(nthr), [&](int w) {
+        for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr) copy_box(bi);
+      });
     } else if (!TI && L.nbox > 1) {
       // boxes are independent and write disjoint key slots: split them over threads
-      const int64_t nthr = std::max<int64_t>(
-          1, std::min<int64_t>({L.nbox - 1, int64_t(std::thread::hardware_concurrency()), 16}));
-      std::vector<std::thread> pool;
+      const int64_t nthr = std::max<int64_t>(1, std::min<int64_t>(L.nbox - 1, pool.size()));
       std::vector<char> ok(nthr, 1);
-      for (int64_t w = 0; w < nthr; ++w)
-        pool.emplace_back([&, w]() {
-          for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr)
-            if (!process(bi)) ok[w] = 0;
-        });
-      for (auto& th : pool) th.join();
+      pool.run(int(nthr), [&](int w) {
+        for (int64_t bi = 1 + w; bi < L.nbox; bi += nthr)
+          if (!process(bi)) ok[w] = 0;
+      });
       for (char c : ok)
         if (!c) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
     }
-    for (int64_t bi = 0; bi < L.nbox; ++bi) {
-      const int64_t b = b0 + bi;
-      const int64_t key = TI ? 0 : bi;
-      auto& S = t->Isk[b];
-      S.resize(L.k);
-      for (int64_t a = 0; a < L.k; ++a) S[a] = t->Isrc[b][L.J[key * L.n + a]];
-    }
+    parallel_range(L.nbox, [&](int64_t i0, int64_t i1) {
+      for (int64_t bi = i0; bi < i1; ++bi) {
+        const int64_t b = b0 + bi;
+        const int64_t key = TI ? 0 : bi;
+        auto& S = t->Isk[b];
+        S.resize(L.k);
+        for (int64_t a = 0; a < L.k; ++a) S[a] = t->Isrc[b][L.J[key * L.n + a]];
+      }
+    });
     for (auto& kb : L.kbox)
       if (kb < 0) kb = int32_t(L.k);
     if (id_level && !TI) ragged_below = true;
