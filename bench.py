@@ -12,13 +12,13 @@ unit-norm complex Gaussian right-hand side.
 `--kernel laplace` runs the SL_L_2D grid of HSS_tree_parameters('SL_L_2D', 0, 1, 8, 64,
 1e-10, 2, 400, 0) instead; it is also reported as a side case.
 
-One step = HSS2D_build_tree_fsym (host index sets + GPU interpolation operators)
-         + HSS2D_compress_fsym (forward operator used by the refinement residual)
-         + HSS2D_invert_fsym + solve to 1e-10 relative residual against the FP64
-           compressed operator (Common/Iterative_Refinement.m).
-Two paths are timed in every step:
+One step runs two paths, each end to end from the parameters:
+  HSS2D_build_tree_fsym (host index sets + GPU interpolation operators)
+  + HSS2D_compress_fsym (forward operator used by the refinement residual)
+  + HSS2D_invert_fsym + solve to 1e-10 relative residual against the FP64 compressed
+    operator (Common/Iterative_Refinement.m),
   mixed: FP32 (complex64) factorization (F^-1, E on MFMA), FP64 refinement -> `value`
-  fp64 : FP64 (complex128) factorization, FP64 refinement (plain path)
+  fp64 : FP64 (complex128) factorization, FP64 refinement (plain path).
 
 Prints one JSON line.  `value` is the mixed path's build + solve seconds for the whole
 job (time-like, lower is better).
@@ -36,6 +36,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "c128": 78.6, "c64": 157.3}  # MI355X dense MFMA
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of a kernel class, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# of `bench.py --no-cpu --no-hss1d --no-laplace --no-batched --profile-step` (one step of the
+# default workload), averaged over the class's launches and scaled by what the counters read
+# for a plain 1 GiB device copy (`bench.py --calibrate-copy`: FETCH_SIZE 512 MiB -> x2,
+# WRITE_SIZE 1 GiB -> x1).  profiles/hss2d_512_helmholtz_traffic.md holds the runs.
+MEASURED_TRAFFIC = {
+    "gemm_c128_nn": 2 * 158916.2 * 1024 + 144034.3 * 1024,
+}
 
 
 def parse():
@@ -57,6 +65,11 @@ def parse():
     ap.add_argument("--no-laplace", action="store_true", help="skip the Laplace side case")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="run one step of both paths only (for counter collection)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 1 GiB device copy three times (scales the FETCH_SIZE / "
+                         "WRITE_SIZE counters to bytes)")
     ap.add_argument("--cpu-stop", type=int, default=3,
                     help="CPU baseline: levels measured down to this one, above extrapolated")
     ap.add_argument("--no-hss1d", action="store_true", help="skip the HSS1D side case")
@@ -302,8 +315,9 @@ def roofline(stats):
         return None
     name, s = max(stats.items(), key=lambda kv: kv[1]["ms"])
     prec = next(t for t in ("f64", "f32", "c128", "c64") if f"_{t}" in name)
-    common = dict(kernel=name, traffic=None, launches=s["launches"],
-                  avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2))
+    common = dict(kernel=name, traffic=MEASURED_TRAFFIC.get(name), launches=s["launches"],
+                  avg_launch_us=round(1e3 * s["ms"] / s["launches"], 2),
+                  algorithmic_bytes_per_launch=s["bytes"] / s["launches"])
     if name.startswith(("gemm", "gj_", "cqr_panel")):
         ach = s["flops"] / (s["ms"] * 1e-3) / 1e12
         return dict(bound="mfma", achieved=round(ach, 3), peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
@@ -327,8 +341,23 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = 1 << 28  # float32 elements: 1 GiB
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 4 * n, "copies": 3}), flush=True)
+        return
     P, b, workload = make_problem(H, torch, args.kernel, args)
     N = P.N
+    if args.profile_step:
+        r = one_step(H, P, torch, b, args.tol)
+        print(json.dumps({"profile_step": {k: round(v["build"] + v["solve"], 4)
+                                           for k, v in r.items()}}), flush=True)
+        return
 
     for _ in range(args.warmup):
         one_step(H, P, torch, b, args.tol)
