@@ -758,7 +758,7 @@ static_assert(CQR_ROWS == 256 && CQR_G % CQR_ROWS == 0, "entry split");
 constexpr int64_t CQR_RACC = 0, CQR_QTOP = CQR_G, CQR_PART0 = 2 * CQR_G;
 
 template <typename T>
-__global__ void __launch_bounds__(CQR_ROWS) cqr_pass_kernel(
+__global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
     int pass, int64_t Mp, int jb, int P, int64_t part1, const T* X, int64_t ldx, int64_t sX,
     T* Qo, int64_t ldq, int64_t sQ, T* __restrict__ ws, int64_t sW, T* V, int64_t ldv,
     int64_t sV, T* __restrict__ A, int64_t lda, int64_t sA, int64_t j0, T* __restrict__ Tfo,
@@ -1367,7 +1367,10 @@ void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T*
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                     int64_t batch, T* part, int64_t kchunk) {
   const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
-  if (nsplit <= 1 || ldc != m) {
+  // split only while the unsplit grid leaves CUs idle: with many matrices (the small boxes of
+  // the low levels) the partial products and their sum only add workgroups and a launch
+  const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N) * batch;
+  if (nsplit <= 1 || ldc != m || tiles >= 512) {
     gemm_h<T>(st, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
     return;
   }
