@@ -401,6 +401,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   // every queue fed whatever the mapping.
   const int kLevelStreams = 2 * hw_queues();
   constexpr int64_t kOuterNb = 128;  // outer panel width
+  constexpr int64_t kMergeCols = 128;  // trailing blocks up to this width take rank-32 updates
   struct Ws {
     S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
       *cq = nullptr, *Ts = nullptr, *GV = nullptr;
@@ -460,6 +461,9 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
           const int64_t nc = n - j0 - jbo;
           const int64_t Mp = M - j0;
           const int64_t sV = Mp * nbo;
+          // a narrow trailing block (the last panels of the small levels) takes each
+          // sub-panel's reflectors directly, without the outer panel's block reflector
+          const bool merge = nc <= kMergeCols;
           k::zero2d<S>(st, Mp, jbo, w.V, Mp, sV, E);
           for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
             const int64_t j1 = j0 + s0, Mps = M - j1;
@@ -467,8 +471,9 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
             S* Vs = w.V + s0 + s0 * Mp;
             S* Tss = w.Ts + (s0 / nb) * nb * nb;
             k::cqr_panel<S>(st, M, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq, E);
-            // this sub-panel's reflectors on the rest of the outer panel
-            const int64_t nin = jbo - s0 - jb;
+            // this sub-panel's reflectors on the rest of the outer panel (and the trailing
+            // block when merged)
+            const int64_t nin = jbo - s0 - jb + (merge ? std::max<int64_t>(nc, 0) : 0);
             if (nin <= 0) continue;
             S* Cin = A + j1 + (j1 + jb) * M;
             k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
@@ -478,7 +483,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
             k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
                        S(1), Cin, M, sA, E);
           }
-          if (nc <= 0) return;
+          if (nc <= 0 || merge) return;
           // T of the outer panel from the sub-panel factors and V^H V
           k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
                                E, w.part, kchunk_thin);
