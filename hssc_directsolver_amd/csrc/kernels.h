@@ -49,7 +49,7 @@ template <typename T>
 void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
                const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta,
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
-               int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn);
+               int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn, int tri = 0);
 
 // C = A^H B with the long K dimension split in chunks of kchunk rows (deterministic sum);
 // part needs batch * ceil(kdim/kchunk) * m * n elements.
@@ -62,9 +62,11 @@ void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T*
 // which the blocked sweep inverts the next diagonal block while st finishes the current
 // step; all work is ordered back onto st before the call returns.
 // work: invert_ws_elems<T>(n, batch) elements (none when one kernel inverts the matrices).
+// sym: the matrices are symmetric (A^T = A, complex symmetric for complex T); complex ones
+// are then swept on their upper tiles only.
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux, T* work);
+                    hipStream_t aux, T* work, bool sym);
 template <typename T>
 int64_t invert_ws_elems(int64_t n, int64_t batch);
 

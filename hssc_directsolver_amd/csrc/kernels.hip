@@ -185,7 +185,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
                                                   const T* B, int64_t ldb, int64_t sB,
                                                   T beta, T* C, int64_t ldc,
                                                   int64_t sC, int64_t rs0, int64_t rsn, int64_t cs0,
-                                                  int64_t csn, int64_t ksplit, int64_t kchunk) {
+                                                  int64_t csn, int64_t ksplit, int64_t kchunk,
+                                                  int tri) {
   using R = real_t<T>;
   constexpr bool CX = is_complex_v<T>;
   constexpr int NP = CX ? 2 : 1;  // LDS planes per operand
@@ -211,6 +212,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
   if (i0 >= rs0) i0 += rsn;
   if (j0 >= cs0) j0 += csn;
   if (i0 >= m || j0 >= n) return;
+  if (tri && i0 > j0) return;  // upper tiles only (symmetric updates)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
   // complex float: three real products per tile (Gauss): P0 = re re, P1 = im im,
@@ -352,15 +354,15 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
                      const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
                      T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
                      int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
-                     int64_t ksplit, int64_t kchunk, bool ca = false);
+                     int64_t ksplit, int64_t kchunk, bool ca = false, int tri = 0);
 
 template <typename T>
 void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t kdim, T alpha,
                const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta,
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
-               int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn) {
+               int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn, int tri) {
   gemm_skip_split<T>(st, ta, tb, m, n, kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
-                     rtiles, ctiles, rs0, rsn, cs0, csn, 1, kdim);
+                     rtiles, ctiles, rs0, rsn, cs0, csn, 1, kdim, false, tri);
 }
 
 template <typename T>
@@ -368,14 +370,17 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
                      const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
                      T beta, T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles,
                      int64_t ctiles, int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn,
-                     int64_t ksplit, int64_t kchunk, bool ca) {
+                     int64_t ksplit, int64_t kchunk, bool ca, int tri) {
   if (m <= 0 || n <= 0 || batch <= 0 || rtiles <= 0 || ctiles <= 0) return;
   dim3 grid(rtiles, ctiles, batch * ksplit);
-  // algorithmic work: rows / cols actually produced (the skipped block excluded); a complex
-  // multiply-add is 8 real flops
+  // algorithmic work: rows / cols actually produced (the skipped block excluded; about half
+  // the tiles with tri); a complex multiply-add is 8 real flops
   const double rows = double(rs0 < m ? m - rsn : m), cols = double(cs0 < n ? n - csn : n);
-  const double fl = (is_complex_v<T> ? 8.0 : 2.0) * rows * cols * double(kdim) * double(batch);
-  const double by = (rows * kdim + kdim * cols + 2 * rows * cols) * sizeof(T) * double(batch);
+  const double part = tri ? 0.5 : 1.0;
+  const double fl =
+      part * (is_complex_v<T> ? 8.0 : 2.0) * rows * cols * double(kdim) * double(batch);
+  const double by =
+      (rows * kdim + kdim * cols + part * 2 * rows * cols) * sizeof(T) * double(batch);
   const bool conjA = is_complex_v<T> && ca;
   // label per kernel variant: gemm_<type>_<op(A)><op(B)>, op = n, t, or h (conjugate transpose)
   const std::string name = lbl<T>("gemm") + "_" + (ta ? (conjA ? "h" : "t") : (conjA ? "c" : "n")) +
@@ -384,7 +389,7 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
   HSSC_TIMED(name.c_str(), st, fl, by,                                                          \
              hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_>), grid, dim3(256), 0, st, m, n,  \
                                 kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, \
-                                cs0, csn, ksplit, kchunk))
+                                cs0, csn, ksplit, kchunk, tri))
   if (conjA) {
     if (ta && !tb) HSSC_GEMM_LAUNCH(true, true, false);
     else if (ta && tb) HSSC_GEMM_LAUNCH(true, true, true);
@@ -656,20 +661,171 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
   if (inv) (void)hipEventDestroy(inv);
 }
 
+// A <- (A + A^T) / 2 in place (32 x 32 tile pairs through LDS).  A matrix that is symmetric
+// only up to rounding (E^-1 = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T forms its two
+// off-diagonal halves by different products) is replaced by its symmetric part before the
+// upper-tile sweep: that drops the antisymmetric rounding to first order, as the reference's
+// symmetrised inverse (Ai + Ai.')/2 does (HSS2D_build_E_fsym.m:255-259).
+template <typename T>
+__global__ void __launch_bounds__(256) sym_average_kernel(int64_t n, T* __restrict__ A,
+                                                          int64_t lda, int64_t sA) {
+  __shared__ T X[32][33];
+  __shared__ T Y[32][33];
+  const int64_t ti = blockIdx.x, tj = blockIdx.y;
+  if (ti > tj) return;
+  A += int64_t(blockIdx.z) * sA;
+  const int64_t i0 = ti * 32, j0 = tj * 32;
+  for (int idx = threadIdx.x; idx < 32 * 32; idx += 256) {
+    const int r = idx % 32, c = idx / 32;
+    if (i0 + r < n && j0 + c < n) X[r][c] = A[(i0 + r) + (j0 + c) * lda];
+    if (j0 + r < n && i0 + c < n) Y[r][c] = A[(j0 + r) + (i0 + c) * lda];
+  }
+  __syncthreads();
+  const T half = T(real_t<T>(0.5));
+  for (int idx = threadIdx.x; idx < 32 * 32; idx += 256) {
+    const int r = idx % 32, c = idx / 32;  // A(i0 + r, j0 + c) and A(j0 + c, i0 + r)
+    if (i0 + r < n && j0 + c < n) {
+      const T v = (X[r][c] + Y[c][r]) * half;
+      A[(i0 + r) + (j0 + c) * lda] = v;
+      A[(j0 + c) + (i0 + r) * lda] = v;
+    }
+  }
+}
+
+// A(i, j) = A(j, i) below the diagonal tiles (64 x 64 tiles, row tile > column tile), through
+// an LDS tile: the lower triangle of a symmetric matrix kept in its upper tiles.
+template <typename T>
+__global__ void __launch_bounds__(256) sym_lower_fill_kernel(int64_t n, T* __restrict__ A,
+                                                             int64_t lda, int64_t sA) {
+  __shared__ T tile[64][65];
+  const int64_t ti = blockIdx.x, tj = blockIdx.y;  // destination tile
+  if (ti <= tj) return;
+  A += int64_t(blockIdx.z) * sA;
+  const int64_t i0 = ti * 64, j0 = tj * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int r = idx % 64, c = idx / 64;  // source A(j0 + r, i0 + c)
+    if (j0 + r < n && i0 + c < n) tile[r][c] = A[(j0 + r) + (i0 + c) * lda];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int r = idx % 64, c = idx / 64;  // destination A(i0 + r, j0 + c)
+    if (i0 + r < n && j0 + c < n) A[(i0 + r) + (j0 + c) * lda] = tile[c][r];
+  }
+}
+
+// Blocked GJ of a symmetric matrix (complex symmetric: A^T = A, no conjugate) on its upper
+// tiles.  With P the blocks already swept and U the others, the swept matrix keeps
+// M_PP = M_PP^T, M_UU = M_UU^T and M_UP = -M_PU^T (A11^-1 and the Schur complement are
+// symmetric; A21' = -A21 A11^-1 = -(A11^-1 A12)^T).  So every operand of step B can be read
+// from the upper tiles, and only they are updated: for X, Y outside B, X <= Y,
+//   X, Y < B:  A_XY += W_X (-A_YB^T)        W_X = -A_XB A_BB^-1   (new A_XB)
+//   X < B < Y: A_XY += W_X A_BY
+//   B < X, Y:  A_XY += (-A_BX^T) W_Y        W_Y = A_BB^-1 A_BY     (new A_BY)
+// with the old A_YB, A_BX, A_BY (W_X and W_Y are written back afterwards).  Half the trailing
+// tiles of the general sweep, which streams the whole matrix once per block; the lower
+// triangle is filled at the end.  Diagonal tiles are updated whole, so the 64 x 64 diagonal
+// blocks inverted by gj_reg are complete.
+template <typename T>
+void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
+                    int64_t batch, T* work) {
+  constexpr int NB = GB_M;
+  const std::string diag = lbl<T>("gj_diag");
+  const int64_t sW = 2 * n * NB;
+  T* Wx = work;           // rows X < B: p x nb, ld n
+  T* Wy = work + n * NB;  // cols Y > B: nb x (n - q), ld NB
+  hipEvent_t upd = nullptr, inv = nullptr;
+  if (aux) {
+    if (hipEventCreateWithFlags(&upd, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&inv, hipEventDisableTiming) != hipSuccess) {
+      if (upd) (void)hipEventDestroy(upd);
+      upd = inv = nullptr;
+      aux = nullptr;
+    }
+  }
+  const int64_t big = INT64_MAX;
+  auto tl = [](int64_t x) { return (x + GB_M - 1) / GB_M; };
+  // the pivoted inverse of a diagonal block is symmetric only to rounding; the sweep reads it
+  // on both sides (A_BB^-1 in the new A_XB and, through the upper tiles, A_BB^-T), so it is
+  // replaced by its symmetric part
+  auto diag_inverse = [&](hipStream_t s, int nbk, int64_t off) {
+    gj_reg<T>(s, nbk, A, lda, sA, off, batch, diag);
+    hipLaunchKernelGGL(sym_average_kernel<T>, dim3((nbk + 31) / 32, (nbk + 31) / 32, batch),
+                       dim3(256), 0, s, int64_t(nbk), A + off, lda, sA);
+  };
+  diag_inverse(st, int(n < NB ? n : NB), 0);
+  for (int64_t p = 0; p < n; p += NB) {
+    const int nb = int(p + NB <= n ? NB : n - p);
+    const int64_t q = p + nb, nr = n - q;
+    if (p > 0 && aux) (void)hipStreamWaitEvent(st, inv, 0);  // inverse of A_BB (aux stream)
+    if (n - nb <= 0) continue;
+    T* Abb = A + p + p * lda;
+    T* Abq = A + p + q * lda;  // A(B, q:n)
+    if (p > 0)
+      gemm_skip<T>(st, false, false, p, nb, nb, T(-1), A + p * lda, lda, sA, Abb, lda, sA, T(0),
+                   Wx, n, sW, batch, tl(p), tl(nb), big, 0, big, 0);
+    if (nr > 0)
+      gemm_skip<T>(st, false, false, nb, nr, nb, T(1), Abb, lda, sA, Abq, lda, sA, T(0), Wy, NB,
+                   sW, batch, tl(nb), tl(nr), big, 0, big, 0);
+    // B < X <= Y: the next block row B' first, so that its inverse can start on aux
+    const int nb2 = int(q < n ? (q + NB <= n ? NB : n - q) : 0);
+    if (nb2 > 0) {
+      gemm_skip<T>(st, true, false, nb2, nr, nb, T(-1), Abq, lda, sA, Wy, NB, sW, T(1),
+                   A + q + q * lda, lda, sA, batch, tl(nb2), tl(nr), big, 0, big, 0, 1);
+      if (aux) {
+        (void)hipEventRecord(upd, st);
+        (void)hipStreamWaitEvent(aux, upd, 0);
+        diag_inverse(aux, nb2, q + q * lda);
+        (void)hipEventRecord(inv, aux);
+      }
+    }
+    const int64_t q2 = q + nb2, nr2 = n - q2;
+    if (nr2 > 0)
+      gemm_skip<T>(st, true, false, nr2, nr2, nb, T(-1), A + p + q2 * lda, lda, sA,
+                   Wy + nb2 * NB, NB, sW, T(1), A + q2 + q2 * lda, lda, sA, batch, tl(nr2),
+                   tl(nr2), big, 0, big, 0, 1);
+    if (p > 0) {
+      // X <= Y < B
+      gemm_skip<T>(st, false, true, p, p, nb, T(-1), Wx, n, sW, A + p * lda, lda, sA, T(1), A,
+                   lda, sA, batch, tl(p), tl(p), big, 0, big, 0, 1);
+      // X < B < Y
+      if (nr > 0)
+        gemm_skip<T>(st, false, false, p, nr, nb, T(1), Wx, n, sW, Abq, lda, sA, T(1),
+                     A + q * lda, lda, sA, batch, tl(p), tl(nr), big, 0, big, 0);
+      copy2d<T, T>(st, p, nb, Wx, n, sW, A + p * lda, lda, sA, batch);
+    }
+    if (nr > 0) copy2d<T, T>(st, nb, nr, Wy, NB, sW, Abq, lda, sA, batch);
+    if (nb2 > 0 && !aux) diag_inverse(st, nb2, q + q * lda);
+  }
+  const int64_t nt = tl(n);
+  hipLaunchKernelGGL(sym_lower_fill_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
+                     sA);
+  if (upd) (void)hipEventDestroy(upd);
+  if (inv) (void)hipEventDestroy(inv);
+}
+
+// The symmetric sweep serves complex matrices (64-wide blocks, the GEMM tile); real ones use
+// the general sweep with 128-wide blocks.
 template <typename T>
 int64_t invert_ws_elems(int64_t n, int64_t batch) {
-  return (n > gj_reg_max<T>() && gj_reg_max<T>() != GB_M) ? 2 * n * gj_reg_max<T>() * batch : 0;
+  return n > gj_reg_max<T>() ? 2 * n * gj_reg_max<T>() * batch : 0;
 }
 
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux, T* work) {
+                    hipStream_t aux, T* work, bool sym) {
   if (n <= 0 || batch <= 0) return;
   if (n <= gj_reg_max<T>()) {
     gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
     return;
   }
-  gj_blocked<T>(st, aux, n, A, lda, sA, batch, work);
+  if (sym && is_complex_v<T>) {
+    const int64_t nt = (n + 31) / 32;
+    hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
+                       sA);
+    gj_blocked_sym<T>(st, aux, n, A, lda, sA, batch, work);
+  } else {
+    gj_blocked<T>(st, aux, n, A, lda, sA, batch, work);
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1419,9 +1575,9 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
                           const T*, int64_t, int64_t, T, T*, int64_t, int64_t, int64_t);           \
   template void gemm_skip<T>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, T, const T*,    \
                              int64_t, int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, \
-                             int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t);        \
+                             int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int);   \
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
-                                  hipStream_t, T*);                                                \
+                                  hipStream_t, T*, bool);                                          \
   template int64_t invert_ws_elems<T>(int64_t, int64_t);                                           \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \

@@ -611,7 +611,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
                        static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, ks);
     }
-    k::invert_batched<T>(st, n, FI, n, n * n, E, aux, Wg);
+    k::invert_batched<T>(st, n, FI, n, n * n, E, aux, Wg, true);  // F is symmetric
     if (l > 0) {
       // EI = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T ;  E = inv(EI)
       T* EI = dalloc<T>(E * kk * kk);
@@ -635,7 +635,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
         k::gemm<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI, kk,
                    kk * kk, E);
       }
-      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg);
+      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg, true);  // EI = R F^-1 R^T
     }
   }
   HIPCHK(hipStreamSynchronize(st));

@@ -643,7 +643,8 @@ def invert(tree, timings=None, stop_level=0):
 
     Leaf: FI = inv(K(I_src, I_src)) (:102-111).  Non-leaf: FI = inv([E1 K12; K12^T E2])
     (build_Finv_fsym.m:93-101).  lev>0: E = inv(R FI R^T) (build_E_fsym.m:43-75; the
-    LR=1 branch is the same product evaluated through T.U, T.V)."""
+    LR=1 branch is the same product evaluated through T.U, T.V), symmetrised as the
+    reference's LOCAL_Invert_sym does: E = (Ai + Ai.')/2 (build_E_fsym.m:216,244,255-259)."""
     params = tree.params
     X = params["X_source"]
     import time as _time
@@ -666,7 +667,8 @@ def invert(tree, timings=None, stop_level=0):
             out.FI[key] = FI
             if lev > 0:
                 R = R_matrix(tree.J[key], tree.k[key], tree.Tdense(b))
-                out.E[key] = np.linalg.inv(R @ FI @ R.T)
+                Ai = np.linalg.inv(R @ FI @ R.T)
+                out.E[key] = 0.5 * (Ai + Ai.T)
         if timings is not None:
             timings[lev] = timings.get(lev, 0.0) + _time.perf_counter() - _t0
     return out
