@@ -34,7 +34,7 @@ def build(verbose: bool = False) -> Path:
     objdir.mkdir(exist_ok=True)
     common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
               "-I", str(ROOT / "include")]
-    # packed FP32 ops are disabled for all device code (build flags)
+    # packed FP32 ops are disabled for all device code (a correctness precaution on gfx950)
     dev = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
     objs = []
     for src in SOURCES:

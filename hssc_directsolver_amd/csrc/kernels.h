@@ -121,11 +121,14 @@ void add2d(hipStream_t st, int64_t m, int64_t n, T a, const T* s1, int64_t ld1, 
 template <typename T>
 void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks);
 
-// Batched ragged interpolative decomposition (id.hip).  Work per matrix: 5 n + m.
+// Batched ragged interpolative decomposition (id.hip).  Work per matrix: 5 n + m.  mmax and
+// nmax (> 0 when every block is known to be at most mmax x nmax) select the register-resident
+// kernel for complex blocks that fit it; a block larger than those maxima gets rank -1.
 template <typename T>
 void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc, T* A,
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
-             T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed);
+             T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed,
+             int64_t mmax = 0, int64_t nmax = 0);
 
 // G[i] ~ N(0,1), deterministic in (seed, base + i).
 void randn(hipStream_t st, int64_t len, uint64_t seed, uint64_t base, double* G);

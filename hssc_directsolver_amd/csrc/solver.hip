@@ -327,11 +327,13 @@ int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t
     k::gen_block<zdouble>(st, M, n, nb, Qx, Qy, Qw, M, Sx, Sy, Sw, n, A, M, M * n, kernel_spec(p));
     HIPCHK(hipMemcpyAsync(Ac, A, size_t(nb * M * n) * sizeof(zdouble), hipMemcpyDeviceToDevice, st));
     k::cpqr_id<zdouble>(st, int(nb), doff, dmr, dnc, A, Ac, p.acc, dkfix, dperm, dpoff, dk, Tt,
-                        dtoff, W, dwoff, 0x5EEDull ^ uint64_t(jb.lev));
+                        dtoff, W, dwoff, 0x5EEDull ^ uint64_t(jb.lev), M, n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipMemcpy(order, dperm, size_t(nb * n) * sizeof(int32_t), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(rank, dk, size_t(nb) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t b = 0; b < nb; ++b)
+      if (rank[b] < 0 || rank[b] > n) return fail(HSSC_ERR_HIP, "skeleton ID returned a bad rank");
     return HSSC_OK;
   });
 }

@@ -13,6 +13,8 @@ ragged tree and this padded form (build_tree(pad=True)).  Pinned here:
   * the polar / interface orientation of J, exactly;
   * the padding of the preset levels above, exactly (same I_src -> same J);
   * the solution against the dense solve and the reference's ragged tree.
+The first two hold for both ID kernels (the default global-memory CPQR and the opt-in
+register-resident one, HSSC_ID_KERNEL=register).
 Helmholtz k = 20, layers 3, acc 1e-10, 4-layer proxy rings.
 """
 import numpy as np
@@ -31,15 +33,26 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def case(Ng, np_, TI):
-    key = (Ng, np_, TI)
+def case(Ng, np_, TI, kernel="global"):
+    """kernel: the skeleton ID's kernel (HSSC_ID_KERNEL): the global-memory CPQR (default)
+    or the register-resident one."""
+    key = (Ng, np_, TI, kernel)
     if key not in _cache:
+        import os
+        old = os.environ.get("HSSC_ID_KERNEL")
+        os.environ["HSSC_ID_KERNEL"] = kernel
         print(f"[id-skeletons] building Ng={Ng} np={np_} TI={TI}", flush=True)
         P = H.HSS_tree_parameters("SL_H_2D", KH, 1, Ng, np_, 1e-10, 3, 800, TI, proxy_layers=4)
         h = 2.0 / (Ng * np_)
         p = o.tree_parameters("SL_H_2D", KH, 1, Ng, np_, 1e-10, 3, 800, TI, corr0=corr0(h, KH))
         p["proxy_layers"] = 4
-        tree = H.HSS2D_build_tree_fsym(P)
+        try:
+            tree = H.HSS2D_build_tree_fsym(P)
+        finally:
+            if old is None:
+                del os.environ["HSSC_ID_KERNEL"]
+            else:
+                os.environ["HSSC_ID_KERNEL"] = old
         _cache[key] = dict(P=P, p=p, tree=tree, opad=o.build_tree(p, interform="dense", pad=True))
     return _cache[key]
 
@@ -59,9 +72,10 @@ def rank_band(A, acc, slack=3.0):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["global", "register"])
 @pytest.mark.parametrize("Ng,np_,TI", CASES)
-def test_id_ranks_and_skeleton_sets_match_oracle(Ng, np_, TI):
-    c = case(Ng, np_, TI)
+def test_id_ranks_and_skeleton_sets_match_oracle(Ng, np_, TI, kernel):
+    c = case(Ng, np_, TI, kernel)
     tree, ot, p = c["tree"], c["opad"], c["p"]
     X = p["X_source"]
     d = tree.depth
@@ -94,9 +108,10 @@ def test_id_ranks_and_skeleton_sets_match_oracle(Ng, np_, TI):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["global", "register"])
 @pytest.mark.parametrize("Ng,np_,TI", CASES)
-def test_id_skeleton_error_and_orientation(Ng, np_, TI):
-    c = case(Ng, np_, TI)
+def test_id_skeleton_error_and_orientation(Ng, np_, TI, kernel):
+    c = case(Ng, np_, TI, kernel)
     tree, p = c["tree"], c["p"]
     X = p["X_source"]
     cent = o.box_centers(tree.depth)
