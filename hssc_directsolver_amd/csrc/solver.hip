@@ -694,6 +694,10 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
   const bool TI = t->p.transinv != 0;
   const int64_t N = t->N;
   w.ensure(t, m);
+  // F^-1 and E of a complex symmetric operator are exactly symmetric (the symmetric sweep
+  // of invert_batched), so their products with one vector run as transposed GEMVs: one wave
+  // per contiguous column instead of one thread per row striding across columns
+  const bool symv = is_complex_v<T> && m == 1 && !TI;
   // upward pass
   for (int l = D; l >= 1; --l) {
     const Level& L = t->levels[l];
@@ -706,7 +710,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     else form_parent_q<T>(t, l, w.PU[l + 1], Q, w.S1, m, st);
     // psi = FI q
     T* PSI = w.S2;
-    level_matvec<T>(st, false, n, n, T(1), FI, n, n * n, Q, n, T(0), nullptr, n, PSI, n, nb, m,
+    level_matvec<T>(st, symv, n, n, T(1), FI, n, n * n, Q, n, T(0), nullptr, n, PSI, n, nb, m,
                     TI, w.work);
     // u = psi_s + T psi_r
     T* U = w.S1;
@@ -716,7 +720,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     else
       k::copy2d<T, T>(st, kk, nb * m, PSI, n, 0, U, kk, 0, 1);
     // phi_up = E u
-    level_matvec<T>(st, false, kk, kk, T(1), E, kk, kk * kk, U, kk, T(0), nullptr, kk, w.PU[l],
+    level_matvec<T>(st, symv, kk, kk, T(1), E, kk, kk * kk, U, kk, T(0), nullptr, kk, w.PU[l],
                     kk, nb, m, TI, w.work);
   }
   // root: phi = FI q
@@ -727,7 +731,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     if (D == 0) k::gather<H, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, 1);
     else form_parent_q<T>(t, 0, w.PU[1], Q, w.S1, m, st);
     T* PHI = w.S2;
-    level_matvec<T>(st, false, n, n, T(1), static_cast<const T*>(inv->FI[0]), n, n * n, Q, n,
+    level_matvec<T>(st, symv, n, n, T(1), static_cast<const T*>(inv->FI[0]), n, n * n, Q, n,
                     T(0), nullptr, n, PHI, n, 1, m, TI, w.work);
     if (D == 0) k::scatter<T, H>(st, n, m, t->dLeafG, n, PHI, n, n * m, Y, N, 0, 1, false);
     else split_to_children<T>(t, 0, PHI, w.PD[1], w.S1, m, st);
@@ -742,7 +746,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     T* Q = w.Q[l];
     // wv = E phi_dn - phi_up
     T* WV = w.S1;
-    level_matvec<T>(st, false, kk, kk, T(1), E, kk, kk * kk, w.PD[l], kk, T(-1), w.PU[l], kk, WV,
+    level_matvec<T>(st, symv, kk, kk, T(1), E, kk, kk * kk, w.PD[l], kk, T(-1), w.PU[l], kk, WV,
                     kk, nb, m, TI, w.work);
     // v = q + [wv ; T^T wv]
     T* V = w.S2;
@@ -751,7 +755,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
       level_matvec<T>(st, true, kk, r, T(1), Tm, kk, kk * r, WV, kk, T(1), Q + kk, n, V + kk, n,
                       nb, m, TI, w.work);
     // phi = FI v  (into Q, which is no longer needed)
-    level_matvec<T>(st, false, n, n, T(1), FI, n, n * n, V, n, T(0), nullptr, n, Q, n, nb, m, TI,
+    level_matvec<T>(st, symv, n, n, T(1), FI, n, n * n, V, n, T(0), nullptr, n, Q, n, nb, m, TI,
                     w.work);
     if (l == D) k::scatter<T, H>(st, n, m, t->dLeafG, n, Q, n, n * m, Y, N, 0, nb, false);
     else split_to_children<T>(t, l, Q, w.PD[l + 1], w.S1, m, st);
