@@ -1314,14 +1314,21 @@ __global__ void __launch_bounds__(256) gemv_n_partial(int64_t m, int64_t n, cons
   for (int64_t c = c0 + threadIdx.x; c < c1; c += 256) xs[c - c0] = x[c];
   __syncthreads();
   if (r >= m) return;
+  // eight column loads issued before their products (memory-level parallelism per thread)
   T acc0 = T(0), acc1 = T(0);
   const T* a = A + r;
   int64_t c = c0;
-  for (; c + 1 < c1; c += 2) {
-    acc0 += a[c * lda] * xs[c - c0];
-    acc1 += a[(c + 1) * lda] * xs[c + 1 - c0];
+  for (; c + 7 < c1; c += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = a[(c + u) * lda];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      acc0 += v[u] * xs[c + u - c0];
+      acc1 += v[u + 1] * xs[c + u + 1 - c0];
+    }
   }
-  if (c < c1) acc0 += a[c * lda] * xs[c - c0];
+  for (; c < c1; ++c) acc0 += a[c * lda] * xs[c - c0];
   part[(e * nsplit + blockIdx.y) * m + r] = acc0 + acc1;
 }
 
@@ -1339,8 +1346,23 @@ __global__ void __launch_bounds__(256) gemv_t_partial(int64_t m, int64_t n, cons
   const int64_t r0 = int64_t(blockIdx.y) * GV_CHUNK;
   const int64_t r1 = (r0 + GV_CHUNK < m) ? r0 + GV_CHUNK : m;
   if (j >= n) return;
-  T acc = T(0);
-  for (int64_t r = r0 + lane; r < r1; r += 64) acc += A[r + j * lda] * x[r];
+  // the chunk's eight loads of this lane issued before their products
+  const T* a = A + j * lda;
+  T acc = T(0), acc1 = T(0);
+  int64_t r = r0 + lane;
+  if (r0 + GV_CHUNK <= m) {
+    T v[GV_CHUNK / 64];
+#pragma unroll
+    for (int u = 0; u < GV_CHUNK / 64; ++u) v[u] = a[r + u * 64];
+#pragma unroll
+    for (int u = 0; u < GV_CHUNK / 64; u += 2) {
+      acc += v[u] * x[r + u * 64];
+      acc1 += v[u + 1] * x[r + (u + 1) * 64];
+    }
+  } else {
+    for (; r < r1; r += 64) acc += a[r] * x[r];
+  }
+  acc += acc1;
   for (int off = 32; off > 0; off >>= 1) acc += shfl_xor_s(acc, off);
   if (lane == 0) part[(e * nsplit + blockIdx.y) * n + j] = acc;
 }
