@@ -1350,6 +1350,34 @@ __global__ void __launch_bounds__(256) gemv_t_partial(int64_t m, int64_t n, cons
   const T* a = A + j * lda;
   T acc = T(0), acc1 = T(0);
   int64_t r = r0 + lane;
+  if constexpr (sizeof(T) == 8) {
+    // 8-byte elements (complex64): each lane takes two adjacent rows in one 16-byte load
+    using V2 = float4;
+    if (r0 + GV_CHUNK <= m && ((reinterpret_cast<uintptr_t>(a + r0) |
+                                reinterpret_cast<uintptr_t>(x + r0)) & 15) == 0) {
+      const V2* av = reinterpret_cast<const V2*>(a + r0) + lane;
+      const V2* xv = reinterpret_cast<const V2*>(x + r0) + lane;
+      V2 v[GV_CHUNK / 128];
+#pragma unroll
+      for (int u = 0; u < GV_CHUNK / 128; ++u) v[u] = av[u * 64];
+#pragma unroll
+      for (int u = 0; u < GV_CHUNK / 128; ++u) {
+        const V2 w = xv[u * 64];
+        T a0, a1, x0, x1;
+        static_assert(sizeof(V2) == 2 * sizeof(T), "two elements per vector");
+        __builtin_memcpy(&a0, &v[u].x, sizeof(T));
+        __builtin_memcpy(&a1, &v[u].z, sizeof(T));
+        __builtin_memcpy(&x0, &w.x, sizeof(T));
+        __builtin_memcpy(&x1, &w.z, sizeof(T));
+        acc += a0 * x0;
+        acc1 += a1 * x1;
+      }
+      acc += acc1;
+      for (int off = 32; off > 0; off >>= 1) acc += shfl_xor_s(acc, off);
+      if (lane == 0) part[(e * nsplit + blockIdx.y) * n + j] = acc;
+      return;
+    }
+  }
   if (r0 + GV_CHUNK <= m) {
     T v[GV_CHUNK / 64];
 #pragma unroll
