@@ -51,6 +51,11 @@ void gemm_skip(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k
                T* C, int64_t ldc, int64_t sC, int64_t batch, int64_t rtiles, int64_t ctiles,
                int64_t rs0, int64_t rsn, int64_t cs0, int64_t csn, int tri = 0);
 
+// Lower 64 x 64 tiles of a symmetric matrix from the upper ones (A(i, j) = A(j, i) for the
+// tiles below the diagonal; the diagonal tiles are left as they are).
+template <typename T>
+void sym_lower_fill(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
+
 // C = A^H B with the long K dimension split in chunks of kchunk rows (deterministic sum);
 // part needs batch * ceil(kdim/kchunk) * m * n elements.
 template <typename T>

@@ -713,6 +713,14 @@ __global__ void __launch_bounds__(256) sym_lower_fill_kernel(int64_t n, T* __res
   }
 }
 
+template <typename T>
+void sym_lower_fill(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch) {
+  if (n <= 0 || batch <= 0) return;
+  const int64_t nt = (n + 63) / 64;
+  hipLaunchKernelGGL(sym_lower_fill_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
+                     sA);
+}
+
 // Blocked GJ of a symmetric matrix (complex symmetric: A^T = A, no conjugate) on its upper
 // tiles.  With P the blocks already swept and U the others, the swept matrix keeps
 // M_PP = M_PP^T, M_UU = M_UU^T and M_UP = -M_PU^T (A11^-1 and the Schur complement are
@@ -1640,6 +1648,7 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
                                   hipStream_t, T*, bool);                                          \
   template int64_t invert_ws_elems<T>(int64_t, int64_t);                                           \
+  template void sym_lower_fill<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t);            \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
   template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \

@@ -632,10 +632,14 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
         k::copy2d<T, T>(st, kk, r, FI + kk * n, n, n * n, Wk, kk, kk * r, E);
         k::gemm<T>(st, false, false, kk, r, r, T(1), Tm, kk, kk * r, FI + kk + kk * n, n, n * n,
                    T(1), Wk, kk, kk * r, E);
-        k::gemm<T>(st, false, false, kk, kk, r, T(1), Tm, kk, kk * r, FI + kk, n, n * n, T(1), EI,
-                   kk, kk * kk, E);
-        k::gemm<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI, kk,
-                   kk * kk, E);
+        // EI is symmetric: the two kk x kk products form its upper 64 x 64 tiles only and the
+        // lower ones are their transposes (Phi_ss, copied whole, is symmetric itself)
+        const int64_t tk = (kk + 63) / 64, big = INT64_MAX;
+        k::gemm_skip<T>(st, false, false, kk, kk, r, T(1), Tm, kk, kk * r, FI + kk, n, n * n, T(1),
+                        EI, kk, kk * kk, E, tk, tk, big, 0, big, 0, 1);
+        k::gemm_skip<T>(st, false, true, kk, kk, r, T(1), Wk, kk, kk * r, Tm, kk, kk * r, T(1), EI,
+                        kk, kk * kk, E, tk, tk, big, 0, big, 0, 1);
+        k::sym_lower_fill<T>(st, kk, EI, kk, kk * kk, E);
       }
       k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg, true);  // EI = R F^-1 R^T
     }
