@@ -80,7 +80,7 @@ def stats_snapshot(H):
     import ctypes as C
     L = H.lib()
     out = {}
-    names = [f"{k}_{t}" for k in ("gemv", "gemvt", "gj_small", "gj_diag", "gen_block",
+    names = [f"{k}_{t}" for k in ("gemv", "gemvt", "symv", "gj_small", "gj_diag", "gen_block",
                                   "gen_parent", "cqr_panel")
              for t in ("f64", "f32", "c128", "c64")]
     names += [f"gemm_{t}_{a}{b}" for t in ("f64", "f32", "c128", "c64") for a in "ntch"

@@ -108,6 +108,14 @@ void gemv(hipStream_t st, bool trans, int64_t m, int64_t n, T alpha, const T* A,
           int64_t batch, T* work);
 constexpr int64_t kGemvChunk = 512;
 
+// y = alpha A x + gamma z for symmetric A (A^T = A), read from its upper triangle only;
+// work needs symv_work_elems(n, batch) elements.
+template <typename T>
+void symv_upper(hipStream_t st, int64_t n, T alpha, const T* A, int64_t lda, int64_t sA,
+                const T* x, int64_t sx, T gamma, const T* z, int64_t sz, T* y, int64_t sy,
+                int64_t batch, T* work);
+int64_t symv_work_elems(int64_t n, int64_t batch);
+
 template <typename TS, typename TD>
 void gather(hipStream_t st, int64_t n, int64_t m, const int32_t* idx, int64_t sidx, const TS* src,
             int64_t ld_src, int64_t s_src, TD* dst, int64_t ld_dst, int64_t s_dst, int64_t batch);

@@ -1403,6 +1403,87 @@ __global__ void __launch_bounds__(256) gemv_t_partial(int64_t m, int64_t n, cons
   if (lane == 0) part[(e * nsplit + blockIdx.y) * n + j] = acc;
 }
 
+// y = alpha A x + gamma z for a symmetric A (A^T = A, no conjugate) read from its upper
+// triangle only, half the bytes of a GEMV: each stored A(r, j), r <= j, gives A(r, j) x_j
+// to y_r and, off the diagonal, A(r, j) x_r to y_j.  Block (cb, rc) takes the columns
+// [64 cb, 64 cb + 64) and rows [256 rc, 256 rc + 256) of one matrix and leaves row partials
+// rp[e][cb][r] (its columns' part of y_r) and column partials cp[e][rc][j] (its rows' part of
+// y_j, a wave sum per column); a tile below the diagonal reads nothing and leaves zeros.
+constexpr int SV_CB = 64, SV_RC = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(256) symv_upper_partial(int64_t n, const T* __restrict__ A,
+                                                          int64_t lda, int64_t sA,
+                                                          const T* __restrict__ x, int64_t sx,
+                                                          T* __restrict__ rp,
+                                                          T* __restrict__ cp) {
+  __shared__ T red[4][SV_RC];
+  const int64_t e = blockIdx.z;
+  const int cb = blockIdx.x, rc = blockIdx.y, ncb = gridDim.x, nrc = gridDim.y;
+  A += e * sA;
+  x += e * sx;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t j0 = int64_t(cb) * SV_CB, r0 = int64_t(rc) * SV_RC;
+  const bool active = r0 < j0 + SV_CB;  // the tile meets the upper triangle
+  constexpr int RU = SV_RC / 64;
+  T ra[RU], xr[RU];
+#pragma unroll
+  for (int u = 0; u < RU; ++u) {
+    const int64_t r = r0 + lane + 64 * u;
+    ra[u] = T(0);
+    xr[u] = (active && r < n) ? x[r] : T(0);
+  }
+  T* cpo = cp + (e * nrc + rc) * n;
+  for (int jj = 0; jj < SV_CB / 4; ++jj) {
+    const int64_t j = j0 + wave * (SV_CB / 4) + jj;  // wave-uniform
+    if (j >= n) break;
+    T ca = T(0);
+    if (active) {
+      const T xj = x[j];
+      const T* col = A + j * lda;
+      T v[RU];
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int64_t r = r0 + lane + 64 * u;
+        v[u] = r <= j ? col[r] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int64_t r = r0 + lane + 64 * u;
+        ra[u] += v[u] * xj;
+        if (r < j) ca += v[u] * xr[u];
+      }
+      for (int off = 32; off > 0; off >>= 1) ca += shfl_xor_s(ca, off);
+    }
+    if (lane == 0) cpo[j] = ca;
+  }
+#pragma unroll
+  for (int u = 0; u < RU; ++u) red[wave][lane + 64 * u] = ra[u];
+  __syncthreads();
+  T* rpo = rp + (e * ncb + cb) * n;
+  const int64_t r = r0 + threadIdx.x;
+  if (r < n) rpo[r] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                      (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) symv_reduce(int64_t n, int ncb, int nrc,
+                                                   const T* __restrict__ rp,
+                                                   const T* __restrict__ cp, T alpha,
+                                                   const T* __restrict__ z, int64_t sz, T gamma,
+                                                   T* __restrict__ y, int64_t sy) {
+  const int64_t e = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  T s = T(0);
+  for (int q = 0; q < ncb; ++q) s += rp[(e * ncb + q) * n + i];
+  for (int q = 0; q < nrc; ++q) s += cp[(e * nrc + q) * n + i];
+  T v = alpha * s;
+  if (gamma != T(0)) v += gamma * z[e * sz + i];
+  y[e * sy + i] = v;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) gemv_reduce(int64_t len, int64_t nsplit,
                                                    const T* __restrict__ part, T alpha,
@@ -1448,6 +1529,27 @@ void gemv(hipStream_t st, bool trans, int64_t m, int64_t n, T alpha, const T* A,
   }
   hipLaunchKernelGGL(gemv_reduce<T>, dim3((out + 255) / 256, batch), dim3(256), 0, st, out, nsplit,
                      work, alpha, z, sz, gamma, y, sy);
+}
+
+int64_t symv_work_elems(int64_t n, int64_t batch) {
+  return batch * ((n + SV_CB - 1) / SV_CB + (n + SV_RC - 1) / SV_RC) * n;
+}
+
+template <typename T>
+void symv_upper(hipStream_t st, int64_t n, T alpha, const T* A, int64_t lda, int64_t sA,
+                const T* x, int64_t sx, T gamma, const T* z, int64_t sz, T* y, int64_t sy,
+                int64_t batch, T* work) {
+  if (batch <= 0 || n <= 0) return;
+  const int ncb = int((n + SV_CB - 1) / SV_CB), nrc = int((n + SV_RC - 1) / SV_RC);
+  T* rp = work;
+  T* cp = work + batch * ncb * n;
+  const double fl = (is_complex_v<T> ? 8.0 : 2.0) * double(n) * n * batch;
+  const double by = (double(n) * (n + 1) / 2 + 2 * n) * sizeof(T) * batch;
+  HSSC_TIMED(lbl<T>("symv").c_str(), st, fl, by,
+             hipLaunchKernelGGL(symv_upper_partial<T>, dim3(ncb, nrc, batch), dim3(256), 0, st, n,
+                                A, lda, sA, x, sx, rp, cp));
+  hipLaunchKernelGGL(symv_reduce<T>, dim3((n + 255) / 256, batch), dim3(256), 0, st, n, ncb, nrc,
+                     rp, cp, alpha, z, sz, gamma, y, sy);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1648,6 +1750,8 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
                                   hipStream_t, T*, bool);                                          \
   template int64_t invert_ws_elems<T>(int64_t, int64_t);                                           \
+  template void symv_upper<T>(hipStream_t, int64_t, T, const T*, int64_t, int64_t, const T*,       \
+                              int64_t, T, const T*, int64_t, T*, int64_t, int64_t, T*);            \
   template void sym_lower_fill<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t);            \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \

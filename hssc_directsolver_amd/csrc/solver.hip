@@ -116,6 +116,20 @@ void level_matvec(hipStream_t st, bool trans, int64_t rows, int64_t cols, T alph
   }
 }
 
+// y = A x + gamma z for the n x n blocks A of a level (stride n^2): from the upper triangle
+// when sym (exactly symmetric blocks, one right-hand side, one block per box), else as
+// level_matvec.
+template <typename T>
+void sym_matvec(hipStream_t st, bool sym, int64_t n, const T* A, const T* x, int64_t ldx,
+                T gamma, const T* z, int64_t ldz, T* y, int64_t ldy, int64_t nbox, int64_t m,
+                bool shared, T* work) {
+  if (sym)
+    k::symv_upper<T>(st, n, T(1), A, n, n * n, x, ldx, gamma, z, ldz, y, ldy, nbox, work);
+  else
+    level_matvec<T>(st, false, n, n, T(1), A, n, n * n, x, ldx, gamma, z, ldz, y, ldy, nbox, m,
+                    shared, work);
+}
+
 int64_t gemv_work_elems(int64_t out, int64_t inner, int64_t batch) {
   return batch * ((inner + k::kGemvChunk - 1) / k::kGemvChunk + 1) * out;
 }
@@ -190,6 +204,7 @@ struct SolveWork {
       PD[l] = dalloc<T>(L.nbox * L.k * mm);
       smax = std::max(smax, L.nbox * L.n * mm);
       wmax = std::max(wmax, gemv_work_elems(L.n, L.n, L.nbox));
+      wmax = std::max(wmax, k::symv_work_elems(L.n, L.nbox));
     }
     S1 = dalloc<T>(smax);
     S2 = dalloc<T>(smax);
@@ -699,8 +714,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
   const int64_t N = t->N;
   w.ensure(t, m);
   // F^-1 and E of a complex symmetric operator are exactly symmetric (the symmetric sweep
-  // of invert_batched), so their products with one vector run as transposed GEMVs: one wave
-  // per contiguous column instead of one thread per row striding across columns
+  // of invert_batched), so their products with one vector read the upper triangle only
   const bool symv = is_complex_v<T> && m == 1 && !TI;
   // upward pass
   for (int l = D; l >= 1; --l) {
@@ -714,8 +728,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     else form_parent_q<T>(t, l, w.PU[l + 1], Q, w.S1, m, st);
     // psi = FI q
     T* PSI = w.S2;
-    level_matvec<T>(st, symv, n, n, T(1), FI, n, n * n, Q, n, T(0), nullptr, n, PSI, n, nb, m,
-                    TI, w.work);
+    sym_matvec<T>(st, symv, n, FI, Q, n, T(0), nullptr, n, PSI, n, nb, m, TI, w.work);
     // u = psi_s + T psi_r
     T* U = w.S1;
     if (r > 0)
@@ -724,8 +737,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     else
       k::copy2d<T, T>(st, kk, nb * m, PSI, n, 0, U, kk, 0, 1);
     // phi_up = E u
-    level_matvec<T>(st, symv, kk, kk, T(1), E, kk, kk * kk, U, kk, T(0), nullptr, kk, w.PU[l],
-                    kk, nb, m, TI, w.work);
+    sym_matvec<T>(st, symv, kk, E, U, kk, T(0), nullptr, kk, w.PU[l], kk, nb, m, TI, w.work);
   }
   // root: phi = FI q
   {
@@ -735,8 +747,8 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     if (D == 0) k::gather<H, T>(st, n, m, t->dLeafG, n, X, N, 0, Q, n, n * m, 1);
     else form_parent_q<T>(t, 0, w.PU[1], Q, w.S1, m, st);
     T* PHI = w.S2;
-    level_matvec<T>(st, symv, n, n, T(1), static_cast<const T*>(inv->FI[0]), n, n * n, Q, n,
-                    T(0), nullptr, n, PHI, n, 1, m, TI, w.work);
+    sym_matvec<T>(st, symv, n, static_cast<const T*>(inv->FI[0]), Q, n, T(0), nullptr, n, PHI,
+                  n, 1, m, TI, w.work);
     if (D == 0) k::scatter<T, H>(st, n, m, t->dLeafG, n, PHI, n, n * m, Y, N, 0, 1, false);
     else split_to_children<T>(t, 0, PHI, w.PD[1], w.S1, m, st);
   }
@@ -750,8 +762,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
     T* Q = w.Q[l];
     // wv = E phi_dn - phi_up
     T* WV = w.S1;
-    level_matvec<T>(st, symv, kk, kk, T(1), E, kk, kk * kk, w.PD[l], kk, T(-1), w.PU[l], kk, WV,
-                    kk, nb, m, TI, w.work);
+    sym_matvec<T>(st, symv, kk, E, w.PD[l], kk, T(-1), w.PU[l], kk, WV, kk, nb, m, TI, w.work);
     // v = q + [wv ; T^T wv]
     T* V = w.S2;
     k::add2d<T>(st, kk, nb * m, T(1), Q, n, 0, T(1), WV, kk, 0, V, n, 0, 1);
@@ -759,8 +770,7 @@ void applyinv_impl(const hssc_inverse* inv, SolveWork<T>& w, const hi_t<T>* X, h
       level_matvec<T>(st, true, kk, r, T(1), Tm, kk, kk * r, WV, kk, T(1), Q + kk, n, V + kk, n,
                       nb, m, TI, w.work);
     // phi = FI v  (into Q, which is no longer needed)
-    level_matvec<T>(st, symv, n, n, T(1), FI, n, n * n, V, n, T(0), nullptr, n, Q, n, nb, m, TI,
-                    w.work);
+    sym_matvec<T>(st, symv, n, FI, V, n, T(0), nullptr, n, Q, n, nb, m, TI, w.work);
     if (l == D) k::scatter<T, H>(st, n, m, t->dLeafG, n, Q, n, n * m, Y, N, 0, nb, false);
     else split_to_children<T>(t, l, Q, w.PD[l + 1], w.S1, m, st);
   }
