@@ -125,7 +125,7 @@ def one_step(H, P, torch, b, tol, split=False):
 
 
 def hss1d_case(H, torch, N=4096, reps=3):
-    """HSS1D: second-kind Laplace double layer on the star contour
+    """HSS1D workload: second-kind Laplace double layer on the star contour
     r(t) = 1 + 0.3 cos(5t), N trapezoidal nodes, leaf 64, ID tol 1e-10, 64 proxy points;
     Gaussian RHS from seed 0.  Returns GPU times and the oracle's CPU time on the same
     problem (all of it: the CPU oracle finishes N = 4096 in about a second)."""
@@ -333,7 +333,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(DESIGN.md describes how the tree would shard)"}))
+                                   "(the tree would shard by subtrees, one per GPU)"}))
         sys.exit(2)
     import torch
     import hssc_directsolver_amd as H

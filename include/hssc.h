@@ -210,8 +210,8 @@ int hssc_lrid_rand(const double* A, int64_t m, int64_t n, int64_t lda, double ac
  * The operator is the second-kind Laplace double layer A = -1/2 I + D on a closed curve
  * given by its N nodes (X, N x 2), outward unit normals (nrm, N x 2), quadrature weights
  * w and curvature kappa (D_jj = -kappa_j w_j / 4pi).  The reference's Kernel_Eval takes
- * positions only; its 'green' compression with this kernel is the BASELINE configuration
- * of the HSS1D path.  Proxy circles carry log charges. */
+ * positions only; its 'green' compression with this kernel is the HSS1D workload measured
+ * here.  Proxy circles carry log charges. */
 typedef struct hssc_hss1d hssc_hss1d;
 
 /* Replaces HSS1D/HSS1D_compress_nsym.m:1 (K_HSS = HSS1D_compress_nsym('green',X,nbox_max,

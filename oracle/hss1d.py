@@ -7,8 +7,8 @@ numpy/scipy restatement of HSSC_DirectSolver's HSS1D solver (nsym, 'green' compr
   HSS1D/HSS1D_apply_nsym.m      forward apply
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU baseline may use it.
 
-Operator (no counterpart in the reference's Kernel_Eval, which takes
-positions only; SURVEY Appendix C): second-kind Laplace double layer on the star contour
+Operator (the HSS1D workload; no counterpart in the reference's Kernel_Eval, which takes
+positions only): second-kind Laplace double layer on the star contour
 r(t) = 1 + a cos(m t), N trapezoidal nodes,
     A = -1/2 I + D,  D_ij = (1/2pi) (x_i - x_j).n_j / |x_i - x_j|^2 w_j  (i != j),
                      D_jj = -kappa_j w_j / (4 pi),
