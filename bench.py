@@ -42,7 +42,7 @@ PEAK_HBM_GBS = 8000.0
 # for a plain 1 GiB device copy (`bench.py --calibrate-copy`: FETCH_SIZE 512 MiB -> x2,
 # WRITE_SIZE 1 GiB -> x1).  profiles/hss2d_512_helmholtz_traffic.md holds the runs.
 MEASURED_TRAFFIC = {
-    "gemm_c128_nn": 2 * 85087.0 * 1024 + 69935.7 * 1024,
+    "gemm_c128_nn": 2 * 85087.3 * 1024 + 69935.5 * 1024,
 }
 
 
