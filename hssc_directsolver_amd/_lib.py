@@ -11,6 +11,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "SINGULAR", 4: "RANK
           5: "HIP_ERROR", 6: "NOT_CONVERGED", 7: "UNSUPPORTED"}
 POT = {"SL_L_2D": 0, "SL_H_2D": 1}
 PREC = {"fp64": 0, "fp32": 1}
+CONTRAST = {"plateau": 0, "gaussian": 1}
 
 
 class HsscParams(C.Structure):
@@ -22,7 +23,9 @@ class HsscParams(C.Structure):
         ("max_particles", C.c_int32), ("interform", C.c_int32), ("k_cut", C.c_int32),
         ("dr_weights_re", C.c_double), ("dr_weights_im", C.c_double),
         ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("preset_all", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("contrast", C.c_int32), ("contrast_amp", C.c_double), ("contrast_width", C.c_double),
+        ("kernel_scale", C.c_double), ("length_scale", C.c_double),
+        ("tree_precision", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

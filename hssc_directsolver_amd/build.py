@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -36,15 +37,20 @@ def build(verbose: bool = False) -> Path:
               "-I", str(ROOT / "include")]
     # packed FP32 ops are disabled for all device code (a correctness precaution on gfx950)
     dev = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = objdir / (src + ".o")
         cmd = [cc, *common, *dev, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
             # host-only translation units (still compiled by hipcc for the HIP headers)
             cmd = [cc, *common, "-c", str(CSRC / src), "-o", str(obj)]
-        _run(cmd, verbose)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # translation units compile independently: one worker each, bounded by the host's cores
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([cc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB)

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -103,8 +104,18 @@ struct hssc_tree {
   std::vector<void*> bpending;
   bool cx = false;  // complex (Helmholtz) kernel: every matrix and vector is complex
   int device = 0;
+  // Lifetime: the caller's handle is one reference, and every inverse and operator made from
+  // the tree (they read its device data) holds another; hssc_tree_destroy drops the caller's,
+  // and whoever drops the last one frees the tree.
+  std::atomic<int> refs{1};
   ~hssc_tree();
 };
+
+namespace hssc {
+// Takes / drops a reference on a tree (dropping the last one frees it).
+void tree_ref(hssc_tree* t);
+void tree_unref(hssc_tree* t);
+}  // namespace hssc
 
 namespace hssc {
 
@@ -129,7 +140,12 @@ void source_points(int Ng, int np, double* X /* N x 2 col-major */);
 // streams and workspaces.  Idempotent.
 void tree_sync(hssc_tree* t);
 
-// Kernel weight of a point (Common/Kernel_Eval.m:111-159, params.proxy == 0).
-double point_weight(const hssc_params& p, double x, double y);
+// Kernel weight of a point (Common/Kernel_Eval.m:111-159, params.proxy == 0): b(x) of a
+// row point or c(x) of a column point (the same for a symmetric kernel).
+double point_weight(const hssc_params& p, double x, double y, bool row = true);
+
+// Checks the options of a params struct that every entry point relies on; returns an
+// HSSC_* status.
+int check_params(const hssc_params& p);
 
 }  // namespace hssc

@@ -19,6 +19,7 @@ struct KernelSpec {
   double scale = 0;            // SL_L_2D: h^2 / 2 (times log r^2); SL_H_2D: h^2 k^2 / 4
   double kh = 0;               // wavenumber
   double wre = 0, wim = 0;     // SL_H_2D diagonal w = 1 + k^2 (i/4) corr0
+  double lshift = 0;           // SL_L_2D: added to log r^2 (log L^2 of a length scale L)
 };
 
 // A(r,c) = K(R_r, C_c) with per-point weights (real T for SL_L_2D, complex for SL_H_2D).
@@ -57,21 +58,28 @@ template <typename T>
 void sym_lower_fill(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
 
 // C = A^H B with the long K dimension split in chunks of kchunk rows (deterministic sum);
-// part needs batch * ceil(kdim/kchunk) * m * n elements.
+// the split needs batch * ceil(kdim/kchunk) * m * n elements of part, whose capacity is
+// part_cap elements: a product whose partial sums would not fit is computed unsplit.
 template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
-                    int64_t batch, T* part, int64_t kchunk);
+                    int64_t batch, T* part, int64_t kchunk, int64_t part_cap);
+// Elements of the split-K buffer of one level's least squares (per matrix): the outer-panel
+// products (jbo x nc, chunks of kchunk rows), the sub-panel products (jb x nin with
+// jb <= kQrNb and nin < n, chunks of kchunk_thin rows) and the panel Gram matrices
+// (nbo x nbo, chunks of kchunk_thin rows).
+int64_t ls_part_elems(int64_t M, int64_t n, int64_t nbo, int64_t kchunk, int64_t kchunk_thin);
 
 // In-place inverse of a batch of n x n matrices.  aux (optional) is a second stream on
 // which the blocked sweep inverts the next diagonal block while st finishes the current
 // step; all work is ordered back onto st before the call returns.
 // work: invert_ws_elems<T>(n, batch) elements (none when one kernel inverts the matrices).
 // sym: the matrices are symmetric (A^T = A, complex symmetric for complex T); complex ones
-// are then swept on their upper tiles only.
+// are then swept on their upper tiles only.  flag (optional, device int): set to nonzero when
+// a pivot of a diagonal block is not finite or at most n eps times the block's largest entry.
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux, T* work, bool sym);
+                    hipStream_t aux, T* work, bool sym, int* flag = nullptr);
 template <typename T>
 int64_t invert_ws_elems(int64_t n, int64_t batch);
 

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <limits>
 #include <cstdint>
@@ -34,7 +35,9 @@ std::string lbl(const char* base) {
 // Kernel-block generation (Common/Kernel_Eval.m:29-96, 2D plane)
 // ------------------------------------------------------------------------------------
 // SL_L_2D: A(r,c) = (den==0) + h^2 * 0.5 * w_r * w_c * log(den),  den = |x_r - x_c|^2
-//          (the log term vanishes at den == 0: log(den + (den==0)) = 0 in the reference).
+//          (the log term vanishes at den == 0: log(den + (den==0)) = 0 in the reference);
+//          with a kernel scale s and length scale L, s (L h)^2 w_r w_c log(L r), i.e. scale
+//          0.5 s L^2 h^2 and log(den) + log L^2.
 // SL_H_2D: A(r,c) = w (den==0) + h^2 k^2 (i/4) w_r w_c H0(k sqrt(den)), w = 1 + k^2 (i/4)
 //          corr0 (:44-49; b and c are not applied on the diagonal).  With H0 = J0 + i Y0,
 //          (i c)(J0 + i Y0) = -c Y0 + i c J0.
@@ -51,10 +54,10 @@ __device__ __forceinline__ T kernel_entry(const KernelSpec& ks, double xr, doubl
       return T(R(-c * y0(z)), R(c * j0(z)));
     }
     if (den == 0.0) return T(R(1));
-    return T(R(ks.scale * wr * wc * log(den)));
+    return T(R(ks.scale * wr * wc * (log(den) + ks.lshift)));
   } else {
     if (den == 0.0) return T(1.0);
-    return T(ks.scale * wr * wc * log(den));
+    return T(ks.scale * wr * wc * (log(den) + ks.lshift));
   }
 }
 
@@ -439,24 +442,33 @@ void gemm_h(hipStream_t st, int64_t m, int64_t n, int64_t kdim, T alpha, const T
 //   * every wave reads the old rows c and p of its own columns by lane shuffles, applies
 //     the swap logically and performs the sweep on its registers.
 // Column unscrambling (swaps undone in reverse order) is folded into the final store.
+// Singularity: a pivot that is not finite, or whose magnitude is at most n * eps times the
+// block's largest entry (no correct digit left in the inverse), sets *flag (when given).
 template <typename T, int S, int CB>
 __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, int64_t lda,
-                                                     int64_t sA, int64_t off) {
+                                                     int64_t sA, int64_t off, int* flag) {
   using R = real_t<T>;
   __shared__ T colbuf[2][64 * S];
   __shared__ int spiv[2];
   __shared__ int pivs[64 * S];
   __shared__ int dest[64 * S];
+  __shared__ R wmax[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   A += int64_t(blockIdx.x) * sA + off;
   T v[S][CB];
+  R amax = 0;
 #pragma unroll
   for (int s = 0; s < S; ++s)
 #pragma unroll
     for (int t = 0; t < CB; ++t) {
       const int i = lane + 64 * s, j = wave * CB + t;
       v[s][t] = (i < n && j < n) ? A[i + int64_t(j) * lda] : T(0);
+      amax = fmax(amax, abs1(v[s][t]));
     }
+  for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o));
+  if (lane == 0) wmax[wave] = amax;
+  // (wmax is read after the first barrier of the sweep)
+  const R tiny_rel = R(n) * std::numeric_limits<R>::epsilon();
   auto publish = [&](int c) {
     // called by the wave owning column c.  Candidates are ranked by a 64-bit key: the
     // single-precision magnitude (non-negative floats order like their bit patterns) above
@@ -492,6 +504,11 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
     const T* cb = colbuf[c & 1];
     const T piv = cb[p];
     const T inv = T(1) / piv;
+    if (flag && threadIdx.x == 0) {
+      const R a = abs1(piv);
+      const R bmax = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+      if (!(a > tiny_rel * bmax) || !(a <= std::numeric_limits<R>::max())) atomicOr(flag, 1);
+    }
     // old rows c and p of this wave's columns
     T rc[CB], rp[CB];
     const int sc = c >> 6, lc = c & 63, sp = p >> 6, lp = p & 63;
@@ -549,16 +566,16 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
 
 template <typename T>
 void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, int64_t batch,
-            const std::string& label) {
+            const std::string& label, int* flag) {
   const double fl = (is_complex_v<T> ? 8.0 : 2.0) * n * double(n) * n * batch;
   if (n <= 64) {
     HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
                hipLaunchKernelGGL((gj_reg_kernel<T, 1, 16>), dim3(batch), dim3(256), 0, st, n, A,
-                                  lda, sA, off));
+                                  lda, sA, off, flag));
   } else if constexpr (!is_complex_v<T>) {
     HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
                hipLaunchKernelGGL((gj_reg_kernel<T, 2, 32>), dim3(batch), dim3(256), 0, st, n, A,
-                                  lda, sA, off));
+                                  lda, sA, off, flag));
   }
 }
 
@@ -586,7 +603,7 @@ constexpr int gj_reg_max() {
 // step, at half the traffic per flop of rank-64 ones.
 template <typename T>
 void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
-                int64_t batch, T* work) {
+                int64_t batch, T* work, int* flag) {
   // p advances in multiples of 64 = the GEMM tile, so no 64-row tile straddles block B; the
   // tile-skip ranges of gemm_skip enumerate the tiles of [0,p) U [p+nb,n).
   // Look-ahead (when an aux stream is given): step (3) first updates the row block B' of
@@ -609,7 +626,7 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
     }
   }
   const int64_t big = INT64_MAX;
-  gj_reg<T>(st, int(n < NB ? n : NB), A, lda, sA, 0, batch, diag);
+  gj_reg<T>(st, int(n < NB ? n : NB), A, lda, sA, 0, batch, diag, flag);
   for (int64_t p = 0; p < n; p += NB) {
     const int nb = int(p + NB <= n ? NB : n - p);
     const int64_t nout = n - nb;
@@ -634,7 +651,7 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
       if (aux) {
         (void)hipEventRecord(upd, st);
         (void)hipStreamWaitEvent(aux, upd, 0);
-        gj_reg<T>(aux, nb2, A, lda, sA, q + q * lda, batch, diag);
+        gj_reg<T>(aux, nb2, A, lda, sA, q + q * lda, batch, diag, flag);
         (void)hipEventRecord(inv, aux);
       }
     }
@@ -655,7 +672,7 @@ void gj_blocked(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, i
       copy2d<T, T>(st, nb, p, Wy, NB, sW, Apb, lda, sA, batch);
       copy2d<T, T>(st, nb, n - q, Wy + q * NB, NB, sW, Apb + q * lda, lda, sA, batch);
     }
-    if (nb2 > 0 && !aux) gj_reg<T>(st, nb2, A, lda, sA, q + q * lda, batch, diag);
+    if (nb2 > 0 && !aux) gj_reg<T>(st, nb2, A, lda, sA, q + q * lda, batch, diag, flag);
   }
   if (upd) (void)hipEventDestroy(upd);
   if (inv) (void)hipEventDestroy(inv);
@@ -735,7 +752,7 @@ void sym_lower_fill(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 // blocks inverted by gj_reg are complete.
 template <typename T>
 void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
-                    int64_t batch, T* work) {
+                    int64_t batch, T* work, int* flag) {
   constexpr int NB = GB_M;
   const std::string diag = lbl<T>("gj_diag");
   const int64_t sW = 2 * n * NB;
@@ -756,7 +773,7 @@ void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t ld
   // on both sides (A_BB^-1 in the new A_XB and, through the upper tiles, A_BB^-T), so it is
   // replaced by its symmetric part
   auto diag_inverse = [&](hipStream_t s, int nbk, int64_t off) {
-    gj_reg<T>(s, nbk, A, lda, sA, off, batch, diag);
+    gj_reg<T>(s, nbk, A, lda, sA, off, batch, diag, flag);
     hipLaunchKernelGGL(sym_average_kernel<T>, dim3((nbk + 31) / 32, (nbk + 31) / 32, batch),
                        dim3(256), 0, s, int64_t(nbk), A + off, lda, sA);
   };
@@ -826,10 +843,10 @@ int64_t invert_ws_elems(int64_t n, int64_t batch) {
 
 template <typename T>
 void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
-                    hipStream_t aux, T* work, bool sym) {
+                    hipStream_t aux, T* work, bool sym, int* flag) {
   if (n <= 0 || batch <= 0) return;
   if (n <= gj_reg_max<T>()) {
-    gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"));
+    gj_reg<T>(st, int(n), A, lda, sA, 0, batch, lbl<T>("gj_small"), flag);
     if (sym && is_complex_v<T>) {  // the pivoted inverse is symmetric only to rounding
       const int64_t nt = (n + 31) / 32;
       hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
@@ -841,9 +858,9 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
     const int64_t nt = (n + 31) / 32;
     hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
                        sA);
-    gj_blocked_sym<T>(st, aux, n, A, lda, sA, batch, work);
+    gj_blocked_sym<T>(st, aux, n, A, lda, sA, batch, work, flag);
   } else {
-    gj_blocked<T>(st, aux, n, A, lda, sA, batch, work);
+    gj_blocked<T>(st, aux, n, A, lda, sA, batch, work, flag);
   }
 }
 
@@ -1692,12 +1709,13 @@ __global__ void __launch_bounds__(256) sum_split_kernel(int64_t len, int64_t nsp
 template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
-                    int64_t batch, T* part, int64_t kchunk) {
+                    int64_t batch, T* part, int64_t kchunk, int64_t part_cap) {
   const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
   // split only while the unsplit grid leaves CUs idle: with many matrices (the small boxes of
-  // the low levels) the partial products and their sum only add workgroups and a launch
+  // the low levels) the partial products and their sum only add workgroups and a launch; and
+  // only when the partial products fit the caller's buffer
   const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N) * batch;
-  if (nsplit <= 1 || ldc != m || tiles >= 512) {
+  if (nsplit <= 1 || ldc != m || tiles >= 512 || batch * nsplit * m * n > part_cap) {
     gemm_h<T>(st, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
     return;
   }
@@ -1708,6 +1726,12 @@ void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T*
                      kchunk, true);
   hipLaunchKernelGGL(sum_split_kernel<T>, dim3((len + 255) / 256, batch), dim3(256), 0, st, len,
                      nsplit, part, C, sC);
+}
+
+int64_t ls_part_elems(int64_t M, int64_t n, int64_t nbo, int64_t kchunk, int64_t kchunk_thin) {
+  const int64_t nsplit = (M + kchunk - 1) / kchunk;
+  const int64_t nsplit_thin = (M + kchunk_thin - 1) / kchunk_thin;
+  return std::max({nsplit * nbo * n, nsplit_thin * int64_t(kQrNb) * n, nsplit_thin * nbo * nbo});
 }
 
 // Sum of squares (for residual norms), deterministic two-phase.
@@ -1748,7 +1772,7 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
                              int64_t, int64_t, const T*, int64_t, int64_t, T, T*, int64_t, int64_t, \
                              int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int);   \
   template void invert_batched<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t,           \
-                                  hipStream_t, T*, bool);                                          \
+                                  hipStream_t, T*, bool, int*);                                    \
   template int64_t invert_ws_elems<T>(int64_t, int64_t);                                           \
   template void symv_upper<T>(hipStream_t, int64_t, T, const T*, int64_t, int64_t, const T*,       \
                               int64_t, T, const T*, int64_t, T*, int64_t, int64_t, T*);            \
@@ -1765,7 +1789,7 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void sumsq<T>(hipStream_t, int64_t, const T*, double*, int);                           \
   template void gemm_tn_splitk<T>(hipStream_t, int64_t, int64_t, int64_t, const T*, int64_t,       \
                                   int64_t, const T*, int64_t, int64_t, T*, int64_t, int64_t,       \
-                                  int64_t, T*, int64_t);                                           \
+                                  int64_t, T*, int64_t, int64_t);                                  \
   template void add2d<T>(hipStream_t, int64_t, int64_t, T, const T*, int64_t, int64_t, T, const T*, \
                          int64_t, int64_t, T*, int64_t, int64_t, int64_t);
 

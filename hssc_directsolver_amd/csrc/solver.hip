@@ -155,6 +155,32 @@ hssc_tree::~hssc_tree() {
 }
 
 namespace hssc {
+int check_params(const hssc_params& p) {
+  if (p.flag_pot != HSSC_POT_SL_L_2D && p.flag_pot != HSSC_POT_SL_H_2D)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown kernel");
+  if (p.flag_pot == HSSC_POT_SL_H_2D && !(p.kh > 0))
+    return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
+  if (!(p.kernel_scale > 0) || !std::isfinite(p.kernel_scale) || !(p.length_scale > 0) ||
+      !std::isfinite(p.length_scale))
+    return fail(HSSC_ERR_INVALID_ARG, "kernel_scale and length_scale must be positive");
+  if (p.flag_pot == HSSC_POT_SL_H_2D && (p.kernel_scale != 1.0 || p.length_scale != 1.0))
+    return fail(HSSC_ERR_UNSUPPORTED, "kernel_scale / length_scale apply to SL_L_2D only");
+  if (p.contrast != HSSC_CONTRAST_PLATEAU && p.contrast != HSSC_CONTRAST_GAUSSIAN)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown contrast");
+  if (p.contrast == HSSC_CONTRAST_GAUSSIAN && (!(p.contrast_amp >= 0) || !(p.contrast_width > 0)))
+    return fail(HSSC_ERR_INVALID_ARG, "Gaussian contrast needs amp >= 0 and width > 0");
+  if (p.tree_precision != HSSC_PREC_FP64 && p.tree_precision != HSSC_PREC_FP32)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown tree precision");
+  return HSSC_OK;
+}
+
+void tree_ref(hssc_tree* t) {
+  if (t) t->refs.fetch_add(1);
+}
+void tree_unref(hssc_tree* t) {
+  if (t && t->refs.fetch_sub(1) == 1) delete t;
+}
+
 void tree_sync(hssc_tree* t) {
   if (!t || t->bstreams.empty()) return;
   for (hipStream_t s : t->bstreams) (void)hipStreamSynchronize(s);
@@ -260,7 +286,9 @@ k::KernelSpec kernel_spec(const hssc_params& p) {
     ks.wre = 1.0 - 0.25 * k2 * p.dr_weights_im;
     ks.wim = 0.25 * k2 * p.dr_weights_re;
   } else {
-    ks.scale = 0.5 * h2;
+    const double L = p.length_scale;
+    ks.scale = 0.5 * h2 * p.kernel_scale * L * L;
+    ks.lshift = 2.0 * std::log(L);
   }
   return ks;
 }
@@ -359,10 +387,12 @@ struct hssc_inverse {
   int prec = HSSC_PREC_FP64;
   std::vector<void*> FI, E;  // per level, nkey x n x n and nkey x k x k (J order)
   mutable hssc::SolveWorks w;
+  explicit hssc_inverse(hssc_tree* t) : tree(t) { hssc::tree_ref(t); }
   ~hssc_inverse() {
     for (void* p : FI) dfree(p);
     for (void* p : E) dfree(p);
     w.release();
+    hssc::tree_unref(tree);
   }
 };
 
@@ -371,9 +401,11 @@ struct hssc_operator {
   // level depth: Bself (nkey x n x n, J order); others: B12 (kc x kc); FP64 real or complex
   std::vector<void*> B;
   mutable hssc::SolveWorks w;
+  explicit hssc_operator(hssc_tree* t) : tree(t) { hssc::tree_ref(t); }
   ~hssc_operator() {
     for (void* p : B) dfree(p);
     w.release();
+    hssc::tree_unref(tree);
   }
 };
 
@@ -451,7 +483,6 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
       Ws& w = ws[l];
       const int64_t sA = M * n;
-      const int64_t nsplit = (M + kchunk - 1) / kchunk;
       // Outer panels of up to 128 columns made of 32-column CholeskyQR3 sub-panels, so the
       // trailing matrix (the HBM-bound part) is streamed once per outer panel.
       const int64_t nbo = std::min<int64_t>(kOuterNb, (kk + nb - 1) / nb * nb);
@@ -460,8 +491,8 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       w.W1 = dalloc<S>(E * nbo * n);
       w.W2 = dalloc<S>(E * nbo * n);
       w.Tf = dalloc<S>(E * nbo * nbo);
-      const int64_t nsplit_thin = (M + kchunk_thin - 1) / kchunk_thin;
-      w.part = dalloc<S>(E * std::max(nsplit * nbo * n, nsplit_thin * nbo * nbo));
+      const int64_t part_cap = E * k::ls_part_elems(M, n, nbo, kchunk, kchunk_thin);
+      w.part = dalloc<S>(part_cap);
       w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
       w.Ts = dalloc<S>(E * kOuterNb * nb);
       w.GV = dalloc<S>(E * nbo * nbo);
@@ -494,7 +525,7 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
             if (nin <= 0) continue;
             S* Cin = A + j1 + (j1 + jb) * M;
             k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
-                                 w.part, kchunk_thin);
+                                 w.part, kchunk_thin, part_cap);
             k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
                          w.W2, jb, jb * nin, E);
             k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
@@ -503,14 +534,14 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
           if (nc <= 0 || merge) return;
           // T of the outer panel from the sub-panel factors and V^H V
           k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
-                               E, w.part, kchunk_thin);
+                               E, w.part, kchunk_thin, part_cap);
           k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
                            nbo * nbo, w.W2, nbo * n, E);
           const int64_t ldt = jbo;
           S* C = A + j0 + (j0 + jbo) * M;
           // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
           k::gemm_tn_splitk<S>(st, jbo, nc, Mp, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
-                               w.part, kchunk);
+                               w.part, kchunk, part_cap);
           k::gemm_h<S>(st, jbo, nc, jbo, S(1), w.Tf, ldt, nbo * nbo, w.W1, jbo, jbo * nc, S(0),
                        w.W2, jbo, jbo * nc, E);
           k::gemm<S>(st, false, false, Mp, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
@@ -606,6 +637,8 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   }
   T* Wk = dalloc<T>(wmax);
   T* Wg = gmax > 0 ? dalloc<T>(gmax) : nullptr;  // blocked Gauss-Jordan workspace
+  int* dflag = dalloc<int>(1);  // set by the sweeps on a singular diagonal block
+  HIPCHK(hipMemsetAsync(dflag, 0, sizeof(int), st));
   // second stream for the look-ahead of the blocked Gauss-Jordan sweeps
   hipStream_t aux = nullptr;
   HIPCHK(hipGetLastError());
@@ -628,7 +661,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       k::gen_parent<T>(st, n, kc, E, L.dJ, TI ? 0 : n, L.dPx, L.dPy, L.dPw, TI ? 0 : n,
                        static_cast<const T*>(inv->E[l + 1]), sEbox, sEpar, FI, n * n, ks);
     }
-    k::invert_batched<T>(st, n, FI, n, n * n, E, aux, Wg, true);  // F is symmetric
+    k::invert_batched<T>(st, n, FI, n, n * n, E, aux, Wg, true, dflag);  // F is symmetric
     if (l > 0) {
       // EI = Phi_ss + T Phi_rs + (Phi_sr + T Phi_rr) T^T ;  E = inv(EI)
       T* EI = dalloc<T>(E * kk * kk);
@@ -656,7 +689,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
                         kk, kk * kk, E, tk, tk, big, 0, big, 0, 1);
         k::sym_lower_fill<T>(st, kk, EI, kk, kk * kk, E);
       }
-      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg, true);  // EI = R F^-1 R^T
+      k::invert_batched<T>(st, kk, EI, kk, kk * kk, E, aux, Wg, true, dflag);  // EI = R F^-1 R^T
     }
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -666,9 +699,15 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
     (void)hipStreamDestroy(aux);
   }
   HIPCHK(hipGetLastError());
+  int singular = 0;
+  HIPCHK(hipMemcpy(&singular, dflag, sizeof(int), hipMemcpyDeviceToHost));
   dfree(Wk);
   dfree(Wg);
+  dfree(dflag);
   tree_sync(t);  // every level's T has been waited for: release the build's resources
+  if (singular)
+    throw HipError{fail(HSSC_ERR_SINGULAR,
+                        "singular or non-finite pivot in the inversion of F or R F^-1 R^T")};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -980,6 +1019,12 @@ int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_
   p->interform = HSSC_INTER_LOWRANK;
   p->k_cut = 2500;
   p->factor_precision = HSSC_PREC_FP64;
+  p->tree_precision = HSSC_PREC_FP64;
+  p->contrast = HSSC_CONTRAST_PLATEAU;
+  p->contrast_amp = 0.5;
+  p->contrast_width = 0.15;
+  p->kernel_scale = 1.0;
+  p->length_scale = 1.0;
   if (kh > 0) {
     // :167-175, params.order = 4: dr_weights = corr0(h, kh)
     if (int s = hssc_dr_corr0(p->h, kh, &p->dr_weights_re, &p->dr_weights_im)) return s;
@@ -1009,15 +1054,16 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
                      int64_t n, double* A, int64_t lda) {
   if (!p || !X1 || !X2 || !A || m < 0 || n < 0 || lda < m)
     return fail(HSSC_ERR_INVALID_ARG, "invalid kernel_eval arguments");
-  if (p->flag_pot == HSSC_POT_SL_H_2D && !(p->kh > 0))
-    return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
+  if (int s = check_params(*p)) return s;
   if (int s = check_device()) return s;
   const bool cx = p->flag_pot == HSSC_POT_SL_H_2D;
   const size_t es = cx ? sizeof(zdouble) : sizeof(double);
   return guarded([&]() {
     std::vector<double> w1(m), w2(n), x1(m), y1(m), x2(n), y2(n);
-    for (int64_t i = 0; i < m; ++i) x1[i] = X1[i], y1[i] = X1[m + i], w1[i] = point_weight(*p, x1[i], y1[i]);
-    for (int64_t j = 0; j < n; ++j) x2[j] = X2[j], y2[j] = X2[n + j], w2[j] = point_weight(*p, x2[j], y2[j]);
+    for (int64_t i = 0; i < m; ++i)
+      x1[i] = X1[i], y1[i] = X1[m + i], w1[i] = point_weight(*p, x1[i], y1[i], true);
+    for (int64_t j = 0; j < n; ++j)
+      x2[j] = X2[j], y2[j] = X2[n + j], w2[j] = point_weight(*p, x2[j], y2[j], false);
     double *dx1 = dupload(x1), *dy1 = dupload(y1), *dw1 = dupload(w1);
     double *dx2 = dupload(x2), *dy2 = dupload(y2), *dw2 = dupload(w2);
     void* dA = dalloc<zdouble>(std::max<int64_t>(1, m * n));
@@ -1039,8 +1085,7 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
 int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
-  if (p->flag_pot == HSSC_POT_SL_H_2D && !(p->kh > 0))
-    return fail(HSSC_ERR_INVALID_ARG, "SL_H_2D needs kh > 0");
+  if (int s = check_params(*p)) return s;
   return guarded([&]() {
     auto* t = new hssc_tree();
     const int s = build_index_sets(*p, t, id_select_gpu);
@@ -1170,7 +1215,9 @@ int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out) {
   });
 }
 
-void hssc_tree_destroy(hssc_tree* t) { delete t; }
+void hssc_tree_destroy(hssc_tree* t) {
+  hssc::tree_unref(t);  // drops the caller's reference; the last inverse or operator frees it
+}
 
 int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss) {
   // HSS2D/levHSS.m:33-65: first level whose skeletons are all <= n_cut (TI: the level k).
@@ -1194,8 +1241,7 @@ int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** out) {
   if (precision != HSSC_PREC_FP64 && precision != HSSC_PREC_FP32)
     return fail(HSSC_ERR_INVALID_ARG, "unknown precision");
   return guarded([&]() {
-    auto* inv = new hssc_inverse();
-    inv->tree = t;
+    auto* inv = new hssc_inverse(t);
     inv->prec = precision;
     try {
       OwnStream os;
@@ -1269,8 +1315,7 @@ int hssc_hss2d_compress(hssc_tree* t, hssc_operator** out) {
   *out = nullptr;
   if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
   return guarded([&]() {
-    auto* op = new hssc_operator();
-    op->tree = t;
+    auto* op = new hssc_operator(t);
     const int D = t->depth;
     op->B.assign(D + 1, nullptr);
     const k::KernelSpec ks = kernel_spec(t->p);

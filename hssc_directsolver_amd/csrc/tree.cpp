@@ -42,13 +42,25 @@ double bump_factor(double h, double x) {
 }
 }  // namespace
 
-double point_weight(const hssc_params& p, double x, double y) {
-  // Common/Kernel_Eval.m:111-159 for params.proxy == 0 and a symmetric kernel:
-  // TI -> 1; NTI Laplace -> b_func(.,0.3,0.6,0.25) (:135,:146); NTI Helmholtz ->
-  // sqrt(Bump_function(.,4,h)) (:133,:143).
+double point_weight(const hssc_params& p, double x, double y, bool row) {
+  // Common/Kernel_Eval.m:111-159 (LOCAL_get_bc) for params.proxy == 0: b on the row points,
+  // c on the column points.
+  //   sym = 0, kh > 0 (:115-128): b = Bump_function(., 4) with h = 0, c = 1;
+  //   TI (:155-158): 1;
+  //   NTI Helmholtz (:133,:143): sqrt(Bump_function(., 4, h)), or sqrt of the Gaussian
+  //     contrast option;
+  //   NTI Laplace (:135,:146-150): b = b_func(., 0.3, 0.6, 0.25); c the same when sym,
+  //     b_func(., 0.2, -0.6, 0.25) otherwise.
+  if (p.sym == 0 && p.kh > 0) return row ? bump_factor(0.0, x) * bump_factor(0.0, y) : 1.0;
   if (p.transinv) return 1.0;
-  if (p.kh > 0) return std::sqrt(bump_factor(p.h, x) * bump_factor(p.h, y));
-  const double d1 = x - 0.3, d2 = y - 0.6;
+  if (p.kh > 0) {
+    if (p.contrast == HSSC_CONTRAST_GAUSSIAN)
+      return std::sqrt(p.contrast_amp *
+                       std::exp(-(x * x + y * y) / (2.0 * p.contrast_width * p.contrast_width)));
+    return std::sqrt(bump_factor(p.h, x) * bump_factor(p.h, y));
+  }
+  const bool second = !row && !p.sym;
+  const double d1 = x - (second ? 0.2 : 0.3), d2 = y - (second ? -0.6 : 0.6);
   return 1.0 + 0.25 * std::exp(-d1 * d1 - d2 * d2);
 }
 
@@ -61,7 +73,9 @@ namespace {
 class WeightTable {
  public:
   explicit WeightTable(const hssc_params& p) : p_(p) {
-    if (p.transinv || !(p.kh > 0) || !(p.h > 0)) return;
+    if (p.transinv || !(p.kh > 0) || !(p.h > 0) || p.contrast != HSSC_CONTRAST_PLATEAU ||
+        !p.sym)
+      return;
     step_ = 0.5 * p.h;
     K_ = int64_t(std::ceil(4.0 / step_));
     if (K_ > (int64_t(1) << 22)) return;

@@ -157,6 +157,8 @@ def HSS1D_solve(K_HSS: HSS1D, f, tol=1e-10, maxit=10):
     """Iterative refinement (Common/Iterative_Refinement.m:20-35): FP32 factors (FP64 if
     only those exist) against the FP64 compressed operator.  Returns (x, solves, relres)."""
     f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    if f.shape[0] != K_HSS.N:
+        raise ValueError(f"right-hand side must have N = {K_HSS.N} entries, got {f.shape[0]}")
     x = np.empty_like(f)
     it, rel = C.c_int32(), C.c_double()
     check(lib().hssc_hss1d_solve(K_HSS.handle, f.ctypes.data_as(_DP), x.ctypes.data_as(_DP),
@@ -166,6 +168,8 @@ def HSS1D_solve(K_HSS: HSS1D, f, tol=1e-10, maxit=10):
 
 def _vec(fn, K_HSS, v):
     v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    if v.shape[0] != K_HSS.N:
+        raise ValueError(f"vector must have N = {K_HSS.N} entries, got {v.shape[0]}")
     out = np.empty_like(v)
     check(fn(K_HSS.handle, v.ctypes.data_as(_DP), out.ctypes.data_as(_DP), 1))
     return out

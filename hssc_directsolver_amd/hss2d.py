@@ -10,10 +10,17 @@ Function names and argument meaning follow HSSC_DirectSolver (DOCUMENTATION_HSS2
     Y         = HSS2D_apply_TI(INFO_TREE, M_HSS, X, params)     # or _fsym
     lHSS      = levHSS(INFO_TREE, n_cut, trinv)
 
-All numerical work runs on the GPU through the C ABI; X and Y are numpy arrays
-(N x m, X_source ordering) or torch tensors already on the GPU (the *_device helpers):
-float64 for 'SL_L_2D', complex128 for the Helmholtz kernel 'SL_H_2D'.
+All numerical work runs on the GPU through the C ABI.  Right-hand sides and solutions are
+N x m column-major blocks in the X_source ordering (one column per right-hand side), as the
+C ABI takes them: numpy arrays of shape (N,) or (N, m) for the host functions, and
+contiguous torch tensors on the GPU of shape (N,) or (m, N) (each row one right-hand side,
+which is the same N x m column-major memory) for the *_device helpers; float64 for
+'SL_L_2D', complex128 for the Helmholtz kernel 'SL_H_2D'.  Shapes, dtypes, devices and
+contiguity are checked before any pointer reaches the library.
 Errors raise HsscError (the reference raises MATLAB runtime errors instead).
+
+Handles keep what they read alive: an Inverse or Operator holds a reference to its Tree,
+whose device data they use until they are destroyed.
 """
 from __future__ import annotations
 
@@ -21,7 +28,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import PREC, POT, HsscParams, check, lib
+from ._lib import CONTRAST, PREC, POT, HsscParams, check, lib
 
 _DP = C.POINTER(C.c_double)
 
@@ -61,13 +68,18 @@ class Params:
 
 
 def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
-                        proxy_layers=None, preset_all=None) -> Params:
+                        proxy_layers=None, preset_all=None, contrast=None, kernel_scale=None,
+                        length_scale=None, tree_precision=None) -> Params:
     """Common/HSS_tree_parameters.m:1.
 
     For kh > 0 the Duan-Rokhlin weight corr0 is computed by the library (``corr0``
     overrides it); ``proxy_layers`` replaces the proxy-ring depth rule at lev > 0;
     ``preset_all=True`` keeps the 'bdry' skeleton rule at the two bottom levels of a kh > 0
-    tree, where the reference selects skeletons by an ID."""
+    tree, where the reference selects skeletons by an ID.  Options without a reference
+    counterpart: ``contrast=("gaussian", amp, width)`` (kh > 0, NTI) instead of the tanh
+    plateau; ``kernel_scale`` s and ``length_scale`` L (SL_L_2D) for the kernel
+    s (L h)^2 log(L r); ``tree_precision="fp32"`` for FP32 interpolation operators (a
+    preconditioner, refined against the exact grid operator)."""
     raw = HsscParams()
     check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
                                  float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
@@ -79,6 +91,19 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
         raw.proxy_layers = int(proxy_layers)
     if preset_all is not None:
         raw.preset_all = int(bool(preset_all))
+    if contrast is not None:
+        if contrast == "plateau":
+            raw.contrast = CONTRAST["plateau"]
+        else:
+            kind, amp, width = contrast
+            raw.contrast = CONTRAST[kind]
+            raw.contrast_amp, raw.contrast_width = float(amp), float(width)
+    if kernel_scale is not None:
+        raw.kernel_scale = float(kernel_scale)
+    if length_scale is not None:
+        raw.length_scale = float(length_scale)
+    if tree_precision is not None:
+        raw.tree_precision = PREC[tree_precision]
     return Params(raw)
 
 
@@ -238,9 +263,12 @@ def HSS2D_compress_fsym(tree: Tree, params: Params = None) -> Operator:
     return Operator(h, tree)
 
 
-def _host_call(fn, handle, X, dtype):
+def _host_call(fn, handle, X, dtype, N):
     X = np.asarray(X, dtype=dtype)
     shape = X.shape
+    if X.ndim not in (1, 2) or shape[0] != N or X.size == 0:
+        raise ValueError(f"right-hand side must have shape (N,) or (N, m) with N = {N}, "
+                         f"got {shape}")
     Xf = np.asfortranarray(X.reshape(shape[0], -1))
     Y = np.empty_like(Xf, order="F")
     check(fn(handle, _dptr(Xf), _dptr(Y), Xf.shape[1]))
@@ -249,7 +277,8 @@ def _host_call(fn, handle, X, dtype):
 
 def HSS2D_applyinv_TI(tree: Tree, A_HSS: Inverse, X, params: Params = None):
     """HSS2D/HSS2D_applyinv_TI.m:1."""
-    return _host_call(lib().hssc_hss2d_applyinv, A_HSS.handle, X, A_HSS.tree.params.dtype)
+    return _host_call(lib().hssc_hss2d_applyinv, A_HSS.handle, X, A_HSS.tree.params.dtype,
+                      A_HSS.tree.params.N)
 
 
 HSS2D_applyinv_fsym = HSS2D_applyinv_TI  # HSS2D/HSS2D_applyinv_fsym.m:1 (same C entry point)
@@ -257,35 +286,62 @@ HSS2D_applyinv_fsym = HSS2D_applyinv_TI  # HSS2D/HSS2D_applyinv_fsym.m:1 (same C
 
 def HSS2D_apply_TI(tree: Tree, M_HSS: Operator, X, params: Params = None):
     """HSS2D/HSS2D_apply_TI.m:1."""
-    return _host_call(lib().hssc_hss2d_apply, M_HSS.handle, X, M_HSS.tree.params.dtype)
+    return _host_call(lib().hssc_hss2d_apply, M_HSS.handle, X, M_HSS.tree.params.dtype,
+                      M_HSS.tree.params.N)
 
 
 HSS2D_apply_fsym = HSS2D_apply_TI  # HSS2D/HSS2D_apply_fsym.m:1
 
 
 # ---- device-resident variants (torch tensors on the GPU, float64 or complex128 for the
-# Helmholtz kernel, N x m column-major) ----
+# Helmholtz kernel, shape (N,) or (m, N): N x m column-major) ----
 
 def _stream_ptr(stream):
     return None if stream is None else C.c_void_p(stream.cuda_stream)
 
 
+def _device_rhs(params: Params, *tensors):
+    """Checks that every tensor is a contiguous GPU tensor of the tree's dtype holding N x m
+    entries (shape (N,) or (m, N)), all of the same shape; returns m."""
+    import torch
+    want = torch.complex128 if params.is_complex else torch.float64
+    N = params.N
+    shape = tensors[0].shape
+    for X in tensors:
+        if not X.is_cuda:
+            raise ValueError("device variants take GPU tensors")
+        if X.dtype != want:
+            raise ValueError(f"expected dtype {want}, got {X.dtype}")
+        if not X.is_contiguous():
+            raise ValueError("device variants take contiguous tensors")
+        if X.shape != shape:
+            raise ValueError(f"shape mismatch: {tuple(X.shape)} vs {tuple(shape)}")
+    if len(shape) == 1 and shape[0] == N:
+        return 1
+    if len(shape) == 2 and shape[1] == N and shape[0] >= 1:
+        return int(shape[0])
+    raise ValueError(f"expected shape (N,) or (m, N) with N = {N}, got {tuple(shape)}")
+
+
 def applyinv_device(A_HSS: Inverse, X, Y, stream=None):
-    """Y = A^-1 X with X, Y torch tensors of shape (m, N) (each row one RHS)."""
-    m = X.shape[0] if X.dim() == 2 else 1
+    """Y = A^-1 X with X, Y torch tensors of shape (N,) or (m, N) (each row one RHS)."""
+    m = _device_rhs(A_HSS.tree.params, X, Y)
     check(lib().hssc_hss2d_applyinv_device(A_HSS.handle, C.c_void_p(X.data_ptr()),
                                            C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
 
 
 def apply_device(M_HSS: Operator, X, Y, stream=None):
-    m = X.shape[0] if X.dim() == 2 else 1
+    """Y = A_HSS X with X, Y torch tensors of shape (N,) or (m, N)."""
+    m = _device_rhs(M_HSS.tree.params, X, Y)
     check(lib().hssc_hss2d_apply_device(M_HSS.handle, C.c_void_p(X.data_ptr()),
                                         C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
 
 
 def solve_device(A_HSS: Inverse, M_HSS: Operator, B, Y, tol=1e-10, maxit=10, stream=None):
     """Common/Iterative_Refinement.m on the GPU; returns (iterations, relative residual)."""
-    m = B.shape[0] if B.dim() == 2 else 1
+    if A_HSS.tree is not M_HSS.tree:
+        raise ValueError("the inverse and the operator must come from the same tree")
+    m = _device_rhs(A_HSS.tree.params, B, Y)
     it = C.c_int32()
     rel = C.c_double()
     check(lib().hssc_hss2d_solve_device(A_HSS.handle, M_HSS.handle, C.c_void_p(B.data_ptr()),

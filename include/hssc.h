@@ -67,8 +67,26 @@ typedef struct hssc_params {
   int32_t preset_all;  /* kh > 0: 0 = skeletons of the two bottom levels chosen by an ID of
                           K(X_proxy, X_src) (HSS2D_build_tree_fsym.m:376, :686-729), as the
                           reference; 1 = the 'bdry' rule at every level (uniform shapes) */
-  int32_t reserved[5];
+  /* Options without a reference counterpart (the benchmark configurations' literal forms;
+     hssc_params_init sets the reference's choices): */
+  int32_t contrast;     /* kh > 0, NTI, sym: HSSC_CONTRAST_PLATEAU = b = c = sqrt of the tanh
+                           plateau (Bump_function option 4, Common/Kernel_Eval.m:133,143);
+                           HSSC_CONTRAST_GAUSSIAN = b = c = sqrt(q),
+                           q(x) = contrast_amp exp(-|x|^2 / (2 contrast_width^2)) */
+  double contrast_amp;
+  double contrast_width;
+  double kernel_scale;  /* SL_L_2D: factor s of the kernel s log r (1 = the reference's
+                           h^2 log r, Kernel_Eval.m:61; 1/(2 pi) = (1/2 pi) log r) */
+  double length_scale;  /* SL_L_2D: physical length L of the reference's unit: spacing L h and
+                           distances L r (1 = [-1,1]^2 as the reference; 1/2 = [0,1]^2) */
+  int32_t tree_precision; /* HSSC_PREC_* of the interpolation operators T (no reference
+                           counterpart: GPU policy); FP32 T serves only as a preconditioner,
+                           with residuals taken against the exact operator (hssc_grid_op) */
+  int32_t reserved[3];
 } hssc_params;
+
+#define HSSC_CONTRAST_PLATEAU 0
+#define HSSC_CONTRAST_GAUSSIAN 1
 
 typedef struct hssc_tree hssc_tree;         /* INFO_TREE + params.X_source            */
 typedef struct hssc_inverse hssc_inverse;   /* A_HSS of HSS2D_invert_* (FI, E, JE)    */
@@ -149,6 +167,9 @@ int hssc_tree_get_proxy(const hssc_tree* t, int64_t box, double* out);
  * matrix (T = T.U * T.V.' in the reference's low-rank form); interleaved complex doubles
  * for SL_H_2D. */
 int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out);
+/* Releases the caller's handle.  Inverses and operators made from the tree read its device
+ * data and keep it alive: the tree's memory is freed when the handle and every such inverse
+ * and operator have been destroyed, in any order. */
 void hssc_tree_destroy(hssc_tree* t);
 
 /* Replaces HSS2D/levHSS.m:1 (lHSS = levHSS(TREE,n_cut,trinv)). */

@@ -103,12 +103,21 @@ def bump_function(x, y, h):
     return f1 * f2
 
 
+def gaussian_contrast(x, y, amp, width):
+    """The Gaussian contrast of the Lippmann-Schwinger configuration (no reference
+    counterpart: Kernel_Eval.m:133,143 hard-wire Bump_function option 4):
+    q = amp * exp(-|x|^2 / (2 width^2))."""
+    return amp * np.exp(-(x * x + y * y) / (2.0 * width * width))
+
+
 def point_weights(X, params, role):
     """Per-point factor b(x) (role 'row') or c(y) (role 'col').
 
     Restates Common/Kernel_Eval.m:111-159 (LOCAL_get_bc) for params.proxy == 0.  The
     reference evaluates b on the row points X1 and c on the column points X2; both depend
-    only on their own point, so they are per-point vectors.
+    only on their own point, so they are per-point vectors.  ``params["contrast"] =
+    ("gaussian", amp, width)`` replaces the tanh plateau of the symmetric kh > 0 case by
+    sqrt(gaussian_contrast) (the configuration's contrast; the reference has no such option).
     """
     x, y = X[:, 0], X[:, 1]
     kh, h = params["kh"], params["h"]
@@ -118,6 +127,9 @@ def point_weights(X, params, role):
         return np.ones_like(x)
     if params["transinv"] == 0:
         if kh > 0:
+            con = params.get("contrast")
+            if con is not None and con[0] == "gaussian":
+                return np.sqrt(gaussian_contrast(x, y, con[1], con[2]))
             return np.sqrt(bump_function(x, y, h))
         if role == "row" or params["sym"]:
             return b_func(x, y, 0.3, 0.6, 0.25)
@@ -131,6 +143,12 @@ def kernel_eval(X1, X2, params):
     SL_L_2D: A = (den==0) + h^2 * 0.5 * b .* log(den + (den==0)) .* c    (:60-61)
     SL_H_2D: A = w*(den==0) + h^2 k^2 (i/4) b .* (H0(k sqrt(den+(den==0))) - H0(k)(den==0)) .* c
              with w = 1 + k^2 (i/4) corr0                                  (:44-49)
+
+    SL_L_2D options without a reference counterpart (the configuration's literal kernel):
+    ``kernel_scale`` s multiplies the kernel log r (1: the reference's h^2 log r; 1/(2 pi):
+    (1/2 pi) log r) and ``length_scale`` L makes the grid's physical spacing L h and
+    distances L r, so the off-diagonal entries become s (L h)^2 log(L r); L = 1/2 maps the
+    reference's [-1,1]^2 onto [0,1]^2.
     """
     from scipy.special import hankel1
 
@@ -143,7 +161,12 @@ def kernel_eval(X1, X2, params):
     z = den == 0
     pot = params["flag_pot"]
     if pot == "SL_L_2D":
-        return z + (h * h) * 0.5 * b[:, None] * np.log(den + z) * c[None, :]
+        s = params.get("kernel_scale", 1.0)
+        L = params.get("length_scale", 1.0)
+        if s == 1.0 and L == 1.0:
+            return z + (h * h) * 0.5 * b[:, None] * np.log(den + z) * c[None, :]
+        logr = 0.5 * np.log(den + z) + np.log(L) * (~z)
+        return z + s * (L * h) ** 2 * b[:, None] * logr * c[None, :]
     if pot == "SL_H_2D":
         kh = params["kh"]
         C = hankel1(0, kh)
@@ -527,6 +550,23 @@ class Tree:
 
 def level_of(b):
     return int(b).bit_length() - 1
+
+
+def levhss(tree, n_cut, trinv):
+    """HSS2D/levHSS.m:33-65: walk down from the root until the smallest skeleton of a level
+    is at most n_cut (never past depth - 1); return that level rounded down to an even one,
+    or -1 if the root's level already qualifies."""
+    lev = 0
+    if trinv == 0:
+        n_min = max(tree.k[tree.key(b)] for b in (1, 2, 3))
+    else:
+        n_min = tree.k[tree.key(1)]
+    while n_min > n_cut and lev < tree.depth - 1:
+        lev += 1
+        n_min = min(tree.k[tree.key(b)] for b in range(2 ** lev, 2 ** (lev + 1)))
+    if lev > 0:
+        return lev - 1 if lev % 2 == 1 else lev
+    return -1
 
 
 def build_index_sets(params):
