@@ -61,6 +61,22 @@ void free_all_locked(Device& d) {
   d.cached = 0;
 }
 
+// HSSC_POOL_POISON=<byte> (diagnostics): every block handed out is first filled with that
+// byte, e.g. 0x4E (about 1.6e70 as a double, 8.7e8 as a float), so that a kernel which reads
+// device memory it has not written gives a result that differs from a run without it.
+int poison_byte() {
+  static const int v = [] {
+    const char* e = std::getenv("HSSC_POOL_POISON");
+    return e ? int(std::strtol(e, nullptr, 0)) & 0xff : -1;
+  }();
+  return v;
+}
+
+void poison(void* p, size_t s) {
+  const int v = poison_byte();
+  if (v >= 0) (void)hipMemset(p, v, s);
+}
+
 bool take_locked(Device& d, size_t s, void** p) {
   auto it = d.ready.find(s);
   if (it == d.ready.end()) return false;
@@ -68,6 +84,7 @@ bool take_locked(Device& d, size_t s, void** p) {
   d.ready.erase(it);
   d.cached -= s;
   d.live[*p] = s;
+  poison(*p, s);
   return true;
 }
 
@@ -96,6 +113,7 @@ hipError_t alloc(void** p, size_t bytes) {
     return e;
   }
   d.live[*p] = s;
+  poison(*p, s);
   return hipSuccess;
 }
 
