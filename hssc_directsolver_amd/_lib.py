@@ -66,6 +66,7 @@ SIGNATURES = [
     ("hssc_hss2d_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),
     ("hssc_hss2d_applyinv_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("hssc_hss2d_apply_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("hssc_hss2d_applyinv_device_noalloc", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("hssc_hss2d_solve_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int32,
                                           C.POINTER(C.c_int32), _DP, _P]),
     ("hssc_id", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int64, C.c_double, _I64P, _DP, _I64P]),

@@ -17,6 +17,7 @@
 #include <functional>
 #include <type_traits>
 #include <new>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -200,41 +201,82 @@ using hi_t = typename std::conditional<is_complex_v<T>, zdouble, double>::type;
 template <typename T>
 using lo_t = typename std::conditional<is_complex_v<T>, zfloat, float>::type;
 
-// Per-RHS-count solve workspace (levels x {Q, PU, PD}) plus scratch.
+// Per-RHS-count solve workspace (levels x {Q, PU, PD}) plus scratch.  It either owns its
+// buffers (ensure: pool allocations kept for the next call with the same m) or lays them out
+// in memory the caller provides (borrow: nothing is allocated or freed).
 template <typename T>
 struct SolveWork {
   int64_t m = 0;
+  bool borrowed = false;
   std::vector<T*> Q, PU, PD;
   T* S1 = nullptr;
   T* S2 = nullptr;
   T* work = nullptr;
   void release() {
-    for (T* p : Q) dfree(p);
-    for (T* p : PU) dfree(p);
-    for (T* p : PD) dfree(p);
+    if (!borrowed) {
+      for (T* p : Q) dfree(p);
+      for (T* p : PU) dfree(p);
+      for (T* p : PD) dfree(p);
+      dfree(S1), dfree(S2), dfree(work);
+    }
     Q.clear(), PU.clear(), PD.clear();
-    dfree(S1), dfree(S2), dfree(work);
     S1 = S2 = work = nullptr;
     m = 0;
+    borrowed = false;
+  }
+  // Visits every buffer of the workspace for mm right-hand sides with its element count.
+  template <typename F>
+  static void layout(const hssc_tree* t, int64_t mm, F&& f) {
+    const int D = t->depth;
+    int64_t smax = 0, wmax = 0;
+    for (int l = 0; l <= D; ++l) {
+      const Level& L = t->levels[l];
+      f(0, l, L.nbox * L.n * mm);
+      f(1, l, L.nbox * L.k * mm);
+      f(2, l, L.nbox * L.k * mm);
+      smax = std::max(smax, L.nbox * L.n * mm);
+      wmax = std::max(wmax, gemv_work_elems(L.n, L.n, L.nbox));
+      wmax = std::max(wmax, k::symv_work_elems(L.n, L.nbox));
+    }
+    f(3, 0, smax);
+    f(4, 0, smax);
+    f(5, 0, wmax);
+  }
+  void assign(int which, int l, T* p) {
+    if (which == 0) Q[l] = p;
+    else if (which == 1) PU[l] = p;
+    else if (which == 2) PD[l] = p;
+    else if (which == 3) S1 = p;
+    else if (which == 4) S2 = p;
+    else work = p;
   }
   void ensure(const hssc_tree* t, int64_t mm) {
     if (m == mm) return;
     release();
     const int D = t->depth;
     Q.assign(D + 1, nullptr), PU.assign(D + 1, nullptr), PD.assign(D + 1, nullptr);
-    int64_t smax = 0, wmax = 0;
-    for (int l = 0; l <= D; ++l) {
-      const Level& L = t->levels[l];
-      Q[l] = dalloc<T>(L.nbox * L.n * mm);
-      PU[l] = dalloc<T>(L.nbox * L.k * mm);
-      PD[l] = dalloc<T>(L.nbox * L.k * mm);
-      smax = std::max(smax, L.nbox * L.n * mm);
-      wmax = std::max(wmax, gemv_work_elems(L.n, L.n, L.nbox));
-      wmax = std::max(wmax, k::symv_work_elems(L.n, L.nbox));
-    }
-    S1 = dalloc<T>(smax);
-    S2 = dalloc<T>(smax);
-    work = dalloc<T>(wmax);
+    layout(t, mm, [&](int which, int l, int64_t cnt) { assign(which, l, dalloc<T>(cnt)); });
+    m = mm;
+  }
+  // Bytes borrow() needs for mm right-hand sides (every buffer 256-byte aligned).
+  static int64_t borrow_bytes(const hssc_tree* t, int64_t mm) {
+    int64_t total = 0;
+    layout(t, mm, [&](int, int, int64_t cnt) {
+      total += (std::max<int64_t>(cnt, 1) * int64_t(sizeof(T)) + 255) / 256 * 256;
+    });
+    return total;
+  }
+  // Lays the workspace out in [base, base + borrow_bytes) (base 256-byte aligned).
+  void borrow(const hssc_tree* t, int64_t mm, void* base) {
+    release();
+    const int D = t->depth;
+    Q.assign(D + 1, nullptr), PU.assign(D + 1, nullptr), PD.assign(D + 1, nullptr);
+    char* cur = static_cast<char*>(base);
+    layout(t, mm, [&](int which, int l, int64_t cnt) {
+      assign(which, l, reinterpret_cast<T*>(cur));
+      cur += (std::max<int64_t>(cnt, 1) * int64_t(sizeof(T)) + 255) / 256 * 256;
+    });
+    borrowed = true;
     m = mm;
   }
 };
@@ -939,6 +981,22 @@ void applyinv_dispatch(const hssc_inverse* inv, const void* X, void* Y, int64_t 
   }
 }
 
+// applyinv with the solve workspace laid out in caller-provided device memory of cap bytes.
+template <typename T>
+int applyinv_borrowed(const hssc_inverse* inv, const void* X, void* Y, int64_t m, hipStream_t st,
+                      void* base, int64_t cap) {
+  const int64_t need = SolveWork<T>::borrow_bytes(inv->tree, m);
+  if (need > cap)
+    return fail(HSSC_ERR_INVALID_ARG,
+                "the solve workspace of this tree and m needs " + std::to_string(need) +
+                    " bytes; the fixed buffer holds " + std::to_string(cap));
+  SolveWork<T> w;
+  w.borrow(inv->tree, m, base);
+  applyinv_impl<T>(inv, w, static_cast<const hi_t<T>*>(X), static_cast<hi_t<T>*>(Y), m, st);
+  w.release();  // borrowed: nothing is freed
+  return HSSC_OK;
+}
+
 // Common/Iterative_Refinement.m:20-35 with A = the FP64 compressed operator.
 template <typename S>
 int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* Y, int64_t m,
@@ -967,6 +1025,37 @@ int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* 
   return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
 }
 
+}  // namespace
+}  // namespace hssc
+
+// ---------------------------------------------------------------------------------------
+// Fixed device workspace of hssc_hss2d_applyinv_device_noalloc
+// ---------------------------------------------------------------------------------------
+// Every GPU the library runs on holds its own copy of this array from the module load until
+// the process ends.  In host code its name only stands in for the symbol: the device address
+// comes from hipGetSymbolAddress, looked up once per device with that device current.
+static constexpr int64_t kNoallocScratchBytes = int64_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace hssc {
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal
+
+// Device address of the current device's scratch array (looked up on its first use there).
+int scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (int64_t(g_scratch_ptr.size()) <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(HSSC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HSSC_OK;
+}
 }  // namespace
 }  // namespace hssc
 
@@ -1366,6 +1455,33 @@ int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY
     applyinv_dispatch(inv, dX, dY, m, st);
     HIPCHK(hipGetLastError());
     return HSSC_OK;
+  });
+}
+
+int hssc_hss2d_applyinv_device_noalloc(const hssc_inverse* inv, const void* dX, void* dY,
+                                       int64_t m, void* stream) {
+  if (!inv || !dX || !dY || m < 1) return fail(HSSC_ERR_INVALID_ARG, "invalid applyinv arguments");
+  const hssc_tree* t = inv->tree;
+  // the inversion has waited for the tree's build; nothing is left to release here
+  if (!t->bstreams.empty())
+    return fail(HSSC_ERR_INVALID_ARG, "the tree's interpolation operators are still pending");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != t->device)
+    return fail(HSSC_ERR_INVALID_ARG, "the current device is not the inverse's device");
+  void* base = nullptr;
+  if (int s = scratch_address(dev, &base)) return s;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool fp64 = inv->prec == HSSC_PREC_FP64;
+  return guarded([&]() {
+    int s;
+    if (t->cx)
+      s = fp64 ? applyinv_borrowed<zdouble>(inv, dX, dY, m, st, base, kNoallocScratchBytes)
+               : applyinv_borrowed<zfloat>(inv, dX, dY, m, st, base, kNoallocScratchBytes);
+    else
+      s = fp64 ? applyinv_borrowed<double>(inv, dX, dY, m, st, base, kNoallocScratchBytes)
+               : applyinv_borrowed<float>(inv, dX, dY, m, st, base, kNoallocScratchBytes);
+    if (s == HSSC_OK) HIPCHK(hipGetLastError());
+    return s;
   });
 }
 

@@ -330,6 +330,15 @@ def applyinv_device(A_HSS: Inverse, X, Y, stream=None):
                                            C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
 
 
+def applyinv_device_noalloc(A_HSS: Inverse, X, Y, stream=None):
+    """applyinv_device with the library's fixed device workspace (no allocation; calls that
+    overlap on one GPU race)."""
+    m = _device_rhs(A_HSS.tree.params, X, Y)
+    check(lib().hssc_hss2d_applyinv_device_noalloc(A_HSS.handle, C.c_void_p(X.data_ptr()),
+                                                   C.c_void_p(Y.data_ptr()), m,
+                                                   _stream_ptr(stream)))
+
+
 def apply_device(M_HSS: Operator, X, Y, stream=None):
     """Y = A_HSS X with X, Y torch tensors of shape (N,) or (m, N)."""
     m = _device_rhs(M_HSS.tree.params, X, Y)

@@ -203,6 +203,20 @@ int hssc_hss2d_applyinv_device(const hssc_inverse* inv, const void* dX, void* dY
 int hssc_hss2d_apply_device(const hssc_operator* op, const void* dX, void* dY, int64_t m,
                             void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * result as hssc_hss2d_applyinv_device, with the solve workspace (the per-level vectors of
+ * the two sweeps) laid out in a fixed device buffer of 128 MiB that every GPU the library runs
+ * on holds until the process ends, instead of library allocations; it calls no allocator.
+ * HSSC_ERR_INVALID_ARG if the workspace of the tree and m does not fit (the message gives both
+ * sizes; the 512^2 Helmholtz tree with m = 1 needs 80 MB in complex double, 40 MB in
+ * complex float), if the current device is not
+ * the inverse's, or if the tree's build is still pending.  Every call on a GPU uses that GPU's
+ * buffer, which keeps the last call's contents: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+int hssc_hss2d_applyinv_device_noalloc(const hssc_inverse* inv, const void* dX, void* dY,
+                                       int64_t m, void* stream);
+
 /* Replaces Common/Iterative_Refinement.m:1 with A = the FP64 compressed operator and
  * Ainv = the stored inverse: Y = Ainv(B); while ||B - A(Y)|| > tol*||B|| and
  * iter < maxit: Y += Ainv(B - A(Y)).  Returns the iterations done and the final relative
