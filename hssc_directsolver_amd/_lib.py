@@ -69,6 +69,12 @@ SIGNATURES = [
     ("hssc_hss2d_applyinv_device_noalloc", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("hssc_hss2d_solve_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int32,
                                           C.POINTER(C.c_int32), _DP, _P]),
+    ("hssc_grid_op_create", C.c_int, [_PP, C.POINTER(_P)]),
+    ("hssc_grid_op_apply", C.c_int, [_P, _DP, _DP, C.c_int64]),
+    ("hssc_grid_op_apply_device", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("hssc_grid_op_destroy", None, [_P]),
+    ("hssc_hss2d_solve_exact_device", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double,
+                                                C.c_int32, C.POINTER(C.c_int32), _DP, _P]),
     ("hssc_id", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int64, C.c_double, _I64P, _DP, _I64P]),
     ("hssc_lrid_rand", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                  C.c_int64, C.c_uint64, _I64P, _I64P, _DP, _DP]),

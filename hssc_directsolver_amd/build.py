@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhssc.so"
-SOURCES = ["tree.cpp", "dr_weights.cpp", "stats.cpp", "pool.cpp", "kernels.hip", "id.hip", "solver.hip", "hss1d.hip", "mlp.hip"]
+SOURCES = ["tree.cpp", "dr_weights.cpp", "stats.cpp", "pool.cpp", "kernels.hip", "id.hip", "solver.hip", "gridop.hip", "hss1d.hip", "mlp.hip"]
 ARCH = os.environ.get("HSSC_OFFLOAD_ARCH", "gfx950")
 
 

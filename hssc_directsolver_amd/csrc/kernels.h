@@ -22,6 +22,34 @@ struct KernelSpec {
   double lshift = 0;           // SL_L_2D: added to log r^2 (log L^2 of a length scale L)
 };
 
+// One entry of Common/Kernel_Eval.m:29-96 (2D plane):
+// SL_L_2D: A(r,c) = (den==0) + h^2 * 0.5 * w_r * w_c * log(den),  den = |x_r - x_c|^2
+//          (the log term vanishes at den == 0: log(den + (den==0)) = 0 in the reference);
+//          with a kernel scale s and length scale L, s (L h)^2 w_r w_c log(L r), i.e. scale
+//          0.5 s L^2 h^2 and log(den) + log L^2.
+// SL_H_2D: A(r,c) = w (den==0) + h^2 k^2 (i/4) w_r w_c H0(k sqrt(den)), w = 1 + k^2 (i/4)
+//          corr0 (:44-49; b and c are not applied on the diagonal).  With H0 = J0 + i Y0,
+//          (i c)(J0 + i Y0) = -c Y0 + i c J0.
+template <typename T>
+__device__ __forceinline__ T kernel_entry(const KernelSpec& ks, double xr, double yr, double wr,
+                                          double xc, double yc, double wc) {
+  const double dx = xr - xc, dy = yr - yc;
+  const double den = dx * dx + dy * dy;
+  if constexpr (is_complex_v<T>) {
+    using R = real_t<T>;
+    if (ks.pot == HSSC_POT_SL_H_2D) {
+      if (den == 0.0) return T(R(ks.wre), R(ks.wim));
+      const double z = ks.kh * sqrt(den), c = ks.scale * wr * wc;
+      return T(R(-c * y0(z)), R(c * j0(z)));
+    }
+    if (den == 0.0) return T(R(1));
+    return T(R(ks.scale * wr * wc * (log(den) + ks.lshift)));
+  } else {
+    if (den == 0.0) return T(1.0);
+    return T(ks.scale * wr * wc * (log(den) + ks.lshift));
+  }
+}
+
 // A(r,c) = K(R_r, C_c) with per-point weights (real T for SL_L_2D, complex for SL_H_2D).
 template <typename T>
 void gen_block(hipStream_t st, int64_t m, int64_t n, int64_t batch, const double* Rx,
@@ -87,9 +115,11 @@ constexpr int kQrNb = 32;  // QR panel width (= QR_NB in kernels.hip)
 
 // One 32-column panel of the blocked Householder QR: shifted CholeskyQR3 with Householder
 // reconstruction.  Writes the panel's R rows into A, the unit lower trapezoidal V (rows
-// j0..M-1, ld ldv) and the compact-WY factor Tf (32 x 32).  ws: batch * cqr_ws_elems(M)
-// elements, laid out per matrix as Racc | Qtop | two sets of ceil(M / kCqrRows) partial Gram
-// blocks | Q_1 (M x 32).
+// j0..M-1, ld ldv) and the compact-WY factor Tf (32 x 32).  The Gram matrices, Cholesky
+// factors, Q_1 and the reconstruction are computed in FP64 also for FP32 data (whose panels
+// can be too ill-conditioned for CholeskyQR in FP32 arithmetic); only A, V and Tf are stored
+// in T.  ws: batch * cqr_ws_elems(M) FP64 elements of the field, laid out per matrix as
+// Racc | Qtop | two sets of ceil(M / kCqrRows) partial Gram blocks | Q_1 (M x 32).
 constexpr int kCqrRows = 256;
 constexpr int64_t cqr_row_blocks(int64_t M) { return (M + kCqrRows - 1) / kCqrRows; }
 constexpr int64_t cqr_ws_elems(int64_t M) {
@@ -97,7 +127,7 @@ constexpr int64_t cqr_ws_elems(int64_t M) {
 }
 template <typename T>
 void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
-               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, T* ws, int64_t batch);
+               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, wide_t<T>* ws, int64_t batch);
 // T (n x n, upper) of H_0 ... H_{ns-1} from the 32 x 32 factors Ts (stride 32*32 per block,
 // sTs per matrix) and the Gram matrix GV = V^H V (n x n); X: n * 32 scratch per matrix.
 template <typename T>
@@ -105,6 +135,12 @@ void t_assemble(hipStream_t st, int n, const T* Ts, int64_t sTs, const T* GV, in
                 int64_t sG, T* To, int64_t ldt, int64_t sT, T* X, int64_t sX, int64_t batch);
 template <typename T>
 void zero2d(hipStream_t st, int64_t m, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch);
+// Regularisation rows below the Mq x n block of each matrix: A(Mq + i, i) = lambda (i < k),
+// lambda = eta times the RMS norm of the block's first min(k, 32) columns (the other entries
+// of the k appended rows must already be zero).
+template <typename T>
+void tikhonov_rows(hipStream_t st, int64_t Mq, int64_t k, T* A, int64_t lda, int64_t sA,
+                   int64_t batch, double eta);
 template <typename T>
 void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
                  int64_t sW, int64_t batch);

@@ -34,32 +34,7 @@ std::string lbl(const char* base) {
 // ------------------------------------------------------------------------------------
 // Kernel-block generation (Common/Kernel_Eval.m:29-96, 2D plane)
 // ------------------------------------------------------------------------------------
-// SL_L_2D: A(r,c) = (den==0) + h^2 * 0.5 * w_r * w_c * log(den),  den = |x_r - x_c|^2
-//          (the log term vanishes at den == 0: log(den + (den==0)) = 0 in the reference);
-//          with a kernel scale s and length scale L, s (L h)^2 w_r w_c log(L r), i.e. scale
-//          0.5 s L^2 h^2 and log(den) + log L^2.
-// SL_H_2D: A(r,c) = w (den==0) + h^2 k^2 (i/4) w_r w_c H0(k sqrt(den)), w = 1 + k^2 (i/4)
-//          corr0 (:44-49; b and c are not applied on the diagonal).  With H0 = J0 + i Y0,
-//          (i c)(J0 + i Y0) = -c Y0 + i c J0.
-template <typename T>
-__device__ __forceinline__ T kernel_entry(const KernelSpec& ks, double xr, double yr, double wr,
-                                          double xc, double yc, double wc) {
-  const double dx = xr - xc, dy = yr - yc;
-  const double den = dx * dx + dy * dy;
-  if constexpr (is_complex_v<T>) {
-    using R = real_t<T>;
-    if (ks.pot == HSSC_POT_SL_H_2D) {
-      if (den == 0.0) return T(R(ks.wre), R(ks.wim));
-      const double z = ks.kh * sqrt(den), c = ks.scale * wr * wc;
-      return T(R(-c * y0(z)), R(c * j0(z)));
-    }
-    if (den == 0.0) return T(R(1));
-    return T(R(ks.scale * wr * wc * (log(den) + ks.lshift)));
-  } else {
-    if (den == 0.0) return T(1.0);
-    return T(ks.scale * wr * wc * (log(den) + ks.lshift));
-  }
-}
+// The kernel entries come from kernel_entry (kernels.h).
 
 // One thread per entry; threadIdx.x runs along rows so column-major stores coalesce.
 template <typename T>
@@ -949,38 +924,39 @@ static_assert(CQR_ROWS == 256 && CQR_G % CQR_ROWS == 0, "entry split");
 // second partial set starts at part1 = CQR_PART0 + cqr_row_blocks(M) * CQR_G
 constexpr int64_t CQR_RACC = 0, CQR_QTOP = CQR_G, CQR_PART0 = 2 * CQR_G;
 
-template <typename T>
+template <typename T, typename XT, typename QT>
 __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
-    int pass, int64_t Mp, int jb, int P, int64_t part1, const T* X, int64_t ldx, int64_t sX,
-    T* Qo, int64_t ldq, int64_t sQ, T* __restrict__ ws, int64_t sW, T* V, int64_t ldv,
+    int pass, int64_t Mp, int jb, int P, int64_t part1, const XT* X, int64_t ldx, int64_t sX,
+    QT* Qo, int64_t ldq, int64_t sQ, wide_t<T>* __restrict__ ws, int64_t sW, T* V, int64_t ldv,
     int64_t sV, T* __restrict__ A, int64_t lda, int64_t sA, int64_t j0, T* __restrict__ Tfo,
     int64_t sT) {
-  using R = real_t<T>;
-  __shared__ T Rm[QR_NB][QR_NB + 1];       // Cholesky factor of this pass
-  __shared__ T Wb[2 * QR_NB][QR_NB + 1];   // Gram staging (64 rows); pass 3: LU | scratch
-  __shared__ T sdiag[QR_NB];               // S of the reconstruction
+  using C = wide_t<T>;  // arithmetic type: FP64 whatever the storage precision
+  using R = real_t<C>;
+  __shared__ C Rm[QR_NB][QR_NB + 1];       // Cholesky factor of this pass
+  __shared__ C Wb[2 * QR_NB][QR_NB + 1];   // Gram staging (64 rows); pass 3: LU | scratch
+  __shared__ C sdiag[QR_NB];               // S of the reconstruction
   __shared__ R rdi[QR_NB];                 // 1 / R_cc
-  __shared__ T udi[QR_NB];                 // 1 / U_cc
+  __shared__ C udi[QR_NB];                 // 1 / U_cc
   const int64_t e = blockIdx.y;
   const int blk = blockIdx.x, tid = threadIdx.x;
   const bool wg0 = blk == 0;
   ws += e * sW;
   X += e * sX;
-  const T* pin = ws + ((pass & 1) ? CQR_PART0 : part1);   // partial Grams of pass - 1
-  T* pout = ws + ((pass & 1) ? part1 : CQR_PART0);        // partial Grams of this pass
+  const C* pin = ws + ((pass & 1) ? CQR_PART0 : part1);   // partial Grams of pass - 1
+  C* pout = ws + ((pass & 1) ? part1 : CQR_PART0);        // partial Grams of this pass
   if (pass > 0) {
     // R = chol(sum of the partial Gram matrices (+ shift)); the partials are loaded eight at
     // a time and summed in block order
-    T sum[CQR_NE];
+    C sum[CQR_NE];
 #pragma unroll
-    for (int m = 0; m < CQR_NE; ++m) sum[m] = T(0);
+    for (int m = 0; m < CQR_NE; ++m) sum[m] = C(0);
     for (int q0 = 0; q0 < P; q0 += 8) {
-      T v[8][CQR_NE];
+      C v[8][CQR_NE];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int m = 0; m < CQR_NE; ++m)
-          v[u][m] = q0 + u < P ? pin[int64_t(q0 + u) * CQR_G + tid + m * CQR_ROWS] : T(0);
+          v[u][m] = q0 + u < P ? pin[int64_t(q0 + u) * CQR_G + tid + m * CQR_ROWS] : C(0);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
 #pragma unroll
@@ -998,13 +974,13 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       const R u = std::numeric_limits<R>::epsilon() / 2;
       const R shift = R(11) * (R(Mp) * R(jb) + R(jb) * R(jb + 1)) * u * tr;
       __syncthreads();
-      if (tid < jb) Rm[tid][tid] += T(shift);
+      if (tid < jb) Rm[tid][tid] += C(shift);
       __syncthreads();
     }
     for (int k = 0; k < jb; ++k) {
       const R d2 = re_s(Rm[k][k]);
       const R d = d2 > R(0) ? sqrt(d2) : std::numeric_limits<R>::min();
-      const T inv2 = T(R(1) / (d * d));
+      const C inv2 = C(R(1) / (d * d));
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
@@ -1012,25 +988,25 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       }
       __syncthreads();
       // row k of R (the next step reads rows > k only)
-      if (tid >= k && tid < jb) Rm[k][tid] = tid == k ? T(d) : Rm[k][tid] * T(R(1) / d);
+      if (tid >= k && tid < jb) Rm[k][tid] = tid == k ? C(d) : Rm[k][tid] * C(R(1) / d);
     }
     __syncthreads();
     if (tid < jb) rdi[tid] = R(1) / re_s(Rm[tid][tid]);
     if (wg0) {
       // Racc <- R Racc (Racc = R_1 after pass 1)
-      T* Racc = ws + CQR_RACC;
-      T* Old = &Wb[QR_NB][0];
+      C* Racc = ws + CQR_RACC;
+      C* Old = &Wb[QR_NB][0];
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-        Wb[QR_NB + i][j] = pass == 1 ? (i == j ? T(1) : T(0)) : Racc[ent];
+        Wb[QR_NB + i][j] = pass == 1 ? (i == j ? C(1) : C(0)) : Racc[ent];
       }
       (void)Old;
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-        T sum = T(0);
+        C sum = C(0);
         if (i <= j && j < jb)
           for (int t = i; t <= j; ++t) sum += Rm[i][t] * Wb[QR_NB + t][j];
         Racc[ent] = sum;
@@ -1040,28 +1016,28 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
   }
   if (pass == 3) {
     // Q_top = Q2_top R^-1 (rows by forward substitution), then LU of Q_top - S
-    const T* Qtop = ws + CQR_QTOP;
+    const C* Qtop = ws + CQR_QTOP;
 #pragma unroll
     for (int m = 0; m < CQR_NE; ++m) {
       const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-      Wb[i][j] = (i < jb && j < jb) ? Qtop[ent] : T(0);
+      Wb[i][j] = (i < jb && j < jb) ? Qtop[ent] : C(0);
     }
     __syncthreads();
     if (tid < jb) {
       const int i = tid;
       for (int c = 0; c < jb; ++c) {
-        T sum = Wb[i][c];
+        C sum = Wb[i][c];
         for (int t = 0; t < c; ++t) sum -= Wb[i][t] * Rm[t][c];
-        Wb[i][c] = sum * T(rdi[c]);
+        Wb[i][c] = sum * C(rdi[c]);
       }
     }
     __syncthreads();
     for (int k = 0; k < jb; ++k) {
-      const T c = Wb[k][k];
+      const C c = Wb[k][k];
       const R a = sqrt(abs2(c));
-      const T sk = a > R(0) ? -(c * T(R(1) / a)) : T(-1);
-      const T piv = c - sk;
-      const T pinv = T(1) / piv;
+      const C sk = a > R(0) ? -(c * C(R(1) / a)) : C(-1);
+      const C piv = c - sk;
+      const C pinv = C(1) / piv;
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
@@ -1073,23 +1049,23 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       if (tid == k) Wb[k][k] = piv, sdiag[k] = sk;
     }
     __syncthreads();
-    if (tid < jb) udi[tid] = T(1) / Wb[tid][tid];
+    if (tid < jb) udi[tid] = C(1) / Wb[tid][tid];
     __syncthreads();
   }
   // rows of this workgroup: x <- x R^-1 (pass 3: then x <- x U^-1)
   const int64_t r = int64_t(blk) * CQR_ROWS + tid;
   const bool live = r < Mp && !(pass == 3 && r < jb);
-  T x[QR_NB];
+  C x[QR_NB];
 #pragma unroll
-  for (int c = 0; c < QR_NB; ++c) x[c] = (live && c < jb) ? X[r + int64_t(c) * ldx] : T(0);
+  for (int c = 0; c < QR_NB; ++c) x[c] = (live && c < jb) ? C(X[r + int64_t(c) * ldx]) : C(0);
   if (pass > 0) {
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) {
       if (c < jb) {
-        T sum = x[c];
+        C sum = x[c];
 #pragma unroll
         for (int i = 0; i < c; ++i) sum -= x[i] * Rm[i][c];
-        x[c] = sum * T(rdi[c]);
+        x[c] = sum * C(rdi[c]);
       }
     }
   }
@@ -1097,7 +1073,7 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) {
       if (c < jb) {
-        T sum = x[c];
+        C sum = x[c];
 #pragma unroll
         for (int i = 0; i < c; ++i) sum -= x[i] * Wb[i][c];
         x[c] = sum * udi[c];
@@ -1108,11 +1084,11 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
     Qo += e * sQ;
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c)
-      if (c < jb) Qo[r + int64_t(c) * ldq] = x[c];
+      if (c < jb) Qo[r + int64_t(c) * ldq] = QT(x[c]);
   }
   if (pass == 2 && wg0 && tid < jb) {
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) ws[CQR_QTOP + tid + c * QR_NB] = c < jb ? x[c] : T(0);
+    for (int c = 0; c < QR_NB; ++c) ws[CQR_QTOP + tid + c * QR_NB] = c < jb ? x[c] : C(0);
   }
   if (pass < 3) {
     // Partial Gram matrix X^H X of this workgroup's rows on MFMA.  The rows are staged
@@ -1121,7 +1097,7 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
     // conj X(t, I0 + l%16), operand b is X(t, J0 + l%16), t = 4 kq + l/16; item r of lane l
     // is G(I0 + row(l, r), J0 + l%16).
     using v4 = typename Mfma<R>::v4;
-    constexpr bool CX = is_complex_v<T>;
+    constexpr bool CX = is_complex_v<C>;
     constexpr int LDP = QR_NB + 1;
     R* Sr = reinterpret_cast<R*>(&Wb[0][0]);
     R* Si = Sr + 64 * LDP;
@@ -1151,18 +1127,18 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
         }
       }
     }
-    T* pp = pout + int64_t(blk) * CQR_G;
+    C* pp = pout + int64_t(blk) * CQR_G;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = I0 + Mfma<R>::row(lane, r), j = J0 + lc;
-      if constexpr (CX) pp[i + j * QR_NB] = T(gr[r], gi[r]);
-      else pp[i + j * QR_NB] = T(gr[r]);
+      if constexpr (CX) pp[i + j * QR_NB] = C(gr[r], gi[r]);
+      else pp[i + j * QR_NB] = C(gr[r]);
     }
     return;
   }
   if (!wg0) return;
   // pass 3, workgroup 0: Tf = -U S^H L^-H, the panel's R rows S Racc, the top rows of V = L
-  T* Racc = ws + CQR_RACC;
+  C* Racc = ws + CQR_RACC;
   Tfo += e * sT;
   A += e * sA;
   V += e * sV;
@@ -1170,46 +1146,48 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
   for (int m = 0; m < CQR_NE; ++m) {
     const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
     if (i < jb && j < jb) {
-      A[(j0 + i) + (j0 + j) * lda] = i <= j ? sdiag[i] * Racc[ent] : T(0);
-      V[i + int64_t(j) * ldv] = i > j ? Wb[i][j] : (i == j ? T(1) : T(0));
+      A[(j0 + i) + (j0 + j) * lda] = T(i <= j ? sdiag[i] * Racc[ent] : C(0));
+      V[i + int64_t(j) * ldv] = T(i > j ? Wb[i][j] : (i == j ? C(1) : C(0)));
     }
   }
   if (tid < QR_NB) {
     // row i of Tf: Tf L^H = B with B = -U S^H (upper); unit upper L^H, ascending columns
     const int i = tid;
-    T* Tr = &Wb[QR_NB + i][0];
+    C* Tr = &Wb[QR_NB + i][0];
     for (int j = 0; j < QR_NB; ++j) {
-      T b = (i <= j && j < jb) ? -(Wb[i][j] * conj_s(sdiag[j])) : T(0);
+      C b = (i <= j && j < jb) ? -(Wb[i][j] * conj_s(sdiag[j])) : C(0);
       if (j < jb)
         for (int t = 0; t < j; ++t) b -= Tr[t] * conj_s(Wb[j][t]);
-      Tr[j] = (i < jb && j < jb) ? b : T(0);
-      Tfo[i + j * QR_NB] = Tr[j];
+      Tr[j] = (i < jb && j < jb) ? b : C(0);
+      Tfo[i + j * QR_NB] = T(Tr[j]);
     }
   }
 }
 
 template <typename T>
 void cqr_panel(hipStream_t st, int64_t M, int64_t j0, int jb, T* A, int64_t lda, int64_t sA,
-               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, T* ws, int64_t batch) {
+               T* V, int64_t ldv, int64_t sV, T* Tf, int64_t sT, wide_t<T>* ws, int64_t batch) {
+  using C = wide_t<T>;
   const int64_t Mp = M - j0, sW = cqr_ws_elems(M), Pm = cqr_row_blocks(M);
   if (Mp <= 0 || jb <= 0 || batch <= 0) return;
   const int P = int((Mp + CQR_ROWS - 1) / CQR_ROWS);
   const int64_t part1 = CQR_PART0 + Pm * CQR_G;
-  T* Q1 = ws + part1 + Pm * CQR_G;
+  C* Q1 = ws + part1 + Pm * CQR_G;
   const T* Ap = A + j0 + j0 * lda;
   const dim3 g(P, batch), b(CQR_ROWS);
   const double fl = (is_complex_v<T> ? 4.0 : 1.0) * 2.0 * 5.0 * double(Mp) * jb * jb * batch;
   const double by = 7.0 * double(Mp) * jb * sizeof(T) * batch;
   HSSC_TIMED(lbl<T>("cqr_panel").c_str(), st, fl, by, {
     // pass 0: Gram of the panel; 1: Q_1 -> workspace; 2: Q_2 -> V; 3: V <- Householder form
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 0, Mp, jb, P, part1, Ap, lda, sA, (T*)nullptr,
-                       int64_t(0), int64_t(0), ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 1, Mp, jb, P, part1, Ap, lda, sA, Q1, Mp, sW, ws,
-                       sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 2, Mp, jb, P, part1, (const T*)Q1, Mp, sW, V,
-                       ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
-    hipLaunchKernelGGL(cqr_pass_kernel<T>, g, b, 0, st, 3, Mp, jb, P, part1, (const T*)V, ldv, sV, V,
-                       ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL((cqr_pass_kernel<T, T, T>), g, b, 0, st, 0, Mp, jb, P, part1, Ap, lda, sA,
+                       (T*)nullptr, int64_t(0), int64_t(0), ws, sW, V, ldv, sV, A, lda, sA, j0, Tf,
+                       sT);
+    hipLaunchKernelGGL((cqr_pass_kernel<T, T, C>), g, b, 0, st, 1, Mp, jb, P, part1, Ap, lda, sA,
+                       Q1, Mp, sW, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL((cqr_pass_kernel<T, C, T>), g, b, 0, st, 2, Mp, jb, P, part1,
+                       (const C*)Q1, Mp, sW, V, ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
+    hipLaunchKernelGGL((cqr_pass_kernel<T, T, T>), g, b, 0, st, 3, Mp, jb, P, part1,
+                       (const T*)V, ldv, sV, V, ldv, sV, ws, sW, V, ldv, sV, A, lda, sA, j0, Tf, sT);
   });
 }
 
@@ -1734,6 +1712,34 @@ int64_t ls_part_elems(int64_t M, int64_t n, int64_t nbo, int64_t kchunk, int64_t
   return std::max({nsplit * nbo * n, nsplit_thin * int64_t(kQrNb) * n, nsplit_thin * nbo * nbo});
 }
 
+// Regularisation rows of the least squares of one box (one workgroup per box): below the
+// Mq x n block, A(Mq + i, i) = lambda for i < k (the rows themselves are zero), with lambda =
+// eta times the RMS norm of the block's first min(k, 32) columns.
+template <typename T>
+__global__ void __launch_bounds__(256) tikhonov_rows_kernel(int64_t Mq, int64_t k,
+                                                            T* __restrict__ A, int64_t lda,
+                                                            int64_t sA, double eta) {
+  __shared__ double red[4];
+  A += int64_t(blockIdx.x) * sA;
+  const int64_t c = k < 32 ? k : 32;
+  double s = 0;
+  for (int64_t idx = threadIdx.x; idx < Mq * c; idx += 256)
+    s += double(abs2(A[(idx % Mq) + (idx / Mq) * lda]));
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double lam = eta * sqrt((red[0] + red[1] + red[2] + red[3]) / double(c > 0 ? c : 1));
+  for (int64_t i = threadIdx.x; i < k; i += 256) A[Mq + i + i * lda] = T(real_t<T>(lam));
+}
+
+template <typename T>
+void tikhonov_rows(hipStream_t st, int64_t Mq, int64_t k, T* A, int64_t lda, int64_t sA,
+                   int64_t batch, double eta) {
+  if (k <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(tikhonov_rows_kernel<T>, dim3(batch), dim3(256), 0, st, Mq, k, A, lda, sA,
+                     eta);
+}
+
 // Sum of squares (for residual norms), deterministic two-phase.
 template <typename T>
 __global__ void __launch_bounds__(256) sumsq_partial(int64_t len, const T* __restrict__ x,
@@ -1780,10 +1786,12 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
   template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \
-                             int64_t, T*, int64_t, T*, int64_t);                                   \
+                             int64_t, T*, int64_t, wide_t<T>*, int64_t);                           \
   template void t_assemble<T>(hipStream_t, int, const T*, int64_t, const T*, int64_t, int64_t, T*,  \
                               int64_t, int64_t, T*, int64_t, int64_t);                             \
   template void zero2d<T>(hipStream_t, int64_t, int64_t, T*, int64_t, int64_t, int64_t);            \
+  template void tikhonov_rows<T>(hipStream_t, int64_t, int64_t, T*, int64_t, int64_t, int64_t,     \
+                                 double);                                                          \
   template void gemv<T>(hipStream_t, bool, int64_t, int64_t, T, const T*, int64_t, int64_t,        \
                         const T*, int64_t, T, const T*, int64_t, T*, int64_t, int64_t, T*);        \
   template void sumsq<T>(hipStream_t, int64_t, const T*, double*, int);                           \

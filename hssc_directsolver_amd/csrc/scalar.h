@@ -121,6 +121,10 @@ __host__ __device__ inline R im_s(const cplx<R>& x) {
   return x.im;
 }
 
+// The FP64 type of the same field: double for real T, complex double for complex T.
+template <typename T>
+using wide_t = typename std::conditional<is_complex_v<T>, cplx<double>, double>::type;
+
 // Wave shuffles for every scalar type.
 __device__ inline double shfl_s(double v, int src) { return __shfl(v, src); }
 __device__ inline float shfl_s(float v, int src) { return __shfl(v, src); }

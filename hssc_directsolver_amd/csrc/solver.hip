@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "gridop.h"
 #include "internal.h"
 #include "kernels.h"
 #include "pool.h"
@@ -492,10 +493,17 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   // every queue fed whatever the mapping.
   const int kLevelStreams = 2 * hw_queues();
   constexpr int64_t kOuterNb = 128;  // outer panel width
+  // regularisation weight of FP32 trees relative to the RMS column norm of K(Xp, X_sk)
+  // (HSSC_TIKHONOV_ETA overrides it)
+  static const double kTikhonovEta = [] {
+    const char* v = std::getenv("HSSC_TIKHONOV_ETA");
+    return v ? std::atof(v) : 1e-5;
+  }();
   constexpr int64_t kMergeCols = 128;  // trailing blocks up to this width take rank-32 updates
   struct Ws {
     S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
-      *cq = nullptr, *Ts = nullptr, *GV = nullptr;
+      *Ts = nullptr, *GV = nullptr;
+    wide_t<S>* cq = nullptr;  // CholeskyQR workspace (FP64 arithmetic for every storage type)
   };
   std::vector<Ws> ws(t->depth + 1);
   std::vector<hipStream_t> streams(kLevelStreams, nullptr);
@@ -517,12 +525,21 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
     std::vector<double> cost(t->depth + 1, 0.0);
     for (int l = t->depth; l >= 1; --l) {
       Level& L = t->levels[l];
-      const int64_t M = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+      // FP32 trees solve the Tikhonov-regularised least squares
+      //   min ||K(Xp, X_sk) T - K(Xp, X_rs)||^2 + lambda^2 ||T||^2
+      // as the plain one of the block with k more rows [lambda I 0] below it: directions in
+      // which K(Xp, X_sk) is smaller than FP32 rounding can resolve (its singular values reach
+      // acc ||K||) are damped instead of amplifying rounding noise into T.
+      const bool tikhonov = sizeof(real_t<S>) == 4;
+      const int64_t Mq = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+      const int64_t M = tikhonov ? Mq + kk : Mq;  // rows of the least squares
       S* dT = dalloc<S>(E * kk * std::max<int64_t>(r, 1));
-      L.dT = dT;
+      // an FP32 tree keeps only the FP32 operator (the copy the FP32 factorization reads)
+      if constexpr (sizeof(real_t<S>) == 4) L.dT32 = dT;
+      else L.dT = dT;
       L.q = r;
       if (r == 0) continue;
-      if (M < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
+      if (Mq < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
       Ws& w = ws[l];
       const int64_t sA = M * n;
       // Outer panels of up to 128 columns made of 32-column CholeskyQR3 sub-panels, so the
@@ -535,15 +552,19 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
       w.Tf = dalloc<S>(E * nbo * nbo);
       const int64_t part_cap = E * k::ls_part_elems(M, n, nbo, kchunk, kchunk_thin);
       w.part = dalloc<S>(part_cap);
-      w.cq = dalloc<S>(E * k::cqr_ws_elems(M));
+      w.cq = dalloc<wide_t<S>>(E * k::cqr_ws_elems(M));
       w.Ts = dalloc<S>(E * kOuterNb * nb);
       w.GV = dalloc<S>(E * nbo * nbo);
       S* A = w.A;
       auto& sl = steps[l];
       // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
       sl.push_back([=, &L](hipStream_t st) {
-        k::gen_block<S>(st, M, n, E, L.dXpx, L.dXpy, L.dXpw, M, L.dPx, L.dPy, L.dPw, n, A, M, sA,
+        k::gen_block<S>(st, Mq, n, E, L.dXpx, L.dXpy, L.dXpw, Mq, L.dPx, L.dPy, L.dPw, n, A, M, sA,
                         ks);
+        if (tikhonov) {
+          k::zero2d<S>(st, kk, n, A + Mq, M, sA, E);
+          k::tikhonov_rows<S>(st, Mq, kk, A, M, sA, E, kTikhonovEta);
+        }
       });
       for (int64_t j0 = 0; j0 < kk; j0 += nbo) {
         sl.push_back([=](hipStream_t st) {
@@ -649,9 +670,33 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   t->has_T = true;
 }
 
+// The least squares in the tree's precision: FP64, or FP32 / complex64 on MFMA (the skeleton
+// IDs stay FP64 either way).
 void compute_interpolation(hssc_tree* t, hipStream_t st0) {
-  if (t->cx) compute_interpolation_impl<zdouble>(t, st0);
-  else compute_interpolation_impl<double>(t, st0);
+  const bool lo = t->p.tree_precision == HSSC_PREC_FP32;
+  if (t->cx) lo ? compute_interpolation_impl<zfloat>(t, st0) : compute_interpolation_impl<zdouble>(t, st0);
+  else lo ? compute_interpolation_impl<float>(t, st0) : compute_interpolation_impl<double>(t, st0);
+}
+
+// FP64 interpolation operators of an FP32 tree (widened once, for the FP64 readers: the FP64
+// factorization, the compressed operator's sweeps, hssc_tree_get_T).  Waits for the build.
+void ensure_T64(hssc_tree* t) {
+  tree_sync(t);
+  for (int l = 1; l <= t->depth; ++l) {
+    Level& L = t->levels[l];
+    if (L.dT || !L.dT32) continue;
+    const int64_t cnt = L.nkey * L.k * std::max<int64_t>(L.n - L.k, 1);
+    if (t->cx) {
+      zdouble* d = dalloc<zdouble>(cnt);
+      k::axpby<zfloat, zdouble>(0, cnt, zdouble(1), static_cast<const zfloat*>(L.dT32), zdouble(0), d);
+      L.dT = d;
+    } else {
+      double* d = dalloc<double>(cnt);
+      k::axpby<float, double>(0, cnt, 1.0, static_cast<const float*>(L.dT32), 0.0, d);
+      L.dT = d;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(0));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -669,6 +714,8 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   const int D = t->depth;
   const k::KernelSpec ks = kernel_spec(t->p);
   const bool TI = t->p.transinv != 0;
+  const bool tree_lo = t->p.tree_precision == HSSC_PREC_FP32;
+  if (tree_lo && sizeof(real_t<T>) == 8) ensure_T64(t);  // FP64 factors of an FP32 tree
   inv->FI.assign(D + 1, nullptr);
   inv->E.assign(D + 1, nullptr);
   int64_t wmax = 0, gmax = 0;
@@ -712,7 +759,7 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       if (r > 0) {
         // T of this level may still be in the making on the build's streams
         if (L.ready) HIPCHK(hipStreamWaitEvent(st, L.ready, 0));
-        if (sizeof(real_t<T>) == 4) {
+        if (sizeof(real_t<T>) == 4 && !tree_lo) {  // an FP32 tree already holds the FP32 T
           Level& Lm = t->levels[l];
           if (!Lm.dT32) Lm.dT32 = dalloc<T>(Lm.nkey * Lm.k * r);
           k::axpby<hi_t<T>, T>(st, Lm.nkey * Lm.k * r, T(1), static_cast<const hi_t<T>*>(Lm.dT),
@@ -945,7 +992,7 @@ void apply_impl(const hssc_operator* op, const S* X, S* Y, int64_t m, hipStream_
 }
 
 void apply_dispatch(const hssc_operator* op, const void* X, void* Y, int64_t m, hipStream_t st) {
-  tree_sync(op->tree);  // the sweeps read T
+  ensure_T64(op->tree);  // the sweeps read the FP64 T
   if (op->tree->cx)
     apply_impl<zdouble>(op, static_cast<const zdouble*>(X), static_cast<zdouble*>(Y), m, st);
   else
@@ -997,10 +1044,11 @@ int applyinv_borrowed(const hssc_inverse* inv, const void* X, void* Y, int64_t m
   return HSSC_OK;
 }
 
-// Common/Iterative_Refinement.m:20-35 with A = the FP64 compressed operator.
-template <typename S>
-int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* Y, int64_t m,
-               double tol, int32_t maxit, int32_t* iters, double* relres, hipStream_t st) {
+// Common/Iterative_Refinement.m:20-35: Y = Ainv(B); while ||B - A(Y)|| > tol ||B||: Y +=
+// Ainv(B - A(Y)).  resid(Y, R) forms R = B - A(Y) for the residual operator A.
+template <typename S, typename Resid>
+int refine_impl(const hssc_inverse* inv, const S* B, S* Y, int64_t m, double tol, int32_t maxit,
+                int32_t* iters, double* relres, hipStream_t st, Resid&& resid) {
   const int64_t len = inv->tree->N * m;
   S* R = dalloc<S>(len);
   S* D = dalloc<S>(len);
@@ -1010,8 +1058,7 @@ int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* 
   int it = 1;
   double rel = 0;
   for (;;) {
-    apply_dispatch(op, Y, R, m, st);
-    k::axpby<S, S>(st, len, S(1), B, S(-1), R);  // R = B - A Y
+    resid(Y, R);
     rel = device_norm<S>(R, len, st, part) / (nb > 0 ? nb : 1.0);
     if (rel <= tol || it >= maxit) break;
     applyinv_dispatch(inv, R, D, m, st);
@@ -1023,6 +1070,26 @@ int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* 
   if (iters) *iters = it;
   if (relres) *relres = rel;
   return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
+}
+
+// The refinement with A = the FP64 compressed operator.
+template <typename S>
+int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* Y, int64_t m,
+               double tol, int32_t maxit, int32_t* iters, double* relres, hipStream_t st) {
+  const int64_t len = inv->tree->N * m;
+  return refine_impl<S>(inv, B, Y, m, tol, maxit, iters, relres, st, [&](const S* y, S* r) {
+    apply_dispatch(op, y, r, m, st);
+    k::axpby<S, S>(st, len, S(1), B, S(-1), r);  // R = B - A Y
+  });
+}
+
+// The refinement with A = the exact grid operator (FFT convolution).
+template <typename S>
+int solve_exact_impl(const hssc_inverse* inv, const hssc_grid_op* A, const S* B, S* Y,
+                     int64_t m, double tol, int32_t maxit, int32_t* iters, double* relres,
+                     hipStream_t st) {
+  return refine_impl<S>(inv, B, Y, m, tol, maxit, iters, relres, st,
+                        [&](const S* y, S* r) { grid_op_apply(A, y, B, r, m, st); });
 }
 
 }  // namespace
@@ -1291,7 +1358,7 @@ int hssc_tree_get_T(const hssc_tree* t, int64_t box, double* out) {
   if (int s = box_level(t, box, &lev)) return s;
   if (!out) return fail(HSSC_ERR_INVALID_ARG, "null output");
   if (!t->has_T) return fail(HSSC_ERR_INVALID_ARG, "tree has no interpolation operators");
-  tree_sync(const_cast<hssc_tree*>(t));
+  ensure_T64(const_cast<hssc_tree*>(t));
   const Level& L = t->levels[lev];
   if (lev == 0) return fail(HSSC_ERR_INVALID_ARG, "the root has no interpolation operator");
   const int64_t key = box_key(t, box, lev), cnt = L.k * (L.n - L.k);
@@ -1541,6 +1608,24 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
                                  m, tol, maxit, iters, relres, st);
     return solve_impl<double>(inv, op, static_cast<const double*>(dB), static_cast<double*>(dY), m,
                               tol, maxit, iters, relres, st);
+  });
+}
+
+int hssc_hss2d_solve_exact_device(const hssc_inverse* inv, const hssc_grid_op* A,
+                                  const void* dB, void* dY, int64_t m, double tol,
+                                  int32_t maxit, int32_t* iters, double* relres, void* stream) {
+  // Common/Iterative_Refinement.m:20-35 with A = the exact grid operator.
+  if (!inv || !A || !dB || !dY || m < 1 || maxit < 1)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid solve arguments");
+  if (grid_op_n(A) != inv->tree->N || grid_op_complex(A) != inv->tree->cx)
+    return fail(HSSC_ERR_INVALID_ARG, "the grid operator and the inverse belong to other grids");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return guarded([&]() {
+    if (inv->tree->cx)
+      return solve_exact_impl<zdouble>(inv, A, static_cast<const zdouble*>(dB),
+                                       static_cast<zdouble*>(dY), m, tol, maxit, iters, relres, st);
+    return solve_exact_impl<double>(inv, A, static_cast<const double*>(dB),
+                                    static_cast<double*>(dY), m, tol, maxit, iters, relres, st);
   });
 }
 

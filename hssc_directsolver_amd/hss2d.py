@@ -357,3 +357,45 @@ def solve_device(A_HSS: Inverse, M_HSS: Operator, B, Y, tol=1e-10, maxit=10, str
                                         C.c_void_p(Y.data_ptr()), m, float(tol), int(maxit),
                                         C.byref(it), C.byref(rel), _stream_ptr(stream)))
     return it.value, rel.value
+
+
+class GridOperator:
+    """The exact operator A = Kernel_Eval(X_source, X_source, params) of the uniform grid,
+    applied by FFT convolution on the GPU (the dense products of
+    HSS2D/Test/Test_HSS2D_InverseCompression.m:300-311 and Test_HSS2D_Matrix_Apply.m:227-235
+    without forming A)."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        h = C.c_void_p()
+        check(lib().hssc_grid_op_create(C.byref(params.raw), C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().hssc_grid_op_destroy(h)
+            self.handle = None
+
+    def apply(self, X):
+        """A X for host arrays of shape (N,) or (N, m)."""
+        return _host_call(lib().hssc_grid_op_apply, self.handle, X, self.params.dtype,
+                          self.params.N)
+
+    def apply_device(self, X, Y, stream=None):
+        """Y = A X for GPU tensors of shape (N,) or (m, N)."""
+        m = _device_rhs(self.params, X, Y)
+        check(lib().hssc_grid_op_apply_device(self.handle, C.c_void_p(X.data_ptr()),
+                                              C.c_void_p(Y.data_ptr()), m, _stream_ptr(stream)))
+
+
+def solve_exact_device(A_HSS: Inverse, A: GridOperator, B, Y, tol=1e-10, maxit=10, stream=None):
+    """Common/Iterative_Refinement.m on the GPU with the exact grid operator as the residual
+    operator; returns (solves, relative residual ||B - A Y|| / ||B||)."""
+    m = _device_rhs(A_HSS.tree.params, B, Y)
+    it = C.c_int32()
+    rel = C.c_double()
+    check(lib().hssc_hss2d_solve_exact_device(A_HSS.handle, A.handle, C.c_void_p(B.data_ptr()),
+                                              C.c_void_p(Y.data_ptr()), m, float(tol), int(maxit),
+                                              C.byref(it), C.byref(rel), _stream_ptr(stream)))
+    return it.value, rel.value

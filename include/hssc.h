@@ -225,6 +225,29 @@ int hssc_hss2d_solve_device(const hssc_inverse* inv, const hssc_operator* op, co
                             void* dY, int64_t m, double tol, int32_t maxit, int32_t* iters,
                             double* relres, void* stream);
 
+/* ---- exact grid operator -----------------------------------------------------------
+ * Replaces the dense products with A = Kernel_Eval(X_source, X_source, params) that the
+ * reference's tests form to check a solve (HSS2D/Test/Test_HSS2D_InverseCompression.m:
+ * 300-311, Test_HSS2D_Matrix_Apply.m:227-235): the exact operator of the uniform grid, applied
+ * in O(N log N) by FFT convolution on the GPU (the kernel depends on point offsets only; the
+ * weights b, c and the diagonal are applied pointwise).  Grids up to 512 x 512 points.  X, Y:
+ * N x m column-major in the X_source ordering (double, or interleaved complex double for
+ * SL_H_2D).  Calls on one handle must be serialised (it owns its work arrays). */
+typedef struct hssc_grid_op hssc_grid_op;
+int hssc_grid_op_create(const hssc_params* p, hssc_grid_op** op);
+int hssc_grid_op_apply(const hssc_grid_op* op, const double* X, double* Y, int64_t m);
+int hssc_grid_op_apply_device(const hssc_grid_op* op, const void* dX, void* dY, int64_t m,
+                              void* stream);
+void hssc_grid_op_destroy(hssc_grid_op* op);
+
+/* Common/Iterative_Refinement.m:20-35 with A = the exact grid operator (so the stopping
+ * residual is the true one, ||B - A Y|| / ||B||) and Ainv = the stored inverse (FP64 or FP32,
+ * built on an FP64 or FP32 tree).  Returns the solves done and the final relative residual;
+ * HSSC_ERR_NOT_CONVERGED if tol is not reached within maxit solves.  Device pointers. */
+int hssc_hss2d_solve_exact_device(const hssc_inverse* inv, const hssc_grid_op* A,
+                                  const void* dB, void* dY, int64_t m, double tol,
+                                  int32_t maxit, int32_t* iters, double* relres, void* stream);
+
 /* ---- interpolative decompositions ------------------------------------------------- */
 /* Replaces Common/ID.m:1 ([T,I] = ID(A,par)): column-pivoted QR on the GPU; par < 1 is the
  * accuracy (rank = #{|R_ii| > par |R_11|} plus the reference's randomized check),
