@@ -13,12 +13,14 @@ unit-norm complex Gaussian right-hand side.
 1e-10, 2, 400, 0) instead; it is also reported as a side case.
 
 One step runs two paths, each end to end from the parameters:
-  HSS2D_build_tree_fsym (host index sets + GPU interpolation operators)
-  + HSS2D_compress_fsym (forward operator used by the refinement residual)
-  + HSS2D_invert_fsym + solve to 1e-10 relative residual against the FP64 compressed
-    operator (Common/Iterative_Refinement.m),
-  mixed: FP32 (complex64) factorization (F^-1, E on MFMA), FP64 refinement -> `value`
-  fp64 : FP64 (complex128) factorization, FP64 refinement (plain path).
+  HSS2D_build_tree_fsym (host index sets, FP64 skeleton IDs, GPU interpolation operators)
+  + the exact grid operator A = Kernel_Eval(X, X, params) (FFT convolution)
+  + HSS2D_invert_fsym + solve refined (Common/Iterative_Refinement.m) until the true
+    relative residual ||b - A y|| / ||b|| <= 1e-10,
+  mixed: complex64 interpolation operators (Tikhonov-regularised FP32 least squares with
+         FP64 panel factorizations) and complex64 factorization (F^-1, E on MFMA), FP64
+         refinement -> `value`
+  fp64 : complex128 throughout (plain path).
 
 Prints one JSON line.  `value` is the mixed path's build + solve seconds for the whole
 job (time-like, lower is better).
@@ -85,6 +87,7 @@ def stats_snapshot(H):
              for t in ("f64", "f32", "c128", "c64")]
     names += [f"gemm_{t}_{a}{b}" for t in ("f64", "f32", "c128", "c64") for a in "ntch"
               for b in "nt"]
+    names += ["fft_c128"]
     for name in names:
         n, fl, by, ms = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
         L.hssc_stats_get(name.encode(), C.byref(n), C.byref(fl), C.byref(by), C.byref(ms))
@@ -94,19 +97,24 @@ def stats_snapshot(H):
 
 
 def one_step(H, P, torch, b, tol, split=False):
-    """Runs each path end to end (tree with its interpolation operators, compressed operator,
-    inverse, refined solve); returns timings (s) and residual info per path.  Without `split`
-    the tree, compression and inversion are called back to back with no host synchronisation,
-    as a caller would: the inversion waits on the device, level by level, for the interpolation
-    operators it needs, so the two overlap.  With `split` every phase is synchronised on its
-    own to report the breakdown."""
+    """Runs each path end to end: the tree with its interpolation operators, the exact grid
+    operator (FFT convolution, the refinement's residual operator), the inverse, and the solve
+    refined until the true relative residual ||b - A y|| / ||b|| <= tol.
+      mixed ("fp32"): FP32 / complex64 interpolation operators (skeleton IDs in FP64) and
+                      FP32 / complex64 factorization;
+      plain ("fp64"): everything in FP64 / complex128.
+    Returns timings (s) and residual info per path.  Without `split` the tree and the inversion
+    are called back to back with no host synchronisation, as a caller would: the inversion
+    waits on the device, level by level, for the interpolation operators it needs, so the two
+    overlap.  With `split` every phase is synchronised on its own to report the breakdown."""
     sync = torch.cuda.synchronize
     res = {}
     for prec in ("fp32", "fp64"):
+        P.raw.tree_precision = 1 if prec == "fp32" else 0
         sync()
         a = time.perf_counter()
         tree = H.HSS2D_build_tree_fsym(P)
-        op = H.HSS2D_compress_fsym(tree)
+        A = H.GridOperator(P)
         if split:
             sync()
         m = time.perf_counter()
@@ -114,13 +122,14 @@ def one_step(H, P, torch, b, tol, split=False):
         sync()
         c = time.perf_counter()
         y = torch.empty_like(b)
-        it, rel = H.solve_device(inv, op, b, y, tol=tol, maxit=12)
+        it, rel = H.solve_exact_device(inv, A, b, y, tol=tol, maxit=12)
         sync()
         d = time.perf_counter()
         res[prec] = dict(build=c - a, solve=d - c, iters=it, relres=rel)
         if split:
             res[prec].update(tree=m - a, invert=c - m)
-        del inv, op, tree
+        del inv, A, tree
+    P.raw.tree_precision = 0
     return res
 
 
@@ -214,14 +223,59 @@ def level_flops(n, k, M, cx):
                                                           + 2 * k**3)
 
 
+def _cpu_run(o, H, P, args, p, stop, cx):
+    """One oracle run on the TI tree (one box per level), levels depth..stop timed: build
+    (least squares + inversion) and the solve's products with F^-1 and E (each applied twice
+    per box in the two sweeps), scaled by the level's box count; the levels above stop are
+    extrapolated at level stop's flop and byte rates."""
+    import numpy as np
+    tb, ti = {}, {}
+    t0 = time.perf_counter()
+    tree = o.build_tree(p, interform="dense", timings=tb, stop_level=stop,
+                        preset_all=args.preset_all)
+    inv = o.invert(tree, timings=ti, stop_level=stop)
+    ts = {}
+    rng = np.random.default_rng(0)
+    for lev in range(tree.depth, stop - 1, -1):
+        key = tree.key(2 ** lev)
+        FI = inv.FI[key]
+        E = inv.E.get(key)
+        v = rng.standard_normal(FI.shape[0]) + (1j * rng.standard_normal(FI.shape[0]) if cx else 0)
+        a = time.perf_counter()
+        for _ in range(2):
+            FI @ v
+            if E is not None:
+                E @ v[:E.shape[0]]
+        ts[lev] = time.perf_counter() - a
+    t1 = time.perf_counter()
+    idx = H.HSS2D_build_index_sets(P)
+    shapes = {l: idx.level_info(l) for l in range(idx.depth + 1)}
+    fT, fI = level_flops(shapes[stop]["n"], shapes[stop]["k"], shapes[stop]["M"], cx)
+    rT, rI = fT / tb[stop], fI / ti[stop]
+    es = 16 if cx else 8
+    bS = 2 * (shapes[stop]["n"] ** 2 + shapes[stop]["k"] ** 2) * es / ts[stop]
+    build = solve = 0.0
+    for l, s in shapes.items():
+        if l >= stop:
+            build += (tb[l] + ti[l]) * s["nbox"]
+            solve += ts[l] * s["nbox"]
+        else:
+            a_, b_ = level_flops(s["n"], s["k"], s["M"], cx)
+            build += (a_ / rT * (l > 0) + b_ / rI) * s["nbox"]
+            solve += 2 * (s["n"] ** 2 + (s["k"] ** 2 if l > 0 else 0)) * es / bS * s["nbox"]
+    return dict(build_s=round(build, 1), solve_s=round(solve, 3), measured_s=round(t1 - t0, 2),
+                lsq_gflops=round(rT / 1e9, 1), inv_gflops=round(rI / 1e9, 1),
+                solve_gbs=round(bS / 1e9, 2))
+
+
 def cpu_baseline(H, P, args):
-    """The oracle (numpy/scipy FP64 restatement of the reference) on the TI tree of the same
-    grid, one box per level: levels depth..cpu_stop are timed, each level's time is scaled by
-    its box count (the NTI build does every box), and the levels above cpu_stop are
-    extrapolated from the flop rates measured on level cpu_stop.  The solve (milliseconds on
-    the GPU, seconds on the CPU) is left out of the CPU figure."""
+    """The oracle (numpy/scipy FP64 restatement of the reference) on the host cores, at all
+    cores and at one thread: build and one solve (the FP64 path's first applyinv), measured
+    on levels depth..stop of the TI tree of the same grid and scaled by box counts (the NTI
+    path processes every box), the levels above extrapolated at the measured rates."""
     from oracle import hss2d as o
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    from threadpoolctl import threadpool_limits
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cx = args.kernel == "helmholtz"
     if cx:
         from oracle.dr_weights import corr0
@@ -231,31 +285,57 @@ def cpu_baseline(H, P, args):
         p["proxy_layers"] = args.proxy_layers
     else:
         p = o.tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, 1)
-    stop = args.cpu_stop
-    tb, ti = {}, {}
-    t0 = time.perf_counter()
-    tree = o.build_tree(p, interform="dense", timings=tb, stop_level=stop,
-                        preset_all=args.preset_all)
-    o.invert(tree, timings=ti, stop_level=stop)
-    t1 = time.perf_counter()
-    idx = H.HSS2D_build_index_sets(P)
-    shapes = {l: idx.level_info(l) for l in range(idx.depth + 1)}
-    fT, fI = level_flops(shapes[stop]["n"], shapes[stop]["k"], shapes[stop]["M"], cx)
-    rT, rI = fT / tb[stop], fI / ti[stop]
-    total = 0.0
-    for l, s in shapes.items():
-        if l >= stop:
-            total += (tb[l] + ti[l]) * s["nbox"]
-        else:
-            a, b = level_flops(s["n"], s["k"], s["M"], cx)
-            total += (a / rT * (l > 0) + b / rI) * s["nbox"]
-    return dict(value=round(total, 1), unit="s", cores=threads, kind="port",
+    with threadpool_limits(limits=cores):
+        allc = _cpu_run(o, H, P, args, p, args.cpu_stop, cx)
+    with threadpool_limits(limits=1):
+        one = _cpu_run(o, H, P, args, p, args.cpu_stop + 2, cx)
+    return dict(value=round(allc["build_s"] + allc["solve_s"], 1), unit="s", cores=cores,
+                kind="port",
                 sample=(f"oracle (numpy/scipy FP64) on the TI tree of the same grid: levels "
-                        f"{idx.depth}..{stop} timed on one box each ({t1 - t0:.1f} s), scaled by "
-                        f"boxes per level; levels {stop - 1}..0 extrapolated at the level-{stop} "
-                        f"rates ({rT / 1e9:.1f} GF/s least squares, {rI / 1e9:.1f} GF/s "
-                        f"inversion); build only"),
-                measured_s=round(t1 - t0, 2))
+                        f"{P.depth}..{args.cpu_stop} timed on one box each "
+                        f"({allc['measured_s']} s at {cores} threads), scaled by boxes per level; "
+                        f"levels above extrapolated at the measured rates; build + one solve"),
+                all_cores=allc, one_thread=dict(
+                    value=round(one["build_s"] + one["solve_s"], 1), levels_timed=(
+                        f"{P.depth}..{args.cpu_stop + 2}"), **one))
+
+
+def phase_rooflines(levels, split, stats, cx):
+    """Per-phase figures of the synchronised step: the inversion and the interpolation
+    operators against the MFMA peak of their precision (algorithmic flops: the symmetric
+    sweep count sum 4 (n^3 + 2 n^2 k + 2 n k^2 + k^3) and the least squares' sum
+    4 (2 M k^2 + 4 M k (n - k)) over boxes; the tree time includes the host index sets and the
+    FP64 skeleton IDs), kernel-entry generation in entries/s and the solve kernels' HBM
+    bytes/s, both from the instrumented step's HIP events."""
+    f = 4 if cx else 1
+    W_inv = W_T = 0.0
+    for lev, (n, k, M) in levels.items():
+        nb = 2 ** int(lev)
+        W_inv += nb * f * (n ** 3 + 2 * n * n * k + 2 * n * k * k + k ** 3)
+        if int(lev) > 0:
+            W_T += nb * f * (2 * M * k * k + 4 * M * k * (n - k))
+    out = {}
+    for prec, peak in (("fp32", 157.3), ("fp64", 78.6)):
+        sp = split[prec]
+        ti, tt = sp["invert"], sp["tree"]
+        out[prec] = dict(
+            invert=dict(bound="mfma", flops=W_inv, achieved_tflops=round(W_inv / ti / 1e12, 2),
+                        peak=peak, frac=round(W_inv / ti / 1e12 / peak, 3)),
+            tree=dict(bound="mfma", flops=W_T, achieved_tflops=round(W_T / tt / 1e12, 2),
+                      peak=peak, frac=round(W_T / tt / 1e12 / peak, 3)))
+    gen = [v for k_, v in stats.items() if k_.startswith(("gen_block", "gen_parent"))]
+    if gen:
+        ent = sum(v["flops"] for v in gen)
+        ms = sum(v["ms"] for v in gen)
+        out["generation"] = dict(bound="valu", entries=ent,
+                                 entries_per_s=round(ent / (ms * 1e-3), 1))
+    sol = [v for k_, v in stats.items() if k_.startswith(("symv", "gemv"))]
+    if sol:
+        by = sum(v["bytes"] for v in sol)
+        ms = sum(v["ms"] for v in sol)
+        out["solve_kernels"] = dict(bound="hbm", bytes=by, achieved_gbs=round(by / (ms * 1e-3) / 1e9, 1),
+                                    peak=PEAK_HBM_GBS, frac=round(by / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 3))
+    return out
 
 
 def level_shapes(H, P):
@@ -390,11 +470,12 @@ def main():
     h1 = hss1d_case(H, torch) if not args.no_hss1d else None
     hb = batched_case(H, torch) if not args.no_batched else None
     cx = args.kernel == "helmholtz"
+    levels = {lev: [v["n"], v["k"], v["M"]] for lev, v in level_shapes(H, P).items()}
 
     out = {
         "metric": "HSS2D N=512² build s + solve ms @1e-10 resid: mixed+batched vs FP64 GPU",
         "value": summ["mixed"]["total_s"],
-        "unit": "s (build + solve, mixed FP32 factorization + FP64 refinement)",
+        "unit": "s (build + solve to 1e-10 true residual, mixed FP32 tree and factorization + FP64 refinement)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -402,18 +483,19 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64 factorization / c128 refinement" if cx
-                  else "f32 factorization / f64 refinement"),
+        "dtype": ("c64 tree and factorization / c128 refinement" if cx
+                  else "f32 tree and factorization / f64 refinement"),
         "data": "synthetic (seeded Gaussian RHS on the reference's uniform grid)",
         "config": {"workload": workload, "N": N},
         **summ,
         "roofline": roofline(stats),
+        "phases": phase_rooflines(levels, split, stats, cx),
         "kernels": {k: dict(launches=v["launches"], ms=round(v["ms"], 3),
                             tflops=round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else None,
                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
                     for k, v in stats.items()},
         "cpu_baseline": cpu,
-        "levels": {lev: [v["n"], v["k"], v["M"]] for lev, v in level_shapes(H, P).items()},
+        "levels": levels,
         "laplace": side,
         "hss1d": h1,
         "hss1d_batched": hb,
