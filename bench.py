@@ -39,12 +39,12 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "c128": 78.6, "c64": 157.3}  # MI355X dense MFMA
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of a kernel class, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
-# of `bench.py --no-cpu --no-hss1d --no-laplace --no-batched --profile-step` (one step of the
+# of `bench.py --no-cpu --no-hss1d --no-batched --no-laplace --profile-step` (one step of the
 # default workload), averaged over the class's launches and scaled by what the counters read
 # for a plain 1 GiB device copy (`bench.py --calibrate-copy`: FETCH_SIZE 512 MiB -> x2,
 # WRITE_SIZE 1 GiB -> x1).  profiles/hss2d_512_helmholtz_traffic.md holds the runs.
 MEASURED_TRAFFIC = {
-    "gemm_c128_nn": 2 * 85087.3 * 1024 + 69935.5 * 1024,
+    "gemm_c128_nn": 2 * 77102.0 * 1024 + 64783.5 * 1024,
 }
 
 
