@@ -87,6 +87,9 @@ SIGNATURES = [
                                               C.c_double, C.c_int32, C.c_int32, _P, C.c_int32,
                                               C.POINTER(_P)]),
     ("hssc_hss1d_box_features", C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
+    ("hssc_hss2d_build_tree_ranked", C.c_int, [_PP, C.c_int32, _P, C.c_int32, C.POINTER(_P)]),
+    ("hssc_tree_id_stats", C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
+    ("hssc_tree_box_features", C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
     ("hssc_hss1d_id_stats", C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
     ("hssc_mlp_create", C.c_int, [C.c_uint64, C.POINTER(_P)]),
     ("hssc_mlp_destroy", None, [_P]),

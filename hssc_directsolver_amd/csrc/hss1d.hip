@@ -930,6 +930,9 @@ int sketch_rows(double pred, int oversample, int m, int n) {
   return std::min(bucket, std::min(m, n));
 }
 
+
+}  // namespace
+
 // HSS2D/predict_rank.m:5-23: rank ~ m_p log10(n) - b_p (r0_p below n = 150) for
 // p = ceil(-log10 acc) clamped to [7, 12].
 double predict_rank_table(int n, double acc) {
@@ -942,8 +945,6 @@ double predict_rank_table(int n, double acc) {
   if (n < 150) return std::min(n, r0[i]);
   return std::min<double>(n, std::ceil(mt[i] * std::log10(double(n)) - bt[i]));
 }
-
-}  // namespace
 }  // namespace hssc
 
 // ---------------------------------------------------------------------------------------

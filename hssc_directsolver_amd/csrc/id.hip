@@ -730,6 +730,119 @@ __global__ void __launch_bounds__(NW * 64, OCC) cpqr_id_reg_kernel(
                    redr, &s_k, Rl);
 }
 
+// Acceptance test of IDs computed on sketches, on the full blocks A_b (m x n, column-major at
+// A + off[b], n <= 256), the check of Common/ID.m:55-59 with one random vector: x a random
+// unit vector over the residual columns and u = T x, e = |A_s u - A_r x| (A_s the k skeleton
+// columns, A_r the others, in the order perm); g a random unit vector over all columns,
+// a = |A g|.  ok[b] = 1 when e <= 10 acc a and the sketch was not saturated (k < s, unless
+// the s sketch rows already cover min(m, n)).  One workgroup per block.
+template <typename T>
+__global__ void __launch_bounds__(256) id_verify_kernel(
+    const int64_t* __restrict__ off, const int* __restrict__ mr, const int* __restrict__ nc,
+    const int* __restrict__ sr, const T* __restrict__ A, const int* __restrict__ perm,
+    const int64_t* __restrict__ poff, const int* __restrict__ kk, const T* __restrict__ Tout,
+    const int64_t* __restrict__ toff, double acc, uint64_t seed, int* __restrict__ ok) {
+  using R = real_t<T>;
+  __shared__ R red[4];
+  __shared__ T xs[256], us[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int m = mr[b], n = nc[b], s = sr[b], k = kk[b], r_ = n - k;
+  const T* M = A + off[b];
+  const int* P = perm + poff[b];
+  const T* Tb = Tout + toff[b];
+  auto bsum = [&](R v) {
+    for (int o = 32; o > 0; o >>= 1) v += shfl_xor_s(v, o);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+  };
+  auto gauss = [&](uint64_t key, uint64_t i) {  // Box-Muller
+    const double u1 = fmax(urand(key, 2 * i), 1e-300), u2 = urand(key, 2 * i + 1);
+    return R(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  };
+  const uint64_t key = seed ^ (uint64_t(b) << 24) ^ 0x7777ull;
+  R v = 0;
+  if (tid < r_) {
+    const R g = gauss(key, tid);
+    xs[tid] = T(g), v = g * g;
+  }
+  const R xn = sqrt(bsum(v));
+  if (tid < r_) xs[tid] = xs[tid] * T(R(1) / xn);
+  __syncthreads();
+  if (tid < k) {
+    T t = T(0);
+    for (int c = 0; c < r_; ++c) t += Tb[tid + int64_t(c) * k] * xs[c];
+    us[tid] = t;
+  }
+  __syncthreads();
+  R e2 = 0;
+  for (int r = tid; r < m; r += 256) {
+    T t = T(0);
+    for (int i = 0; i < k; ++i) t += M[r + int64_t(P[i]) * m] * us[i];
+    for (int c = 0; c < r_; ++c) t -= M[r + int64_t(P[k + c]) * m] * xs[c];
+    e2 += abs2(t);
+  }
+  e2 = bsum(e2);
+  __syncthreads();
+  v = 0;
+  if (tid < n) {
+    const R g = gauss(key ^ 0x5555ull, tid);
+    xs[tid] = T(g), v = g * g;
+  }
+  const R gn = sqrt(bsum(v));
+  if (tid < n) xs[tid] = xs[tid] * T(R(1) / gn);
+  __syncthreads();
+  R a2 = 0;
+  for (int r = tid; r < m; r += 256) {
+    T t = T(0);
+    for (int c = 0; c < n; ++c) t += M[r + int64_t(c) * m] * xs[c];
+    a2 += abs2(t);
+  }
+  a2 = bsum(a2);
+  if (tid == 0) {
+    const int full = m < n ? m : n;
+    const bool saturated = k >= s && s < full;
+    ok[b] = (n <= 256 && !saturated && sqrt(double(e2)) <= 10.0 * acc * sqrt(double(a2))) ? 1 : 0;
+  }
+}
+
+template <typename T>
+void id_verify(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc,
+               const int* sr, const T* A, const int* perm, const int64_t* poff, const int* kk,
+               const T* Tout, const int64_t* toff, double acc, uint64_t seed, int* ok) {
+  if (nmat <= 0) return;
+  hipLaunchKernelGGL(id_verify_kernel<T>, dim3(nmat), dim3(256), 0, st, off, mr, nc, sr, A, perm,
+                     poff, kk, Tout, toff, acc, seed, ok);
+}
+template void id_verify<zdouble>(hipStream_t, int, const int64_t*, const int*, const int*,
+                                 const int*, const zdouble*, const int*, const int64_t*,
+                                 const int*, const zdouble*, const int64_t*, double, uint64_t,
+                                 int*);
+
+// D_j (rows[j] x n, ld rows[j], at doff[j]) = the first rows[j] rows of S_j (ld lds, at
+// soff[j]).  One workgroup per block.
+template <typename T>
+__global__ void __launch_bounds__(256) pack_rows_kernel(
+    const int64_t* __restrict__ soff, int64_t lds, const int64_t* __restrict__ doff,
+    const int* __restrict__ rows, int64_t n, const T* __restrict__ S, T* __restrict__ D) {
+  const int j = blockIdx.x;
+  const int64_t r = rows[j], len = r * n;
+  const T* s = S + soff[j];
+  T* d = D + doff[j];
+  for (int64_t e = threadIdx.x; e < len; e += 256) d[e] = s[(e % r) + (e / r) * lds];
+}
+
+template <typename T>
+void pack_rows(hipStream_t st, int nj, const int64_t* soff, int64_t lds, const int64_t* doff,
+               const int* rows, int64_t n, const T* S, T* D) {
+  if (nj <= 0) return;
+  hipLaunchKernelGGL(pack_rows_kernel<T>, dim3(nj), dim3(256), 0, st, soff, lds, doff, rows, n, S,
+                     D);
+}
+template void pack_rows<zdouble>(hipStream_t, int, const int64_t*, int64_t, const int64_t*,
+                                 const int*, int64_t, const zdouble*, zdouble*);
+
 template <typename T>
 void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc, T* A,
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,

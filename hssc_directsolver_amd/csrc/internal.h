@@ -108,6 +108,10 @@ struct hssc_tree {
   // the tree (they read its device data) holds another; hssc_tree_destroy drops the caller's,
   // and whoever drops the last one frees the tree.
   std::atomic<int> refs{1};
+  // counters of the skeleton IDs (ID-selected levels): blocks, blocks whose sketched ID failed
+  // the acceptance test and were redone by the plain ID, sketch rows, rows of the blocks
+  // factored by the plain ID
+  int64_t id_boxes = 0, id_fallbacks = 0, id_sketch_rows = 0, id_full_rows = 0;
   ~hssc_tree();
 };
 
@@ -131,6 +135,15 @@ struct IdJob {
 // ranks); returns an HSSC_* status.
 using IdSelect = std::function<int(const hssc_params&, const IdJob&, int32_t* order,
                                    int32_t* rank)>;
+
+// HSS2D/predict_rank.m:5-23: the reference's closed-form rank guess of an n-column block.
+double predict_rank_table(int n, double acc);
+
+// Number of features per box fed to the rank network.
+constexpr int kRankFeatures = 8;
+// Features of the boxes of one ID job (nbox x kRankFeatures, column-major): level, log2 n,
+// log2 M, kh times the box width, mean / min / max of the points' kernel weights, -log10 acc.
+void id_job_features(const hssc_params& p, const IdJob& jb, float* X);
 
 // Host builders (tree.cpp).  sel runs the IDs of the levels whose skeletons come from one.
 int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel = nullptr);

@@ -187,6 +187,20 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
              T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed,
              int64_t mmax = 0, int64_t nmax = 0);
 
+// Acceptance of IDs computed on sketches with sr[b] rows, checked on the full blocks (n <= 256):
+// ok[b] = 1 when |A_s T x - A_r x| <= 10 acc |A g| for random unit x, g (Common/ID.m:55-59)
+// and the sketch was not saturated.  T of block b at Tout + toff[b], leading dimension kk[b].
+template <typename T>
+void id_verify(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc,
+               const int* sr, const T* A, const int* perm, const int64_t* poff, const int* kk,
+               const T* Tout, const int64_t* toff, double acc, uint64_t seed, int* ok);
+
+// D_j (rows[j] x n, ld rows[j], at D + doff[j]) = the first rows[j] rows of S_j (ld lds, at
+// S + soff[j]), j < nj: ragged sketches out of a uniform batch.
+template <typename T>
+void pack_rows(hipStream_t st, int nj, const int64_t* soff, int64_t lds, const int64_t* doff,
+               const int* rows, int64_t n, const T* S, T* D);
+
 // G[i] ~ N(0,1), deterministic in (seed, base + i).
 void randn(hipStream_t st, int64_t len, uint64_t seed, uint64_t base, double* G);
 

@@ -18,6 +18,7 @@
 #include <type_traits>
 #include <new>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -371,19 +372,105 @@ struct OwnStream {
   OwnStream& operator=(const OwnStream&) = delete;
 };
 
+void id_job_features(const hssc_params& p, const IdJob& jb, float* X) {
+  const int64_t nb = jb.nbox, n = jb.n;
+  for (int64_t b = 0; b < nb; ++b) {
+    const double* sx = jb.Sx + b * n;
+    const double* sy = jb.Sy + b * n;
+    const double* sw = jb.Sw + b * n;
+    double x0 = sx[0], x1 = sx[0], y0 = sy[0], y1 = sy[0], wm = 0, wlo = sw[0], whi = sw[0];
+    for (int64_t i = 0; i < n; ++i) {
+      x0 = std::min(x0, sx[i]), x1 = std::max(x1, sx[i]);
+      y0 = std::min(y0, sy[i]), y1 = std::max(y1, sy[i]);
+      wm += sw[i], wlo = std::min(wlo, sw[i]), whi = std::max(whi, sw[i]);
+    }
+    const double width = std::max(x1 - x0, y1 - y0) + p.h;
+    const float f[kRankFeatures] = {float(jb.lev), float(std::log2(double(n))),
+                                    float(std::log2(double(jb.M))), float(p.kh * width),
+                                    float(wm / double(n)), float(wlo), float(whi),
+                                    float(-std::log10(p.acc))};
+    for (int q = 0; q < kRankFeatures; ++q) X[b + q * nb] = f[q];
+  }
+}
+
+// How the skeleton IDs of a tree are computed: mode 0 = the plain ID of every block; 1 / 2 =
+// a randomized ID whose sketch size comes from the closed-form rank guess / from the rank
+// network, judged on the full block, with the plain ID as the fallback.
+struct IdPolicy {
+  int mode = 0;
+  hssc_mlp* mlp = nullptr;
+  int oversample = 8;
+  hssc_tree* tree = nullptr;  // receives the counters
+};
+
 // Skeleton selection of one ID level (LOCAL_Non_Preset_Skeleton_Info,
 // HSS2D/HSS2D_build_tree_fsym.m:686-699): A_b = K(X_proxy, X_src) of every key box is
 // generated on the GPU and factored by the batched column-pivoted QR ID with the adaptive
-// rank of Common/ID.m:40-65 (|R_ii| > acc |R_11|, then the randomized check).  Returns the
-// pivot order of every box (skeleton first) and its rank.
-int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank) {
+// rank of Common/ID.m:40-65 (|R_ii| > acc |R_11|, then the randomized check).  With a rank
+// policy the ID first runs on Y_b = G_b A_b (Common/LRID_rand.m:10-40): G_b is the first s_b
+// rows of a Gaussian shared by the level's boxes, s_b = kSketchMargin x the predicted rank of
+// box b (at most n) plus oversampling, rounded up to 8 rows (the box's rank bucket); one
+// batched MFMA product for the level at the largest s_b, then each box's rows are packed to its
+// own leading dimension.  A box whose sketched ID fails the check of ID.m:55-59 on its full
+// block (or whose sketch saturated), or whose bucket has no fewer rows than the block, gets the
+// plain ID.  Returns the pivot order of every box (skeleton first) and its rank.
+constexpr double kSketchMargin = 1.05;  // covers the rank network's typical error (a few percent)
+
+static int id_select_chunk(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank,
+                           const IdPolicy& pol);
+
+// The level's boxes go through in chunks whose device buffers (the blocks, their copies, the
+// sketches and the ID work) stay under kIdChunkBytes.
+constexpr int64_t kIdChunkBytes = int64_t(8) << 30;
+
+int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank,
+                  const IdPolicy& pol) {
   if (int s = check_device()) return s;
   if (p.flag_pot != HSSC_POT_SL_H_2D)
     return fail(HSSC_ERR_UNSUPPORTED, "ID-selected skeletons are defined for kh > 0 only");
   if (jb.n < 1 || jb.M < 1 || jb.n > INT32_MAX || jb.M > INT32_MAX)
     return fail(HSSC_ERR_INVALID_ARG, "empty or oversized ID block");
+  const int64_t n = jb.n, M = jb.M;
+  const int64_t per_box = int64_t(sizeof(zdouble)) * (5 * M * n + n * n / 4 + 7 * n + M);
+  const int64_t chunk = std::max<int64_t>(1, kIdChunkBytes / per_box);
+  for (int64_t b0 = 0; b0 < jb.nbox; b0 += chunk) {
+    IdJob sub = jb;
+    sub.nbox = std::min(chunk, jb.nbox - b0);
+    sub.Sx += b0 * n, sub.Sy += b0 * n, sub.Sw += b0 * n;
+    sub.Qx += b0 * M, sub.Qy += b0 * M, sub.Qw += b0 * M;
+    if (int s = id_select_chunk(p, sub, order + b0 * n, rank + b0, pol)) return s;
+  }
+  return HSSC_OK;
+}
+
+static int id_select_chunk(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank,
+                           const IdPolicy& pol) {
   return guarded([&]() {
     const int64_t nb = jb.nbox, n = jb.n, M = jb.M;
+    const uint64_t seed = 0x5EEDull ^ uint64_t(jb.lev);
+    // sketch rows of every box (0: plain ID)
+    std::vector<int32_t> sb(nb, 0);
+    int64_t s = 0;  // the largest
+    if (pol.mode != 0) {
+      std::vector<double> pred(nb, 1.0);
+      if (pol.mode == 1) {
+        std::fill(pred.begin(), pred.end(), predict_rank_table(int(n), p.acc));
+      } else {
+        std::vector<float> X(nb * kRankFeatures), y(nb);
+        id_job_features(p, jb, X.data());
+        if (int st = hssc_mlp_predict(pol.mlp, X.data(), int32_t(nb), y.data()))
+          throw HipError{st};
+        for (int64_t b = 0; b < nb; ++b) pred[b] = std::exp2(double(y[b]));
+      }
+      for (int64_t b = 0; b < nb; ++b) {
+        // the rank is at most n: a sketch of n + oversample rows keeps the whole column space
+        const double r = std::min(std::max(1.0, kSketchMargin * pred[b]), double(n)) +
+                         pol.oversample;
+        const int64_t sr = (int64_t(std::ceil(r)) + 7) / 8 * 8;
+        sb[b] = sr < M ? int32_t(sr) : 0;  // no fewer rows than the block: the plain ID
+        s = std::max<int64_t>(s, sb[b]);
+      }
+    }
     const int64_t tsz = n * n / 4 + n, wsz = 6 * n + M;  // T is k x (n-k) <= n^2/4
     DevBufs B;
     const double* Sx = B.upload(jb.Sx, nb * n);
@@ -396,7 +483,6 @@ int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t
     std::vector<int32_t> mr(nb, int32_t(M)), nc(nb, int32_t(n)), kfix(nb, 0);
     for (int64_t b = 0; b < nb; ++b)
       off[b] = b * M * n, poff[b] = b * n, toff[b] = b * tsz, woff[b] = b * wsz;
-    const int64_t* doff = B.upload(off.data(), nb);
     const int64_t* dpoff = B.upload(poff.data(), nb);
     const int64_t* dtoff = B.upload(toff.data(), nb);
     const int64_t* dwoff = B.upload(woff.data(), nb);
@@ -404,22 +490,91 @@ int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t
     const int32_t* dnc = B.upload(nc.data(), nb);
     const int32_t* dkfix = B.upload(kfix.data(), nb);
     zdouble* A = B.get<zdouble>(nb * M * n);
-    zdouble* Ac = B.get<zdouble>(nb * M * n);
     zdouble* Tt = B.get<zdouble>(nb * tsz);
     zdouble* W = B.get<zdouble>(nb * wsz);
     int32_t* dperm = B.get<int32_t>(nb * n);
-    int32_t* dk = B.get<int32_t>(nb);
     const hipStream_t st = 0;
     k::gen_block<zdouble>(st, M, n, nb, Qx, Qy, Qw, M, Sx, Sy, Sw, n, A, M, M * n, kernel_spec(p));
-    HIPCHK(hipMemcpyAsync(Ac, A, size_t(nb * M * n) * sizeof(zdouble), hipMemcpyDeviceToDevice, st));
-    k::cpqr_id<zdouble>(st, int(nb), doff, dmr, dnc, A, Ac, p.acc, dkfix, dperm, dpoff, dk, Tt,
-                        dtoff, W, dwoff, 0x5EEDull ^ uint64_t(jb.lev), M, n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> ok(nb, 0);
+    std::vector<int64_t> sj;  // the boxes that get a sketch
+    for (int64_t b = 0; b < nb; ++b)
+      if (sb[b] > 0) sj.push_back(b);
+    const int64_t ns = int64_t(sj.size());
+    int64_t sketch_rows = 0;
+    if (ns > 0) {
+      // Ytmp_b = G A_b (G: s x M Gaussian shared by the level), then box b keeps its first
+      // s_b rows in Y at leading dimension s_b
+      std::vector<zdouble> G(s * M);
+      std::mt19937_64 rng(seed ^ 0x9E3779B97F4A7C15ull);
+      std::normal_distribution<double> nd;
+      for (auto& g : G) g = zdouble(nd(rng));
+      const zdouble* dG = B.upload(G.data(), s * M);
+      zdouble* Ytmp = B.get<zdouble>(nb * s * n);
+      k::gemm<zdouble>(st, false, false, s, n, M, zdouble(1), dG, s, 0, A, M, M * n, zdouble(0),
+                       Ytmp, s, s * n, nb);
+      std::vector<int64_t> soff(ns), yoff(ns), aoff(ns), jpoff(ns), jtoff(ns), jwoff(ns);
+      std::vector<int32_t> sr(ns);
+      for (int64_t j = 0; j < ns; ++j) {
+        const int64_t b = sj[j];
+        soff[j] = b * s * n, yoff[j] = sketch_rows * n, sr[j] = sb[b];
+        aoff[j] = off[b], jpoff[j] = poff[b], jtoff[j] = toff[b], jwoff[j] = woff[b];
+        sketch_rows += sb[b];
+      }
+      zdouble* Y = B.get<zdouble>(sketch_rows * n);
+      zdouble* Yc = B.get<zdouble>(sketch_rows * n);
+      const int32_t* dsr = B.upload(sr.data(), ns);
+      const int64_t* dyoff = B.upload(yoff.data(), ns);
+      k::pack_rows<zdouble>(st, int(ns), B.upload(soff.data(), ns), s, dyoff, dsr, n, Ytmp, Y);
+      HIPCHK(hipMemcpyAsync(Yc, Y, size_t(sketch_rows * n) * sizeof(zdouble),
+                            hipMemcpyDeviceToDevice, st));
+      const int64_t* djp = B.upload(jpoff.data(), ns);
+      const int64_t* djt = B.upload(jtoff.data(), ns);
+      int32_t* dks = B.get<int32_t>(ns);
+      int32_t* dok = B.get<int32_t>(ns);
+      k::cpqr_id<zdouble>(st, int(ns), dyoff, dsr, dnc, Y, Yc, p.acc, dkfix, dperm, djp, dks, Tt,
+                          djt, W, B.upload(jwoff.data(), ns), seed, s, n);
+      k::id_verify<zdouble>(st, int(ns), B.upload(aoff.data(), ns), dmr, dnc, dsr, A, dperm, djp,
+                            dks, Tt, djt, p.acc, seed ^ 0xABCDull, dok);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      std::vector<int32_t> oks(ns), ks(ns);
+      HIPCHK(hipMemcpy(oks.data(), dok, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ks.data(), dks, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int64_t j = 0; j < ns; ++j) ok[sj[j]] = oks[j], rank[sj[j]] = ks[j];
+    }
+    // the plain ID (on the full blocks) for every box without an accepted sketch
+    std::vector<int64_t> fall;
+    for (int64_t b = 0; b < nb; ++b)
+      if (!ok[b]) fall.push_back(b);
+    if (!fall.empty()) {
+      const int64_t nf = int64_t(fall.size());
+      std::vector<int64_t> foff(nf), fpoff(nf), ftoff(nf), fwoff(nf);
+      for (int64_t j = 0; j < nf; ++j)
+        foff[j] = off[fall[j]], fpoff[j] = poff[fall[j]], ftoff[j] = toff[fall[j]],
+        fwoff[j] = woff[fall[j]];
+      zdouble* Ac = B.get<zdouble>(nb * M * n);
+      HIPCHK(hipMemcpyAsync(Ac, A, size_t(nb * M * n) * sizeof(zdouble), hipMemcpyDeviceToDevice, st));
+      int32_t* dkf = B.get<int32_t>(nf);
+      k::cpqr_id<zdouble>(st, int(nf), B.upload(foff.data(), nf), dmr, dnc, A, Ac, p.acc, dkfix,
+                          dperm, B.upload(fpoff.data(), nf), dkf, Tt, B.upload(ftoff.data(), nf), W,
+                          B.upload(fwoff.data(), nf), seed, M, n);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      std::vector<int32_t> kf(nf);
+      HIPCHK(hipMemcpy(kf.data(), dkf, size_t(nf) * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int64_t j = 0; j < nf; ++j) rank[fall[j]] = kf[j];
+    }
     HIPCHK(hipMemcpy(order, dperm, size_t(nb * n) * sizeof(int32_t), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(rank, dk, size_t(nb) * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t b = 0; b < nb; ++b)
       if (rank[b] < 0 || rank[b] > n) return fail(HSSC_ERR_HIP, "skeleton ID returned a bad rank");
+    if (pol.tree) {
+      int64_t failed = 0;
+      for (int64_t b : sj) failed += ok[b] ? 0 : 1;
+      pol.tree->id_boxes += nb;
+      pol.tree->id_fallbacks += failed;
+      pol.tree->id_sketch_rows += sketch_rows;
+      pol.tree->id_full_rows += int64_t(fall.size()) * M;
+    }
     return HSSC_OK;
   });
 }
@@ -1238,13 +1393,17 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
   });
 }
 
-int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
+static int build_index_sets_policy(const hssc_params* p, hssc_tree** tree, IdPolicy pol) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
   if (int s = check_params(*p)) return s;
+  if (pol.mode < 0 || pol.mode > 2 || (pol.mode == 2 && !pol.mlp) || pol.oversample < 0)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid rank prediction arguments");
   return guarded([&]() {
     auto* t = new hssc_tree();
-    const int s = build_index_sets(*p, t, id_select_gpu);
+    pol.tree = t;
+    const int s = build_index_sets(*p, t, [&](const hssc_params& pp, const IdJob& jb, int32_t* o,
+                                              int32_t* r) { return id_select_gpu(pp, jb, o, r, pol); });
     if (s != HSSC_OK) {
       delete t;
       return s;
@@ -1254,12 +1413,12 @@ int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
   });
 }
 
-int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree) {
+static int build_tree_policy(const hssc_params* p, hssc_tree** tree, const IdPolicy& pol) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
   if (int s = check_device()) return s;
   pool::reclaim();  // the device is expected idle here: released blocks become reusable
-  int s = hssc_hss2d_build_index_sets(p, tree);
+  int s = build_index_sets_policy(p, tree, pol);
   if (s != HSSC_OK) return s;
   hssc_tree* t = *tree;
   s = guarded([&]() {
@@ -1272,6 +1431,40 @@ int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree) {
     *tree = nullptr;
   }
   return s;
+}
+
+int hssc_hss2d_build_index_sets(const hssc_params* p, hssc_tree** tree) {
+  return build_index_sets_policy(p, tree, IdPolicy{});
+}
+
+int hssc_hss2d_build_tree(const hssc_params* p, hssc_tree** tree) {
+  return build_tree_policy(p, tree, IdPolicy{});
+}
+
+int hssc_hss2d_build_tree_ranked(const hssc_params* p, int32_t rank_mode, hssc_mlp* mlp,
+                                 int32_t oversample, hssc_tree** tree) {
+  IdPolicy pol;
+  pol.mode = rank_mode, pol.mlp = mlp, pol.oversample = oversample;
+  return build_tree_policy(p, tree, pol);
+}
+
+int hssc_tree_id_stats(const hssc_tree* t, int64_t* boxes, int64_t* fallbacks,
+                       int64_t* sketch_rows, int64_t* full_rows) {
+  if (!t) return fail(HSSC_ERR_INVALID_ARG, "null tree");
+  if (boxes) *boxes = t->id_boxes;
+  if (fallbacks) *fallbacks = t->id_fallbacks;
+  if (sketch_rows) *sketch_rows = t->id_sketch_rows;
+  if (full_rows) *full_rows = t->id_full_rows;
+  return HSSC_OK;
+}
+
+int hssc_tree_box_features(const hssc_tree* t, int32_t lev, float* X) {
+  if (!t || !X || lev < 0 || lev > t->depth) return fail(HSSC_ERR_INVALID_ARG, "bad level");
+  const Level& L = t->levels[lev];
+  IdJob jb{lev, L.nkey, L.n, L.M, L.PNx.data(), L.PNy.data(), L.PNw.data(),
+           L.Xpx.data(), L.Xpy.data(), L.Xpw.data()};
+  id_job_features(t->p, jb, X);
+  return HSSC_OK;
 }
 
 int hssc_tree_depth(const hssc_tree* t, int32_t* depth) {

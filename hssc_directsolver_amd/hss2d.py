@@ -152,6 +152,21 @@ class Tree:
                                         out.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(idl)))
         return out, bool(idl.value)
 
+    def id_stats(self):
+        """(blocks, fallbacks to the plain ID, sketch rows, rows factored by the plain ID) of
+        the skeleton IDs."""
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().hssc_tree_id_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("blocks", "fallbacks", "sketch_rows", "full_rows"), (x.value for x in v)))
+
+    def box_features(self, lev):
+        """Rank-network features (nkey x 8) of the key boxes of a level."""
+        nkey = self.level_info(lev)["nkey"]
+        X = np.empty((8, nkey), dtype=np.float32)  # column-major nkey x 8
+        check(lib().hssc_tree_box_features(self.handle, int(lev),
+                                           X.ctypes.data_as(C.POINTER(C.c_float))))
+        return X.T.copy()
+
     def _i64(self, fn, box, size):
         out = np.empty(size, dtype=np.int64)
         check(fn(self.handle, int(box), out.ctypes.data_as(C.POINTER(C.c_int64))))
@@ -184,6 +199,9 @@ class Tree:
         return buf.reshape((r, k)).T.copy() if k * r else out
 
 
+RANK_MODES = {"id": 0, "table": 1, "mlp": 2}
+
+
 def _tree(params: Params, index_only=False) -> Tree:
     h = C.c_void_p()
     fn = lib().hssc_hss2d_build_index_sets if index_only else lib().hssc_hss2d_build_tree
@@ -191,13 +209,28 @@ def _tree(params: Params, index_only=False) -> Tree:
     return Tree(h, params)
 
 
-def HSS2D_build_tree_fsym(params: Params) -> Tree:
-    """HSS2D/HSS2D_build_tree_fsym.m:1 (index sets on the host, T on the GPU)."""
-    return _tree(params)
+def HSS2D_build_tree_fsym(params: Params, rank="id", mlp=None, oversample=8) -> Tree:
+    """HSS2D/HSS2D_build_tree_fsym.m:1 (index sets on the host, T on the GPU).
+
+    ``rank`` sizes the skeleton IDs of a kh > 0 tree: 'id' runs the plain ID on every block
+    (the reference's HSS2D_build_tree_fsym.m:686-729); 'table' sketches each level with the
+    closed-form rank guess of HSS2D/predict_rank.m; 'mlp' with the prediction of ``mlp`` (a
+    RankMLP trained on box_features).  Sketched IDs that fail the check on their full block
+    fall back to the plain ID."""
+    if rank == "id":
+        return _tree(params)
+    if rank == "mlp" and mlp is None:
+        raise ValueError("rank='mlp' needs a trained RankMLP")
+    h = C.c_void_p()
+    check(lib().hssc_hss2d_build_tree_ranked(C.byref(params.raw), RANK_MODES[rank],
+                                             mlp.handle if mlp is not None else None,
+                                             int(oversample), C.byref(h)))
+    return Tree(h, params)
 
 
 def HSS2D_build_index_sets(params: Params) -> Tree:
-    """Tree geometry and index sets only (no interpolation operators, no GPU needed)."""
+    """Tree geometry and index sets only (no interpolation operators; the ID-selected skeleton
+    levels run on the GPU)."""
     return _tree(params, index_only=True)
 
 

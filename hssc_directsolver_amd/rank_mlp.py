@@ -125,3 +125,49 @@ def train_rank_mlp(X, t, seed=0, epochs=3000, lr=3e-3):
     m.fit_scaling(X, t)
     loss = m.train(X, t, epochs=epochs, lr=lr)
     return m, loss
+
+
+# ---- HSS2D: ranks of the ID-selected skeleton levels (kh > 0) -----------------------------
+
+def hss2d_geometry(rng, Ng_choices=(6, 8, 10), np_choices=(4, 8, 16, 32, 64),
+                   kh_range=(4.0, 40.0)):
+    """A seeded Lippmann-Schwinger configuration: grid (Ng, np), wavenumber log-uniform in
+    kh_range, contrast the tanh plateau or a Gaussian of random amplitude and width."""
+    from .hss2d import HSS_tree_parameters
+    Ng = int(rng.choice(Ng_choices))
+    np_ = int(rng.choice(np_choices))
+    kh = float(np.exp(rng.uniform(np.log(kh_range[0]), np.log(kh_range[1]))))
+    contrast = None
+    if rng.uniform() < 0.5:
+        contrast = ("gaussian", float(rng.uniform(0.2, 1.0)), float(rng.uniform(0.1, 0.5)))
+    # the Duan-Rokhlin weight needs h k <= 2
+    kh = min(kh, 1.9 * Ng * np_ / 2)
+    return HSS_tree_parameters("SL_H_2D", kh, 1, Ng, np_, 1e-10, 3, 800, 0, proxy_layers=4,
+                               contrast=contrast)
+
+
+def hss2d_box_dataset(tree):
+    """(features, log2 rank) of the boxes of the ID-selected levels of a tree."""
+    Xs, ts = [], []
+    for lev in range(tree.depth + 1):
+        ranks, idl = tree.box_ranks(lev)
+        if not idl:
+            continue
+        Xs.append(tree.box_features(lev))
+        ts.append(np.log2(np.maximum(ranks, 1)).astype(np.float32))
+    return np.concatenate(Xs), np.concatenate(ts)
+
+
+def collect_hss2d(ngeom, seed=0, max_boxes=384, **kw):
+    """Ranks the plain ID produces at the ID-selected levels of `ngeom` seeded geometries, at
+    most `max_boxes` boxes drawn from each (large grids would outweigh the others)."""
+    from .hss2d import HSS2D_build_index_sets
+    rng = np.random.default_rng(seed)
+    Xs, ts = [], []
+    for _ in range(ngeom):
+        X, t = hss2d_box_dataset(HSS2D_build_index_sets(hss2d_geometry(rng, **kw)))
+        if len(t) > max_boxes:
+            pick = rng.choice(len(t), max_boxes, replace=False)
+            X, t = X[pick], t[pick]
+        Xs.append(X), ts.append(t)
+    return np.concatenate(Xs), np.concatenate(ts)

@@ -300,6 +300,25 @@ int hssc_mlp_grads(hssc_mlp* m, const float* X, const float* t, int32_t B, doubl
 int hssc_mlp_train(hssc_mlp* m, const float* X, const float* t, int32_t B, int32_t epochs,
                    float lr, double* loss);
 
+/* HSS2D_build_tree_fsym (hssc_hss2d_build_tree) with rank-predicted randomized IDs for the
+ * ID-selected skeletons (kh > 0, two bottom levels; HSS2D_build_tree_fsym.m:686-729), in
+ * place of the closed-form guesses of HSS2D/predict_rank.m:5-23 and
+ * interpolation_operator.m:101-117 that size the reference's randomized IDs.  rank_mode 0: the
+ * plain ID (as hssc_hss2d_build_tree); 1: sketch rows from the closed-form guess; 2: from the
+ * network `mlp` (features of hssc_tree_box_features).  A level's boxes share one Gaussian
+ * sketch of s rows (the largest predicted rank plus `oversample`, rounded up to 16) applied by
+ * one batched complex MFMA product; each sketched ID is judged on its full block by the check
+ * of Common/ID.m:55-59 and redone by the plain ID when it fails. */
+int hssc_hss2d_build_tree_ranked(const hssc_params* p, int32_t rank_mode, hssc_mlp* mlp,
+                                 int32_t oversample, hssc_tree** tree);
+/* Counters of the skeleton IDs of a tree: blocks, blocks redone by the plain ID, sketch rows
+ * used and rows of the blocks factored by the plain ID. */
+int hssc_tree_id_stats(const hssc_tree* t, int64_t* boxes, int64_t* fallbacks,
+                       int64_t* sketch_rows, int64_t* full_rows);
+/* Rank-network features of the key boxes of a level (nkey x 8, column-major): level, log2 n,
+ * log2 M, kh times the box width, mean / min / max of the points' kernel weights, -log10 acc. */
+int hssc_tree_box_features(const hssc_tree* t, int32_t lev, float* X);
+
 /* HSS1D compression with rank-predicted randomized IDs.  rank_mode 0: the plain ID
  * (hssc_hss1d_compress); 1: sketch rows from the closed-form rank guess of
  * HSS2D/predict_rank.m:5-23; 2: from the network `mlp`.  Each block's sketch has the
