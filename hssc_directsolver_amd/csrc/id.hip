@@ -562,6 +562,9 @@ __global__ void __launch_bounds__(NW * 64, OCC) cpqr_id_reg_kernel(
       }
       for (int i = RREG; i < RW; ++i) colbuf[i * NW + wave] = lrow ? la[lidx(i, p)] : T(0);
     }
+    // every wave has read at[] and cur[] for this step before they change: a wave that read
+    // them mid-swap would take another pivot column than the rest
+    __syncthreads();
     if (tid == 0) {  // swap positions j and cur[p]
       const int cj = at[j], pp = cur[p];
       at[pp] = cj, cur[cj] = pp, at[j] = p, cur[p] = j;
