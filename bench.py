@@ -75,6 +75,11 @@ def parse():
     ap.add_argument("--cpu-stop", type=int, default=3,
                     help="CPU baseline: levels measured down to this one, above extrapolated")
     ap.add_argument("--no-hss1d", action="store_true", help="skip the HSS1D side case")
+    ap.add_argument("--rank", choices=("id", "table", "mlp"), default="mlp",
+                    help="skeleton IDs of the ID-selected levels: plain ID, or randomized IDs "
+                         "with sketches sized by the closed-form guess / by the rank network "
+                         "(trained before the timed steps on the plain-ID ranks of seeded "
+                         "geometries)")
     return ap.parse_args()
 
 
@@ -96,13 +101,14 @@ def stats_snapshot(H):
     return out
 
 
-def one_step(H, P, torch, b, tol, split=False):
+def one_step(H, P, torch, b, tol, split=False, rank="id", mlp=None):
     """Runs each path end to end: the tree with its interpolation operators, the exact grid
     operator (FFT convolution, the refinement's residual operator), the inverse, and the solve
     refined until the true relative residual ||b - A y|| / ||b|| <= tol.
       mixed ("fp32"): FP32 / complex64 interpolation operators (skeleton IDs in FP64) and
                       FP32 / complex64 factorization;
       plain ("fp64"): everything in FP64 / complex128.
+    `rank` / `mlp` select the skeleton IDs of the ID-selected levels (HSS2D_build_tree_fsym).
     Returns timings (s) and residual info per path.  Without `split` the tree and the inversion
     are called back to back with no host synchronisation, as a caller would: the inversion
     waits on the device, level by level, for the interpolation operators it needs, so the two
@@ -113,7 +119,7 @@ def one_step(H, P, torch, b, tol, split=False):
         P.raw.tree_precision = 1 if prec == "fp32" else 0
         sync()
         a = time.perf_counter()
-        tree = H.HSS2D_build_tree_fsym(P)
+        tree = H.HSS2D_build_tree_fsym(P, rank=rank, mlp=mlp)
         A = H.GridOperator(P)
         if split:
             sync()
@@ -125,7 +131,7 @@ def one_step(H, P, torch, b, tol, split=False):
         it, rel = H.solve_exact_device(inv, A, b, y, tol=tol, maxit=12)
         sync()
         d = time.perf_counter()
-        res[prec] = dict(build=c - a, solve=d - c, iters=it, relres=rel)
+        res[prec] = dict(build=c - a, solve=d - c, iters=it, relres=rel, ids=tree.id_stats())
         if split:
             res[prec].update(tree=m - a, invert=c - m)
         del inv, A, tree
@@ -358,7 +364,10 @@ def make_problem(H, torch, kernel, args):
               f"{args.proxy_layers} dense blocks, tanh-plateau contrast (Bump_function "
               "option 4), "
               + ("'bdry' skeletons at every level" if args.preset_all else
-                 "ID skeletons at the two bottom levels"))
+                 "ID skeletons at the two bottom levels" + {
+                     "id": "",
+                     "table": " (randomized IDs, sketches sized by the closed-form rank guess)",
+                     "mlp": " (randomized IDs, sketches sized by the rank network)"}[args.rank]))
     else:
         P = H.HSS_tree_parameters("SL_L_2D", 0, 1, args.Ng, args.np_, 1e-10, 2, 400, args.TI)
         b = torch.randn(P.N, generator=gen, dtype=torch.float64)
@@ -433,29 +442,48 @@ def main():
         return
     P, b, workload = make_problem(H, torch, args.kernel, args)
     N = P.N
+    rank = args.rank if args.kernel == "helmholtz" and not args.preset_all else "id"
+    mlp, rank_info = None, None
+    if rank == "mlp":
+        # the rank network, trained before the timed steps on the plain-ID ranks of 32 seeded
+        # geometries (grids 24^2..640^2, kh 4..40, plateau or Gaussian contrast)
+        from hssc_directsolver_amd.rank_mlp import collect_hss2d, train_rank_mlp
+        t0 = time.perf_counter()
+        Xr, tr = collect_hss2d(32, seed=7)
+        mlp, loss = train_rank_mlp(Xr, tr, seed=0, epochs=2000)
+        rank_info = dict(samples=len(tr), train_loss=round(loss, 6),
+                         collect_and_train_s=round(time.perf_counter() - t0, 2))
+    step = lambda **kw: one_step(H, P, torch, b, args.tol, rank=rank, mlp=mlp, **kw)  # noqa: E731
     if args.profile_step:
-        r = one_step(H, P, torch, b, args.tol)
+        r = step()
         print(json.dumps({"profile_step": {k: round(v["build"] + v["solve"], 4)
                                            for k, v in r.items()}}), flush=True)
         return
 
     for _ in range(args.warmup):
-        one_step(H, P, torch, b, args.tol)
+        step()
 
     L = H.lib()
     torch.cuda.synchronize()
     T0 = time.perf_counter()
     runs = []
     for _ in range(args.steps):
-        runs.append(one_step(H, P, torch, b, args.tol))
+        runs.append(step())
     torch.cuda.synchronize()
     T1 = time.perf_counter()
-    split = one_step(H, P, torch, b, args.tol, split=True)
+    split = step(split=True)
+    if rank != "id":
+        # the same synchronised step with plain skeleton IDs: what the rank buckets save
+        plain = one_step(H, P, torch, b, args.tol, split=True)
+        rank_info = dict(mode=rank, **(rank_info or {}),
+                         ids=split["fp32"]["ids"], plain_ids=plain["fp32"]["ids"],
+                         tree_s={"buckets": {p_: round(split[p_]["tree"], 4) for p_ in split},
+                                 "plain": {p_: round(plain[p_]["tree"], 4) for p_ in plain}})
     # One more step of the same workload with per-kernel HIP-event instrumentation (same launches
     # and streams as the timed steps, so event times match a kernel trace of those steps).
     L.hssc_stats_reset()
     L.hssc_stats_enable(1)
-    one_step(H, P, torch, b, args.tol)
+    step()
     torch.cuda.synchronize()
     L.hssc_stats_enable(0)
     stats = stats_snapshot(H)
@@ -495,6 +523,7 @@ def main():
                             gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None)
                     for k, v in stats.items()},
         "cpu_baseline": cpu,
+        "rank_buckets": rank_info,
         "levels": levels,
         "laplace": side,
         "hss1d": h1,
