@@ -647,7 +647,12 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
   // streams sharing a queue run one after the other); twice as many streams as queues keeps
   // every queue fed whatever the mapping.
   const int kLevelStreams = 2 * hw_queues();
-  constexpr int64_t kOuterNb = 128;  // outer panel width
+  // outer panel width (a multiple of the 32-column sub-panel; HSSC_LS_OUTER_NB overrides it)
+  static const int64_t kOuterNb = [] {
+    const char* v = std::getenv("HSSC_LS_OUTER_NB");
+    const int64_t w = v ? std::atoll(v) : 128;
+    return std::min<int64_t>(512, std::max<int64_t>(32, w / 32 * 32));
+  }();
   // regularisation weight of FP32 trees relative to the RMS column norm of K(Xp, X_sk)
   // (HSSC_TIKHONOV_ETA overrides it)
   static const double kTikhonovEta = [] {
