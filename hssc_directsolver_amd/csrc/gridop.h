@@ -18,5 +18,7 @@ void grid_op_apply(const hssc_grid_op* op, const void* X, const void* B, void* R
 int64_t grid_op_n(const hssc_grid_op* op);
 bool grid_op_complex(const hssc_grid_op* op);
 int grid_op_device(const hssc_grid_op* op);
+// Distinct for every operator the process creates (no two live or past operators share one).
+uint64_t grid_op_uid(const hssc_grid_op* op);
 
 }  // namespace hssc

@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -43,6 +45,7 @@ struct hssc_grid_op {
   hssc::zdouble* tw = nullptr;    // L / 2 twiddles exp(-2 pi i k / L)
   hssc::zdouble* U = nullptr;     // L x L work arrays
   hssc::zdouble* V = nullptr;
+  uint64_t uid = 0;               // unique per operator of the process
   ~hssc_grid_op() {
     (void)hipDeviceSynchronize();
     for (void* q : {(void*)b, (void*)c, (void*)khat, (void*)tw, (void*)U, (void*)V})
@@ -257,6 +260,7 @@ void grid_op_apply(const hssc_grid_op* op, const void* X, const void* B, void* R
 int64_t grid_op_n(const hssc_grid_op* op) { return op->N; }
 bool grid_op_complex(const hssc_grid_op* op) { return op->cx; }
 int grid_op_device(const hssc_grid_op* op) { return op->device; }
+uint64_t grid_op_uid(const hssc_grid_op* op) { return op->uid; }
 
 }  // namespace hssc
 
@@ -279,6 +283,8 @@ int hssc_grid_op_create(const hssc_params* p, hssc_grid_op** out) {
     return fail(HSSC_ERR_HIP, "no HIP device is available");
   return guarded_grid([&]() {
     auto* op = new hssc_grid_op();
+    static std::atomic<uint64_t> next_uid{1};
+    op->uid = next_uid.fetch_add(1);
     try {
       op->p = *p;
       GCHK(hipGetDevice(&op->device));

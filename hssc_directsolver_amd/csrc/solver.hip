@@ -578,6 +578,32 @@ static int id_select_chunk(const hssc_params& p, const IdJob& jb, int32_t* order
     return HSSC_OK;
   });
 }
+// Refinement steps replayed from HIP graphs, for repeated solves with one inverse and one
+// residual operator: "first" = {Y = Ainv B; R = B - A Y; partial sums of |R|^2 and |B|^2} and
+// "step" = {D = Ainv R; Y += D; R = B - A Y; partial sums of |R|^2}, on buffers (and a solve
+// workspace) of their own; the caller's B is copied in and Y out.
+struct RefineGraphs {
+  uint64_t op = 0;  // the residual operator's uid
+  int64_t m = 0;
+  int calls = 0;    // solves with this operator and m
+  hipStream_t cs = nullptr;
+  hipGraphExec_t first = nullptr, step = nullptr;
+  void *B = nullptr, *Y = nullptr, *R = nullptr, *D = nullptr;
+  double* part = nullptr;  // 512 partial sums: |R|^2, then |B|^2
+  SolveWorks w;
+  void release() {
+    if (cs) (void)hipStreamSynchronize(cs);
+    if (first) (void)hipGraphExecDestroy(first);
+    if (step) (void)hipGraphExecDestroy(step);
+    if (cs) (void)hipStreamDestroy(cs);
+    dfree(B), dfree(Y), dfree(R), dfree(D), dfree(part);
+    w.release();
+    cs = nullptr, first = step = nullptr;
+    B = Y = R = D = nullptr, part = nullptr;
+    op = 0, m = 0, calls = 0;
+  }
+  ~RefineGraphs() { release(); }
+};
 }  // namespace hssc
 
 struct hssc_inverse {
@@ -585,8 +611,10 @@ struct hssc_inverse {
   int prec = HSSC_PREC_FP64;
   std::vector<void*> FI, E;  // per level, nkey x n x n and nkey x k x k (J order)
   mutable hssc::SolveWorks w;
+  mutable hssc::RefineGraphs graphs;
   explicit hssc_inverse(hssc_tree* t) : tree(t) { hssc::tree_ref(t); }
   ~hssc_inverse() {
+    graphs.release();
     for (void* p : FI) dfree(p);
     for (void* p : E) dfree(p);
     w.release();
@@ -1171,21 +1199,45 @@ double device_norm(const S* x, int64_t len, hipStream_t st, double* part_d) {
   return std::sqrt(s);
 }
 
-void applyinv_dispatch(const hssc_inverse* inv, const void* X, void* Y, int64_t m,
-                       hipStream_t st) {
+// applyinv with the solve workspace ws (the inverse's own, or a graph's).
+void applyinv_with(const hssc_inverse* inv, SolveWorks& ws, const void* X, void* Y, int64_t m,
+                   hipStream_t st) {
   tree_sync(inv->tree);
   const bool fp64 = inv->prec == HSSC_PREC_FP64;
   if (inv->tree->cx) {
     const auto* Xz = static_cast<const zdouble*>(X);
     auto* Yz = static_cast<zdouble*>(Y);
-    if (fp64) applyinv_impl<zdouble>(inv, inv->w.get<zdouble>(), Xz, Yz, m, st);
-    else applyinv_impl<zfloat>(inv, inv->w.get<zfloat>(), Xz, Yz, m, st);
+    if (fp64) applyinv_impl<zdouble>(inv, ws.get<zdouble>(), Xz, Yz, m, st);
+    else applyinv_impl<zfloat>(inv, ws.get<zfloat>(), Xz, Yz, m, st);
   } else {
     const auto* Xd = static_cast<const double*>(X);
     auto* Yd = static_cast<double*>(Y);
-    if (fp64) applyinv_impl<double>(inv, inv->w.get<double>(), Xd, Yd, m, st);
-    else applyinv_impl<float>(inv, inv->w.get<float>(), Xd, Yd, m, st);
+    if (fp64) applyinv_impl<double>(inv, ws.get<double>(), Xd, Yd, m, st);
+    else applyinv_impl<float>(inv, ws.get<float>(), Xd, Yd, m, st);
   }
+}
+
+void applyinv_dispatch(const hssc_inverse* inv, const void* X, void* Y, int64_t m,
+                       hipStream_t st) {
+  applyinv_with(inv, inv->w, X, Y, m, st);
+}
+
+// The workspace of applyinv_with for m right-hand sides, allocated ahead (graph capture
+// allocates nothing).
+void ensure_works(const hssc_inverse* inv, SolveWorks& ws, int64_t m) {
+  const bool fp64 = inv->prec == HSSC_PREC_FP64;
+  if (inv->tree->cx) fp64 ? ws.get<zdouble>().ensure(inv->tree, m) : ws.get<zfloat>().ensure(inv->tree, m);
+  else fp64 ? ws.get<double>().ensure(inv->tree, m) : ws.get<float>().ensure(inv->tree, m);
+}
+
+// Solves replayed from graphs: on (HSSC_SOLVE_GRAPHS unset or 1) unless kernel statistics
+// are being collected (their events do not belong in a graph).
+bool solve_graphs_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("HSSC_SOLVE_GRAPHS");
+    return !v || std::atoi(v) != 0;
+  }();
+  return on && !stats::enabled();
 }
 
 // applyinv with the solve workspace laid out in caller-provided device memory of cap bytes.
@@ -1243,11 +1295,102 @@ int solve_impl(const hssc_inverse* inv, const hssc_operator* op, const S* B, S* 
   });
 }
 
-// The refinement with A = the exact grid operator (FFT convolution).
+// The refinement of refine_impl replayed from the inverse's graphs (captured on first use
+// for this residual operator and m).  resid(y, b, r, s) forms r = b - A y on stream s.
+template <typename S, typename Resid>
+int refine_graphed(const hssc_inverse* inv, RefineGraphs& G, const S* B, S* Y, int64_t m,
+                   double tol, int32_t maxit, int32_t* iters, double* relres, hipStream_t st,
+                   Resid&& resid) {
+  const int64_t len = inv->tree->N * m;
+  const size_t bytes = size_t(len) * sizeof(S);
+  if (!G.first) {
+    tree_sync(inv->tree);
+    HIPCHK(hipStreamCreateWithFlags(&G.cs, hipStreamNonBlocking));
+    G.B = dalloc<S>(len), G.Y = dalloc<S>(len), G.R = dalloc<S>(len), G.D = dalloc<S>(len);
+    G.part = dalloc<double>(512);
+    ensure_works(inv, G.w, m);
+    S* gB = static_cast<S*>(G.B);
+    S* gY = static_cast<S*>(G.Y);
+    S* gR = static_cast<S*>(G.R);
+    S* gD = static_cast<S*>(G.D);
+    auto capture = [&](hipGraphExec_t* out, auto&& body) {
+      HIPCHK(hipStreamBeginCapture(G.cs, hipStreamCaptureModeThreadLocal));
+      hipGraph_t g = nullptr;
+      try {
+        body();
+      } catch (...) {
+        (void)hipStreamEndCapture(G.cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(G.cs, &g));
+      const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
+    };
+    capture(&G.first, [&]() {
+      applyinv_with(inv, G.w, gB, gY, m, G.cs);
+      resid(gY, gB, gR, G.cs);
+      k::sumsq<S>(G.cs, len, gR, G.part, 256);
+      k::sumsq<S>(G.cs, len, gB, G.part + 256, 256);
+    });
+    capture(&G.step, [&]() {
+      applyinv_with(inv, G.w, gR, gD, m, G.cs);
+      k::axpby<S, S>(G.cs, len, S(1), gD, S(1), gY);
+      resid(gY, gB, gR, G.cs);
+      k::sumsq<S>(G.cs, len, gR, G.part, 256);
+    });
+  }
+  // after the caller's work on st
+  hipEvent_t ev = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(ev, st));
+  HIPCHK(hipStreamWaitEvent(G.cs, ev, 0));
+  (void)hipEventDestroy(ev);
+  HIPCHK(hipMemcpyAsync(G.B, B, bytes, hipMemcpyDeviceToDevice, G.cs));
+  HIPCHK(hipGraphLaunch(G.first, G.cs));
+  double part[512];
+  HIPCHK(hipMemcpyAsync(part, G.part, sizeof(part), hipMemcpyDeviceToHost, G.cs));
+  HIPCHK(hipStreamSynchronize(G.cs));
+  double sr = 0, sb = 0;
+  for (int i = 0; i < 256; ++i) sr += part[i], sb += part[256 + i];
+  const double nb = std::sqrt(sb) > 0 ? std::sqrt(sb) : 1.0;
+  double rel = std::sqrt(sr) / nb;
+  int it = 1;
+  while (!(rel <= tol) && it < maxit) {
+    HIPCHK(hipGraphLaunch(G.step, G.cs));
+    HIPCHK(hipMemcpyAsync(part, G.part, 256 * sizeof(double), hipMemcpyDeviceToHost, G.cs));
+    HIPCHK(hipStreamSynchronize(G.cs));
+    sr = 0;
+    for (int i = 0; i < 256; ++i) sr += part[i];
+    rel = std::sqrt(sr) / nb;
+    ++it;
+  }
+  HIPCHK(hipMemcpyAsync(Y, G.Y, bytes, hipMemcpyDeviceToDevice, G.cs));
+  HIPCHK(hipStreamSynchronize(G.cs));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  return rel <= tol ? HSSC_OK : fail(HSSC_ERR_NOT_CONVERGED, "refinement did not reach tol");
+}
+
+// The refinement with A = the exact grid operator (FFT convolution).  The first solve with a
+// given operator and m runs its launches directly; later ones replay the inverse's graphs.
 template <typename S>
 int solve_exact_impl(const hssc_inverse* inv, const hssc_grid_op* A, const S* B, S* Y,
                      int64_t m, double tol, int32_t maxit, int32_t* iters, double* relres,
                      hipStream_t st) {
+  RefineGraphs& G = inv->graphs;
+  const uint64_t uid = grid_op_uid(A);
+  if (G.op != uid || G.m != m) {
+    G.release();
+    G.op = uid, G.m = m;
+  }
+  if (++G.calls >= 2 && solve_graphs_enabled())
+    return refine_graphed<S>(inv, G, B, Y, m, tol, maxit, iters, relres, st,
+                             [&](const S* y, const S* b, S* r, hipStream_t s) {
+                               grid_op_apply(A, y, b, r, m, s);
+                             });
   return refine_impl<S>(inv, B, Y, m, tol, maxit, iters, relres, st,
                         [&](const S* y, S* r) { grid_op_apply(A, y, B, r, m, st); });
 }
