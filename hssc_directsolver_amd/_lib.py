@@ -25,7 +25,7 @@ class HsscParams(C.Structure):
         ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("preset_all", C.c_int32),
         ("contrast", C.c_int32), ("contrast_amp", C.c_double), ("contrast_width", C.c_double),
         ("kernel_scale", C.c_double), ("length_scale", C.c_double),
-        ("tree_precision", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("tree_precision", C.c_int32), ("proxy", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -80,6 +80,10 @@ SIGNATURES = [
                                  C.c_int64, C.c_uint64, _I64P, _I64P, _DP, _DP]),
     ("hssc_hss1d_compress", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32, C.c_double,
                                       C.c_int32, C.POINTER(_P)]),
+    ("hssc_hss1d_compress_kernel", C.c_int, [_PP, _DP, C.c_int64, C.c_int32, C.c_double,
+                                             C.c_int32, C.POINTER(_P)]),
+    ("hssc_hss1d_compress_dense", C.c_int, [_DP, C.c_int64, C.c_int64, C.c_int32, C.c_double,
+                                            C.POINTER(_P)]),
     ("hssc_hss1d_compress_ranked", C.c_int, [_DP, _DP, _DP, _DP, C.c_int64, C.c_int32,
                                              C.c_double, C.c_int32, C.c_int32, _P, C.c_int32,
                                              C.POINTER(_P)]),

@@ -8,8 +8,12 @@
 // (integer / geometry work) exactly as the reference does; the GPU generates every kernel
 // block, runs the interpolative decompositions, the inversion and the sweeps.
 //
-// Operator: A = -1/2 I + D, D the Laplace double layer with per-point normals, weights and
-// curvature (diagonal -kappa w / 4pi); proxy sources carry log charges (1/2pi) log|x - z|.
+// Operators (hssc_hss1d.kind):
+//   0  A = -1/2 I + D, D the Laplace double layer with per-point normals, weights and
+//      curvature (diagonal -kappa w / 4pi); proxy sources carry log charges (1/2pi) log|x - z|;
+//   1  Kernel_Eval(X, X, params) on arbitrary points (Common/Kernel_Eval.m, real kernels),
+//      proxy rows and columns by the same kernel ('green' as the reference runs it);
+//   2  a given dense matrix ('brute', HSS1D_compress_nsym.m:158-181), no geometry.
 
 #include <hip/hip_runtime.h>
 
@@ -30,6 +34,7 @@
 #include "stats.h"
 
 namespace hssc {
+k::KernelSpec kernel_spec(const hssc_params& p);
 namespace {
 
 struct HipErr {
@@ -96,6 +101,17 @@ struct CurveDev {
   const double *x, *y, *nx, *ny, *w, *kap;
 };
 
+// Where the entries of an operator come from (hssc_hss1d.kind).
+struct Gen1D {
+  int kind;           // 0 double layer, 1 Kernel_Eval on points, 2 dense matrix
+  CurveDev C;         // points (kinds 0, 1), normals / weights / curvature (kind 0)
+  k::KernelSpec ks;   // kind 1
+  const double* wb;   // kind 1: Kernel_Eval's row factor b at every point
+  const double* wc;   // kind 1: column factor c at every point
+  const double* K;    // kind 2: N x N, column-major, leading dimension ldk
+  int64_t ldk;
+};
+
 // Double layer at target (tx,ty) from curve source j; tgt = index of the target if it is a
 // curve point (self term), -1 otherwise.
 __device__ __forceinline__ double dl_entry(const CurveDev& C, double tx, double ty, int tgt, int j) {
@@ -107,39 +123,56 @@ __device__ __forceinline__ double dl_entry(const CurveDev& C, double tx, double 
 
 // Ragged block generation.  Matrix b is m x n at A + off[b] (leading dimension m).
 //   mode 0 (A_out rows x sources): rows r < mc[b] are curve targets ridx[ro[b]+r], the other
-//          rows proxy targets (px, py at pofs[b] + r - mc); columns are curve sources cidx.
+//          rows proxy targets (px, py, weight pw at pofs[b] + r - mc); columns are curve
+//          sources cidx.
 //   mode 1 (A_in^T): rows r < mc[b] are curve sources ridx[..], the other rows proxy
-//          sources (log charges); columns are curve targets cidx.
+//          sources (log charges for the double layer); columns are curve targets cidx.
+// Kernel_Eval entries take b at the target and c at the source (Common/Kernel_Eval.m:35);
+// a proxy point's own factor is pw.
 __global__ void __launch_bounds__(256) gen1d_kernel(
-    CurveDev C, int mode, const int64_t* __restrict__ off, const int* __restrict__ mr,
+    Gen1D G, int mode, const int64_t* __restrict__ off, const int* __restrict__ mr,
     const int* __restrict__ nc, const int* __restrict__ mcv, const int* __restrict__ ridx,
     const int64_t* __restrict__ ro, const int* __restrict__ cidx, const int64_t* __restrict__ co,
-    const double* __restrict__ px, const double* __restrict__ py, const int64_t* __restrict__ pofs,
-    double* __restrict__ A) {
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pw,
+    const int64_t* __restrict__ pofs, double* __restrict__ A) {
   const int b = blockIdx.y;
   const int m = mr[b], n = nc[b], mc = mcv[b];
+  const CurveDev& C = G.C;
   double* M = A + off[b];
   for (int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x; idx < int64_t(m) * n;
        idx += int64_t(gridDim.x) * 256) {
     const int r = int(idx % m), c = int(idx / m);
     const int cj = cidx[co[b] + c];
     double v;
-    if (mode == 0) {
+    if (G.kind == 2) {
+      const int ri = ridx[ro[b] + r];  // dense: no proxy rows (mc == m)
+      v = mode == 0 ? G.K[ri + int64_t(cj) * G.ldk] : G.K[cj + int64_t(ri) * G.ldk];
+    } else if (mode == 0) {
       if (r < mc) {
         const int ti = ridx[ro[b] + r];
-        v = dl_entry(C, C.x[ti], C.y[ti], ti, cj);
+        v = G.kind == 0 ? dl_entry(C, C.x[ti], C.y[ti], ti, cj)
+                        : k::kernel_entry<double>(G.ks, C.x[ti], C.y[ti], G.wb[ti], C.x[cj],
+                                                  C.y[cj], G.wc[cj]);
       } else {
         const int64_t q = pofs[b] + r - mc;
-        v = dl_entry(C, px[q], py[q], -1, cj);
+        v = G.kind == 0 ? dl_entry(C, px[q], py[q], -1, cj)
+                        : k::kernel_entry<double>(G.ks, px[q], py[q], pw[q], C.x[cj], C.y[cj],
+                                                  G.wc[cj]);
       }
     } else {
       if (r < mc) {
         const int sj = ridx[ro[b] + r];
-        v = dl_entry(C, C.x[cj], C.y[cj], cj, sj);
+        v = G.kind == 0 ? dl_entry(C, C.x[cj], C.y[cj], cj, sj)
+                        : k::kernel_entry<double>(G.ks, C.x[cj], C.y[cj], G.wb[cj], C.x[sj],
+                                                  C.y[sj], G.wc[sj]);
       } else {
         const int64_t q = pofs[b] + r - mc;
-        const double dx = C.x[cj] - px[q], dy = C.y[cj] - py[q];
-        v = log(dx * dx + dy * dy) / (4 * kPi);
+        if (G.kind == 0) {
+          const double dx = C.x[cj] - px[q], dy = C.y[cj] - py[q];
+          v = log(dx * dx + dy * dy) / (4 * kPi);
+        } else {
+          v = k::kernel_entry<double>(G.ks, C.x[cj], C.y[cj], G.wb[cj], px[q], py[q], pw[q]);
+        }
       }
     }
     M[idx] = v;
@@ -360,15 +393,20 @@ struct SweepOp {
 };
 
 // The matrices come from one arena (Tm: double, or float for the FP32 inverse) at offset -
-// sofs; vectors and accumulation stay in double.
+// sofs; vectors and accumulation stay in double.  blockIdx.y is the right-hand side: column
+// j of the global vectors starts at j * ldg, of the level vectors at j * ldv.
 template <typename Tm>
 __global__ void __launch_bounds__(256) hss1d_sweep_kernel(const SweepOp* __restrict__ ops,
                                                           const Tm* __restrict__ S, int64_t sofs,
                                                           double* __restrict__ V,
                                                           const double* __restrict__ gin,
-                                                          double* __restrict__ gout) {
+                                                          double* __restrict__ gout, int64_t ldg,
+                                                          int64_t ldv) {
   const SweepOp O = ops[blockIdx.x];
   S -= sofs;
+  V += int64_t(blockIdx.y) * ldv;
+  gin += int64_t(blockIdx.y) * ldg;
+  gout += int64_t(blockIdx.y) * ldg;
   for (int i = threadIdx.x; i < O.rows; i += blockDim.x) {
     double s = 0;
     for (int t = 0; t < O.nterm; ++t) {
@@ -432,7 +470,13 @@ struct hssc_hss1d {
   int64_t N = 0;
   double acc = 0;
   int nproxy = 0;
+  int kind = 0;                 // 0 double layer, 1 Kernel_Eval on points, 2 dense matrix
+  hssc_params prm{};            // kind 1
   std::vector<double> X, Y, NX, NY, W, KAP;
+  std::vector<double> WB, WC;   // kind 1: Kernel_Eval's b and c at every point
+  double *dwb = nullptr, *dwc = nullptr;
+  const double* dK = nullptr;   // kind 2: the caller's matrix on the device (compression only)
+  int64_t ldk = 0;
   // tree (1-based boxes, index 0 unused)
   std::vector<int> level, parent, c1, c2, start, count;
   int nboxes = 0, depth = 0;
@@ -469,11 +513,18 @@ struct hssc_hss1d {
   int64_t id_boxes = 0, id_fallbacks = 0, id_sketch_rows = 0, id_full_rows = 0;
   ~hssc_hss1d() {
     dfree1(S), dfree1(I), dfree1(dx), dfree1(dy), dfree1(dnx), dfree1(dny), dfree1(dw),
-        dfree1(dkap), dfree1(W32);
+        dfree1(dkap), dfree1(W32), dfree1(dwb), dfree1(dwc);
   }
   bool leaf(int b) const { return c1[b] <= 0 && c2[b] <= 0; }
   bool root(int b) const { return parent[b] == 0; }
-  CurveDev curve() const { return CurveDev{dx, dy, dnx, dny, dw, dkap}; }
+  Gen1D gen() const {
+    Gen1D g{};
+    g.kind = kind;
+    g.C = CurveDev{dx, dy, dnx, dny, dw, dkap};
+    if (kind == 1) g.ks = kernel_spec(prm);
+    g.wb = dwb, g.wc = dwc, g.K = dK, g.ldk = ldk;
+    return g;
+  }
 };
 
 namespace hssc {
@@ -600,10 +651,12 @@ struct Ragged {
   std::vector<int> m, n, mc;
   std::vector<int> ridx, cidx;
   std::vector<int64_t> ro, co, pofs;
-  std::vector<double> px, py;
+  std::vector<double> px, py, pw;
   int64_t total = 0;
+  // pw: the proxy points' own Kernel_Eval factors (b as targets, c as sources); empty = 1
   void add(const std::vector<int>& rows, const std::vector<double>& zx,
-           const std::vector<double>& zy, const std::vector<int>& cols) {
+           const std::vector<double>& zy, const std::vector<int>& cols,
+           const std::vector<double>& zw = {}) {
     off.push_back(total);
     const int mm = int(rows.size() + zx.size()), nn = int(cols.size());
     m.push_back(mm), n.push_back(nn), mc.push_back(int(rows.size()));
@@ -613,6 +666,8 @@ struct Ragged {
     cidx.insert(cidx.end(), cols.begin(), cols.end());
     px.insert(px.end(), zx.begin(), zx.end());
     py.insert(py.end(), zy.begin(), zy.end());
+    if (zw.empty()) pw.insert(pw.end(), zx.size(), 1.0);
+    else pw.insert(pw.end(), zw.begin(), zw.end());
     total += int64_t(mm) * nn;
   }
 };
@@ -632,14 +687,15 @@ double* gen_ragged(const hssc_hss1d* h, const Ragged& R, int mode, std::vector<v
   int64_t* dpo = static_cast<int64_t*>(keep(dput(R.pofs)));
   double* dpx = static_cast<double*>(keep(dput(R.px)));
   double* dpy = static_cast<double*>(keep(dput(R.py)));
+  double* dpw = static_cast<double*>(keep(dput(R.pw)));
   double* A = dmalloc<double>(R.total);
   if (nm > 0) {
     int64_t mx = 1;
     for (int i = 0; i < nm; ++i) mx = std::max<int64_t>(mx, int64_t(R.m[i]) * R.n[i]);
     dim3 grid(unsigned(std::min<int64_t>((mx + 255) / 256, 64)), nm);
     HSSC_TIMED("gen1d_f64", 0, double(R.total), double(R.total) * 8,
-               hipLaunchKernelGGL(gen1d_kernel, grid, dim3(256), 0, 0, h->curve(), mode, doff, dm,
-                                  dn, dmc, dri, dro, dci, dco, dpx, dpy, dpo, A));
+               hipLaunchKernelGGL(gen1d_kernel, grid, dim3(256), 0, 0, h->gen(), mode, doff, dm,
+                                  dn, dmc, dri, dro, dci, dco, dpx, dpy, dpw, dpo, A));
   }
   return A;
 }
@@ -952,41 +1008,89 @@ double predict_rank_table(int n, double acc) {
 // ---------------------------------------------------------------------------------------
 
 namespace {
-int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, const double* kappa,
-                        const std::vector<int64_t>& Ns, int32_t nbox_max, double acc,
-                        int32_t nproxy, int rank_mode, hssc_mlp* mlp, int oversample,
+// What an HSS1D compression reads (hssc_hss1d.kind): curves with normals, weights and
+// curvature (0), points with Kernel_Eval's params (1), or a dense matrix (2).
+struct Source1D {
+  int kind = 0;
+  const double *Xc = nullptr, *nrm = nullptr, *w = nullptr, *kappa = nullptr;  // 0; Xc for 1
+  const hssc_params* p = nullptr;                                              // 1
+  const double* K = nullptr;                                                   // 2 (host)
+  int64_t ldk = 0;
+};
+
+int hss1d_compress_impl(const Source1D& src, const std::vector<int64_t>& Ns, int32_t nbox_max,
+                        double acc, int32_t nproxy, int rank_mode, hssc_mlp* mlp, int oversample,
                         hssc_hss1d** out) {
   int64_t N = 0;
   for (int64_t n : Ns) {
     if (n < 4) return fail(HSSC_ERR_INVALID_ARG, "every curve needs at least 4 nodes");
     N += n;
   }
-  if (!Xc || !nrm || !w || !kappa || !out || Ns.empty() || nbox_max < 2 ||
-      !(acc > 0 && acc < 1) || nproxy < 4)
+  const int kind = src.kind;
+  const bool geo = kind != 2;  // points, neighbour lists and proxy circles
+  if (!out || Ns.empty() || nbox_max < 2 || !(acc > 0 && acc < 1) || (geo && nproxy < 4) ||
+      (kind == 0 && (!src.Xc || !src.nrm || !src.w || !src.kappa)) ||
+      (kind == 1 && (!src.Xc || !src.p)) || (kind == 2 && (!src.K || src.ldk < N)) ||
+      kind < 0 || kind > 2)
     return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d_compress arguments");
+  if (kind == 1) {
+    if (int st = check_params(*src.p)) return st;
+    if (src.p->flag_pot != HSSC_POT_SL_L_2D)
+      return fail(HSSC_ERR_UNSUPPORTED, "HSS1D operators are real: Kernel_Eval needs SL_L_2D");
+  }
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
     return fail(HSSC_ERR_HIP, "no HIP device is available");
   hssc_hss1d* h = nullptr;
+  double* dKown = nullptr;
+  struct KFree {
+    double** p;
+    ~KFree() { dfree1(*p); }
+  } kfree{&dKown};
   try {
     h = new hssc_hss1d();
-    h->N = N, h->acc = acc, h->nproxy = nproxy;
-    h->X.assign(Xc, Xc + N), h->Y.assign(Xc + N, Xc + 2 * N);
-    h->NX.assign(nrm, nrm + N), h->NY.assign(nrm + N, nrm + 2 * N);
-    h->W.assign(w, w + N), h->KAP.assign(kappa, kappa + N);
+    h->N = N, h->acc = acc, h->nproxy = geo ? nproxy : 0, h->kind = kind;
+    if (kind == 0) {
+      const double *Xc = src.Xc, *nrm = src.nrm, *w = src.w, *kappa = src.kappa;
+      h->X.assign(Xc, Xc + N), h->Y.assign(Xc + N, Xc + 2 * N);
+      h->NX.assign(nrm, nrm + N), h->NY.assign(nrm + N, nrm + 2 * N);
+      h->W.assign(w, w + N), h->KAP.assign(kappa, kappa + N);
+    } else {
+      if (kind == 1) h->X.assign(src.Xc, src.Xc + N), h->Y.assign(src.Xc + N, src.Xc + 2 * N);
+      else h->X.assign(N, 0.0), h->Y.assign(N, 0.0);
+      h->NX.assign(N, 0.0), h->NY.assign(N, 0.0), h->W.assign(N, 0.0), h->KAP.assign(N, 0.0);
+    }
     h->dx = dput(h->X), h->dy = dput(h->Y), h->dnx = dput(h->NX), h->dny = dput(h->NY);
     h->dw = dput(h->W), h->dkap = dput(h->KAP);
+    if (kind == 1) {
+      h->prm = *src.p;
+      h->WB.resize(N), h->WC.resize(N);
+      for (int64_t i = 0; i < N; ++i) {
+        h->WB[i] = point_weight(h->prm, h->X[i], h->Y[i], true);
+        h->WC[i] = point_weight(h->prm, h->X[i], h->Y[i], false);
+      }
+      h->dwb = dput(h->WB), h->dwc = dput(h->WC);
+    }
+    if (kind == 2) {
+      // the caller's matrix on the device for the compression (leading dimension N)
+      dKown = dmalloc<double>(size_t(N) * N);
+      H1CHK(hipMemcpy2D(dKown, N * 8, src.K, src.ldk * 8, N * 8, N, hipMemcpyHostToDevice));
+      h->dK = dKown, h->ldk = N;
+    }
     build_tree1d(h, nbox_max, Ns);
     for (int r : h->roots)
       if (h->leaf(r)) throw HipErr{fail(HSSC_ERR_INVALID_ARG, "tree needs at least 2 levels")};
-    neighbours(h);
+    if (geo) neighbours(h);
     const int nb = h->nboxes;
     h->Rroot.assign(nb + 1, 1.0);
     for (int r : h->roots) {
       double cx, cy;
-      circum(h, range_of(h, r), &cx, &cy, &h->Rroot[r]);
+      if (geo) circum(h, range_of(h, r), &cx, &cy, &h->Rroot[r]);
     }
+    // 'brute': per level, the skeletons of its boxes in the order the reference compresses
+    // them (nboxes down to 2), I_src{lev,1:2} of HSS1D_compress_nsym.m:210-214
+    std::vector<std::vector<int>> isrc_up(h->depth + 2), isrc_dn(h->depth + 2);
     h->feat.assign(nb + 1, {});
     h->isk_up.assign(nb + 1, {}), h->isk_dn.assign(nb + 1, {});
     h->J_up.assign(nb + 1, {}), h->J_dn.assign(nb + 1, {});
@@ -1027,6 +1131,34 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
           out.insert(out.end(), h->isk_up[h->c2[b]].begin(), h->isk_up[h->c2[b]].end());
           inn = h->isk_dn[h->c1[b]];
           inn.insert(inn.end(), h->isk_dn[h->c2[b]].begin(), h->isk_dn[h->c2[b]].end());
+        }
+        if (!geo) {
+          // HSS1D_compress_nsym.m:158-181: A_out = K(ind_offd_in, indskel_out),
+          // A_in = K(indskel_in, ind_offd_out); a leaf's rows are every index outside it, a
+          // parent's the level below's skeletons outside it
+          const int lo = h->start[b], hi = h->start[b] + h->count[b];
+          std::vector<int>& oin = oins[q];
+          std::vector<int>& iin = iins[q];
+          if (h->leaf(b)) {
+            for (int i = 0; i < int(N); ++i)
+              if (i < lo || i >= hi) oin.push_back(i);
+            iin = oin;
+          } else {
+            for (int i : isrc_up[lev + 1])
+              if (i < lo || i >= hi) oin.push_back(i);
+            for (int i : isrc_dn[lev + 1])
+              if (i < lo || i >= hi) iin.push_back(i);
+          }
+          auto& f = h->feat[b];
+          f = {};
+          f[0] = float(lev);
+          f[1] = float(std::log2(double(h->count[b])));
+          f[2] = float(std::log2(double(out.size())));
+          f[6] = float(std::log2(double(std::max<size_t>(iin.size(), 1))));
+          f[7] = float(-std::log10(acc));
+          outs[q].swap(out), inns[q].swap(inn);
+          Rbox[q] = 0;
+          return;
         }
         std::vector<int> uni = out;
         uni.insert(uni.end(), inn.begin(), inn.end());
@@ -1080,8 +1212,15 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
         Rbox[q] = R;
       });
       for (int q = 0; q < nq0; ++q) {
-        Rout.add(iins[q], zxs[q], zys[q], outs[q]);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
-        Rin.add(oins[q], zxs[q], zys[q], inns[q]);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
+        // Kernel_Eval factors of the proxy points: b as targets of A_out, c as sources of A_in
+        std::vector<double> wbz, wcz;
+        if (kind == 1)
+          for (size_t t = 0; t < zxs[q].size(); ++t) {
+            wbz.push_back(point_weight(h->prm, zxs[q][t], zys[q][t], true));
+            wcz.push_back(point_weight(h->prm, zxs[q][t], zys[q][t], false));
+          }
+        Rout.add(iins[q], zxs[q], zys[q], outs[q], wbz);  // A_out = [K(X_in_inside, X_out); K(Zp, X_out)]
+        Rin.add(oins[q], zxs[q], zys[q], inns[q], wcz);   // A_in^T = [K(X_in, X_out_inside), K(X_in, Zp)]^T
       }
       std::vector<void*> tmp;
       double* Aout = gen_ragged(h, Rout, 0, &tmp);
@@ -1196,6 +1335,12 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
         level_blocks.push_back(blk);
         level_sizes.push_back(aoff[nq]);
       }
+      if (!geo)
+        for (int q = nq - 1; q >= 0; --q) {
+          const int b = boxes[q];
+          isrc_up[lev].insert(isrc_up[lev].end(), h->isk_up[b].begin(), h->isk_up[b].end());
+          isrc_dn[lev].insert(isrc_dn[lev].end(), h->isk_dn[b].begin(), h->isk_dn[b].end());
+        }
     }
     // M_self at leaves, M_sib at children of every parent (HSS1D_compress_nsym.m:220-271)
     Ragged Rm;
@@ -1246,6 +1391,7 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
     h->vlen = vv;
     H1CHK(hipDeviceSynchronize());
     for (void* p : tmp) dfree1(p);
+    h->dK = nullptr;  // the matrix copy is released on return
     *out = h;
     return HSSC_OK;
   } catch (const HipErr& e) {
@@ -1258,12 +1404,35 @@ int hss1d_compress_impl(const double* Xc, const double* nrm, const double* w, co
 }
 }  // namespace
 
+namespace {
+Source1D curves(const double* Xc, const double* nrm, const double* w, const double* kappa) {
+  Source1D s;
+  s.kind = 0, s.Xc = Xc, s.nrm = nrm, s.w = w, s.kappa = kappa;
+  return s;
+}
+}  // namespace
+
 extern "C" {
 
 int hssc_hss1d_compress(const double* Xc, const double* nrm, const double* w, const double* kappa,
                         int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
                         hssc_hss1d** out) {
-  return hss1d_compress_impl(Xc, nrm, w, kappa, {N}, nbox_max, acc, nproxy, 0, nullptr, 0, out);
+  return hss1d_compress_impl(curves(Xc, nrm, w, kappa), {N}, nbox_max, acc, nproxy, 0, nullptr,
+                             0, out);
+}
+
+int hssc_hss1d_compress_kernel(const hssc_params* p, const double* X, int64_t N,
+                               int32_t nbox_max, double acc, int32_t nproxy, hssc_hss1d** out) {
+  Source1D s;
+  s.kind = 1, s.Xc = X, s.p = p;
+  return hss1d_compress_impl(s, {N}, nbox_max, acc, nproxy, 0, nullptr, 0, out);
+}
+
+int hssc_hss1d_compress_dense(const double* K, int64_t N, int64_t ldk, int32_t nbox_max,
+                              double acc, hssc_hss1d** out) {
+  Source1D s;
+  s.kind = 2, s.K = K, s.ldk = ldk;
+  return hss1d_compress_impl(s, {N}, nbox_max, acc, 0, 0, nullptr, 0, out);
 }
 
 int hssc_hss1d_compress_ranked(const double* Xc, const double* nrm, const double* w,
@@ -1272,8 +1441,8 @@ int hssc_hss1d_compress_ranked(const double* Xc, const double* nrm, const double
                                int32_t oversample, hssc_hss1d** out) {
   if (rank_mode < 0 || rank_mode > 2 || (rank_mode == 2 && !mlp) || oversample < 0)
     return fail(HSSC_ERR_INVALID_ARG, "invalid rank prediction arguments");
-  return hss1d_compress_impl(Xc, nrm, w, kappa, {N}, nbox_max, acc, nproxy, rank_mode, mlp,
-                             oversample, out);
+  return hss1d_compress_impl(curves(Xc, nrm, w, kappa), {N}, nbox_max, acc, nproxy, rank_mode,
+                             mlp, oversample, out);
 }
 
 int hssc_hss1d_compress_batched(int32_t nprob, const int64_t* Ns, const double* Xc,
@@ -1283,8 +1452,8 @@ int hssc_hss1d_compress_batched(int32_t nprob, const int64_t* Ns, const double* 
   if (nprob < 1 || !Ns || rank_mode < 0 || rank_mode > 2 || (rank_mode == 2 && !mlp) ||
       oversample < 0)
     return fail(HSSC_ERR_INVALID_ARG, "invalid batched hss1d_compress arguments");
-  return hss1d_compress_impl(Xc, nrm, w, kappa, std::vector<int64_t>(Ns, Ns + nprob), nbox_max,
-                             acc, nproxy, rank_mode, mlp, oversample, out);
+  return hss1d_compress_impl(curves(Xc, nrm, w, kappa), std::vector<int64_t>(Ns, Ns + nprob),
+                             nbox_max, acc, nproxy, rank_mode, mlp, oversample, out);
 }
 
 int hssc_hss1d_box_features(const hssc_hss1d* h, int32_t box, float* f) {
@@ -1422,24 +1591,27 @@ static SweepTerm term(int64_t mat, int ld, int cols, int64_t in, int len1, int64
 }
 
 static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool inverse,
-                   bool fp32 = false) {
+                   bool fp32 = false, int64_t m = 1) {
   // inverse: HSS1D/HSS1D_applyinv_nsym.m:43-86; forward: HSS1D/HSS1D_apply_nsym.m:38-101.
-  // One launch per level and direction.  fp32: the inverse blocks of the FP32 inverse.
+  // One launch per level and direction for all m right-hand sides (N x m column-major).
+  // fp32: the inverse blocks of the FP32 inverse.
   const int nb = h->nboxes;
-  double* V = dmalloc<double>(h->vlen + 1);
+  const int64_t ldv = h->vlen + 1;
+  double* V = dmalloc<double>(ldv * m);
   std::vector<void*> tmp{V};
   auto launch = [&](const std::vector<SweepOp>& ops) {
     if (ops.empty()) return;
     SweepOp* d = dput(ops);
     tmp.push_back(d);
+    const dim3 grid(unsigned(ops.size()), unsigned(m));
     if (fp32) {
       HSSC_TIMED("hss1d_sweep_f32", 0, 0.0, 0.0,
-                 hipLaunchKernelGGL(hss1d_sweep_kernel<float>, dim3(ops.size()), dim3(256), 0, 0,
-                                    d, (const float*)h->W32, h->inv_lo, V, dIn, dOut));
+                 hipLaunchKernelGGL(hss1d_sweep_kernel<float>, grid, dim3(256), 0, 0, d,
+                                    (const float*)h->W32, h->inv_lo, V, dIn, dOut, h->N, ldv));
     } else {
       HSSC_TIMED("hss1d_sweep_f64", 0, 0.0, 0.0,
-                 hipLaunchKernelGGL(hss1d_sweep_kernel<double>, dim3(ops.size()), dim3(256), 0,
-                                    0, d, (const double*)h->S, int64_t(0), V, dIn, dOut));
+                 hipLaunchKernelGGL(hss1d_sweep_kernel<double>, grid, dim3(256), 0, 0, d,
+                                    (const double*)h->S, int64_t(0), V, dIn, dOut, h->N, ldv));
     }
   };
   auto up_op = [&](int b, int64_t mat) {
@@ -1536,10 +1708,11 @@ static int sweep1d(const hssc_hss1d* h, const double* dIn, double* dOut, bool in
 }
 
 int hssc_hss1d_applyinv_device(const hssc_hss1d* h, const void* dF, void* dX, int64_t m) {
-  if (!h || !dF || !dX || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
+  if (!h || !dF || !dX || m < 1 || m > 65535)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
   if (!h->inverted) return fail(HSSC_ERR_INVALID_ARG, "hss1d operator is not inverted");
   try {
-    return sweep1d(h, static_cast<const double*>(dF), static_cast<double*>(dX), true);
+    return sweep1d(h, static_cast<const double*>(dF), static_cast<double*>(dX), true, false, m);
   } catch (const HipErr& e) {
     return e.status;
   }
@@ -1612,13 +1785,15 @@ int hssc_hss1d_solve(const hssc_hss1d* h, const double* F, double* X, double tol
 }
 
 int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t m) {
-  if (!h || !F || !X || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
+  if (!h || !F || !X || m < 1 || m > 65535)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d applyinv arguments");
   try {
-    double* dF = dmalloc<double>(h->N);
-    double* dX = dmalloc<double>(h->N);
-    H1CHK(hipMemcpy(dF, F, h->N * 8, hipMemcpyHostToDevice));
+    const int64_t len = h->N * m;
+    double* dF = dmalloc<double>(len);
+    double* dX = dmalloc<double>(len);
+    H1CHK(hipMemcpy(dF, F, len * 8, hipMemcpyHostToDevice));
     int s = hssc_hss1d_applyinv_device(h, dF, dX, m);
-    if (s == HSSC_OK) H1CHK(hipMemcpy(X, dX, h->N * 8, hipMemcpyDeviceToHost));
+    if (s == HSSC_OK) H1CHK(hipMemcpy(X, dX, len * 8, hipMemcpyDeviceToHost));
     dfree1(dF), dfree1(dX);
     return s;
   } catch (const HipErr& e) {
@@ -1627,22 +1802,25 @@ int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t
 }
 
 int hssc_hss1d_apply_device(const hssc_hss1d* h, const void* dQ, void* dU, int64_t m) {
-  if (!h || !dQ || !dU || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
+  if (!h || !dQ || !dU || m < 1 || m > 65535)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
   try {
-    return sweep1d(h, static_cast<const double*>(dQ), static_cast<double*>(dU), false);
+    return sweep1d(h, static_cast<const double*>(dQ), static_cast<double*>(dU), false, false, m);
   } catch (const HipErr& e) {
     return e.status;
   }
 }
 
 int hssc_hss1d_apply(const hssc_hss1d* h, const double* Q, double* U, int64_t m) {
-  if (!h || !Q || !U || m != 1) return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
+  if (!h || !Q || !U || m < 1 || m > 65535)
+    return fail(HSSC_ERR_INVALID_ARG, "invalid hss1d apply arguments");
   try {
-    double* dQ = dmalloc<double>(h->N);
-    double* dU = dmalloc<double>(h->N);
-    H1CHK(hipMemcpy(dQ, Q, h->N * 8, hipMemcpyHostToDevice));
+    const int64_t len = h->N * m;
+    double* dQ = dmalloc<double>(len);
+    double* dU = dmalloc<double>(len);
+    H1CHK(hipMemcpy(dQ, Q, len * 8, hipMemcpyHostToDevice));
     int s = hssc_hss1d_apply_device(h, dQ, dU, m);
-    if (s == HSSC_OK) H1CHK(hipMemcpy(U, dU, h->N * 8, hipMemcpyDeviceToHost));
+    if (s == HSSC_OK) H1CHK(hipMemcpy(U, dU, len * 8, hipMemcpyDeviceToHost));
     dfree1(dQ), dfree1(dU);
     return s;
   } catch (const HipErr& e) {

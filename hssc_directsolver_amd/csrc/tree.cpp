@@ -43,16 +43,18 @@ double bump_factor(double h, double x) {
 }  // namespace
 
 double point_weight(const hssc_params& p, double x, double y, bool row) {
-  // Common/Kernel_Eval.m:111-159 (LOCAL_get_bc) for params.proxy == 0: b on the row points,
-  // c on the column points.
-  //   sym = 0, kh > 0 (:115-128): b = Bump_function(., 4) with h = 0, c = 1;
+  // Common/Kernel_Eval.m:111-159 (LOCAL_get_bc): b on the row points, c on the column points.
+  //   sym = 0, kh > 0 (:115-128): b = Bump_function(., 4) with h = 0 (1 when proxy > 0), c = 1;
   //   TI (:155-158): 1;
-  //   NTI Helmholtz (:133,:143): sqrt(Bump_function(., 4, h)), or sqrt of the Gaussian
+  //   NTI (:129-154): b = 1 when proxy > 0, c = 1 when proxy < 0; otherwise
+  //     Helmholtz (:133,:143): sqrt(Bump_function(., 4, h)), or sqrt of the Gaussian
   //     contrast option;
-  //   NTI Laplace (:135,:146-150): b = b_func(., 0.3, 0.6, 0.25); c the same when sym,
+  //     Laplace (:135,:146-150): b = b_func(., 0.3, 0.6, 0.25); c the same when sym,
   //     b_func(., 0.2, -0.6, 0.25) otherwise.
-  if (p.sym == 0 && p.kh > 0) return row ? bump_factor(0.0, x) * bump_factor(0.0, y) : 1.0;
+  if (p.sym == 0 && p.kh > 0)
+    return (row && p.proxy <= 0) ? bump_factor(0.0, x) * bump_factor(0.0, y) : 1.0;
   if (p.transinv) return 1.0;
+  if ((row && p.proxy > 0) || (!row && p.proxy < 0)) return 1.0;
   if (p.kh > 0) {
     if (p.contrast == HSSC_CONTRAST_GAUSSIAN)
       return std::sqrt(p.contrast_amp *
@@ -351,6 +353,8 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
   if (p.np < 2) return fail(HSSC_ERR_INVALID_ARG, "np must be at least 2");
   if (p.sym != 1)
     return fail(HSSC_ERR_UNSUPPORTED, "only symmetric kernels (HSS2D_*_fsym) are supported");
+  if (p.proxy != 0)
+    return fail(HSSC_ERR_UNSUPPORTED, "HSS2D trees need params.proxy = 0 (b = c)");
   t->p = p;
   t->cx = p.flag_pot == HSSC_POT_SL_H_2D;
   const int depth = p.depth;

@@ -1,12 +1,16 @@
 """Python mirror of the reference's HSS1D MATLAB entry points, backed by libhssc.so.
 
-    K_HSS = HSS1D_compress_nsym('green', curve, nbox_max, acc)   # HSS1D_compress_nsym.m
-    HSS1D_invert_nsym(K_HSS)                                      # HSS1D_invert_nsym.m
-    x = HSS1D_applyinv_nsym(K_HSS, f)                             # HSS1D_applyinv_nsym.m
-    u = HSS1D_apply_nsym(K_HSS, q)                                # HSS1D_apply_nsym.m
+    K_HSS = HSS1D_compress_nsym('green', X, nbox_max, acc, params)  # HSS1D_compress_nsym.m
+    K_HSS = HSS1D_compress_nsym('brute', K, nbox_max, acc)
+    K_HSS = HSS1D_compress_nsym('green', curve, nbox_max, acc)
+    HSS1D_invert_nsym(K_HSS)                                         # HSS1D_invert_nsym.m
+    x = HSS1D_applyinv_nsym(K_HSS, f)                                # HSS1D_applyinv_nsym.m
+    u = HSS1D_apply_nsym(K_HSS, q)                                   # HSS1D_apply_nsym.m
 
-``curve`` is a Curve (points, outward normals, quadrature weights, curvature); the
-operator is the second-kind Laplace double layer -1/2 I + D (see include/hssc.h).
+'green' with points X (N x 2) and a params object compresses Kernel_Eval(X, X, params), as
+the reference does; 'brute' compresses a given dense matrix; 'green' with a Curve (points,
+outward normals, quadrature weights, curvature) compresses the second-kind Laplace double
+layer -1/2 I + D (see include/hssc.h).  Vectors are N or N x m (m right-hand sides).
 """
 from __future__ import annotations
 
@@ -98,20 +102,43 @@ class HSS1D:
 RANK_MODES = {"id": 0, "table": 1, "mlp": 2}
 
 
-def HSS1D_compress_nsym(type_, curve: Curve, nbox_max, acc, nproxy=64, rank="id", mlp=None,
-                        oversample=8) -> HSS1D:
-    """HSS1D/HSS1D_compress_nsym.m:1 with type 'green'.
+def HSS1D_compress_nsym(type_, curve, nbox_max, acc, params=None, nproxy=None, rank="id",
+                        mlp=None, oversample=8) -> HSS1D:
+    """HSS1D/HSS1D_compress_nsym.m:1 (K_HSS = HSS1D_compress_nsym(type,X,TREE,acc,params)).
 
-    ``rank`` picks how the IDs are sized: 'id' runs the plain ID on every block; 'table'
-    sketches each block with the closed-form rank guess of HSS2D/predict_rank.m; 'mlp' with
-    the prediction of ``mlp`` (a RankMLP).  Sketches that fail fall back to the plain ID."""
+    type 'green': ``curve`` is an N x 2 point array with ``params`` (a Params; the kernel is
+    Kernel_Eval(X, X, params), proxy circles of ``nproxy`` points, 50 as the reference), or
+    a Curve (the double layer, 64 proxy points by default).  type 'brute': ``curve`` is the
+    dense N x N matrix.  For curves, ``rank`` picks how the IDs are sized: 'id' runs the
+    plain ID on every block; 'table' sketches each block with the closed-form rank guess of
+    HSS2D/predict_rank.m; 'mlp' with the prediction of ``mlp`` (a RankMLP).  Sketches that
+    fail fall back to the plain ID."""
+    h = C.c_void_p()
+    if type_ == "brute":
+        K = np.asfortranarray(curve, dtype=np.float64)
+        if K.ndim != 2 or K.shape[0] != K.shape[1]:
+            raise ValueError("'brute' needs a square matrix")
+        check(lib().hssc_hss1d_compress_dense(K.ctypes.data_as(_DP), K.shape[0], K.shape[0],
+                                              int(nbox_max), float(acc), C.byref(h)))
+        return HSS1D(h, K.shape[0])
     if type_ != "green":
-        raise ValueError("only the 'green' (proxy) compression is implemented")
+        raise ValueError(f"unknown compression type {type_!r} ('green' or 'brute')")
+    if not isinstance(curve, Curve):
+        if params is None:
+            raise ValueError("'green' on points needs the kernel's params")
+        X = np.asfortranarray(curve, dtype=np.float64)
+        if X.ndim != 2 or X.shape[1] != 2:
+            raise ValueError("points must be N x 2")
+        raw = params.raw if hasattr(params, "raw") else params
+        check(lib().hssc_hss1d_compress_kernel(C.byref(raw), X.ctypes.data_as(_DP), X.shape[0],
+                                               int(nbox_max), float(acc),
+                                               int(50 if nproxy is None else nproxy), C.byref(h)))
+        return HSS1D(h, X.shape[0])
+    nproxy = 64 if nproxy is None else nproxy
     X = np.asfortranarray(curve.X, dtype=np.float64)
     Nm = np.asfortranarray(curve.nrm, dtype=np.float64)
     w = np.ascontiguousarray(curve.w, dtype=np.float64)
     k = np.ascontiguousarray(curve.kappa, dtype=np.float64)
-    h = C.c_void_p()
     check(lib().hssc_hss1d_compress_ranked(
         X.ctypes.data_as(_DP), Nm.ctypes.data_as(_DP), w.ctypes.data_as(_DP),
         k.ctypes.data_as(_DP), X.shape[0], int(nbox_max), float(acc), int(nproxy),
@@ -167,19 +194,21 @@ def HSS1D_solve(K_HSS: HSS1D, f, tol=1e-10, maxit=10):
 
 
 def _vec(fn, K_HSS, v):
-    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
-    if v.shape[0] != K_HSS.N:
-        raise ValueError(f"vector must have N = {K_HSS.N} entries, got {v.shape[0]}")
-    out = np.empty_like(v)
-    check(fn(K_HSS.handle, v.ctypes.data_as(_DP), out.ctypes.data_as(_DP), 1))
+    v = np.asarray(v, dtype=np.float64)
+    if v.ndim not in (1, 2) or v.shape[0] != K_HSS.N:
+        raise ValueError(f"vectors must be N = {K_HSS.N} or N x m, got shape {v.shape}")
+    m = 1 if v.ndim == 1 else v.shape[1]
+    vf = np.asfortranarray(v)
+    out = np.empty_like(vf, order="F")
+    check(fn(K_HSS.handle, vf.ctypes.data_as(_DP), out.ctypes.data_as(_DP), m))
     return out
 
 
 def HSS1D_applyinv_nsym(K_HSS: HSS1D, f):
-    """HSS1D/HSS1D_applyinv_nsym.m:1 (one right-hand side)."""
+    """HSS1D/HSS1D_applyinv_nsym.m:1 (N or N x m right-hand sides)."""
     return _vec(lib().hssc_hss1d_applyinv, K_HSS, f)
 
 
 def HSS1D_apply_nsym(K_HSS: HSS1D, q):
-    """HSS1D/HSS1D_apply_nsym.m:1 (one vector)."""
+    """HSS1D/HSS1D_apply_nsym.m:1 (N or N x m vectors)."""
     return _vec(lib().hssc_hss1d_apply, K_HSS, q)

@@ -47,6 +47,14 @@ class Params:
         raw = object.__getattribute__(self, "raw")
         return getattr(raw, name)
 
+    def __setattr__(self, name, value):
+        # params.<field> = value sets the struct field (e.g. params.proxy = 1 as
+        # HSS1D/Test/Run_HSS1D_tests.m:19 does)
+        if name != "raw" and name in dict(HsscParams._fields_):
+            setattr(self.raw, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
     @property
     def is_complex(self) -> bool:
         return self.raw.flag_pot == POT["SL_H_2D"]
@@ -69,7 +77,7 @@ class Params:
 
 def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
                         proxy_layers=None, preset_all=None, contrast=None, kernel_scale=None,
-                        length_scale=None, tree_precision=None) -> Params:
+                        length_scale=None, tree_precision=None, proxy=None) -> Params:
     """Common/HSS_tree_parameters.m:1.
 
     For kh > 0 the Duan-Rokhlin weight corr0 is computed by the library (``corr0``
@@ -79,7 +87,8 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
     counterpart: ``contrast=("gaussian", amp, width)`` (kh > 0, NTI) instead of the tanh
     plateau; ``kernel_scale`` s and ``length_scale`` L (SL_L_2D) for the kernel
     s (L h)^2 log(L r); ``tree_precision="fp32"`` for FP32 interpolation operators (a
-    preconditioner, refined against the exact grid operator)."""
+    preconditioner, refined against the exact grid operator).  ``proxy`` sets params.proxy
+    (Kernel_Eval's b / c factors; HSS2D trees need 0)."""
     raw = HsscParams()
     check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
                                  float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
@@ -104,6 +113,8 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
         raw.length_scale = float(length_scale)
     if tree_precision is not None:
         raw.tree_precision = PREC[tree_precision]
+    if proxy is not None:
+        raw.proxy = int(proxy)
     return Params(raw)
 
 

@@ -82,7 +82,11 @@ typedef struct hssc_params {
   int32_t tree_precision; /* HSSC_PREC_* of the interpolation operators T (no reference
                            counterpart: GPU policy); FP32 T serves only as a preconditioner,
                            with residuals taken against the exact operator (hssc_grid_op) */
-  int32_t reserved[3];
+  int32_t proxy;        /* params.proxy (Common/HSS_tree_parameters.m:138, default 0): in
+                           Kernel_Eval's LOCAL_get_bc (Common/Kernel_Eval.m:111-159), > 0 drops
+                           the row factor b, < 0 the column factor c (HSS1D tests use 1); the
+                           HSS2D (fsym) entry points need 0 */
+  int32_t reserved[2];
 } hssc_params;
 
 #define HSSC_CONTRAST_PLATEAU 0
@@ -278,6 +282,20 @@ typedef struct hssc_hss1d hssc_hss1d;
 int hssc_hss1d_compress(const double* X, const double* nrm, const double* w, const double* kappa,
                         int64_t N, int32_t nbox_max, double acc, int32_t nproxy,
                         hssc_hss1d** out);
+/* Replaces HSS1D/HSS1D_compress_nsym.m:1 with type 'green' on the reference's own kernels:
+ * every block, proxy rows (K(Zp, X_out)) and proxy columns (K(X_in, Zp)) included, is
+ * Kernel_Eval(X1, X2, params) (Common/Kernel_Eval.m; :152-156, :222-249), as
+ * HSS1D/Test/Test_HSS1D_compress.m runs it on the skeleton points of a 2D box.  X is N x 2
+ * column-major; nproxy = 50 is the reference's (:52).  Real kernels only (SL_L_2D; the
+ * params.proxy option selects Kernel_Eval's b / c factors): HSSC_ERR_UNSUPPORTED for SL_H_2D. */
+int hssc_hss1d_compress_kernel(const hssc_params* p, const double* X, int64_t N,
+                               int32_t nbox_max, double acc, int32_t nproxy, hssc_hss1d** out);
+/* Replaces HSS1D/HSS1D_compress_nsym.m:1 with type 'brute' (:158-181): compression of the
+ * given dense N x N matrix K (column-major, leading dimension ldk >= N).  A leaf's block row
+ * and column are every index outside it; a parent's are the skeletons of the level below
+ * outside it (:164-177, :210-214).  K is read during the call only. */
+int hssc_hss1d_compress_dense(const double* K, int64_t N, int64_t ldk, int32_t nbox_max,
+                              double acc, hssc_hss1d** out);
 /* ---- rank prediction (HSS2D/predict_rank.m:5-23, interpolation_operator.m:101-117) --- */
 /* A small network (8 -> 64 -> 64 -> 1, GELU, FP32) predicting log2 of a box's ID rank from
  * geometric features; trained and evaluated by the library's own GPU kernels.  It takes
@@ -361,7 +379,8 @@ int hssc_hss1d_solve(const hssc_hss1d* h, const double* F, double* X, double tol
                      int32_t maxit, int32_t* iters, double* relres);
 int hssc_hss1d_solve_device(const hssc_hss1d* h, const void* dF, void* dX, double tol,
                             int32_t maxit, int32_t* iters, double* relres);
-/* Replaces HSS1D/HSS1D_applyinv_nsym.m:1 and HSS1D/HSS1D_apply_nsym.m:1 (one RHS, m = 1). */
+/* Replaces HSS1D/HSS1D_applyinv_nsym.m:1 and HSS1D/HSS1D_apply_nsym.m:1: m right-hand sides
+ * (1 <= m <= 65535), N x m column-major (X = A^-1 F, U = A Q). */
 int hssc_hss1d_applyinv(const hssc_hss1d* h, const double* F, double* X, int64_t m);
 int hssc_hss1d_apply(const hssc_hss1d* h, const double* Q, double* U, int64_t m);
 int hssc_hss1d_applyinv_device(const hssc_hss1d* h, const void* dF, void* dX, int64_t m);

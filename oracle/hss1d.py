@@ -1,22 +1,26 @@
 """CPU oracle for the HSS1D compress / invert / apply path (test infrastructure only).
 
-numpy/scipy restatement of HSSC_DirectSolver's HSS1D solver (nsym, 'green' compression):
+numpy/scipy restatement of HSSC_DirectSolver's HSS1D solver (nsym):
   HSS1D/HSS1D_compress_nsym.m   tree, neighbour lists, proxy circles, ID skeletons, blocks
+                                ('green' compression, :84-154) and the 'brute' compression
+                                of a given dense matrix (:158-181, :210-214)
   HSS1D/HSS1D_invert_nsym.m     telescoping inverse (Finv, E, Linv, Rinv, Dinv, FTinv)
   HSS1D/HSS1D_applyinv_nsym.m   inverse apply
   HSS1D/HSS1D_apply_nsym.m      forward apply
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU baseline may use it.
 
-Operator (the HSS1D workload; no counterpart in the reference's Kernel_Eval, which takes
-positions only): second-kind Laplace double layer on the star contour
-r(t) = 1 + a cos(m t), N trapezoidal nodes,
-    A = -1/2 I + D,  D_ij = (1/2pi) (x_i - x_j).n_j / |x_i - x_j|^2 w_j  (i != j),
-                     D_jj = -kappa_j w_j / (4 pi),
-and proxy sources on the circles use the log kernel (1/2pi) log|x - z| (equivalent
-charges), proxy targets the double-layer kernel.
+Kernels of the 'green' compression (``compress``):
+  * PointKernel: the reference's own, Common/Kernel_Eval.m on point sets (proxy rows and
+    columns by the same Kernel_Eval, :152-156), e.g. SL_L_2D with params.proxy = 1 on the
+    skeleton points of a 2D box as HSS1D/Test/Test_HSS1D_compress.m does;
+  * a Curve: the second-kind Laplace double layer on a closed contour (the HSS1D workload of
+    the benchmark configurations; no counterpart in Kernel_Eval, which takes positions
+    only), A = -1/2 I + D,  D_ij = (1/2pi) (x_i - x_j).n_j / |x_i - x_j|^2 w_j  (i != j),
+    D_jj = -kappa_j w_j / (4 pi); proxy sources on the circles use the log kernel
+    (1/2pi) log|x - z| (equivalent charges), proxy targets the double-layer kernel.
 Pinned by: dense A @ x and dense A \\ b (Test_HSS1D_compress.m:79-109 semantics), the
-Gauss-Legendre-free identity D 1 = -1/2 (Gauss' lemma: the double layer of the constant
-density is -1/2 inside), and the ID reconstruction bound.
+identity D 1 = -1/2 (Gauss' lemma: the double layer of the constant density is -1/2
+inside), the ID reconstruction bound, and 'green' against 'brute' on the same matrix.
 """
 from __future__ import annotations
 
@@ -92,9 +96,46 @@ def log_to_points(C: Curve, I, Z):
     return np.log(dx * dx + dy * dy) / (4 * np.pi)
 
 
-def dense_matrix(C: Curve):
+def dense_matrix(C):
     I = np.arange(C.X.shape[0])
+    if isinstance(C, PointKernel):
+        return C.block(I, I)
     return dl_block(C, I, I)
+
+
+class DoubleLayer:
+    """The double-layer operator of a Curve as the three kinds of blocks the 'green'
+    compression evaluates: curve x curve, proxy targets x curve, curve x proxy sources."""
+
+    def __init__(self, C: Curve):
+        self.C, self.X = C, C.X
+
+    def block(self, I, J):
+        return dl_block(self.C, I, J)
+
+    def rows_from(self, Z, J):
+        return dl_from_points(self.C, Z, J)
+
+    def cols_to(self, I, Z):
+        return log_to_points(self.C, I, Z)
+
+
+class PointKernel:
+    """Kernel_Eval(X1, X2, params) (Common/Kernel_Eval.m:29-159) on the points X: every block,
+    proxy rows and proxy columns included, is one Kernel_Eval call (HSS1D_compress_nsym.m:
+    152-156, 222-249)."""
+
+    def __init__(self, X, params):
+        self.X, self.params = np.asarray(X, dtype=np.float64), params
+
+    def block(self, I, J):
+        return _h2.kernel_eval(self.X[I], self.X[J], self.params)
+
+    def rows_from(self, Z, J):
+        return _h2.kernel_eval(Z, self.X[J], self.params)
+
+    def cols_to(self, I, Z):
+        return _h2.kernel_eval(self.X[I], Z, self.params)
 
 
 # ----------------------------------------------------------------------------------------
@@ -192,7 +233,7 @@ def id_fixed(A, k):
 
 @dataclass
 class HSS1D:
-    C: Curve
+    K: object            # DoubleLayer / PointKernel ('green') or the dense matrix ('brute')
     boxes: list
     nproxy: int
     acc: float
@@ -215,12 +256,14 @@ class HSS1D:
         return np.arange(B.start, B.start + B.count)
 
 
-def compress(C: Curve, nbox_max, acc, nproxy=64, seed=0):
-    """HSS1D_compress_nsym('green', X, nbox_max, acc, params), :55-273."""
+def compress(C, nbox_max, acc, nproxy=64, seed=0):
+    """HSS1D_compress_nsym('green', X, nbox_max, acc, params), :55-273.  ``C`` is a Curve
+    (the double layer) or a PointKernel (Kernel_Eval; the reference's nproxy is 50, :52)."""
     rng = np.random.default_rng(seed)
-    X = C.X
+    kern = C if isinstance(C, PointKernel) else DoubleLayer(C)
+    X = kern.X
     boxes = get_tree(X.shape[0], nbox_max)
-    H = HSS1D(C, boxes, nproxy, acc)
+    H = HSS1D(kern, boxes, nproxy, acc)
     nei = neighbour_lists(X, boxes)
     nb = len(boxes) - 1
     for b in range(nb, 1, -1):
@@ -247,25 +290,76 @@ def compress(C: Curve, nbox_max, acc, nproxy=64, seed=0):
         rr = (RADIUS_REL * R) ** 2
         out_inside = out_maybe[((X[out_maybe] - xxc) ** 2).sum(1) < rr]
         in_inside = in_maybe[((X[in_maybe] - xxc) ** 2).sum(1) < rr]
-        A_out = np.concatenate([dl_block(C, in_inside, out), dl_from_points(C, Zp, out)])
-        A_in = np.concatenate([dl_block(C, inn, out_inside), log_to_points(C, inn, Zp)], axis=1)
-        Tout, Jout = _h2.id_acc(A_out, acc, rng)
-        Tin, Jin = _h2.id_acc(A_in.T, acc, rng)
-        k = max(Tout.shape[0], Tin.shape[0])
-        if k != Tout.shape[0]:
-            Tout, Jout = id_fixed(A_out, k)
-        elif k != Tin.shape[0]:
-            Tin, Jin = id_fixed(A_in.T, k)
-        H.isk_up[b], H.T_up[b], H.J_up[b] = out[Jout[:k]], Tout, Jout
-        H.isk_dn[b], H.T_dn[b], H.J_dn[b] = inn[Jin[:k]], Tin, Jin
-        H.nskel[b], H.kskel[b] = out.size, k
+        A_out = np.concatenate([kern.block(in_inside, out), kern.rows_from(Zp, out)])
+        A_in = np.concatenate([kern.block(inn, out_inside), kern.cols_to(inn, Zp)], axis=1)
+        _skeletons(H, b, out, inn, A_out, A_in, rng)
     for b in range(nb, 0, -1):
         B = boxes[b]
         if H.leaf(b):
-            H.M_self[b] = dl_block(C, H.ind(b), H.ind(b))
+            H.M_self[b] = kern.block(H.ind(b), H.ind(b))
         else:
-            H.M_sib[B.c1] = dl_block(C, H.isk_dn[B.c1], H.isk_up[B.c2])
-            H.M_sib[B.c2] = dl_block(C, H.isk_dn[B.c2], H.isk_up[B.c1])
+            H.M_sib[B.c1] = kern.block(H.isk_dn[B.c1], H.isk_up[B.c2])
+            H.M_sib[B.c2] = kern.block(H.isk_dn[B.c2], H.isk_up[B.c1])
+    return H
+
+
+def _skeletons(H, b, out, inn, A_out, A_in, rng):
+    """:185-208: IDs of A_out and A_in^T, ranks equalised by a fixed-rank ID of the side
+    with the smaller rank, skeleton sets and interpolation matrices recorded."""
+    acc = H.acc
+    Tout, Jout = _h2.id_acc(A_out, acc, rng)
+    Tin, Jin = _h2.id_acc(A_in.T, acc, rng)
+    k = max(Tout.shape[0], Tin.shape[0])
+    if k != Tout.shape[0]:
+        Tout, Jout = id_fixed(A_out, k)
+    elif k != Tin.shape[0]:
+        Tin, Jin = id_fixed(A_in.T, k)
+    H.isk_up[b], H.T_up[b], H.J_up[b] = out[Jout[:k]], Tout, Jout
+    H.isk_dn[b], H.T_dn[b], H.J_dn[b] = inn[Jin[:k]], Tin, Jin
+    H.nskel[b], H.kskel[b] = out.size, k
+
+
+def compress_brute(K, nbox_max, acc, seed=0):
+    """HSS1D_compress_nsym('brute', K, nbox_max, acc), :158-181, :210-214, :220-271.
+
+    The off-diagonal row of a leaf is every index outside it; that of a parent box is the
+    union of the skeletons of the level below (I_src{lev+1}, accumulated in the order the
+    boxes are compressed, nboxes down to 2) without the box's own indices."""
+    rng = np.random.default_rng(seed)
+    K = np.asarray(K)
+    N = K.shape[0]
+    boxes = get_tree(N, nbox_max)
+    H = HSS1D(K, boxes, 0, acc)
+    nb = len(boxes) - 1
+    I_src = {}
+    for b in range(nb, 1, -1):
+        B = boxes[b]
+        lev = B.level
+        lo, hi = B.start, B.start + B.count
+        if H.leaf(b):
+            out = H.ind(b)
+            inn = out
+            off_out = np.concatenate([np.arange(0, lo), np.arange(hi, N)])
+            off_in = off_out
+        else:
+            out = np.concatenate([H.isk_up[B.c1], H.isk_up[B.c2]])
+            inn = np.concatenate([H.isk_dn[B.c1], H.isk_dn[B.c2]])
+            s1, s2 = I_src[lev + 1]
+            off_out = s1[(s1 < lo) | (s1 >= hi)]
+            off_in = s2[(s2 < lo) | (s2 >= hi)]
+        A_out = K[np.ix_(off_in, out)]
+        A_in = K[np.ix_(inn, off_out)]
+        _skeletons(H, b, out, inn, A_out, A_in, rng)
+        if lev > 0:
+            s1, s2 = I_src.get(lev, (np.zeros(0, int), np.zeros(0, int)))
+            I_src[lev] = (np.concatenate([s1, H.isk_up[b]]), np.concatenate([s2, H.isk_dn[b]]))
+    for b in range(nb, 0, -1):
+        B = boxes[b]
+        if H.leaf(b):
+            H.M_self[b] = K[np.ix_(H.ind(b), H.ind(b))]
+        else:
+            H.M_sib[B.c1] = K[np.ix_(H.isk_dn[B.c1], H.isk_up[B.c2])]
+            H.M_sib[B.c2] = K[np.ix_(H.isk_dn[B.c2], H.isk_up[B.c1])]
     return H
 
 

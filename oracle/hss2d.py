@@ -113,19 +113,23 @@ def gaussian_contrast(x, y, amp, width):
 def point_weights(X, params, role):
     """Per-point factor b(x) (role 'row') or c(y) (role 'col').
 
-    Restates Common/Kernel_Eval.m:111-159 (LOCAL_get_bc) for params.proxy == 0.  The
-    reference evaluates b on the row points X1 and c on the column points X2; both depend
-    only on their own point, so they are per-point vectors.  ``params["contrast"] =
+    Restates Common/Kernel_Eval.m:111-159 (LOCAL_get_bc).  The reference evaluates b on the
+    row points X1 and c on the column points X2; both depend only on their own point, so
+    they are per-point vectors.  ``params["proxy"]`` (default 0) > 0 replaces b by ones
+    (:117-121, :131-139), < 0 replaces c by ones (:141-154).  ``params["contrast"] =
     ("gaussian", amp, width)`` replaces the tanh plateau of the symmetric kh > 0 case by
     sqrt(gaussian_contrast) (the configuration's contrast; the reference has no such option).
     """
     x, y = X[:, 0], X[:, 1]
     kh, h = params["kh"], params["h"]
+    proxy = params.get("proxy", 0)
     if params["sym"] == 0 and kh > 0:
-        if role == "row":
+        if role == "row" and proxy <= 0:
             return bump_function(x, y, 0)  # Bump_function(...,opts) with h=0 (:118)
         return np.ones_like(x)
     if params["transinv"] == 0:
+        if (role == "row" and proxy > 0) or (role == "col" and proxy < 0):
+            return np.ones_like(x)
         if kh > 0:
             con = params.get("contrast")
             if con is not None and con[0] == "gaussian":
