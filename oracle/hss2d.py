@@ -420,7 +420,11 @@ def id_acc(A, acc, rng):
 
 
 def id_fixed(A, k):
-    """Common/ID.m:106-113 (fixed rank): T = R11^-1 R12 of the column-pivoted QR."""
+    """Fixed-rank ID as the library computes it: T = R11^-1 R12 of the column-pivoted QR
+    (back substitution).  The reference's skel_col_fixed (Common/ID.m:106-113) applies the
+    SVD pseudo-inverse of R11 with singular values above 1e-12 instead; the two agree while
+    R11 is well conditioned and otherwise differ only along directions at rounding level
+    (the HSS1D oracle, oracle/hss1d.py id_fixed, follows the reference's form)."""
     Q, R, I = sla.qr(A, mode="economic", pivoting=True)
     return _tsolve(R[:k, :k], R[:k, k:]), I
 
