@@ -12,6 +12,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "SINGULAR", 4: "RANK
 POT = {"SL_L_2D": 0, "SL_H_2D": 1}
 PREC = {"fp64": 0, "fp32": 1}
 CONTRAST = {"plateau": 0, "gaussian": 1}
+PROXY_RULE = {"bdry": 0, "rand": 1}
 
 
 class HsscParams(C.Structure):
@@ -25,7 +26,9 @@ class HsscParams(C.Structure):
         ("factor_precision", C.c_int32), ("proxy_layers", C.c_int32), ("preset_all", C.c_int32),
         ("contrast", C.c_int32), ("contrast_amp", C.c_double), ("contrast_width", C.c_double),
         ("kernel_scale", C.c_double), ("length_scale", C.c_double),
-        ("tree_precision", C.c_int32), ("proxy", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("tree_precision", C.c_int32), ("proxy", C.c_int32), ("proxy_rule", C.c_int32),
+        ("skel_extra_pts", C.c_int32), ("reserved0", C.c_int32), ("lambda_", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

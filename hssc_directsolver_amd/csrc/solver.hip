@@ -174,6 +174,10 @@ int check_params(const hssc_params& p) {
     return fail(HSSC_ERR_INVALID_ARG, "Gaussian contrast needs amp >= 0 and width > 0");
   if (p.tree_precision != HSSC_PREC_FP64 && p.tree_precision != HSSC_PREC_FP32)
     return fail(HSSC_ERR_INVALID_ARG, "unknown tree precision");
+  if (p.proxy_rule != HSSC_PROXY_BDRY && p.proxy_rule != HSSC_PROXY_RAND)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown proxy rule");
+  if (p.proxy_rule == HSSC_PROXY_RAND && !(p.lambda > 0))
+    return fail(HSSC_ERR_INVALID_ARG, "the 'rand' proxy rule needs lambda > 0");
   return HSSC_OK;
 }
 
@@ -1484,6 +1488,8 @@ int hssc_params_init(hssc_params* p, int32_t pot, double kh, int32_t sym, int32_
   p->contrast_width = 0.15;
   p->kernel_scale = 1.0;
   p->length_scale = 1.0;
+  p->proxy_rule = HSSC_PROXY_BDRY;
+  p->lambda = 2.0;
   if (kh > 0) {
     // :167-175, params.order = 4: dr_weights = corr0(h, kh)
     if (int s = hssc_dr_corr0(p->h, kh, &p->dr_weights_re, &p->dr_weights_im)) return s;

@@ -12,6 +12,7 @@
 #include <functional>
 #include <mutex>
 #include <numeric>
+#include <random>
 #include <thread>
 
 #include "internal.h"
@@ -268,6 +269,72 @@ void box_proxy(double cx, double cy, double colrad, double rowrad, int layers, d
   py->insert(py->end(), ey.begin(), ey.end());
 }
 
+// MATLAB's rand after rng('default'): the Mersenne Twister MT19937 seeded with 5489, each
+// double made from two 32-bit outputs a, b as ((a >> 5) 2^26 + (b >> 6)) / 2^53.
+class DefaultRand {
+ public:
+  double operator()() {
+    const uint32_t a = uint32_t(g_()) >> 5, b = uint32_t(g_()) >> 6;
+    return (double(a) * 67108864.0 + double(b)) / 9007199254740992.0;
+  }
+
+ private:
+  std::mt19937 g_{5489u};
+};
+
+// LOCAL_random_sampler (HSS2D/HSS2D_build_tree_fsym.m:979-1006): keeps the points within
+// lay*h of the box and the others with probability 2^(-lambda (sz - Ld)), Ld the dyadic shell
+// of their distance D (Chebyshev distance to the box) relative to the largest one, drawn
+// with a fresh rng('default') stream.
+void random_sampler(std::vector<double>* px, std::vector<double>* py, double cx, double cy,
+                    double colrad, double rowrad, double lambda, int lay, double h) {
+  const size_t n = px->size();
+  if (n == 0) return;
+  std::vector<double> D(n);
+  double rad = 0;
+  for (size_t i = 0; i < n; ++i) {
+    D[i] = std::fabs(std::min(colrad - std::fabs((*px)[i] - cx), rowrad - std::fabs((*py)[i] - cy)));
+    rad = std::max(rad, D[i]);
+  }
+  const double width = lay * h;
+  const int sz = -int(std::floor(std::log2(width / rad))) + 1;
+  std::vector<char> keep(n);
+  for (size_t i = 0; i < n; ++i) keep[i] = D[i] < width;
+  if (sz > 0) {
+    DefaultRand rnd;
+    for (size_t i = 0; i < n; ++i) {
+      const int Ld = keep[i] ? sz
+                             : std::min(-int(std::ceil(std::log2(D[i] / rad))), sz - 2) + 1;
+      const double P = std::exp2(-lambda * double(sz - Ld));  // prob(Ld), :995
+      keep[i] = rnd() < P;
+    }
+  }
+  size_t o = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (keep[i]) (*px)[o] = (*px)[i], (*py)[o] = (*py)[i], ++o;
+  px->resize(o), py->resize(o);
+}
+
+// LOCAL_Build_Proxy (:886-977) for the 'bdry' and 'rand' rules.  'rand' thins the ring
+// [X_proxy; X_extra] until it holds k0 = ceil(2/3 n) (even levels) or ceil(3/4 n) (odd)
+// points, at most ten times, each time with lambda x 0.9 (:899-921; every pass thins the
+// previous pass's points, as the reference does).
+void build_proxy(double cx, double cy, double colrad, double rowrad, int lev, int64_t n_skel,
+                 int lay_prox, const hssc_params& p, std::vector<double>* px,
+                 std::vector<double>* py) {
+  box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, px, py);
+  if (p.proxy_rule != HSSC_PROXY_RAND) return;
+  int64_t k0 = int64_t(std::ceil((lev % 2 == 0 ? 2.0 / 3.0 : 3.0 / 4.0) * double(n_skel)));
+  k0 = std::min<int64_t>(k0, int64_t(px->size()));
+  double lambda = p.lambda;
+  int64_t k = 0;
+  for (int c = 0; k0 > k && c < 10; ++c) {
+    random_sampler(px, py, cx, cy, colrad, rowrad, lambda, p.layers, p.h);
+    k = int64_t(px->size());
+    if (k0 > k) lambda *= 0.9;
+  }
+}
+
 int proxy_layers(int lev, const hssc_params& p) {
   // lay_prox of HSS2D/HSS2D_build_tree_fsym.m:339-347 (params.m = 1).
   double colrad, rowrad;
@@ -463,7 +530,8 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       if (!sel) return fail(HSSC_ERR_UNSUPPORTED, "ID skeletons (kh > 0) need the GPU");
       const int64_t n = L.n;
       std::vector<double> qx0, qy0;
-      box_proxy(t->cent[2 * b0], t->cent[2 * b0 + 1], colrad, rowrad, lay_prox, p.h, &qx0, &qy0);
+      build_proxy(t->cent[2 * b0], t->cent[2 * b0 + 1], colrad, rowrad, lev, n, lay_prox, p, &qx0,
+                  &qy0);
       const int64_t M = int64_t(qx0.size());
       hvec<double> Sx(L.nkey * n), Sy(L.nkey * n), Sw(L.nkey * n);
       hvec<double> Qx(L.nkey * M), Qy(L.nkey * M), Qw(L.nkey * M);
@@ -473,7 +541,8 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
         for (int64_t i = 0; i < n; ++i)
           Sx[bi * n + i] = Xx[I[i]], Sy[bi * n + i] = Xy[I[i]], Sw[bi * n + i] = wgt[I[i]];
         std::vector<double> qx, qy;
-        box_proxy(t->cent[2 * b], t->cent[2 * b + 1], colrad, rowrad, lay_prox, p.h, &qx, &qy);
+        build_proxy(t->cent[2 * b], t->cent[2 * b + 1], colrad, rowrad, lev, n, lay_prox, p, &qx,
+                    &qy);
         if (int64_t(qx.size()) != M) return false;
         for (int64_t r = 0; r < M; ++r)
           Qx[bi * M + r] = qx[r], Qy[bi * M + r] = qy[r], Qw[bi * M + r] = weight(qx[r], qy[r]);
@@ -518,7 +587,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
         preset_skeleton(xs, ys, cx, cy, lev, p, pad_to, &J, &k);
       }
       std::vector<double> qx, qy;
-      box_proxy(cx, cy, colrad, rowrad, lay_prox, p.h, &qx, &qy);
+      build_proxy(cx, cy, colrad, rowrad, lev, n, lay_prox, p, &qx, &qy);
       if (bi == 0) {
         L.k = k;
         L.M = int64_t(qx.size());

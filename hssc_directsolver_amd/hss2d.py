@@ -28,7 +28,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import CONTRAST, PREC, POT, HsscParams, check, lib
+from ._lib import CONTRAST, PREC, POT, PROXY_RULE, HsscParams, check, lib
 
 _DP = C.POINTER(C.c_double)
 
@@ -77,7 +77,8 @@ class Params:
 
 def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
                         proxy_layers=None, preset_all=None, contrast=None, kernel_scale=None,
-                        length_scale=None, tree_precision=None, proxy=None) -> Params:
+                        length_scale=None, tree_precision=None, proxy=None, proxy_rule=None,
+                        lam=None, skel_extra_pts=None) -> Params:
     """Common/HSS_tree_parameters.m:1.
 
     For kh > 0 the Duan-Rokhlin weight corr0 is computed by the library (``corr0``
@@ -88,7 +89,9 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
     plateau; ``kernel_scale`` s and ``length_scale`` L (SL_L_2D) for the kernel
     s (L h)^2 log(L r); ``tree_precision="fp32"`` for FP32 interpolation operators (a
     preconditioner, refined against the exact grid operator).  ``proxy`` sets params.proxy
-    (Kernel_Eval's b / c factors; HSS2D trees need 0)."""
+    (Kernel_Eval's b / c factors; HSS2D trees need 0); ``proxy_rule`` ('bdry' or 'rand'),
+    ``lam`` (params.lambda) and ``skel_extra_pts`` are the reference's fields of those
+    names."""
     raw = HsscParams()
     check(lib().hssc_params_init(C.byref(raw), POT[pot] if isinstance(pot, str) else int(pot),
                                  float(kh), int(sym), int(Ng), int(np_), float(acc), int(lay),
@@ -115,6 +118,12 @@ def HSS_tree_parameters(pot, kh, sym, Ng, np_, acc, lay, n_cut, TI, corr0=None,
         raw.tree_precision = PREC[tree_precision]
     if proxy is not None:
         raw.proxy = int(proxy)
+    if proxy_rule is not None:
+        raw.proxy_rule = PROXY_RULE[proxy_rule]
+    if lam is not None:
+        raw.lambda_ = float(lam)
+    if skel_extra_pts is not None:
+        raw.skel_extra_pts = int(bool(skel_extra_pts))
     return Params(raw)
 
 

@@ -86,8 +86,18 @@ typedef struct hssc_params {
                            Kernel_Eval's LOCAL_get_bc (Common/Kernel_Eval.m:111-159), > 0 drops
                            the row factor b, < 0 the column factor c (HSS1D tests use 1); the
                            HSS2D (fsym) entry points need 0 */
-  int32_t reserved[2];
+  int32_t proxy_rule;   /* params.proxy_rule (:123): HSSC_PROXY_BDRY (default) or
+                           HSSC_PROXY_RAND (HSS2D_build_tree_fsym.m:899-921) */
+  int32_t skel_extra_pts; /* params.skel_extra_pts (:122, default 0): 1 adds skeleton points
+                           chosen by a randomized ID of the interpolation residual
+                           (LOCAL_ID_rand_update, HSS2D_build_tree_fsym.m:1069-1124) */
+  int32_t reserved0;
+  double lambda;        /* params.lambda (:124, default 2): decay of the 'rand' proxy density */
+  int32_t reserved[4];
 } hssc_params;
+
+#define HSSC_PROXY_BDRY 0
+#define HSSC_PROXY_RAND 1
 
 #define HSSC_CONTRAST_PLATEAU 0
 #define HSSC_CONTRAST_GAUSSIAN 1

@@ -365,6 +365,47 @@ def box_proxy(cB, colrad, rowrad, layers, h):
     return np.concatenate([X1[~I1], X2[~I2], X3[~I3], X4[~I4], X1[I1], X2[I2], X3[I3], X4[I4]])
 
 
+def default_rand(n):
+    """MATLAB's rand(n, 1) right after rng('default'): MT19937 seeded 5489 with 53-bit doubles,
+    the stream numpy's RandomState(5489) produces."""
+    return np.random.RandomState(5489).random_sample(n)
+
+
+def random_sampler(X, cB, colrad, rowrad, lam, lay, h):
+    """LOCAL_random_sampler, HSS2D_build_tree_fsym.m:979-1006."""
+    D = np.abs(np.minimum(colrad - np.abs(X[:, 0] - cB[0]), rowrad - np.abs(X[:, 1] - cB[1])))
+    width = lay * h
+    rad = D.max()
+    sz = -math.floor(math.log2(width / rad)) + 1
+    prob = 2.0 ** (-lam * np.arange(sz - 1, -1, -1, dtype=float))
+    B = D < width
+    if sz > 0:
+        Ld = np.minimum(-np.ceil(np.log2(D / rad)), sz - 2) + 1
+        Ld[B] = sz
+        P = prob[Ld.astype(int) - 1]
+        B = default_rand(P.size) < P
+    return X[B]
+
+
+def build_proxy(cB, colrad, rowrad, lev, n_skel, lay_prox, params):
+    """LOCAL_Build_Proxy, HSS2D_build_tree_fsym.m:886-977, for the 'bdry' and 'rand' rules
+    (:893-921): 'rand' thins the ring until it holds k0 points, each pass on the previous
+    pass's points with lambda x 0.9."""
+    Xe = box_proxy(cB, colrad, rowrad, lay_prox, params["h"])
+    if params["proxy_rule"] != "rand":
+        return Xe
+    k0 = math.ceil((2 / 3 if lev % 2 == 0 else 3 / 4) * n_skel)
+    k0 = min(k0, Xe.shape[0])
+    lam, k, c = params["lambda_"], 0, 0
+    while k0 > k and c < 10:
+        Xe = random_sampler(Xe, cB, colrad, rowrad, lam, params["layers"], params["h"])
+        k = Xe.shape[0]
+        if k0 > k:
+            lam = 0.9 * lam
+        c += 1
+    return Xe
+
+
 def proxy_layers(lev, params):
     """lay_prox of HSS2D/HSS2D_build_tree_fsym.m:339-347.
 
@@ -584,7 +625,7 @@ def build_index_sets(params):
 
 def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False,
                stop_level=0, pad=False):
-    """HSS2D_build_tree_fsym (uniform tree, 'bdry' proxies; 'bdry' or ID skeletons).
+    """HSS2D_build_tree_fsym (uniform tree, 'bdry' or 'rand' proxies; 'bdry' or ID skeletons).
 
     Restates :81-287 (tree), :292-623 (upward skeleton pass) and
     LOCAL_Interpolation_Operators :1010-1064.  ``interform`` defaults to params
@@ -597,8 +638,9 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
     sets to the level's largest count.  ``stop_level`` > 0 ends the upward pass there
     (levels below the root only; used to time a bounded sample of a large tree).
     """
-    if params["skel_rule"] != "bdry" or params["proxy_rule"] != "bdry":
-        raise NotImplementedError("only the 'bdry' skeleton/proxy rules are restated")
+    if params["skel_rule"] != "bdry" or params["proxy_rule"] not in ("bdry", "rand"):
+        raise NotImplementedError("only the 'bdry' skeleton rule and the 'bdry' / 'rand' "
+                                  "proxy rules are restated")
     interform = interform or params["INTERform"]
     rng = np.random.default_rng(seed)
     depth = params["depth"]
@@ -622,7 +664,8 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
         keyboxes = [2 ** lev] if TI else list(boxes)
         pad_to = None
         if pad and id_level:
-            ranks = [id_acc(kernel_eval(box_proxy(cent[b], colrad, rowrad, lay_prox, params["h"]),
+            ranks = [id_acc(kernel_eval(build_proxy(cent[b], colrad, rowrad, lev,
+                                                    tree.I_src[b].size, lay_prox, params),
                                         X[tree.I_src[b]], params), params["acc"],
                             np.random.default_rng([seed, b]))[0].shape[0]
                      for b in keyboxes]
@@ -634,8 +677,8 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
             key = tree.key(b)
             if not TI or b == 2 ** lev:
                 cB = cent[b]
-                Xp = box_proxy(cB, colrad, rowrad, lay_prox, params["h"])
                 n = Xsrc.shape[0]
+                Xp = build_proxy(cB, colrad, rowrad, lev, n, lay_prox, params)
                 if id_level:
                     J, k, Tid, rank = non_preset_skeleton(Xp, Xsrc, cB, lev, params,
                                                           np.random.default_rng([seed, b]), pad_to)
