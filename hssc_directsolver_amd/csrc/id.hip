@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
     T* __restrict__ A, const T* __restrict__ Acopy, double acc, const int* __restrict__ kfix,
     int* __restrict__ perm, const int64_t* __restrict__ poff, int* __restrict__ kout,
     T* __restrict__ Tout, const int64_t* __restrict__ toff, T* __restrict__ work,
-    const int64_t* __restrict__ woff, uint64_t seed) {
+    const int64_t* __restrict__ woff, uint64_t seed, const int* __restrict__ kpin) {
   using R = real_t<T>;
   __shared__ R redr[ID_NT / 64];
   __shared__ R sbest[ID_NT / 64];
@@ -345,6 +345,7 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
   const int b = blockIdx.x;
   if (b >= nmat) return;
   const int m = mr[b], n = nc[b];
+  const int pin = kpin ? kpin[b] : 0;  // leading columns taken in order, without pivoting
   T* M = A + off[b];
   const T* Mc = Acopy + off[b];
   int* P = perm + poff[b];
@@ -362,21 +363,23 @@ __global__ void __launch_bounds__(ID_NT) cpqr_id_kernel(
   }
   __syncthreads();
   for (int j = 0; j < kmax; ++j) {
-    // pivot: largest remaining column norm (ties -> lowest index)
+    // pivot: largest remaining column norm (ties -> lowest index); pinned columns in order
     R best = R(-1);
     int bi = j;
-    for (int c = j + tid; c < n; c += ID_NT)
-      if (re_s(nrm[c]) > best) best = re_s(nrm[c]), bi = c;
-    for (int o = 32; o > 0; o >>= 1) {
-      const R ob = shfl_xor_s(best, o);
-      const int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
+    if (j >= pin) {  // uniform over the workgroup
+      for (int c = j + tid; c < n; c += ID_NT)
+        if (re_s(nrm[c]) > best) best = re_s(nrm[c]), bi = c;
+      for (int o = 32; o > 0; o >>= 1) {
+        const R ob = shfl_xor_s(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) best = ob, bi = oi;
+      }
+      if (lane == 0) sbest[wave] = best, sidx[wave] = bi;
+      __syncthreads();
+      best = sbest[0], bi = sidx[0];
+      for (int w = 1; w < ID_NT / 64; ++w)
+        if (sbest[w] > best || (sbest[w] == best && sidx[w] < bi)) best = sbest[w], bi = sidx[w];
     }
-    if (lane == 0) sbest[wave] = best, sidx[wave] = bi;
-    __syncthreads();
-    best = sbest[0], bi = sidx[0];
-    for (int w = 1; w < ID_NT / 64; ++w)
-      if (sbest[w] > best || (sbest[w] == best && sidx[w] < bi)) best = sbest[w], bi = sidx[w];
     const int p = bi;
     __syncthreads();
     if (p != j) {
@@ -850,7 +853,7 @@ template <typename T>
 void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc, T* A,
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
              T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed,
-             int64_t mmax, int64_t nmax) {
+             int64_t mmax, int64_t nmax, const int* kpin) {
   if (nmat <= 0) return;
   // The register-resident kernels run when HSSC_ID_KERNEL=register (read at every call).
   // They give IDs of the same quality 1.7x faster, but their sums round differently, and the
@@ -859,7 +862,7 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
   // the skeletons chosen that way condition the upper levels worse (complex64 refinement
   // stalls), so the global-memory kernel stays the default.
   const char* sel = std::getenv("HSSC_ID_KERNEL");
-  const bool use_reg = sel && std::strcmp(sel, "register") == 0;
+  const bool use_reg = sel && std::strcmp(sel, "register") == 0 && !kpin;
   if constexpr (is_complex_v<T>) {
     if (!use_reg) mmax = 0;
     if (mmax > 0 && nmax > 0 && mmax <= IDR_NW * IDS_RW && nmax <= 64 * IDS_CL) {
@@ -906,17 +909,17 @@ void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const 
   HSSC_TIMED(is_complex_v<T> ? "cpqr_id_c128" : "cpqr_id_f64", st, 0.0, 0.0,
              hipLaunchKernelGGL(cpqr_id_kernel<T>, dim3(nmat), dim3(ID_NT), 0, st, nmat, off, mr,
                                 nc, A, Acopy, acc, kfix, perm, poff, kout, Tout, toff, work, woff,
-                                seed));
+                                seed, kpin));
 }
 
 template void cpqr_id<double>(hipStream_t, int, const int64_t*, const int*, const int*, double*,
                               const double*, double, const int*, int*, const int64_t*, int*,
                               double*, const int64_t*, double*, const int64_t*, uint64_t,
-                              int64_t, int64_t);
+                              int64_t, int64_t, const int*);
 template void cpqr_id<zdouble>(hipStream_t, int, const int64_t*, const int*, const int*, zdouble*,
                                const zdouble*, double, const int*, int*, const int64_t*, int*,
                                zdouble*, const int64_t*, zdouble*, const int64_t*, uint64_t,
-                               int64_t, int64_t);
+                               int64_t, int64_t, const int*);
 
 // Standard normal numbers (Box-Muller on the counter-based uniforms): G[i], i < len,
 // element i drawn from (seed, base + i).

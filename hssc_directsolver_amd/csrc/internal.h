@@ -130,6 +130,8 @@ struct IdJob {
   int64_t nbox, n, M;
   const double *Sx, *Sy, *Sw;  // nbox x n
   const double *Qx, *Qy, *Qw;  // nbox x M
+  int64_t pin = 0;  // > 0: the first pin columns are a preset skeleton that the job extends
+                    // with the residual's skeleton points (params.skel_extra_pts)
 };
 // Fills order (nbox x n, the ID's column order, skeleton first) and rank (nbox adaptive
 // ranks); returns an HSSC_* status.

@@ -178,14 +178,17 @@ void add2d(hipStream_t st, int64_t m, int64_t n, T a, const T* s1, int64_t ld1, 
 template <typename T>
 void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks);
 
-// Batched ragged interpolative decomposition (id.hip).  Work per matrix: 5 n + m.  mmax and
-// nmax (> 0 when every block is known to be at most mmax x nmax) select the register-resident
-// kernel for complex blocks that fit it; a block larger than those maxima gets rank -1.
+// Batched ragged interpolative decomposition (id.hip).  Work per matrix: 5 n + m; after the
+// call work[woff[b] + n + j] holds |R_jj| (real part) for j < min(m, n).  mmax and nmax (> 0
+// when every block is known to be at most mmax x nmax) select the register-resident kernel for
+// complex blocks that fit it; a block larger than those maxima gets rank -1.  kpin[b] (when
+// given) leading columns of block b are factored in order before column pivoting starts (the
+// global-memory kernel runs then).
 template <typename T>
 void cpqr_id(hipStream_t st, int nmat, const int64_t* off, const int* mr, const int* nc, T* A,
              const T* Acopy, double acc, const int* kfix, int* perm, const int64_t* poff, int* kout,
              T* Tout, const int64_t* toff, T* work, const int64_t* woff, uint64_t seed,
-             int64_t mmax = 0, int64_t nmax = 0);
+             int64_t mmax = 0, int64_t nmax = 0, const int* kpin = nullptr);
 
 // Acceptance of IDs computed on sketches with sr[b] rows, checked on the full blocks (n <= 256):
 // ok[b] = 1 when |A_s T x - A_r x| <= 10 acc |A g| for random unit x, g (Common/ID.m:55-59)

@@ -427,9 +427,89 @@ static int id_select_chunk(const hssc_params& p, const IdJob& jb, int32_t* order
 // sketches and the ID work) stay under kIdChunkBytes.
 constexpr int64_t kIdChunkBytes = int64_t(8) << 30;
 
+// Extra skeleton points of a preset level (params.skel_extra_pts; LOCAL_ID_rand_update,
+// HSS2D/HSS2D_build_tree_fsym.m:1069-1124).  A_b = K(X_proxy, X_src) of every key box, columns
+// in the preset J order, is factored by the column-pivoted QR with its first jb.pin (skeleton)
+// columns taken in order: after them the trailing block is the interpolation residual
+// K(Xp, X_rs) - K(Xp, X_sk) T (T the least-squares interpolation operator) in an orthonormal
+// basis, so the later pivots and |R_jj| are the residual's own column-pivoted QR.  The
+// reference counts the ranks of a Gaussian sketch of the residual with q0 = 8 layers
+// ceil(log2 k0) samples against the absolute tolerance acc (QR_findrank.m:45-92); a sketch
+// scales the residual's singular values by about sqrt(q0), so the residual's own pivots count
+// while |R_jj| > acc / sqrt(q0).  order = the pivot order, rank = pin + the added points.
+template <typename S>
+static int extra_points_chunk(const hssc_params& p, const IdJob& jb, int32_t* order,
+                              int32_t* rank) {
+  return guarded([&]() {
+    const int64_t nb = jb.nbox, n = jb.n, M = jb.M, pin = jb.pin;
+    const int64_t kq = std::min(M, n);
+    const int64_t tsz = std::max<int64_t>(1, kq * (n - kq)), wsz = 6 * n + M;
+    DevBufs B;
+    const double* Sx = B.upload(jb.Sx, nb * n);
+    const double* Sy = B.upload(jb.Sy, nb * n);
+    const double* Sw = B.upload(jb.Sw, nb * n);
+    const double* Qx = B.upload(jb.Qx, nb * M);
+    const double* Qy = B.upload(jb.Qy, nb * M);
+    const double* Qw = B.upload(jb.Qw, nb * M);
+    std::vector<int64_t> off(nb), poff(nb), toff(nb), woff(nb);
+    std::vector<int32_t> mr(nb, int32_t(M)), nc(nb, int32_t(n)), kfix(nb, int32_t(kq)),
+        kpin(nb, int32_t(pin));
+    for (int64_t b = 0; b < nb; ++b)
+      off[b] = b * M * n, poff[b] = b * n, toff[b] = b * tsz, woff[b] = b * wsz;
+    S* A = B.get<S>(nb * M * n);
+    S* Ac = B.get<S>(nb * M * n);
+    S* Tt = B.get<S>(nb * tsz);
+    S* W = B.get<S>(nb * wsz);
+    int32_t* dperm = B.get<int32_t>(nb * n);
+    int32_t* dk = B.get<int32_t>(nb);
+    const hipStream_t st = 0;
+    k::gen_block<S>(st, M, n, nb, Qx, Qy, Qw, M, Sx, Sy, Sw, n, A, M, M * n, kernel_spec(p));
+    HIPCHK(hipMemcpyAsync(Ac, A, size_t(nb * M * n) * sizeof(S), hipMemcpyDeviceToDevice, st));
+    k::cpqr_id<S>(st, int(nb), B.upload(off.data(), nb), B.upload(mr.data(), nb),
+                  B.upload(nc.data(), nb), A, Ac, -1.0, B.upload(kfix.data(), nb), dperm,
+                  B.upload(poff.data(), nb), dk, Tt, B.upload(toff.data(), nb), W,
+                  B.upload(woff.data(), nb), 0x5EEDull ^ uint64_t(jb.lev), 0, 0,
+                  B.upload(kpin.data(), nb));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<S> w(size_t(nb * wsz));
+    HIPCHK(hipMemcpy(w.data(), W, w.size() * sizeof(S), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(order, dperm, size_t(nb * n) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const double q0 = 8.0 * p.layers * std::ceil(std::log2(double(std::max<int64_t>(pin, 2))));
+    const double thr = p.acc / std::sqrt(q0);
+    for (int64_t b = 0; b < nb; ++b) {
+      int64_t k1 = 0;
+      for (int64_t j = pin; j < kq; ++j) k1 += re_s(w[b * wsz + n + j]) > thr;
+      rank[b] = int32_t(pin + k1);
+    }
+    return HSSC_OK;
+  });
+}
+
+static int extra_points_gpu(const hssc_params& p, const IdJob& jb, int32_t* order,
+                            int32_t* rank) {
+  if (jb.n < 1 || jb.M < 1 || jb.n > INT32_MAX || jb.M > INT32_MAX || jb.pin > jb.n)
+    return fail(HSSC_ERR_INVALID_ARG, "empty or oversized extra-points block");
+  const bool cx = p.flag_pot == HSSC_POT_SL_H_2D;
+  const int64_t es = cx ? int64_t(sizeof(zdouble)) : int64_t(sizeof(double));
+  const int64_t per_box = es * (2 * jb.M * jb.n + jb.n * jb.n / 4 + 7 * jb.n + jb.M);
+  const int64_t chunk = std::max<int64_t>(1, kIdChunkBytes / per_box);
+  for (int64_t b0 = 0; b0 < jb.nbox; b0 += chunk) {
+    IdJob sub = jb;
+    sub.nbox = std::min(chunk, jb.nbox - b0);
+    sub.Sx += b0 * jb.n, sub.Sy += b0 * jb.n, sub.Sw += b0 * jb.n;
+    sub.Qx += b0 * jb.M, sub.Qy += b0 * jb.M, sub.Qw += b0 * jb.M;
+    const int s = cx ? extra_points_chunk<zdouble>(p, sub, order + b0 * jb.n, rank + b0)
+                     : extra_points_chunk<double>(p, sub, order + b0 * jb.n, rank + b0);
+    if (s) return s;
+  }
+  return HSSC_OK;
+}
+
 int id_select_gpu(const hssc_params& p, const IdJob& jb, int32_t* order, int32_t* rank,
                   const IdPolicy& pol) {
   if (int s = check_device()) return s;
+  if (jb.pin > 0) return extra_points_gpu(p, jb, order, rank);
   if (p.flag_pot != HSSC_POT_SL_H_2D)
     return fail(HSSC_ERR_UNSUPPORTED, "ID-selected skeletons are defined for kh > 0 only");
   if (jb.n < 1 || jb.M < 1 || jb.n > INT32_MAX || jb.M > INT32_MAX)

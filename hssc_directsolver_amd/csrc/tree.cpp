@@ -568,6 +568,65 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       });
       for (int64_t bi = 0; bi < L.nkey; ++bi) pad_to = std::max<int64_t>(pad_to, L.kbox[bi]);
     }
+    // params.skel_extra_pts (:380-391 selects the preset levels below the root; :545-564 NTI,
+    // :582-597 TI): the preset skeleton of every key is extended by the skeleton of its
+    // interpolation residual K(Xp, X_rs) - K(Xp, X_sk) T (LOCAL_ID_rand_update, :1069-1124).
+    // The GPU job factors K(Xp, X_src) in the preset J order with the skeleton columns pinned
+    // first, so its later pivots are the residual's; every key then keeps the level's largest
+    // count, as at the ID levels.
+    const bool extra_level = p.skel_extra_pts && !id_level && lev > 0;
+    if (extra_level) {
+      if (!sel) return fail(HSSC_ERR_UNSUPPORTED, "skel_extra_pts needs the GPU");
+      const int64_t n = L.n;
+      std::vector<std::vector<int32_t>> J0(L.nkey);
+      std::vector<int64_t> k0(L.nkey);
+      std::vector<double> qx0, qy0;
+      build_proxy(t->cent[2 * b0], t->cent[2 * b0 + 1], colrad, rowrad, lev, n, lay_prox, p, &qx0,
+                  &qy0);
+      const int64_t M = int64_t(qx0.size());
+      hvec<double> Sx(L.nkey * n), Sy(L.nkey * n), Sw(L.nkey * n);
+      hvec<double> Qx(L.nkey * M), Qy(L.nkey * M), Qw(L.nkey * M);
+      const bool same = parallel_keys([&](int64_t bi) {
+        const int64_t b = b0 + bi;
+        const auto& I = t->Isrc[b];
+        const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
+        std::vector<double> xs(n), ys(n);
+        for (int64_t i = 0; i < n; ++i) xs[i] = Xx[I[i]], ys[i] = Xy[I[i]];
+        preset_skeleton(xs, ys, cx, cy, lev, p, pad_to, &J0[bi], &k0[bi]);
+        for (int64_t a = 0; a < n; ++a) {
+          const int32_t j = J0[bi][a];
+          Sx[bi * n + a] = xs[j], Sy[bi * n + a] = ys[j], Sw[bi * n + a] = wgt[I[j]];
+        }
+        std::vector<double> qx, qy;
+        build_proxy(cx, cy, colrad, rowrad, lev, n, lay_prox, p, &qx, &qy);
+        if (int64_t(qx.size()) != M) return false;
+        for (int64_t r = 0; r < M; ++r)
+          Qx[bi * M + r] = qx[r], Qy[bi * M + r] = qy[r], Qw[bi * M + r] = weight(qx[r], qy[r]);
+        return true;
+      });
+      if (!same || std::any_of(k0.begin(), k0.end(), [&](int64_t v) { return v != k0[0]; }))
+        return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
+      if (k0[0] < n) {
+        IdJob job{lev, L.nkey, n, M, Sx.data(), Sy.data(), Sw.data(), Qx.data(), Qy.data(),
+                  Qw.data()};
+        job.pin = k0[0];
+        std::vector<int32_t> ord(L.nkey * n), rank(L.nkey);
+        if (int s = sel(p, job, ord.data(), rank.data())) return s;
+        id_order.resize(L.nkey * n);
+        pad_to = 0;
+        for (int64_t bi = 0; bi < L.nkey; ++bi) {
+          for (int64_t a = 0; a < n; ++a) id_order[bi * n + a] = J0[bi][ord[bi * n + a]];
+          L.kbox[bi] = rank[bi];
+          pad_to = std::max<int64_t>(pad_to, rank[bi]);
+        }
+      } else {
+        id_order.resize(L.nkey * n);
+        for (int64_t bi = 0; bi < L.nkey; ++bi)
+          std::copy(J0[bi].begin(), J0[bi].end(), id_order.begin() + bi * n);
+        pad_to = k0[0];
+      }
+    }
+    const bool ordered = id_level || extra_level;  // skeleton = the head of id_order
     // skeleton, J order and proxy of one box (key slot = bi for NTI, 0 for TI)
     auto process = [&](int64_t bi) -> bool {
       const int64_t b = b0 + bi;
@@ -578,7 +637,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       const double cx = t->cent[2 * b], cy = t->cent[2 * b + 1];
       std::vector<int32_t> J;
       int64_t k;
-      if (id_level) {
+      if (ordered) {
         const int32_t* o = id_order.data() + (TI ? 0 : bi) * n;
         const std::vector<int32_t> sk(o, o + pad_to), rs(o + pad_to, o + n);
         orient(sk, rs, xs, ys, cx, cy, lev, &J);
@@ -617,7 +676,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       return true;
     };
     if (!process(0)) return fail(HSSC_ERR_UNSUPPORTED, "non-uniform skeleton or proxy sizes");
-    if (!TI && L.nbox > 1 && dyadic && !id_level && !ragged_below) {
+    if (!TI && L.nbox > 1 && dyadic && !ordered && !ragged_below) {
       // All coordinates are exact dyadic rationals, so every box of the level is an exact
       // translate of box 0: the same J (same comparisons on bit-identical relative
       // coordinates) and the same proxy ring shifted by the centre difference.
@@ -671,7 +730,7 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
     });
     for (auto& kb : L.kbox)
       if (kb < 0) kb = int32_t(L.k);
-    if (id_level && !TI) ragged_below = true;
+    if (ordered && !TI) ragged_below = true;
     L.q = L.n - L.k;
   }
   // Leaf gather map: global index of each J-ordered leaf point.

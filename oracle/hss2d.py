@@ -510,12 +510,12 @@ def qr_findrank(m, n, matvecAt, k0, C, rng, eps):
             Y = np.concatenate([Y[:, :krank], np.zeros((Y.shape[0], krank + num - krank),
                                                        dtype=Y.dtype)], axis=1)
         Y[:, krank:krank + num] = Ynew
-        M = Ynew - Q @ (Q.T @ Ynew)
+        M = Ynew - Q @ (Q.conj().T @ Ynew)
         Qc, R, _ = sla.qr(M, mode="economic", pivoting=True)
         D = np.abs(np.diag(R))
         residual = D[-1] if D.size else 0.0
         krank += int(np.sum(D > eps))
-        Qc = Qc - Q @ (Q.T @ Qc)
+        Qc = Qc - Q @ (Q.conj().T @ Qc)
         Q = np.concatenate([Q, Qc], axis=1)
     return krank, Y[:, :krank]
 
@@ -523,10 +523,34 @@ def qr_findrank(m, n, matvecAt, k0, C, rng, eps):
 def skel_colk(Z, k):
     """Common/ID_rand.m:98-106: fixed-rank ID of Z via CPQR + SVD pseudo-inverse of R11."""
     Q, R, I = sla.qr(Z, mode="economic", pivoting=True)
+    if k == 0:
+        return np.zeros((0, Z.shape[1]), dtype=Z.dtype), I
     U, D, Vh = np.linalg.svd(R[:k, :k])
     q = int(np.sum(D > 1e-12))
-    T = Vh[:q].T @ ((U[:, :q].T @ R[:k, k:]) / D[:q, None])
+    T = Vh[:q].conj().T @ ((U[:, :q].conj().T @ R[:k, k:]) / D[:q, None])
     return T, I
+
+
+def id_rand_update(T0, J0, k0, Xp, Xsrc, params, rng):
+    """LOCAL_ID_rand_update, HSS2D_build_tree_fsym.m:1069-1124, dense T (T.V = I): a
+    randomized ID (ID_rand 'norm' with q0 = 8 layers ceil(log2 k0) samples and increments
+    C = q0 / 2, absolute tolerance acc, :1093-1097) of the interpolation residual
+    Res = K(Xp, X_rs) - K(Xp, X_sk) T0 adds its k1 skeleton columns to the skeleton set:
+    J = [J0_sk; J0_rs(J1)], T = [T0(:, J1_rs) - T0(:, J1_sk) T1; T1] (:1100-1106).
+    Returns (T, J, k)."""
+    J0 = np.asarray(J0)
+    J0_sk, J0_rs = J0[:k0], J0[k0:]
+    A_rs = kernel_eval(Xp, Xsrc[J0_rs], params)
+    A_sk = kernel_eval(Xp, Xsrc[J0_sk], params)
+    Res = A_rs - A_sk @ T0
+    m, n = Res.shape
+    q0 = 8 * params["layers"] * math.ceil(math.log2(k0))
+    krank, Y = qr_findrank(m, n, lambda G: Res.T @ G, q0, q0 // 2, rng, params["acc"])
+    T1, J1 = skel_colk(Y.T, krank)
+    k1 = T1.shape[0]
+    J1_sk, J1_rs = J1[:k1], J1[k1:]
+    Tf = np.concatenate([T0[:, J1_rs] - T0[:, J1_sk] @ T1, T1])
+    return Tf, np.concatenate([J0_sk, J0_rs[J1]]), k0 + k1
 
 
 def lrid_rand(K21, k0, C, rng, tol):
@@ -623,6 +647,34 @@ def build_index_sets(params):
     return build_tree(params, with_T=False)
 
 
+def residual_pivots(A, k0, params):
+    """The library's extra-point choice for a preset box (no reference counterpart; it
+    replaces the randomized ID of id_rand_update, whose Gaussian stream is MATLAB's): the
+    column-pivoted QR of the interpolation residual A_rs - A_sk T = (I - P_sk) A_rs, its
+    points counted while |R_jj| > acc / sqrt(q0) (a Gaussian sketch of q0 samples, as the
+    reference draws, scales the residual's singular values by about sqrt(q0)).
+    Returns (pivot order of the residual columns, count)."""
+    Q, _ = np.linalg.qr(A[:, :k0])
+    Res = A[:, k0:] - Q @ (Q.conj().T @ A[:, k0:])
+    _, R, P = sla.qr(Res, mode="economic", pivoting=True)
+    q0 = 8 * params["layers"] * math.ceil(math.log2(max(k0, 2)))
+    return P, int(np.sum(np.abs(np.diag(R)) > params["acc"] / math.sqrt(q0)))
+
+
+def extra_points(T0, J0, k0, Xp, Xsrc, cB, lev, b, params, rng, TI):
+    """The skel_extra_pts update of a preset box (HSS2D_build_tree_fsym.m:545-564 NTI,
+    :582-597 TI): id_rand_update, then the skeleton re-oriented in polar order from th0 (TI
+    and even boxes) or th1 = th0 - pi (odd boxes), the residual along the interface (TI) or
+    left in the ID's order (NTI).  Returns dense (T, J, k)."""
+    T, J, k = id_rand_update(T0, J0, k0, Xp, Xsrc, params, rng)
+    nrm, tan, th0 = level_orientation(lev)
+    th = th0 if (TI or b % 2 == 0) else th0 - np.pi
+    Jsk, Jrs = J[:k], J[k:]
+    Js = orient_skeleton(Xsrc[Jsk], cB, th)
+    Jr = orient_residual(Xsrc[Jrs], nrm, tan) if TI else np.arange(Jrs.size)
+    return T[np.ix_(Js, Jr)], np.concatenate([Jsk[Js], Jrs[Jr]]), k
+
+
 def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset_all=False,
                stop_level=0, pad=False):
     """HSS2D_build_tree_fsym (uniform tree, 'bdry' or 'rand' proxies; 'bdry' or ID skeletons).
@@ -663,6 +715,7 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
                 tree.I_src[b] = np.concatenate([tree.I_sk[2 * b], tree.I_sk[2 * b + 1]])
         keyboxes = [2 ** lev] if TI else list(boxes)
         pad_to = None
+
         if pad and id_level:
             ranks = [id_acc(kernel_eval(build_proxy(cent[b], colrad, rowrad, lev,
                                                     tree.I_src[b].size, lay_prox, params),
@@ -672,6 +725,22 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
             pad_to = max(ranks)
         elif pad and ragged_below and not TI:
             pad_to = max(bdry_count(X[tree.I_src[b]], cent[b], lev, params) for b in keyboxes)
+        extra_order = {}
+        if pad and params.get("skel_extra_pts", 0) and not id_level and lev > 0:
+            # the library's uniform extra points: every key keeps the level's largest count,
+            # the next pivots of its own residual order (residual_pivots)
+            counts = {}
+            for b in keyboxes:
+                Xsrc = X[tree.I_src[b]]
+                J0, k0 = preset_skeleton(Xsrc, cent[b], lev, params, pad_to or 0)
+                Xp = build_proxy(cent[b], colrad, rowrad, lev, Xsrc.shape[0], lay_prox, params)
+                if Xsrc.shape[0] > k0:
+                    P, k1 = residual_pivots(kernel_eval(Xp, Xsrc[J0], params), k0, params)
+                else:
+                    P, k1 = np.zeros(0, int), 0
+                extra_order[b] = (J0, k0, P)
+                counts[b] = k0 + k1
+            pad_to = max(counts.values())
         for b in boxes:
             Xsrc = X[tree.I_src[b]]
             key = tree.key(b)
@@ -684,18 +753,34 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
                                                           np.random.default_rng([seed, b]), pad_to)
                     tree.box_rank[key] = rank
                     U, V = (Tid, None) if with_T else (None, None)
+                elif b in extra_order:
+                    J0, k0, P = extra_order[b]
+                    nrm, tan, th0 = level_orientation(lev)
+                    order = np.concatenate([J0[:k0], J0[k0:][P]])
+                    I_sk, I_rs = order[:pad_to], order[pad_to:]
+                    Js = orient_skeleton(Xsrc[I_sk], cB, th0)
+                    Jr = orient_residual(Xsrc[I_rs], nrm, tan)
+                    J, k = np.concatenate([I_sk[Js], I_rs[Jr]]), pad_to
+                    tree.box_rank[key] = counts[b]
+                    U, V = (interpolation_operator(params, Xp, Xsrc[J[:k]], Xsrc[J[k:]], False,
+                                                   q, lev, rng) if with_T and n > k
+                            else (np.zeros((k, 0)), None) if with_T else (None, None))
                 else:
                     J, k = preset_skeleton(Xsrc, cB, lev, params, pad_to or 0)
                     tree.box_rank[key] = bdry_count(Xsrc, cB, lev, params)
                     if not with_T:
                         U, V = None, None
                     elif n > k:
-                        lowrank = interform == "lowrank" and lev < depth - 1
+                        extra = params.get("skel_extra_pts", 0) and lev > 0 and not pad
+                        lowrank = interform == "lowrank" and lev < depth - 1 and not extra
                         # k >= k_cut selects interpolation_operator_HSS in the reference:
                         # the same T to tolerance, restated by the dense least squares
                         # (the HSS1D least squares is out of scope)
                         U, V = interpolation_operator(params, Xp, Xsrc[J[:k]], Xsrc[J[k:]],
                                                       lowrank, q, lev, rng)
+                        if extra:
+                            U, J, k = extra_points(U, J, k, Xp, Xsrc, cB, lev, b, params, rng,
+                                                   TI)
                     else:
                         U, V = np.zeros((k, 0)), None
                 tree.J[key], tree.k[key], tree.T[key], tree.Xp[key] = J, k, (U, V), Xp
@@ -703,7 +788,7 @@ def build_tree(params, interform=None, seed=0, with_T=True, timings=None, preset
                     q[lev] = max(q[lev], U.shape[1]) if not TI else U.shape[1]
             J, k = tree.J[key], tree.k[key]
             tree.I_sk[b] = tree.I_src[b][J[:k]]
-        if id_level and not TI:
+        if (id_level or extra_order) and not TI:
             ragged_below = True
         if timings is not None:
             timings[lev] = timings.get(lev, 0.0) + _time.perf_counter() - _t0
