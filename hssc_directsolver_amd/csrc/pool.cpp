@@ -1,6 +1,7 @@
 // Device memory cache (see pool.h).
 #include "pool.h"
 
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -74,7 +75,69 @@ int poison_byte() {
 
 void poison(void* p, size_t s) {
   const int v = poison_byte();
-  if (v >= 0) (void)hipMemset(p, v, s);
+  if (v < 0) return;
+  // complete before the block is handed out: work that the caller then queues on streams that
+  // do not synchronise with the null stream must not race with the fill
+  (void)hipMemsetAsync(p, v, s, nullptr);
+  (void)hipStreamSynchronize(nullptr);
+}
+
+// HSSC_POOL_GUARD=<KiB> (diagnostics): every block is allocated on its own with guard zones
+// of that size before and after the requested bytes, filled with a pattern; a release waits
+// for the device and reports to stderr any guard byte that changed (a write out of bounds of
+// the block), then frees the block.  The cache is bypassed.
+size_t guard_bytes() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HSSC_POOL_GUARD");
+    return e ? size_t(std::strtoull(e, nullptr, 0)) * 1024 : size_t(0);
+  }();
+  return v;
+}
+constexpr unsigned char kGuard = 0xA5;
+struct Guarded {
+  void* raw;
+  size_t bytes;
+  uint64_t serial;
+};
+std::unordered_map<void*, Guarded>& guarded_blocks() {
+  static auto* m = new std::unordered_map<void*, Guarded>();
+  return *m;
+}
+
+hipError_t guarded_alloc(void** p, size_t bytes) {
+  static uint64_t serial = 0;
+  const size_t g = guard_bytes(), rb = (bytes + 255) / 256 * 256;
+  void* raw = nullptr;
+  hipError_t e = hipMalloc(&raw, rb + 2 * g);
+  if (e != hipSuccess) return e;
+  (void)hipMemsetAsync(raw, kGuard, rb + 2 * g, nullptr);
+  (void)hipStreamSynchronize(nullptr);
+  *p = static_cast<char*>(raw) + g;
+  guarded_blocks()[*p] = Guarded{raw, bytes, ++serial};
+  return hipSuccess;
+}
+
+void guarded_release(void* p, const Guarded& b) {
+  const size_t g = guard_bytes(), rb = (b.bytes + 255) / 256 * 256;
+  (void)hipDeviceSynchronize();
+  std::vector<unsigned char> front(g), back(g + rb - b.bytes);
+  (void)hipMemcpy(front.data(), b.raw, g, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(back.data(), static_cast<char*>(p) + b.bytes, back.size(),
+                  hipMemcpyDeviceToHost);
+  size_t nf = 0, nb = 0, last_f = 0, first_b = 0, last_b = 0;
+  for (size_t i = 0; i < g; ++i)
+    if (front[i] != kGuard) ++nf, last_f = g - i;
+  for (size_t i = 0; i < back.size(); ++i)
+    if (back[i] != kGuard) {
+      if (!nb) first_b = i;
+      ++nb, last_b = i;
+    }
+  if (nf || nb)
+    std::fprintf(stderr,
+                 "[hssc pool guard] block #%llu of %zu bytes: %zu bytes written before it (up to "
+                 "%zu before), %zu after it (from +%zu to +%zu past the end)\n",
+                 (unsigned long long)b.serial, b.bytes, nf, last_f, nb, first_b, last_b);
+  (void)hipFree(b.raw);
 }
 
 bool take_locked(Device& d, size_t s, void** p) {
@@ -94,6 +157,7 @@ hipError_t alloc(void** p, size_t bytes) {
   *p = nullptr;
   const size_t s = round_size(bytes);
   std::lock_guard<std::mutex> g(mu);
+  if (guard_bytes()) return guarded_alloc(p, bytes);
   Device& d = current();
   if (take_locked(d, s, p)) return hipSuccess;
   for (auto& b : d.released)
@@ -120,6 +184,15 @@ hipError_t alloc(void** p, size_t bytes) {
 void release(void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> g(mu);
+  if (guard_bytes()) {
+    auto it = guarded_blocks().find(p);
+    if (it != guarded_blocks().end()) {
+      const Guarded b = it->second;
+      guarded_blocks().erase(it);
+      guarded_release(p, b);
+      return;
+    }
+  }
   Device& d = current();
   auto it = d.live.find(p);
   if (it == d.live.end()) {

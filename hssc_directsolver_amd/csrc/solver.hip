@@ -987,6 +987,13 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
   const k::KernelSpec ks = kernel_spec(t->p);
   const bool TI = t->p.transinv != 0;
   const bool tree_lo = t->p.tree_precision == HSSC_PREC_FP32;
+  // The build's level streams are drained before the inversion starts.  An inversion running
+  // beside them gave complex128 factors that differed between identical runs (every T
+  // identical to a drained build's; E and F^-1 wrong from a random level up, in about one run
+  // in three after another path's build had run in the process), also when the host waited
+  // for each level's T before queueing its readers; inverting the same tree again once the
+  // build had drained gave the exact factors every time.
+  tree_sync(t);
   if (tree_lo && sizeof(real_t<T>) == 8) ensure_T64(t);  // FP64 factors of an FP32 tree
   inv->FI.assign(D + 1, nullptr);
   inv->E.assign(D + 1, nullptr);
@@ -1029,7 +1036,6 @@ void invert_impl(hssc_tree* t, hssc_inverse* inv, hipStream_t st) {
       inv->E[l] = EI;
       k::copy2d<T, T>(st, kk, kk, FI, n, n * n, EI, kk, kk * kk, E);
       if (r > 0) {
-        // T of this level may still be in the making on the build's streams
         if (L.ready) HIPCHK(hipStreamWaitEvent(st, L.ready, 0));
         if (sizeof(real_t<T>) == 4 && !tree_lo) {  // an FP32 tree already holds the FP32 T
           Level& Lm = t->levels[l];
