@@ -140,11 +140,13 @@ struct hssc_mlp {
   int cap = 0;
   float *Xs = nullptr, *Z1 = nullptr, *A1 = nullptr, *Z2 = nullptr, *A2 = nullptr, *y = nullptr,
         *d = nullptr, *t = nullptr, *Xd = nullptr, *D2 = nullptr, *D1 = nullptr, *part = nullptr;
+  float* kpart = nullptr;  // partial products of the split-K weight gradients
+  int64_t kpart_cap = 0;
   void free_ws() {
-    for (float* q : {Xs, Z1, A1, Z2, A2, y, d, t, Xd, D2, D1, part})
+    for (float* q : {Xs, Z1, A1, Z2, A2, y, d, t, Xd, D2, D1, part, kpart})
       if (q) (void)hipFree(q);
-    Xs = Z1 = A1 = Z2 = A2 = y = d = t = Xd = D2 = D1 = part = nullptr;
-    cap = 0;
+    Xs = Z1 = A1 = Z2 = A2 = y = d = t = Xd = D2 = D1 = part = kpart = nullptr;
+    cap = 0, kpart_cap = 0;
   }
   ~hssc_mlp() {
     free_ws();
@@ -155,6 +157,17 @@ struct hssc_mlp {
 
 namespace {
 
+// Weight gradients are sums over the batch (K = B): the batch is split into chunks of this
+// many samples, one workgroup each, and the partial products are summed in chunk order.
+constexpr int kGradChunk = 128;
+
+// C (m x n) = A^T B over the batch, split over K.
+void grad_gemm(hssc_mlp* m, int64_t rows, int64_t cols, int B, const float* A, const float* Bm,
+               float* C) {
+  hssc::k::gemm_tn_splitk<float>(0, rows, cols, B, A, B, 0, Bm, B, 0, C, rows, 0, 1, m->kpart,
+                                  kGradChunk, m->kpart_cap);
+}
+
 int ensure(hssc_mlp* m, int B) {
   if (B <= m->cap) return HSSC_OK;
   m->free_ws();
@@ -163,8 +176,10 @@ int ensure(hssc_mlp* m, int B) {
   if (al(&m->Xs, b * kIn) || al(&m->Z1, b * kHid) || al(&m->A1, b * kHid) ||
       al(&m->Z2, b * kHid) || al(&m->A2, b * kHid) || al(&m->y, b) || al(&m->d, b) ||
       al(&m->t, b) || al(&m->Xd, b * kIn) || al(&m->D2, b * kHid) || al(&m->D1, b * kHid) ||
-      al(&m->part, (b + 255) / 256 + 1))
+      al(&m->part, (b + 255) / 256 + 1) ||
+      al(&m->kpart, size_t((B + kGradChunk - 1) / kGradChunk) * kHid * kHid))
     return hssc::fail(HSSC_ERR_OUT_OF_MEMORY, "mlp workspace");
+  m->kpart_cap = int64_t((B + kGradChunk - 1) / kGradChunk) * kHid * kHid;
   m->cap = B;
   return HSSC_OK;
 }
@@ -195,22 +210,19 @@ int backward(hssc_mlp* m, int B, double* sse) {
   const int nb = (B + 255) / 256;
   hipLaunchKernelGGL(mse_grad_kernel, dim3(nb), dim3(256), 0, 0, B, m->y, m->t, m->d, m->part);
   // output layer: dw3 = A2^T d, db3 = sum d, dA2 = d w3^T
-  gemm<float>(0, true, false, kHid, 1, B, 1.0f, m->A2, B, 0, m->d, B, 0, 0.0f, G + Offs::w3, kHid,
-              0, 1);
+  grad_gemm(m, kHid, 1, B, m->A2, m->d, G + Offs::w3);
   hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, 0, B, m->d, G + Offs::b3);
   gemm<float>(0, false, true, B, kHid, 1, 1.0f, m->d, B, 0, P + Offs::w3, kHid, 0, 0.0f, m->D2, B,
               0, 1);
   hipLaunchKernelGGL(gelu_back_kernel, g1(B * kHid), dim3(256), 0, 0, B, kHid, m->Z2, m->D2);
   // hidden layer 2: dW2 = A1^T D2, db2 = colsum D2, dA1 = D2 W2^T
-  gemm<float>(0, true, false, kHid, kHid, B, 1.0f, m->A1, B, 0, m->D2, B, 0, 0.0f, G + Offs::W2,
-              kHid, 0, 1);
+  grad_gemm(m, kHid, kHid, B, m->A1, m->D2, G + Offs::W2);
   hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, 0, B, m->D2, G + Offs::b2);
   gemm<float>(0, false, true, B, kHid, kHid, 1.0f, m->D2, B, 0, P + Offs::W2, kHid, 0, 0.0f, m->D1,
               B, 0, 1);
   hipLaunchKernelGGL(gelu_back_kernel, g1(B * kHid), dim3(256), 0, 0, B, kHid, m->Z1, m->D1);
   // hidden layer 1: dW1 = Xs^T D1, db1 = colsum D1
-  gemm<float>(0, true, false, kIn, kHid, B, 1.0f, m->Xs, B, 0, m->D1, B, 0, 0.0f, G + Offs::W1,
-              kIn, 0, 1);
+  grad_gemm(m, kIn, kHid, B, m->Xs, m->D1, G + Offs::W1);
   hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, 0, B, m->D1, G + Offs::b1);
   if (sse) {
     std::vector<float> part(nb);
