@@ -936,6 +936,7 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
   __shared__ C Wb[2 * QR_NB][QR_NB + 1];   // Gram staging (64 rows); pass 3: LU | scratch
   __shared__ C sdiag[QR_NB];               // S of the reconstruction
   __shared__ R rdi[QR_NB];                 // 1 / R_cc
+  __shared__ R rdg[QR_NB];                 // R_cc
   __shared__ C udi[QR_NB];                 // 1 / U_cc
   const int64_t e = blockIdx.y;
   const int blk = blockIdx.x, tid = threadIdx.x;
@@ -977,38 +978,70 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       if (tid < jb) Rm[tid][tid] += C(shift);
       __syncthreads();
     }
+    // Elimination on the unscaled rows: step k subtracts conj(U_ki) U_kj / U_kk from the
+    // trailing entries (U_kk = R_kk^2 is final from step k on); one barrier per step, and
+    // each thread's operands are read together before its updates.  The rows are scaled
+    // by 1 / R_kk afterwards.
     for (int k = 0; k < jb; ++k) {
       const R d2 = re_s(Rm[k][k]);
-      const R d = d2 > R(0) ? sqrt(d2) : std::numeric_limits<R>::min();
-      const C inv2 = C(R(1) / (d * d));
+      C ua[CQR_NE], ub[CQR_NE], uw[CQR_NE];
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-        if (i > k && i <= j && j < jb) Rm[i][j] -= conj_s(Rm[k][i]) * Rm[k][j] * inv2;
+        ua[m] = Rm[k][i], ub[m] = Rm[k][j], uw[m] = Rm[i][j];
+      }
+      const R inv2 = d2 > R(0) ? R(1) / d2 : std::numeric_limits<R>::infinity();
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        if (i > k && i <= j && j < jb) Rm[i][j] = uw[m] - scale_s(conj_s(ua[m]) * ub[m], inv2);
       }
       __syncthreads();
-      // row k of R (the next step reads rows > k only)
-      if (tid >= k && tid < jb) Rm[k][tid] = tid == k ? C(d) : Rm[k][tid] * C(R(1) / d);
+    }
+    // R_kk, and 1 / R_kk (0 past the panel: the row solves below run over all QR_NB
+    // columns without branches, the padded columns of x and R being zero)
+    if (tid < QR_NB) {
+      R d = R(0), rd = R(0);
+      if (tid < jb) {
+        const R d2 = re_s(Rm[tid][tid]);
+        d = d2 > R(0) ? sqrt(d2) : std::numeric_limits<R>::min();
+        rd = R(1) / d;
+      }
+      rdg[tid] = d, rdi[tid] = rd;
     }
     __syncthreads();
-    if (tid < jb) rdi[tid] = R(1) / re_s(Rm[tid][tid]);
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) {
+      const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+      if (i <= j && j < jb) Rm[i][j] = i == j ? C(rdg[i]) : scale_s(Rm[i][j], rdi[i]);
+    }
+    __syncthreads();
     if (wg0) {
       // Racc <- R Racc (Racc = R_1 after pass 1)
       C* Racc = ws + CQR_RACC;
-      C* Old = &Wb[QR_NB][0];
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
         Wb[QR_NB + i][j] = pass == 1 ? (i == j ? C(1) : C(0)) : Racc[ent];
       }
-      (void)Old;
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
-        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        // lanes run along j here (Rm[i][t] read by half a wave at once, Wb rows contiguous);
+        // blocks of eight terms loaded together (ascending t, as one sum over [i, j])
+        const int e2 = tid + m * CQR_ROWS, i = e2 / QR_NB, j = e2 % QR_NB;
+        const int ent = i + j * QR_NB;
         C sum = C(0);
-        if (i <= j && j < jb)
-          for (int t = i; t <= j; ++t) sum += Rm[i][t] * Wb[QR_NB + t][j];
+        if (i <= j && j < jb) {
+          for (int t0 = i & ~7; t0 <= j; t0 += 8) {
+            C av[8], bv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[u] = Rm[i][t0 + u], bv[u] = Wb[QR_NB + t0 + u][j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (t0 + u >= i && t0 + u <= j) sum += av[u] * bv[u];
+          }
+        }
         Racc[ent] = sum;
       }
     }
@@ -1023,33 +1056,63 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       Wb[i][j] = (i < jb && j < jb) ? Qtop[ent] : C(0);
     }
     __syncthreads();
-    if (tid < jb) {
-      const int i = tid;
+    {
+      // forward substitution of row i on 8 threads (t = s + 8 u, u < 4): a 32-step chain of
+      // short partial sums and a 3-step lane reduction
+      const int i = tid >> 3, sl = tid & 7;
+      C w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = Wb[i][sl + 8 * u];
       for (int c = 0; c < jb; ++c) {
-        C sum = Wb[i][c];
-        for (int t = 0; t < c; ++t) sum -= Wb[i][t] * Rm[t][c];
-        Wb[i][c] = sum * C(rdi[c]);
+        C part = C(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (sl + 8 * u < c) part += w[u] * Rm[sl + 8 * u][c];
+        part = sum8_lanes(part);
+        if (sl == (c & 7)) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (u == (c >> 3)) w[u] = (w[u] - part) * C(rdi[c]);
+        }
+      }
+      if (i < jb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (sl + 8 * u < jb) Wb[i][sl + 8 * u] = w[u];
       }
     }
     __syncthreads();
+    // Step k eliminates with the unscaled column k; the columns of L (column k times
+    // 1 / U_kk) and the pivots U_kk = c_k - s_k are written afterwards.  |U_kk| = |c_k| + 1.
     for (int k = 0; k < jb; ++k) {
       const C c = Wb[k][k];
-      const R a = sqrt(abs2(c));
-      const C sk = a > R(0) ? -(c * C(R(1) / a)) : C(-1);
-      const C piv = c - sk;
-      const C pinv = C(1) / piv;
+      C la[CQR_NE], lb[CQR_NE], lw[CQR_NE];
 #pragma unroll
       for (int m = 0; m < CQR_NE; ++m) {
         const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
-        if (i > k && j > k && i < jb && j < jb) Wb[i][j] -= Wb[i][k] * pinv * Wb[k][j];
+        la[m] = Wb[i][k], lb[m] = Wb[k][j], lw[m] = Wb[i][j];
       }
+      const R a = sqrt(abs2(c));
+      const C sk = a > R(0) ? scale_s(c, R(-1) / a) : C(-1);
+      const C piv = c - sk;
+      const C pinv = scale_s(conj_s(piv), R(1) / abs2(piv));
+#pragma unroll
+      for (int m = 0; m < CQR_NE; ++m) {
+        const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+        if (i > k && j > k && i < jb && j < jb) Wb[i][j] = lw[m] - la[m] * pinv * lb[m];
+      }
+      if (tid == k) sdiag[k] = sk, udi[k] = pinv;
       __syncthreads();
-      // column k of L and the pivot (the next step reads column k + 1 and row k + 1 only)
-      if (tid > k && tid < jb) Wb[tid][k] = Wb[tid][k] * pinv;
-      if (tid == k) Wb[k][k] = piv, sdiag[k] = sk;
     }
-    __syncthreads();
-    if (tid < jb) udi[tid] = C(1) / Wb[tid][tid];
+#pragma unroll
+    for (int m = 0; m < CQR_NE; ++m) {
+      const int ent = tid + m * CQR_ROWS, i = ent % QR_NB, j = ent / QR_NB;
+      if (i < jb && j < jb) {
+        if (i > j) Wb[i][j] = Wb[i][j] * udi[j];
+        else if (i == j) Wb[i][i] = Wb[i][i] - sdiag[i];
+      }
+    }
+    if (tid >= jb && tid < QR_NB) udi[tid] = C(0);
     __syncthreads();
   }
   // rows of this workgroup: x <- x R^-1 (pass 3: then x <- x U^-1)
@@ -1058,26 +1121,23 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
   C x[QR_NB];
 #pragma unroll
   for (int c = 0; c < QR_NB; ++c) x[c] = (live && c < jb) ? C(X[r + int64_t(c) * ldx]) : C(0);
+  // Row substitutions over all QR_NB columns, with no branches (one basic block)
   if (pass > 0) {
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) {
-      if (c < jb) {
-        C sum = x[c];
+      C sum = x[c];
 #pragma unroll
-        for (int i = 0; i < c; ++i) sum -= x[i] * Rm[i][c];
-        x[c] = sum * C(rdi[c]);
-      }
+      for (int i = 0; i < c; ++i) sum -= x[i] * Rm[i][c];
+      x[c] = scale_s(sum, rdi[c]);
     }
   }
   if (pass == 3) {
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) {
-      if (c < jb) {
-        C sum = x[c];
+      C sum = x[c];
 #pragma unroll
-        for (int i = 0; i < c; ++i) sum -= x[i] * Wb[i][c];
-        x[c] = sum * udi[c];
-      }
+      for (int i = 0; i < c; ++i) sum -= x[i] * Wb[i][c];
+      x[c] = sum * udi[c];
     }
   }
   if (Qo && live) {
@@ -1150,16 +1210,29 @@ __global__ void __launch_bounds__(CQR_ROWS, 2) cqr_pass_kernel(
       V[i + int64_t(j) * ldv] = T(i > j ? Wb[i][j] : (i == j ? C(1) : C(0)));
     }
   }
-  if (tid < QR_NB) {
-    // row i of Tf: Tf L^H = B with B = -U S^H (upper); unit upper L^H, ascending columns
-    const int i = tid;
-    C* Tr = &Wb[QR_NB + i][0];
+  {
+    // row i of Tf: Tf L^H = B with B = -U S^H (upper); unit upper L^H, ascending columns,
+    // on 8 threads per row (t = s + 8 u) as the forward substitution above
+    const int i = tid >> 3, sl = tid & 7;
+    C tr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) tr[u] = C(0);
     for (int j = 0; j < QR_NB; ++j) {
-      C b = (i <= j && j < jb) ? -(Wb[i][j] * conj_s(sdiag[j])) : C(0);
-      if (j < jb)
-        for (int t = 0; t < j; ++t) b -= Tr[t] * conj_s(Wb[j][t]);
-      Tr[j] = (i < jb && j < jb) ? b : C(0);
-      Tfo[i + j * QR_NB] = T(Tr[j]);
+      C part = C(0);
+      if (j < jb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (sl + 8 * u < j) part += tr[u] * conj_s(Wb[j][sl + 8 * u]);
+      }
+      part = sum8_lanes(part);
+      if (sl == (j & 7)) {
+        const C b = (i <= j && j < jb) ? -(Wb[i][j] * conj_s(sdiag[j])) : C(0);
+        const C v = (i < jb && j < jb) ? b - part : C(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u == (j >> 3)) tr[u] = v;
+        Tfo[i + j * QR_NB] = T(v);
+      }
     }
   }
 }

@@ -114,6 +114,13 @@ template <typename R>
 __host__ __device__ inline R re_s(const cplx<R>& x) {
   return x.re;
 }
+// x times a real scalar (two products for complex x, not a complex product)
+__host__ __device__ inline double scale_s(double x, double s) { return x * s; }
+__host__ __device__ inline float scale_s(float x, float s) { return x * s; }
+template <typename R>
+__host__ __device__ inline cplx<R> scale_s(const cplx<R>& x, R s) {
+  return cplx<R>(x.re * s, x.im * s);
+}
 __host__ __device__ inline double im_s(double) { return 0.0; }
 __host__ __device__ inline float im_s(float) { return 0.0f; }
 template <typename R>
@@ -152,6 +159,33 @@ __device__ inline double readlane_s(double v, int l) {
 template <typename R>
 __device__ inline cplx<R> readlane_s(const cplx<R>& v, int l) {
   return cplx<R>(readlane_s(v.re, l), readlane_s(v.im, l));
+}
+
+// Sum over each aligned group of eight lanes, the result in all eight (DPP moves on the VALU:
+// pairs, quads, then the two quads of the group through the half-row mirror).  Every lane
+// of a group forms the same sums in the same order up to commutation, so all eight hold the
+// same bits.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL, typename R>
+__device__ inline cplx<R> dpp_move(const cplx<R>& v) {
+  return cplx<R>(dpp_move<CTRL>(v.re), dpp_move<CTRL>(v.im));
+}
+template <typename T>
+__device__ inline T sum8_lanes(T v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_move<0x141>(v);  // row_half_mirror: lane 7 - lane within the group of eight
+  return v;
 }
 
 }  // namespace hssc
