@@ -157,7 +157,10 @@ constexpr int GB_M = 64, GB_N = 64, GB_K = 16;
 // Tile origins at or beyond rs0 (cs0) are shifted by rsn (csn): the blocked Gauss-Jordan
 // steps use this to run over the index set [0, p) U [p+nb, n) with 64-aligned p.
 // CA conjugates op(A) (complex only: C = A^H B for the Householder updates).
-template <typename T, bool TA, bool CA, bool TB>
+// PRE reads the C tile (beta != 0) before the K loop, so that its latency hides behind the
+// products: worth it for the short K of the rank-32 and rank-64 updates, where the loop runs
+// two to four tiles; for longer K the extra registers cost more than the latency saved.
+template <typename T, bool TA, bool CA, bool TB, bool PRE>
 __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t kdim, T alpha,
                                                   const T* A, int64_t lda, int64_t sA,
                                                   const T* B, int64_t ldb, int64_t sB,
@@ -253,6 +256,21 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
     }
   };
   gload(0);
+  T cpre[PRE ? 2 : 1][PRE ? 2 : 1][PRE ? 4 : 1];
+  if constexpr (PRE) {
+    if (beta != T(0)) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t gi = i0 + wi + x * 16 + (lane & 15);
+            const int64_t gj = j0 + wj + y * 16 + Mfma<R>::row(lane, r);
+            cpre[x][y][r] = (gi < m && gj < n) ? C[gi + gj * ldc] : T(0);
+          }
+    }
+  }
   lstore(0);
   __syncthreads();
   int cur = 0;
@@ -322,7 +340,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(int64_t m, int64_t n, int64_t
           else
             a = acc[0][x][y][r];
           const T v = alpha * a;
-          *c = (beta == T(0)) ? v : v + beta * *c;
+          if constexpr (PRE) *c = (beta == T(0)) ? v : v + beta * cpre[x][y][r];
+          else *c = (beta == T(0)) ? v : v + beta * *c;
         }
       }
 }
@@ -363,11 +382,18 @@ void gemm_skip_split(hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int
   // label per kernel variant: gemm_<type>_<op(A)><op(B)>, op = n, t, or h (conjugate transpose)
   const std::string name = lbl<T>("gemm") + "_" + (ta ? (conjA ? "h" : "t") : (conjA ? "c" : "n")) +
                            (tb ? "t" : "n");
-#define HSSC_GEMM_LAUNCH(TA_, CA_, TB_)                                                         \
-  HSSC_TIMED(name.c_str(), st, fl, by,                                                          \
-             hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_>), grid, dim3(256), 0, st, m, n,  \
-                                kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, rs0, rsn, \
-                                cs0, csn, ksplit, kchunk, tri))
+  // the C tile read ahead for short K (at most four K tiles per output), single precision
+  // only: with FP64 entries the extra registers halve the waves per SIMD
+  constexpr bool kPre = sizeof(real_t<T>) == 4;
+  const bool pre = kPre && beta != T(0) && (ksplit > 1 ? kchunk : kdim) <= 4 * GB_K;
+#define HSSC_GEMM_LAUNCH(TA_, CA_, TB_)                                                          \
+  HSSC_TIMED(name.c_str(), st, fl, by,                                                           \
+             if (pre) hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_, kPre>), grid, dim3(256),  \
+                                         0, st, m, n, kdim, alpha, A, lda, sA, B, ldb, sB, beta,  \
+                                         C, ldc, sC, rs0, rsn, cs0, csn, ksplit, kchunk, tri);    \
+             else hipLaunchKernelGGL((gemm_kernel<T, TA_, CA_, TB_, false>), grid, dim3(256), 0,  \
+                                     st, m, n, kdim, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, \
+                                     sC, rs0, rsn, cs0, csn, ksplit, kchunk, tri))
   if (conjA) {
     if (ta && !tb) HSSC_GEMM_LAUNCH(true, true, false);
     else if (ta && tb) HSSC_GEMM_LAUNCH(true, true, true);
