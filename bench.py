@@ -45,12 +45,13 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "c128": 78.6, "c64": 157.3}  # MI355X dense MFMA
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of a kernel class, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
-# of `bench.py --no-cpu --no-hss1d --no-batched --no-laplace --profile-step` (one step of the
-# default workload), averaged over the class's launches and scaled by what the counters read
+# of `bench.py --no-cpu --no-hss1d --no-batched --no-laplace --mlp-cache F --profile-step`
+# (one step of the default workload, the rank network loaded from a file written by an
+# unprofiled run), averaged over the class's launches and scaled by what the counters read
 # for a plain 1 GiB device copy (`bench.py --calibrate-copy`: FETCH_SIZE 512 MiB -> x2,
 # WRITE_SIZE 1 GiB -> x1).  profiles/hss2d_512_helmholtz_traffic.md holds the runs.
 MEASURED_TRAFFIC = {
-    "gemm_c128_nn": 2 * 77102.0 * 1024 + 64783.5 * 1024,
+    "gemm_c128_nn": 2 * 78164.6 * 1024 + 65175.7 * 1024,
 }
 
 
@@ -81,6 +82,9 @@ def parse():
     ap.add_argument("--cpu-stop", type=int, default=3,
                     help="CPU baseline: levels measured down to this one, above extrapolated")
     ap.add_argument("--no-hss1d", action="store_true", help="skip the HSS1D side case")
+    ap.add_argument("--mlp-cache", default=None,
+                    help="npz file of the trained rank network: loaded when present, else "
+                         "written after training")
     ap.add_argument("--rank", choices=("id", "table", "mlp"), default="mlp",
                     help="skeleton IDs of the ID-selected levels: plain ID, or randomized IDs "
                          "with sketches sized by the closed-form guess / by the rank network "
@@ -462,12 +466,18 @@ def main():
     if rank == "mlp":
         # the rank network, trained before the timed steps on the plain-ID ranks of 32 seeded
         # geometries (grids 24^2..640^2, kh 4..40, plateau or Gaussian contrast)
-        from hssc_directsolver_amd.rank_mlp import collect_hss2d, train_rank_mlp
-        t0 = time.perf_counter()
-        Xr, tr = collect_hss2d(32, seed=7)
-        mlp, loss = train_rank_mlp(Xr, tr, seed=0, epochs=2000)
-        rank_info = dict(samples=len(tr), train_loss=round(loss, 6),
-                         collect_and_train_s=round(time.perf_counter() - t0, 2))
+        from hssc_directsolver_amd.rank_mlp import RankMLP, collect_hss2d, train_rank_mlp
+        if args.mlp_cache and os.path.exists(args.mlp_cache):
+            mlp = RankMLP.load(args.mlp_cache)
+            rank_info = dict(loaded=args.mlp_cache)
+        else:
+            t0 = time.perf_counter()
+            Xr, tr = collect_hss2d(32, seed=7)
+            mlp, loss = train_rank_mlp(Xr, tr, seed=0, epochs=2000)
+            rank_info = dict(samples=len(tr), train_loss=round(loss, 6),
+                             collect_and_train_s=round(time.perf_counter() - t0, 2))
+            if args.mlp_cache:
+                mlp.save(args.mlp_cache)
     step = lambda **kw: one_step(H, P, torch, b, args.tol, rank=rank, mlp=mlp, **kw)  # noqa: E731
     if args.profile_step:
         r = step()

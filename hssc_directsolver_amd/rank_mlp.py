@@ -56,6 +56,21 @@ class RankMLP:
     def set_scaling(self, mu, sd):
         mu, sd = _f32(mu), _f32(sd)
         check(lib().hssc_mlp_set_scaling(self.handle, mu.ctypes.data_as(_FP), sd.ctypes.data_as(_FP)))
+        self.scaling = (mu.copy(), sd.copy())
+
+    def save(self, path):
+        """Writes the weights and the feature scaling to a .npz file (plain arrays, no pickles)."""
+        mu, sd = self.scaling
+        np.savez(path, params=self.get_params(), mu=mu, sd=sd)
+
+    @classmethod
+    def load(cls, path):
+        """A network with the weights and scaling that save() wrote."""
+        m = cls()
+        with np.load(path, allow_pickle=False) as f:
+            m.set_params(f["params"])
+            m.set_scaling(f["mu"], f["sd"])
+        return m
 
     def fit_scaling(self, X, t):
         mu = np.concatenate([X.mean(0), [t.mean()]])

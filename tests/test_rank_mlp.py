@@ -65,7 +65,7 @@ def test_gpu_forward_and_gradients_match_oracle():
 
 
 @pytest.mark.gpu
-def test_training_fits_a_smooth_target():
+def test_training_fits_a_smooth_target(tmp_path):
     from hssc_directsolver_amd.rank_mlp import RankMLP
     X, t = _data(2, B=2000)
     m = RankMLP(seed=5)
@@ -75,6 +75,12 @@ def test_training_fits_a_smooth_target():
     assert l1 < 0.02 * l0
     Xv, tv = _data(9, B=500)
     assert np.sqrt(np.mean((m.predict(Xv) - tv) ** 2)) < 0.1 * np.std(tv)
+    # the saved network predicts the same bits once loaded into a fresh handle
+    path = tmp_path / "rank_mlp.npz"
+    m.save(path)
+    m2 = RankMLP.load(path)
+    assert np.array_equal(m2.get_params(), m.get_params())
+    assert np.array_equal(m2.predict(Xv), m.predict(Xv))
 
 
 @pytest.mark.gpu
