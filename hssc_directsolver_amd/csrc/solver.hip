@@ -842,18 +842,27 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
         sl.push_back([=](hipStream_t st) {
           const int jbo = int(std::min(nbo, kk - j0));
           const int64_t nc = n - j0 - jbo;
-          const int64_t Mp = M - j0;
+          const int64_t Mp = M - j0;  // leading dimension of V
           const int64_t sV = Mp * nbo;
+          // Rows that can be nonzero below row j of the columns >= j once the columns before
+          // jend are factored: every data row, and of the Tikhonov rows only those of the
+          // factored columns (the reflector of column j reaches the Tikhonov rows of the
+          // columns <= j only, so Tikhonov row i stays lambda e_i until column i is reached).
+          // The panels and their products skip the untouched rows.
+          auto rows_to = [=](int64_t jend) { return tikhonov ? Mq + std::min(jend, kk) : M; };
+          const int64_t Mpe = rows_to(j0 + jbo) - j0;
           // a narrow trailing block (the last panels of the small levels) takes each
           // sub-panel's reflectors directly, without the outer panel's block reflector
           const bool merge = nc <= kMergeCols;
-          k::zero2d<S>(st, Mp, jbo, w.V, Mp, sV, E);
+          k::zero2d<S>(st, Mpe, jbo, w.V, Mp, sV, E);
           for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
-            const int64_t j1 = j0 + s0, Mps = M - j1;
+            const int64_t j1 = j0 + s0;
             const int jb = int(std::min<int64_t>(nb, jbo - s0));
+            const int64_t Mps = rows_to(j1 + jb) - j1;
             S* Vs = w.V + s0 + s0 * Mp;
             S* Tss = w.Ts + (s0 / nb) * nb * nb;
-            k::cqr_panel<S>(st, M, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq, E);
+            k::cqr_panel<S>(st, j1 + Mps, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq,
+                            E);
             // this sub-panel's reflectors on the rest of the outer panel (and the trailing
             // block when merged)
             const int64_t nin = jbo - s0 - jb + (merge ? std::max<int64_t>(nc, 0) : 0);
@@ -868,18 +877,18 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
           }
           if (nc <= 0 || merge) return;
           // T of the outer panel from the sub-panel factors and V^H V
-          k::gemm_tn_splitk<S>(st, jbo, jbo, Mp, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
+          k::gemm_tn_splitk<S>(st, jbo, jbo, Mpe, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
                                E, w.part, kchunk_thin, part_cap);
           k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
                            nbo * nbo, w.W2, nbo * n, E);
           const int64_t ldt = jbo;
           S* C = A + j0 + (j0 + jbo) * M;
           // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
-          k::gemm_tn_splitk<S>(st, jbo, nc, Mp, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
+          k::gemm_tn_splitk<S>(st, jbo, nc, Mpe, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
                                w.part, kchunk, part_cap);
           k::gemm_h<S>(st, jbo, nc, jbo, S(1), w.Tf, ldt, nbo * nbo, w.W1, jbo, jbo * nc, S(0),
                        w.W2, jbo, jbo * nc, E);
-          k::gemm<S>(st, false, false, Mp, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
+          k::gemm<S>(st, false, false, Mpe, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
                      S(1), C, M, sA, E);
         });
       }
