@@ -60,6 +60,8 @@ SIGNATURES = [
     ("hssc_tree_destroy", None, [_P]),
     ("hssc_levhss", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     ("hssc_hss2d_invert", C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    ("hssc_dense_invert_device", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, _P,
+                                          C.c_int64, C.c_int64, C.c_int64]),
     ("hssc_inverse_get_FI", C.c_int, [_P, C.c_int64, _DP]),
     ("hssc_inverse_get_E", C.c_int, [_P, C.c_int64, _DP]),
     ("hssc_inverse_destroy", None, [_P]),

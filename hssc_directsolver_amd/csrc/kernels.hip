@@ -565,14 +565,186 @@ __global__ void __launch_bounds__(256) gj_reg_kernel(int n, T* __restrict__ A, i
     }
 }
 
+// Reciprocal of a pivot: the hardware estimate (v_rcp) refined by Newton's iteration
+// r <- r + r (1 - x r), once in single precision, twice in double (each step squares the
+// relative error); complex: the exponent of max(|re|, |im|) is scaled out exactly first, so
+// |z|^2 neither overflows nor underflows.  Zero, infinite or NaN pivots give non-finite
+// results (flagged by the sweep).
+__device__ inline float rcp_nr(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ inline double recip_s(double x) { return rcp_nr(x); }
+__device__ inline float recip_s(float x) { return rcp_nr(x); }
+template <typename R>
+__device__ inline cplx<R> recip_s(const cplx<R>& z) {
+  int e;
+  (void)frexp(fmax(fabs(z.re), fabs(z.im)), &e);
+  const R a = ldexp(z.re, -e), b = ldexp(z.im, -e);
+  const R r = rcp_nr(a * a + b * b);
+  return cplx<R>(ldexp(a * r, -e), ldexp(-b * r, -e));
+}
+__device__ inline double fnma_s(double g, double b, double c) { return fma(-g, b, c); }
+__device__ inline float fnma_s(float g, float b, float c) { return fmaf(-g, b, c); }
+template <typename R>
+__device__ inline cplx<R> fnma_s(const cplx<R>& g, const cplx<R>& b, const cplx<R>& c) {
+  return cplx<R>(fma(-g.re, b.re, fma(g.im, b.im, c.re)), fma(-g.re, b.im, fma(-g.im, b.re, c.im)));
+}
+
+// Largest value over the wave: DPP within each row of 16 lanes (pairs, quads, the two quads
+// of a group of eight through the half-row mirror, the two halves of the row through the row
+// mirror), then the four rows through v_readlane.
+__device__ inline float wave_max_f(float v) {
+  v = fmaxf(v, dpp_move<0xB1>(v));
+  v = fmaxf(v, dpp_move<0x4E>(v));
+  v = fmaxf(v, dpp_move<0x141>(v));
+  v = fmaxf(v, dpp_move<0x140>(v));
+  const float r0 = readlane_s(v, 0), r1 = readlane_s(v, 16), r2 = readlane_s(v, 32),
+              r3 = readlane_s(v, 48);
+  return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// Gauss-Jordan inversion of an n x n block (n <= 64) per workgroup, in registers and without
+// row interchanges.  Lanes are rows; wave w of NW holds columns [CB w, CB w + CB),
+// CB = 64 / NW.  Step c pivots column c on the unused row p of largest |re| + |im| (LAPACK
+// izamax's magnitude, compared in single precision; ties to the lowest row) and sweeps: row p
+// becomes (row p) / a_pc with 1 / a_pc in column c, every other row i subtracts
+// (a_ic / a_pc) (row p) and takes -a_ic / a_pc in column c.  Once every column is swept, row
+// i of the result is row c(i) of the inverse with its columns in pivot-row order:
+// inv(c(i), p_j) = B(i, j), c(i) the column pivoted on row i and p_j the pivot row of column
+// j; the stores apply the permutation.
+// Per step: one barrier; the wave that owns column c + 1 updates it first, writes it to LDS
+// and finds its pivot row (a DPP wave max and a ballot) and the pivot's reciprocal, which
+// the others read right after the barrier; the pivot row's entries come from lane p through
+// scalar registers.  Singularity: a pivot that is not finite, or whose magnitude is at most
+// n eps times the block's largest entry, sets *flag (when given).
+template <typename T, int NW>
+__global__ void __launch_bounds__(NW * 64) gj64_kernel(int n, T* __restrict__ A, int64_t lda,
+                                                       int64_t sA, int64_t off, int* flag) {
+  constexpr int CB = 64 / NW;  // columns per wave
+  using R = real_t<T>;
+  __shared__ T colbuf[2][64];
+  __shared__ int spiv[2];
+  __shared__ T sinv[2];
+  __shared__ int prow_of[64];  // pivot row of column c
+  __shared__ int col_of[64];   // column pivoted on row i
+  __shared__ R wmax[NW];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  A += int64_t(blockIdx.x) * sA + off;
+  T v[CB];
+  R amax = R(0);
+#pragma unroll
+  for (int t = 0; t < CB; ++t) {
+    const int j = wave * CB + t;
+    v[t] = (lane < n && j < n) ? A[lane + int64_t(j) * lda] : T(0);
+    amax = fmax(amax, abs1(v[t]));
+  }
+  for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o));
+  if (lane == 0) wmax[wave] = amax;
+  bool used = lane >= n;  // rows past the block never pivot
+  const R tiny_rel = R(n) * std::numeric_limits<R>::epsilon();
+  // the entry of column c (uniform c) of this lane's row: a switch keeps v in registers
+  auto pick = [&](int c) {
+    T x = v[CB - 1];
+    switch (c % CB) {
+#define HSSC_GJ_CASE(t) \
+  case t:               \
+    if (t < CB) x = v[t < CB ? t : 0]; \
+    break;
+      HSSC_GJ_CASE(0) HSSC_GJ_CASE(1) HSSC_GJ_CASE(2) HSSC_GJ_CASE(3) HSSC_GJ_CASE(4)
+      HSSC_GJ_CASE(5) HSSC_GJ_CASE(6) HSSC_GJ_CASE(7) HSSC_GJ_CASE(8) HSSC_GJ_CASE(9)
+      HSSC_GJ_CASE(10) HSSC_GJ_CASE(11) HSSC_GJ_CASE(12) HSSC_GJ_CASE(13) HSSC_GJ_CASE(14)
+#undef HSSC_GJ_CASE
+      default: break;
+    }
+    return x;
+  };
+  // column x into LDS, its pivot row and the pivot's reciprocal (by the wave owning the
+  // column, so that the others read both right after the barrier)
+  auto search = [&](const T& x, bool done, int buf) {
+    colbuf[buf][lane] = x;
+    const float mag = done ? -1.0f : float(abs1(x));
+    const float mx = wave_max_f(mag);
+    unsigned long long ball = __ballot(!done && mag == mx);
+    if (!ball) ball = __ballot(!done);  // every candidate is NaN: flagged below
+    const int p = __builtin_ctzll(ball);
+    const T piv = readlane_s(x, p);
+    if (lane == 0) {
+      spiv[buf] = p;
+      sinv[buf] = recip_s(piv);
+      const R a = abs1(piv);
+      R bmax = wmax[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) bmax = fmax(bmax, wmax[w]);
+      if (flag && (!(a > tiny_rel * bmax) || !(a <= std::numeric_limits<R>::max())))
+        atomicOr(flag, 1);
+    }
+  };
+  __syncthreads();  // wmax
+  if (wave == 0) search(pick(0), used, 0);
+  for (int c = 0; c < n; ++c) {
+    const int buf = c & 1;
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(spiv[buf]);
+    const T inv = sinv[buf];
+    const T f = colbuf[buf][lane];
+    if (threadIdx.x == 0) prow_of[c] = p, col_of[p] = c;
+    const bool isp = lane == p;
+    // row i: v - g (row p) with g = a_ic / a_pc; row p: 0 - (-1 / a_pc) (row p)
+    const T g = isp ? -inv : f * inv;
+    // the next column first (the same formula as the sweep below, so the same bits), and
+    // its pivot search, which the next step waits for
+    if (c + 1 < n && wave == (c + 1) / CB) {
+      const T vo = pick(c + 1);
+      search(fnma_s(g, readlane_s(vo, p), isp ? T(0) : vo), used || isp, buf ^ 1);
+    }
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      const T pr = readlane_s(v[t], p);
+      v[t] = fnma_s(g, pr, isp ? T(0) : v[t]);
+    }
+    if (wave == c / CB) {  // column c: -a_ic / a_pc, and 1 / a_pc on row p (= -g there)
+      switch (c % CB) {
+#define HSSC_GJ_CASE(t) \
+  case t:               \
+    if (t < CB) v[t < CB ? t : 0] = -g; \
+    break;
+        HSSC_GJ_CASE(0) HSSC_GJ_CASE(1) HSSC_GJ_CASE(2) HSSC_GJ_CASE(3) HSSC_GJ_CASE(4)
+        HSSC_GJ_CASE(5) HSSC_GJ_CASE(6) HSSC_GJ_CASE(7) HSSC_GJ_CASE(8) HSSC_GJ_CASE(9)
+        HSSC_GJ_CASE(10) HSSC_GJ_CASE(11) HSSC_GJ_CASE(12) HSSC_GJ_CASE(13) HSSC_GJ_CASE(14)
+#undef HSSC_GJ_CASE
+        default: v[CB - 1] = -g;
+      }
+    }
+    used = used || isp;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < CB; ++t) {
+    const int j = wave * CB + t;
+    if (lane < n && j < n) A[col_of[lane] + int64_t(prow_of[j]) * lda] = v[t];
+  }
+}
+
+// Waves per block of the register Gauss-Jordan kernel (8 columns each).  One 64 x 64 block:
+// 62 / 52 / 47 us (complex64) and 75 / 63 / 62 us (complex128) with 4 / 8 / 16 waves; 4096
+// blocks in one launch: 448 / 430 / 524 us and 700 / 673 / 776 us.
+constexpr int kGjWaves = 8;
+
 template <typename T>
 void gj_reg(hipStream_t st, int n, T* A, int64_t lda, int64_t sA, int64_t off, int64_t batch,
             const std::string& label, int* flag) {
   const double fl = (is_complex_v<T> ? 8.0 : 2.0) * n * double(n) * n * batch;
   if (n <= 64) {
     HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
-               hipLaunchKernelGGL((gj_reg_kernel<T, 1, 16>), dim3(batch), dim3(256), 0, st, n, A,
-                                  lda, sA, off, flag));
+               hipLaunchKernelGGL((gj64_kernel<T, kGjWaves>), dim3(batch), dim3(kGjWaves * 64), 0,
+                                  st, n, A, lda, sA, off, flag));
   } else if constexpr (!is_complex_v<T>) {
     HSSC_TIMED(label.c_str(), st, fl, 2.0 * n * n * sizeof(T) * batch,
                hipLaunchKernelGGL((gj_reg_kernel<T, 2, 32>), dim3(batch), dim3(256), 0, st, n, A,

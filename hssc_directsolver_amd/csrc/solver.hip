@@ -1532,6 +1532,24 @@ int scratch_address(int dev, void** out) {
 // C ABI
 // ---------------------------------------------------------------------------------------
 
+// Dense block inverses of hssc_dense_invert_device, on streams of the call.
+template <typename T>
+static void dense_invert(bool sym, int64_t n, T* A, int64_t lda, int64_t sA, int64_t batch,
+                         int* singular) {
+  OwnStream os, aux;
+  const int64_t ws = k::invert_ws_elems<T>(n, batch);
+  T* W = ws > 0 ? dalloc<T>(ws) : nullptr;
+  int* dflag = dalloc<int>(1);
+  HIPCHK(hipMemsetAsync(dflag, 0, sizeof(int), os.s));
+  k::invert_batched<T>(os.s, n, A, lda, sA, batch, aux.s, W, sym, dflag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(os.s));
+  HIPCHK(hipStreamSynchronize(aux.s));
+  HIPCHK(hipMemcpy(singular, dflag, sizeof(int), hipMemcpyDeviceToHost));
+  dfree(W);
+  dfree(dflag);
+}
+
 extern "C" {
 
 const char* hssc_last_error(void) { return hssc::g_last_error.c_str(); }
@@ -1829,6 +1847,31 @@ int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss) {
   }
   if (lev > 0) *lhss = (lev % 2 == 1) ? lev - 1 : lev;
   else *lhss = -1;
+  return HSSC_OK;
+}
+
+int hssc_dense_invert_device(int32_t complex_entries, int32_t precision, int32_t sym, int64_t n,
+                             void* A, int64_t lda, int64_t sA, int64_t batch) {
+  if (!A || n < 1 || lda < n || batch < 1 || (batch > 1 && sA < lda * n))
+    return fail(HSSC_ERR_INVALID_ARG, "invalid dense inverse arguments");
+  if (precision != HSSC_PREC_FP64 && precision != HSSC_PREC_FP32)
+    return fail(HSSC_ERR_INVALID_ARG, "unknown precision");
+  if (int s = check_device()) return s;
+  int singular = 0;
+  int st = guarded([&]() {
+    const bool s = sym != 0;
+    if (complex_entries && precision == HSSC_PREC_FP64)
+      dense_invert(s, n, static_cast<zdouble*>(A), lda, sA, batch, &singular);
+    else if (complex_entries)
+      dense_invert(s, n, static_cast<zfloat*>(A), lda, sA, batch, &singular);
+    else if (precision == HSSC_PREC_FP64)
+      dense_invert(false, n, static_cast<double*>(A), lda, sA, batch, &singular);
+    else
+      dense_invert(false, n, static_cast<float*>(A), lda, sA, batch, &singular);
+    return HSSC_OK;
+  });
+  if (st != HSSC_OK) return st;
+  if (singular) return fail(HSSC_ERR_SINGULAR, "singular or non-finite pivot in a dense inverse");
   return HSSC_OK;
 }
 

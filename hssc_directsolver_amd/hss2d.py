@@ -295,6 +295,29 @@ def HSS2D_invert_fsym(tree: Tree, params: Params = None, precision: str = "fp64"
     return Inverse(h, tree, precision)
 
 
+def dense_inv_device(A, sym=False):
+    """MATLAB's inv on a batch of dense blocks, in place, on the GPU: A is a contiguous
+    torch tensor on the device of shape (batch, n, n) or (n, n), float64 / float32 /
+    complex128 / complex64, each block stored column-major (A[b] holds the transpose of
+    block b, which is block b itself when it is symmetric).  sym=True: complex symmetric
+    blocks (upper-tile sweep).  Raises HsscError (HSSC_ERR_SINGULAR) on a zero, non-finite
+    or negligible pivot."""
+    import torch
+    if not (A.is_cuda and A.is_contiguous() and A.dim() in (2, 3) and A.shape[-1] == A.shape[-2]):
+        raise ValueError("A must be a contiguous CUDA tensor of shape (batch, n, n) or (n, n)")
+    kinds = {torch.float64: (0, "fp64"), torch.float32: (0, "fp32"),
+             torch.complex128: (1, "fp64"), torch.complex64: (1, "fp32")}
+    if A.dtype not in kinds:
+        raise ValueError(f"unsupported dtype {A.dtype}")
+    cx, prec = kinds[A.dtype]
+    n = A.shape[-1]
+    batch = A.shape[0] if A.dim() == 3 else 1
+    torch.cuda.synchronize(A.device)
+    check(lib().hssc_dense_invert_device(cx, PREC[prec], int(bool(sym)), n,
+                                         C.c_void_p(A.data_ptr()), n, n * n, batch))
+    return A
+
+
 class Operator:
     """A_HSS of HSS2D_compress_fsym (dense Bself / B12 blocks)."""
 

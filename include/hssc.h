@@ -192,6 +192,18 @@ int hssc_levhss(const hssc_tree* t, int32_t n_cut, int32_t* lhss);
 /* Replaces HSS2D/HSS2D_invert_fsym.m:1 with params.Fopts = 'dense' (dense F^-1 and E per
  * box, or per level when transinv).  precision = HSSC_PREC_FP64 or HSSC_PREC_FP32. */
 int hssc_hss2d_invert(hssc_tree* t, int32_t precision, hssc_inverse** inv);
+
+/* Replaces MATLAB's inv on the dense blocks of the inversion (HSS2D_invert_fsym.m:108,
+ * HSS2D_build_Finv_fsym.m:101, HSS2D_build_E_fsym.m:51,64): each of `batch` n x n blocks of
+ * device memory A (column-major, leading dimension lda, block stride sA elements) is replaced
+ * by its inverse, by the library's Gauss-Jordan sweeps (pivoting inside each diagonal block
+ * of 64 columns, or 128 for real entries).  complex_entries != 0: interleaved complex;
+ * precision = HSSC_PREC_FP64 or HSSC_PREC_FP32 selects the entry type; sym != 0 (complex
+ * only) for exactly symmetric blocks (A^T = A): upper-tile sweeps and a symmetric result.
+ * Returns HSSC_ERR_SINGULAR when a pivot is zero, not finite or at most n eps max|A_block|
+ * (the blocks are overwritten all the same). */
+int hssc_dense_invert_device(int32_t complex_entries, int32_t precision, int32_t sym, int64_t n,
+                             void* A, int64_t lda, int64_t sA, int64_t batch);
 /* F^-1 (n x n, rows/cols in I_src order) and E (k x k) of box b (or its level), as doubles
  * or interleaved complex doubles (FP32 factorizations are widened). */
 int hssc_inverse_get_FI(const hssc_inverse* inv, int64_t box, double* out);
