@@ -923,14 +923,19 @@ void sym_lower_fill(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 // tiles of the general sweep, which streams the whole matrix once per block; the lower
 // triangle is filled at the end.  Diagonal tiles are updated whole, so the 64 x 64 diagonal
 // blocks inverted by gj_reg are complete.
-template <typename T>
+// NB = 128 sweeps two 64-column blocks at once: the 128 x 128 diagonal block is inverted by
+// the same sweep with NB = 64 (so the pivots are chosen inside the same 64 x 64 blocks as
+// there; sweeping B1 and then B2 equals sweeping B1 u B2), and the rest of the matrix takes
+// rank-128 updates: half the steps, launches and trailing-matrix traffic of the 64-wide sweep.
+template <typename T, int NB = GB_M>
 void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t lda, int64_t sA,
                     int64_t batch, T* work, int* flag) {
-  constexpr int NB = GB_M;
+  static_assert(NB == GB_M || NB == 2 * GB_M, "blocks of one or two GEMM tiles");
   const std::string diag = lbl<T>("gj_diag");
   const int64_t sW = 2 * n * NB;
   T* Wx = work;           // rows X < B: p x nb, ld n
   T* Wy = work + n * NB;  // cols Y > B: nb x (n - q), ld NB
+  T* Wd = work + sW * batch;  // NB = 128: workspace of the diagonal blocks' own sweeps
   hipEvent_t upd = nullptr, inv = nullptr;
   if (aux) {
     if (hipEventCreateWithFlags(&upd, hipEventDisableTiming) != hipSuccess ||
@@ -946,6 +951,11 @@ void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t ld
   // on both sides (A_BB^-1 in the new A_XB and, through the upper tiles, A_BB^-T), so it is
   // replaced by its symmetric part
   auto diag_inverse = [&](hipStream_t s, int nbk, int64_t off) {
+    if (nbk > GB_M) {  // NB = 128: the 64-wide sweep of the block, on this stream
+      if constexpr (NB > GB_M)
+        gj_blocked_sym<T, GB_M>(s, nullptr, nbk, A + off, lda, sA, batch, Wd, flag);
+      return;
+    }
     gj_reg<T>(s, nbk, A, lda, sA, off, batch, diag, flag);
     hipLaunchKernelGGL(sym_average_kernel<T>, dim3((nbk + 31) / 32, (nbk + 31) / 32, batch),
                        dim3(256), 0, s, int64_t(nbk), A + off, lda, sA);
@@ -1007,11 +1017,16 @@ void gj_blocked_sym(hipStream_t st, hipStream_t aux, int64_t n, T* A, int64_t ld
   if (inv) (void)hipEventDestroy(inv);
 }
 
-// The symmetric sweep serves complex matrices (64-wide blocks, the GEMM tile); real ones use
-// the general sweep with 128-wide blocks.
+// The symmetric sweep serves complex matrices: 64-wide blocks (the GEMM tile), 128-wide ones
+// from kWideSweep columns on, where the rank-128 updates of the large matrices pay for the
+// diagonal blocks' own sweeps; real matrices use the general sweep with 128-wide blocks.
+constexpr int64_t kWideSweep = 2048;
 template <typename T>
 int64_t invert_ws_elems(int64_t n, int64_t batch) {
-  return n > gj_reg_max<T>() ? 2 * n * gj_reg_max<T>() * batch : 0;
+  if (n <= gj_reg_max<T>()) return 0;
+  if (is_complex_v<T> && n >= kWideSweep)
+    return (2 * n * 2 * GB_M + 2 * 2 * GB_M * GB_M) * batch;
+  return 2 * n * gj_reg_max<T>() * batch;
 }
 
 template <typename T>
@@ -1031,7 +1046,8 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
     const int64_t nt = (n + 31) / 32;
     hipLaunchKernelGGL(sym_average_kernel<T>, dim3(nt, nt, batch), dim3(256), 0, st, n, A, lda,
                        sA);
-    gj_blocked_sym<T>(st, aux, n, A, lda, sA, batch, work, flag);
+    if (n >= kWideSweep) gj_blocked_sym<T, 2 * GB_M>(st, aux, n, A, lda, sA, batch, work, flag);
+    else gj_blocked_sym<T, GB_M>(st, aux, n, A, lda, sA, batch, work, flag);
   } else {
     gj_blocked<T>(st, aux, n, A, lda, sA, batch, work, flag);
   }

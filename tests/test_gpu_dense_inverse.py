@@ -1,8 +1,9 @@
 """The dense block inverse (MATLAB's inv on F and R F^-1 R^T: HSS2D_invert_fsym.m:108,
 HSS2D_build_Finv_fsym.m:101, HSS2D_build_E_fsym.m:51,64) through hssc_dense_invert_device:
 the register kernel for one block of up to 64 columns (128 for real entries) and the blocked
-sweeps above, against LAPACK's inverse (numpy), on matrices that need row pivoting, complex
-symmetric blocks, and the singular-pivot flag."""
+sweeps above (for complex symmetric blocks of 2048 columns or more, 128-wide steps whose
+diagonal blocks take the 64-wide sweep), against LAPACK's inverse (numpy), on matrices that
+need row pivoting, complex symmetric blocks, and the singular-pivot flag."""
 import numpy as np
 import pytest
 
@@ -67,11 +68,11 @@ def test_blocked_sweep(kind, n):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["c64", "c128"])
-@pytest.mark.parametrize("n", [48, 64, 200])
+@pytest.mark.parametrize("n", [48, 64, 200, 2240])
 def test_complex_symmetric_blocks(kind, n):
     print(f"dense inverse symmetric {kind} n={n}", flush=True)
     rng = np.random.default_rng(n)
-    B = rand(rng, (2, n, n), kind)
+    B = rand(rng, (2 if n < 1000 else 1, n, n), kind)
     blocks = B + np.swapaxes(B, 1, 2) + 3 * np.sqrt(n) * np.eye(n)
     X = run(blocks, kind, sym=True)
     check(blocks, X, kind)
