@@ -147,8 +147,12 @@ constexpr int kRankFeatures = 8;
 // log2 M, kh times the box width, mean / min / max of the points' kernel weights, -log10 acc.
 void id_job_features(const hssc_params& p, const IdJob& jb, float* X);
 
+// Called with each level once its index sets, proxies and points are final (levels in build
+// order, the leaves first); a nonzero HSSC_* status stops the build and is returned.
+using LevelDone = std::function<int(int lev)>;
 // Host builders (tree.cpp).  sel runs the IDs of the levels whose skeletons come from one.
-int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel = nullptr);
+int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel = nullptr,
+                     const LevelDone& done = nullptr);
 void source_points(int Ng, int np, double* X /* N x 2 col-major */);
 
 // Waits for the tree's asynchronous build (interpolation operators) and releases its

@@ -20,6 +20,7 @@ struct Device {
 };
 
 std::mutex mu;
+int no_sync = 0;  // live NoSync scopes (guarded by mu)
 // never destroyed: blocks outlive static destruction order (the runtime may be gone by then)
 std::map<int, Device>& devices() {
   static auto* d = new std::map<int, Device>();
@@ -161,7 +162,7 @@ hipError_t alloc(void** p, size_t bytes) {
   Device& d = current();
   if (take_locked(d, s, p)) return hipSuccess;
   for (auto& b : d.released)
-    if (b.second == s) {
+    if (b.second == s && no_sync == 0) {
       make_ready_locked(d);
       if (take_locked(d, s, p)) return hipSuccess;
       break;
@@ -220,6 +221,15 @@ void release(void* p) {
       d.ready.erase(last);
     }
   }
+}
+
+NoSync::NoSync() {
+  std::lock_guard<std::mutex> g(mu);
+  ++no_sync;
+}
+NoSync::~NoSync() {
+  std::lock_guard<std::mutex> g(mu);
+  --no_sync;
 }
 
 void reclaim() {

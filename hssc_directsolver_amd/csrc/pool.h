@@ -28,5 +28,15 @@ void trim();
 // Bytes held in the cache (released, not yet freed) on the current device.
 size_t cached_bytes();
 
+// While a NoSync object lives, alloc() never synchronises the device to reuse released blocks
+// (it takes a ready block or calls hipMalloc): code that queues work on its own streams and
+// keeps allocating meanwhile (the pipelined tree build) is not stalled behind that work.
+struct NoSync {
+  NoSync();
+  ~NoSync();
+  NoSync(const NoSync&) = delete;
+  NoSync& operator=(const NoSync&) = delete;
+};
+
 }  // namespace pool
 }  // namespace hssc

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <type_traits>
 #include <new>
 #include <mutex>
@@ -726,16 +727,12 @@ namespace {
 // Tree upload and interpolation operators
 // ---------------------------------------------------------------------------------------
 
-void upload_tree(hssc_tree* t) {
-  HIPCHK(hipGetDevice(&t->device));
-  for (auto& L : t->levels) {
-    L.dJ = dupload(L.J), L.dJinv = dupload(L.Jinv);
-    L.dPx = dupload(L.Px), L.dPy = dupload(L.Py), L.dPw = dupload(L.Pw);
-    L.dPNx = dupload(L.PNx), L.dPNy = dupload(L.PNy), L.dPNw = dupload(L.PNw);
-    L.dXpx = dupload(L.Xpx), L.dXpy = dupload(L.Xpy), L.dXpw = dupload(L.Xpw);
-  }
-  t->dLeafG = dupload(t->leafG);
-  t->on_device = true;
+// Uploads one level's index sets: J, J^-1, the points in J and in natural order, the proxies.
+void upload_level(Level& L) {
+  L.dJ = dupload(L.J), L.dJinv = dupload(L.Jinv);
+  L.dPx = dupload(L.Px), L.dPy = dupload(L.Py), L.dPw = dupload(L.Pw);
+  L.dPNx = dupload(L.PNx), L.dPNy = dupload(L.PNy), L.dPNw = dupload(L.PNw);
+  L.dXpx = dupload(L.Xpx), L.dXpy = dupload(L.Xpy), L.dXpw = dupload(L.Xpw);
 }
 
 // T = K(Xp, X_sk) \ K(Xp, X_rs) for every key of every level
@@ -743,22 +740,77 @@ void upload_tree(hssc_tree* t) {
 // by blocked Householder QR of the augmented M x n block [K11 K21] on its first k
 // columns, then back substitution with R11.  The root's T is never used by invert or the
 // sweeps (the root has no E), so it is not formed.  S = double or complex double.
+//
+// The build follows the host's index sets level by level (the leaves first): add(l) uploads
+// level l and queues its whole least-squares sequence on a stream of its own, so the GPU
+// factors the lower levels while the host still builds the upper ones.  finish() uploads the
+// leaf map and hands the streams and workspaces to the tree, whose readers drain them
+// (tree_sync); a build destroyed unfinished drains and frees its own.
+class InterpBuild {
+ public:
+  virtual ~InterpBuild() = default;
+  virtual void add(int l) = 0;
+  virtual void finish() = 0;
+};
+
 template <typename S>
-void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
+class InterpBuildT final : public InterpBuild {
+ public:
+  explicit InterpBuildT(hssc_tree* t) : t_(t), ws_(t->depth + 1) {
+    HIPCHK(hipGetDevice(&t->device));
+    // Streams are mapped onto the process's few hardware queues by the runtime (kernels of
+    // streams sharing a queue run one after the other); twice as many streams as queues keeps
+    // every queue fed whatever the mapping.
+    streams_.assign(2 * hw_queues(), nullptr);
+  }
+  ~InterpBuildT() override {
+    if (handed_) return;
+    for (auto& s : streams_)
+      if (s) (void)hipStreamSynchronize(s);
+    stats::flush();
+    for (auto& s : streams_)
+      if (s) (void)hipStreamDestroy(s);
+    for (auto& w : ws_)
+      for (void* p : w.all()) dfree(p);
+  }
+  void add(int l) override;
+  void finish() override {
+    t_->dLeafG = dupload(t_->leafG);
+    t_->on_device = true;
+    for (auto& s : streams_)
+      if (s) t_->bstreams.push_back(s);
+    for (auto& w : ws_)
+      for (void* p : w.all())
+        if (p) t_->bpending.push_back(p);
+    handed_ = true;
+    t_->has_T = true;
+  }
+
+ private:
+  struct Ws {
+    S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
+      *Ts = nullptr, *GV = nullptr;
+    wide_t<S>* cq = nullptr;  // CholeskyQR workspace (FP64 arithmetic for every storage type)
+    std::vector<void*> all() const { return {A, V, W1, W2, Tf, part, Ts, GV, cq}; }
+  };
+  hssc_tree* t_;
+  std::vector<Ws> ws_;
+  std::vector<hipStream_t> streams_;
+  size_t next_ = 0;
+  bool handed_ = false;
+};
+
+template <typename S>
+void InterpBuildT<S>::add(int l) {
+  hssc_tree* t = t_;
+  Level& L = t->levels[l];
+  upload_level(L);
+  if (l == 0) return;  // the root's T is not formed
   const k::KernelSpec ks = kernel_spec(t->p);
   const int64_t nb = k::kQrNb;
   const int64_t kchunk = 512;
   // K chunk of the thin products (at most 128 x 128 outputs): more, shorter chunks
   const int64_t kchunk_thin = 128;
-  // Levels are independent.  Each level's QR sequence is cut into steps (block generation,
-  // one step per 32-column panel, one per back-substitution block); the levels are spread
-  // over kLevelStreams streams by longest-processing-time on an estimated cost and their
-  // steps are submitted round-robin, so no level waits behind another's whole sequence when
-  // streams share a hardware queue.
-  // Streams are mapped onto the process's few hardware queues by the runtime (kernels of
-  // streams sharing a queue run one after the other); twice as many streams as queues keeps
-  // every queue fed whatever the mapping.
-  const int kLevelStreams = 2 * hw_queues();
   // outer panel width (a multiple of the 32-column sub-panel; HSSC_LS_OUTER_NB overrides it)
   static const int64_t kOuterNb = [] {
     const char* v = std::getenv("HSSC_LS_OUTER_NB");
@@ -772,191 +824,138 @@ void compute_interpolation_impl(hssc_tree* t, hipStream_t st0) {
     return v ? std::atof(v) : 1e-5;
   }();
   constexpr int64_t kMergeCols = 128;  // trailing blocks up to this width take rank-32 updates
-  struct Ws {
-    S *A = nullptr, *V = nullptr, *W1 = nullptr, *W2 = nullptr, *Tf = nullptr, *part = nullptr,
-      *Ts = nullptr, *GV = nullptr;
-    wide_t<S>* cq = nullptr;  // CholeskyQR workspace (FP64 arithmetic for every storage type)
-  };
-  std::vector<Ws> ws(t->depth + 1);
-  std::vector<hipStream_t> streams(kLevelStreams, nullptr);
-  tree_sync(t);
-  HIPCHK(hipStreamSynchronize(st0));  // the tree upload is complete before the level streams start
-  for (auto& s : streams) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  auto cleanup = [&]() {
-    for (auto& s : streams)
-      if (s) (void)hipStreamSynchronize(s);
-    stats::flush();
-    for (auto& s : streams)
-      if (s) (void)hipStreamDestroy(s);
-    for (auto& w : ws)
-      dfree(w.A), dfree(w.V), dfree(w.W1), dfree(w.W2), dfree(w.Tf), dfree(w.part), dfree(w.cq),
-          dfree(w.Ts), dfree(w.GV);
-  };
-  try {
-    std::vector<std::vector<std::function<void(hipStream_t)>>> steps(t->depth + 1);
-    std::vector<double> cost(t->depth + 1, 0.0);
-    for (int l = t->depth; l >= 1; --l) {
-      Level& L = t->levels[l];
-      // FP32 trees solve the Tikhonov-regularised least squares
-      //   min ||K(Xp, X_sk) T - K(Xp, X_rs)||^2 + lambda^2 ||T||^2
-      // as the plain one of the block with k more rows [lambda I 0] below it: directions in
-      // which K(Xp, X_sk) is smaller than FP32 rounding can resolve (its singular values reach
-      // acc ||K||) are damped instead of amplifying rounding noise into T.
-      const bool tikhonov = sizeof(real_t<S>) == 4;
-      const int64_t Mq = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
-      const int64_t M = tikhonov ? Mq + kk : Mq;  // rows of the least squares
-      S* dT = dalloc<S>(E * kk * std::max<int64_t>(r, 1));
-      // an FP32 tree keeps only the FP32 operator (the copy the FP32 factorization reads)
-      if constexpr (sizeof(real_t<S>) == 4) L.dT32 = dT;
-      else L.dT = dT;
-      L.q = r;
-      if (r == 0) continue;
-      if (Mq < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
-      Ws& w = ws[l];
-      const int64_t sA = M * n;
-      // Outer panels of up to 128 columns made of 32-column CholeskyQR3 sub-panels, so the
-      // trailing matrix (the HBM-bound part) is streamed once per outer panel.
-      const int64_t nbo = std::min<int64_t>(kOuterNb, (kk + nb - 1) / nb * nb);
-      w.A = dalloc<S>(E * sA);
-      w.V = dalloc<S>(E * M * nbo);
-      w.W1 = dalloc<S>(E * nbo * n);
-      w.W2 = dalloc<S>(E * nbo * n);
-      w.Tf = dalloc<S>(E * nbo * nbo);
-      const int64_t part_cap = E * k::ls_part_elems(M, n, nbo, kchunk, kchunk_thin);
-      w.part = dalloc<S>(part_cap);
-      w.cq = dalloc<wide_t<S>>(E * k::cqr_ws_elems(M));
-      w.Ts = dalloc<S>(E * kOuterNb * nb);
-      w.GV = dalloc<S>(E * nbo * nbo);
-      S* A = w.A;
-      auto& sl = steps[l];
-      // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
-      sl.push_back([=, &L](hipStream_t st) {
-        k::gen_block<S>(st, Mq, n, E, L.dXpx, L.dXpy, L.dXpw, Mq, L.dPx, L.dPy, L.dPw, n, A, M, sA,
-                        ks);
-        if (tikhonov) {
-          k::zero2d<S>(st, kk, n, A + Mq, M, sA, E);
-          k::tikhonov_rows<S>(st, Mq, kk, A, M, sA, E, kTikhonovEta);
-        }
-      });
-      for (int64_t j0 = 0; j0 < kk; j0 += nbo) {
-        sl.push_back([=](hipStream_t st) {
-          const int jbo = int(std::min(nbo, kk - j0));
-          const int64_t nc = n - j0 - jbo;
-          const int64_t Mp = M - j0;  // leading dimension of V
-          const int64_t sV = Mp * nbo;
-          // Rows that can be nonzero below row j of the columns >= j once the columns before
-          // jend are factored: every data row, and of the Tikhonov rows only those of the
-          // factored columns (the reflector of column j reaches the Tikhonov rows of the
-          // columns <= j only, so Tikhonov row i stays lambda e_i until column i is reached).
-          // The panels and their products skip the untouched rows.
-          auto rows_to = [=](int64_t jend) { return tikhonov ? Mq + std::min(jend, kk) : M; };
-          const int64_t Mpe = rows_to(j0 + jbo) - j0;
-          // a narrow trailing block (the last panels of the small levels) takes each
-          // sub-panel's reflectors directly, without the outer panel's block reflector
-          const bool merge = nc <= kMergeCols;
-          k::zero2d<S>(st, Mpe, jbo, w.V, Mp, sV, E);
-          for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
-            const int64_t j1 = j0 + s0;
-            const int jb = int(std::min<int64_t>(nb, jbo - s0));
-            const int64_t Mps = rows_to(j1 + jb) - j1;
-            S* Vs = w.V + s0 + s0 * Mp;
-            S* Tss = w.Ts + (s0 / nb) * nb * nb;
-            k::cqr_panel<S>(st, j1 + Mps, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq,
-                            E);
-            // this sub-panel's reflectors on the rest of the outer panel (and the trailing
-            // block when merged)
-            const int64_t nin = jbo - s0 - jb + (merge ? std::max<int64_t>(nc, 0) : 0);
-            if (nin <= 0) continue;
-            S* Cin = A + j1 + (j1 + jb) * M;
-            k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
-                                 w.part, kchunk_thin, part_cap);
-            k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
-                         w.W2, jb, jb * nin, E);
-            k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
-                       S(1), Cin, M, sA, E);
-          }
-          if (nc <= 0 || merge) return;
-          // T of the outer panel from the sub-panel factors and V^H V
-          k::gemm_tn_splitk<S>(st, jbo, jbo, Mpe, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
-                               E, w.part, kchunk_thin, part_cap);
-          k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
-                           nbo * nbo, w.W2, nbo * n, E);
-          const int64_t ldt = jbo;
-          S* C = A + j0 + (j0 + jbo) * M;
-          // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
-          k::gemm_tn_splitk<S>(st, jbo, nc, Mpe, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
-                               w.part, kchunk, part_cap);
-          k::gemm_h<S>(st, jbo, nc, jbo, S(1), w.Tf, ldt, nbo * nbo, w.W1, jbo, jbo * nc, S(0),
-                       w.W2, jbo, jbo * nc, E);
-          k::gemm<S>(st, false, false, Mpe, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
-                     S(1), C, M, sA, E);
-        });
+  // FP32 trees solve the Tikhonov-regularised least squares
+  //   min ||K(Xp, X_sk) T - K(Xp, X_rs)||^2 + lambda^2 ||T||^2
+  // as the plain one of the block with k more rows [lambda I 0] below it: directions in
+  // which K(Xp, X_sk) is smaller than FP32 rounding can resolve (its singular values reach
+  // acc ||K||) are damped instead of amplifying rounding noise into T.
+  const bool tikhonov = sizeof(real_t<S>) == 4;
+  const int64_t Mq = L.M, n = L.n, kk = L.k, r = n - kk, E = L.nkey;
+  const int64_t M = tikhonov ? Mq + kk : Mq;  // rows of the least squares
+  S* dT = dalloc<S>(E * kk * std::max<int64_t>(r, 1));
+  // an FP32 tree keeps only the FP32 operator (the copy the FP32 factorization reads)
+  if constexpr (sizeof(real_t<S>) == 4) L.dT32 = dT;
+  else L.dT = dT;
+  L.q = r;
+  if (r == 0) return;
+  if (Mq < kk) throw HipError{fail(HSSC_ERR_UNSUPPORTED, "proxy smaller than skeleton (M < k)")};
+  Ws& w = ws_[l];
+  const int64_t sA = M * n;
+  // Outer panels of up to 128 columns made of 32-column CholeskyQR3 sub-panels, so the
+  // trailing matrix (the HBM-bound part) is streamed once per outer panel.
+  const int64_t nbo = std::min<int64_t>(kOuterNb, (kk + nb - 1) / nb * nb);
+  w.A = dalloc<S>(E * sA);
+  w.V = dalloc<S>(E * M * nbo);
+  w.W1 = dalloc<S>(E * nbo * n);
+  w.W2 = dalloc<S>(E * nbo * n);
+  w.Tf = dalloc<S>(E * nbo * nbo);
+  const int64_t part_cap = E * k::ls_part_elems(M, n, nbo, kchunk, kchunk_thin);
+  w.part = dalloc<S>(part_cap);
+  w.cq = dalloc<wide_t<S>>(E * k::cqr_ws_elems(M));
+  w.Ts = dalloc<S>(E * kOuterNb * nb);
+  w.GV = dalloc<S>(E * nbo * nbo);
+  S* A = w.A;
+  std::vector<std::function<void(hipStream_t)>> sl;
+  // K(Xp, P) with P in J order: columns [0,k) = skeleton, [k,n) = residual
+  sl.push_back([=, &L](hipStream_t st) {
+    k::gen_block<S>(st, Mq, n, E, L.dXpx, L.dXpy, L.dXpw, Mq, L.dPx, L.dPy, L.dPw, n, A, M, sA,
+                    ks);
+    if (tikhonov) {
+      k::zero2d<S>(st, kk, n, A + Mq, M, sA, E);
+      k::tikhonov_rows<S>(st, Mq, kk, A, M, sA, E, kTikhonovEta);
+    }
+  });
+  for (int64_t j0 = 0; j0 < kk; j0 += nbo) {
+    sl.push_back([=](hipStream_t st) {
+      const int jbo = int(std::min(nbo, kk - j0));
+      const int64_t nc = n - j0 - jbo;
+      const int64_t Mp = M - j0;  // leading dimension of V
+      const int64_t sV = Mp * nbo;
+      // Rows that can be nonzero below row j of the columns >= j once the columns before
+      // jend are factored: every data row, and of the Tikhonov rows only those of the
+      // factored columns (the reflector of column j reaches the Tikhonov rows of the
+      // columns <= j only, so Tikhonov row i stays lambda e_i until column i is reached).
+      // The panels and their products skip the untouched rows.
+      auto rows_to = [=](int64_t jend) { return tikhonov ? Mq + std::min(jend, kk) : M; };
+      const int64_t Mpe = rows_to(j0 + jbo) - j0;
+      // a narrow trailing block (the last panels of the small levels) takes each
+      // sub-panel's reflectors directly, without the outer panel's block reflector
+      const bool merge = nc <= kMergeCols;
+      k::zero2d<S>(st, Mpe, jbo, w.V, Mp, sV, E);
+      for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
+        const int64_t j1 = j0 + s0;
+        const int jb = int(std::min<int64_t>(nb, jbo - s0));
+        const int64_t Mps = rows_to(j1 + jb) - j1;
+        S* Vs = w.V + s0 + s0 * Mp;
+        S* Tss = w.Ts + (s0 / nb) * nb * nb;
+        k::cqr_panel<S>(st, j1 + Mps, j1, jb, A, M, sA, Vs, Mp, sV, Tss, kOuterNb * nb, w.cq,
+                        E);
+        // this sub-panel's reflectors on the rest of the outer panel (and the trailing
+        // block when merged)
+        const int64_t nin = jbo - s0 - jb + (merge ? std::max<int64_t>(nc, 0) : 0);
+        if (nin <= 0) continue;
+        S* Cin = A + j1 + (j1 + jb) * M;
+        k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
+                             w.part, kchunk_thin, part_cap);
+        k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
+                     w.W2, jb, jb * nin, E);
+        k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
+                   S(1), Cin, M, sA, E);
       }
-      // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
-      S* C1 = A + kk * M;
-      const int64_t last = ((kk - 1) / nb) * nb;
-      for (int64_t p = last; p >= 0; p -= nb) {
-        sl.push_back([=](hipStream_t st) {
-          const int bsz = int(std::min(nb, kk - p));
-          k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
-          // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
-          k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
-                     C1 + p, M, sA, E);
-          if (p > 0)
-            k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1),
-                       C1, M, sA, E);
-        });
-      }
-      sl.push_back([=](hipStream_t st) { k::copy2d<S, S>(st, kk, r, C1, M, sA, dT, kk, kk * r, E); });
-      // estimated cost: least-squares flops plus a latency term per panel row
-      const double npan = double((kk + nb - 1) / nb);
-      cost[l] = double(E) * (2.0 * M * kk * kk + 4.0 * M * kk * r) + npan * double(M) * 2e6;
-    }
-    // longest processing time first onto the least loaded stream
-    std::vector<int> order;
-    for (int l = 1; l <= t->depth; ++l)
-      if (!steps[l].empty()) order.push_back(l);
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<double> load(kLevelStreams, 0.0);
-    std::vector<int> lst(t->depth + 1, 0);
-    for (int l : order) {
-      const int sidx = int(std::min_element(load.begin(), load.end()) - load.begin());
-      lst[l] = sidx;
-      load[sidx] += cost[l];
-    }
-    size_t maxsteps = 0;
-    for (int l : order) maxsteps = std::max(maxsteps, steps[l].size());
-    for (int l : order) {
-      Level& L = t->levels[l];
-      if (!L.ready) HIPCHK(hipEventCreateWithFlags(&L.ready, hipEventDisableTiming));
-    }
-    for (size_t i = 0; i < maxsteps; ++i)
-      for (int l : order)
-        if (i < steps[l].size()) {
-          steps[l][i](streams[lst[l]]);
-          if (i + 1 == steps[l].size())
-            HIPCHK(hipEventRecord(t->levels[l].ready, streams[lst[l]]));
-        }
-    HIPCHK(hipGetLastError());
-  } catch (...) {
-    cleanup();
-    throw;
+      if (nc <= 0 || merge) return;
+      // T of the outer panel from the sub-panel factors and V^H V
+      k::gemm_tn_splitk<S>(st, jbo, jbo, Mpe, w.V, Mp, sV, w.V, Mp, sV, w.GV, jbo, jbo * jbo,
+                           E, w.part, kchunk_thin, part_cap);
+      k::t_assemble<S>(st, jbo, w.Ts, kOuterNb * nb, w.GV, jbo, jbo * jbo, w.Tf, jbo,
+                       nbo * nbo, w.W2, nbo * n, E);
+      const int64_t ldt = jbo;
+      S* C = A + j0 + (j0 + jbo) * M;
+      // C <- Q_panel^H C:  W1 = V^H C ; W2 = Tf^H W1 ; C -= V W2
+      k::gemm_tn_splitk<S>(st, jbo, nc, Mpe, w.V, Mp, sV, C, M, sA, w.W1, jbo, jbo * nc, E,
+                           w.part, kchunk, part_cap);
+      k::gemm_h<S>(st, jbo, nc, jbo, S(1), w.Tf, ldt, nbo * nbo, w.W1, jbo, jbo * nc, S(0),
+                   w.W2, jbo, jbo * nc, E);
+      k::gemm<S>(st, false, false, Mpe, nc, jbo, S(-1), w.V, Mp, sV, w.W2, jbo, jbo * nc,
+                 S(1), C, M, sA, E);
+    });
   }
-  // the work runs on: the tree keeps the streams and workspaces until tree_sync
-  for (auto& s : streams) t->bstreams.push_back(s);
-  for (auto& w : ws)
-    for (void* p : {(void*)w.A, (void*)w.V, (void*)w.W1, (void*)w.W2, (void*)w.Tf,
-                    (void*)w.part, (void*)w.cq, (void*)w.Ts, (void*)w.GV})
-      if (p) t->bpending.push_back(p);
-  t->has_T = true;
+  // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
+  S* C1 = A + kk * M;
+  const int64_t last = ((kk - 1) / nb) * nb;
+  for (int64_t p = last; p >= 0; p -= nb) {
+    sl.push_back([=](hipStream_t st) {
+      const int bsz = int(std::min(nb, kk - p));
+      k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
+      // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
+      k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
+                 C1 + p, M, sA, E);
+      if (p > 0)
+        k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1),
+                   C1, M, sA, E);
+    });
+  }
+  sl.push_back([=](hipStream_t st) { k::copy2d<S, S>(st, kk, r, C1, M, sA, dT, kk, kk * r, E); });
+  // the level's uploads (pageable copies on the null stream) have landed before its stream
+  // reads them
+  HIPCHK(hipStreamSynchronize(nullptr));
+  hipStream_t& st = streams_[next_++ % streams_.size()];
+  if (!st) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  if (!L.ready) HIPCHK(hipEventCreateWithFlags(&L.ready, hipEventDisableTiming));
+  for (auto& f : sl) f(st);
+  HIPCHK(hipEventRecord(L.ready, st));
+  HIPCHK(hipGetLastError());
 }
 
 // The least squares in the tree's precision: FP64, or FP32 / complex64 on MFMA (the skeleton
 // IDs stay FP64 either way).
-void compute_interpolation(hssc_tree* t, hipStream_t st0) {
+std::unique_ptr<InterpBuild> interp_build(hssc_tree* t) {
   const bool lo = t->p.tree_precision == HSSC_PREC_FP32;
-  if (t->cx) lo ? compute_interpolation_impl<zfloat>(t, st0) : compute_interpolation_impl<zdouble>(t, st0);
-  else lo ? compute_interpolation_impl<float>(t, st0) : compute_interpolation_impl<double>(t, st0);
+  if (t->cx) {
+    if (lo) return std::unique_ptr<InterpBuild>(new InterpBuildT<zfloat>(t));
+    return std::unique_ptr<InterpBuild>(new InterpBuildT<zdouble>(t));
+  }
+  if (lo) return std::unique_ptr<InterpBuild>(new InterpBuildT<float>(t));
+  return std::unique_ptr<InterpBuild>(new InterpBuildT<double>(t));
 }
 
 // FP64 interpolation operators of an FP32 tree (widened once, for the FP64 readers: the FP64
@@ -1660,7 +1659,9 @@ int hssc_kernel_eval(const hssc_params* p, const double* X1, int64_t m, const do
   });
 }
 
-static int build_index_sets_policy(const hssc_params* p, hssc_tree** tree, IdPolicy pol) {
+static int build_index_sets_policy(
+    const hssc_params* p, hssc_tree** tree, IdPolicy pol,
+    const std::function<int(hssc_tree*, int)>& level_done = nullptr) {
   if (!p || !tree) return fail(HSSC_ERR_INVALID_ARG, "null argument");
   *tree = nullptr;
   if (int s = check_params(*p)) return s;
@@ -1669,8 +1670,14 @@ static int build_index_sets_policy(const hssc_params* p, hssc_tree** tree, IdPol
   return guarded([&]() {
     auto* t = new hssc_tree();
     pol.tree = t;
-    const int s = build_index_sets(*p, t, [&](const hssc_params& pp, const IdJob& jb, int32_t* o,
-                                              int32_t* r) { return id_select_gpu(pp, jb, o, r, pol); });
+    LevelDone done = nullptr;
+    if (level_done) done = [&](int lev) { return level_done(t, lev); };
+    const int s = build_index_sets(
+        *p, t,
+        [&](const hssc_params& pp, const IdJob& jb, int32_t* o, int32_t* r) {
+          return id_select_gpu(pp, jb, o, r, pol);
+        },
+        done);
     if (s != HSSC_OK) {
       delete t;
       return s;
@@ -1685,18 +1692,35 @@ static int build_tree_policy(const hssc_params* p, hssc_tree** tree, const IdPol
   *tree = nullptr;
   if (int s = check_device()) return s;
   pool::reclaim();  // the device is expected idle here: released blocks become reusable
-  int s = build_index_sets_policy(p, tree, pol);
-  if (s != HSSC_OK) return s;
-  hssc_tree* t = *tree;
-  s = guarded([&]() {
-    upload_tree(t);
-    compute_interpolation(t, 0);
-    return HSSC_OK;
-  });
-  if (s != HSSC_OK) {
-    delete t;
-    *tree = nullptr;
+  // Each level's interpolation operators are queued as soon as its index sets are final, so
+  // the GPU works on the levels below while the host builds the ones above.
+  std::unique_ptr<InterpBuild> ib;
+  int s;
+  {
+    pool::NoSync ns;  // allocations meanwhile do not wait for the queued levels
+    s = build_index_sets_policy(p, tree, pol, [&](hssc_tree* t, int lev) {
+      return guarded([&]() {
+        if (!ib) ib = interp_build(t);
+        ib->add(lev);
+        return HSSC_OK;
+      });
+    });
+    if (s == HSSC_OK) {
+      hssc_tree* t = *tree;
+      s = guarded([&]() {
+        if (!ib) ib = interp_build(t);
+        ib->finish();
+        return HSSC_OK;
+      });
+      if (s != HSSC_OK) {
+        ib.reset();
+        delete t;
+        *tree = nullptr;
+      }
+    }
   }
+  // a failed build drains its queued levels before their workspaces return to the pool
+  ib.reset();
   return s;
 }
 

@@ -414,7 +414,8 @@ class WorkPool {
 
 }  // namespace
 
-int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
+int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel,
+                     const LevelDone& done) {
   if (p.Ng < 1 || p.np < 1 || (p.np & (p.np - 1)) != 0)
     return fail(HSSC_ERR_INVALID_ARG, "np must be a power of two and Ng >= 1");
   if (p.np < 2) return fail(HSSC_ERR_INVALID_ARG, "np must be at least 2");
@@ -732,6 +733,8 @@ int build_index_sets(const hssc_params& p, hssc_tree* t, const IdSelect& sel) {
       if (kb < 0) kb = int32_t(L.k);
     if (ordered && !TI) ragged_below = true;
     L.q = L.n - L.k;
+    if (done)
+      if (int s = done(lev)) return s;
   }
   // Leaf gather map: global index of each J-ordered leaf point.
   const Level& LL = t->levels[depth];
