@@ -119,10 +119,11 @@ def one_step(H, P, torch, b, tol, split=False, rank="id", mlp=None):
                       FP32 / complex64 factorization;
       plain ("fp64"): everything in FP64 / complex128.
     `rank` / `mlp` select the skeleton IDs of the ID-selected levels (HSS2D_build_tree_fsym).
-    Returns timings (s) and residual info per path.  Without `split` the tree and the inversion
-    are called back to back with no host synchronisation, as a caller would: the inversion
-    waits on the device, level by level, for the interpolation operators it needs, so the two
-    overlap.  With `split` every phase is synchronised on its own to report the breakdown."""
+    Returns timings (s) and residual info per path.  Without `split` the tree, the grid
+    operator and the inversion are called back to back with no host synchronisation, as a
+    caller would: the tree build returns with its upper levels still queued, the grid operator
+    is set up meanwhile, and the inversion drains the build before it starts.  With `split`
+    every phase is synchronised on its own to report the breakdown."""
     sync = torch.cuda.synchronize
     res = {}
     for prec in ("fp32", "fp64"):

@@ -21,6 +21,7 @@
 #include <mutex>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gridop.h"
@@ -742,10 +743,13 @@ void upload_level(Level& L) {
 // sweeps (the root has no E), so it is not formed.  S = double or complex double.
 //
 // The build follows the host's index sets level by level (the leaves first): add(l) uploads
-// level l and queues its whole least-squares sequence on a stream of its own, so the GPU
-// factors the lower levels while the host still builds the upper ones.  finish() uploads the
-// leaf map and hands the streams and workspaces to the tree, whose readers drain them
-// (tree_sync); a build destroyed unfinished drains and frees its own.
+// level l, allocates its workspace and hands its whole least-squares sequence (hundreds of
+// launches on the upper levels) to a launcher thread of its own, which queues it on a stream
+// of its own.  The GPU factors the lower levels while the host still builds the upper ones,
+// and the launches of the levels do not queue behind each other on the host: the top level,
+// the longest chain, starts as soon as its index sets exist.  finish() joins the launchers,
+// uploads the leaf map and hands the streams and workspaces to the tree, whose readers drain
+// them (tree_sync); a build destroyed unfinished joins, drains and frees its own.
 class InterpBuild {
  public:
   virtual ~InterpBuild() = default;
@@ -764,6 +768,7 @@ class InterpBuildT final : public InterpBuild {
     streams_.assign(2 * hw_queues(), nullptr);
   }
   ~InterpBuildT() override {
+    join();
     if (handed_) return;
     for (auto& s : streams_)
       if (s) (void)hipStreamSynchronize(s);
@@ -775,6 +780,8 @@ class InterpBuildT final : public InterpBuild {
   }
   void add(int l) override;
   void finish() override {
+    join();
+    if (err_) throw HipError{fail(err_, err_msg_)};
     t_->dLeafG = dupload(t_->leafG);
     t_->on_device = true;
     for (auto& s : streams_)
@@ -793,9 +800,18 @@ class InterpBuildT final : public InterpBuild {
     wide_t<S>* cq = nullptr;  // CholeskyQR workspace (FP64 arithmetic for every storage type)
     std::vector<void*> all() const { return {A, V, W1, W2, Tf, part, Ts, GV, cq}; }
   };
+  void join() {
+    for (auto& th : launchers_)
+      if (th.joinable()) th.join();
+    launchers_.clear();
+  }
   hssc_tree* t_;
   std::vector<Ws> ws_;
   std::vector<hipStream_t> streams_;
+  std::vector<std::thread> launchers_;
+  std::mutex err_mu_;
+  int err_ = HSSC_OK;  // first launcher failure (status and message), guarded by err_mu_
+  std::string err_msg_;
   size_t next_ = 0;
   bool handed_ = false;
 };
@@ -938,12 +954,26 @@ void InterpBuildT<S>::add(int l) {
   // the level's uploads (pageable copies on the null stream) have landed before its stream
   // reads them
   HIPCHK(hipStreamSynchronize(nullptr));
-  hipStream_t& st = streams_[next_++ % streams_.size()];
-  if (!st) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipStream_t& sref = streams_[next_++ % streams_.size()];
+  if (!sref) HIPCHK(hipStreamCreateWithFlags(&sref, hipStreamNonBlocking));
   if (!L.ready) HIPCHK(hipEventCreateWithFlags(&L.ready, hipEventDisableTiming));
-  for (auto& f : sl) f(st);
-  HIPCHK(hipEventRecord(L.ready, st));
-  HIPCHK(hipGetLastError());
+  const hipStream_t st = sref;
+  const hipEvent_t ev = L.ready;
+  const int dev = t->device;
+  launchers_.emplace_back([this, st, ev, dev, steps = std::move(sl)]() {
+    try {
+      HIPCHK(hipSetDevice(dev));  // the current device is per host thread
+      for (auto& f : steps) f(st);
+      HIPCHK(hipEventRecord(ev, st));
+      HIPCHK(hipGetLastError());
+    } catch (const HipError& e) {
+      std::lock_guard<std::mutex> g(err_mu_);
+      if (!err_) err_ = e.status, err_msg_ = g_last_error;
+    } catch (...) {
+      std::lock_guard<std::mutex> g(err_mu_);
+      if (!err_) err_ = HSSC_ERR_HIP, err_msg_ = "least-squares launcher failed";
+    }
+  });
 }
 
 // The least squares in the tree's precision: FP64, or FP32 / complex64 on MFMA (the skeleton
