@@ -92,6 +92,14 @@ template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                     int64_t batch, T* part, int64_t kchunk, int64_t part_cap);
+// W2 = Ts^H (A^H B) for a sub-panel update (m <= 32 rows): when the product is split along K,
+// the reduction of the chunks and the product with the triangular factor Ts (m x m) run as one
+// launch; otherwise W1 = A^H B and W2 = Ts^H W1 as two products.  W1, W2: m x n, ld m.
+template <typename T>
+void gemm_tn_splitk_th(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A,
+                       int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, const T* Ts,
+                       int64_t ldt, int64_t sT, T* W1, T* W2, int64_t sW, int64_t batch, T* part,
+                       int64_t kchunk, int64_t part_cap);
 // Elements of the split-K buffer of one level's least squares (per matrix): the outer-panel
 // products (jbo x nc, chunks of kchunk rows), the sub-panel products (jb x nin with
 // jb <= kQrNb and nin < n, chunks of kchunk_thin rows) and the panel Gram matrices

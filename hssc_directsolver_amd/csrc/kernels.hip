@@ -1968,6 +1968,76 @@ __global__ void __launch_bounds__(256) sum_split_kernel(int64_t len, int64_t nsp
   out[e * sout + i] = s;
 }
 
+// W2[e] = Ts[e]^H W1[e], W1[e] = sum_{q < nsplit} part[e*nsplit + q] (m x n, m <= QR_NB, ld
+// m): the split-K reduction of a sub-panel's V^H C (the chunks summed in the order of
+// sum_split_kernel) and its product with the sub-panel's triangular factor, in one launch.
+// Sixteen columns per workgroup; Ts and the reduced columns are staged in LDS.
+constexpr int kThCols = 16;
+template <typename T>
+__global__ void __launch_bounds__(256) sum_split_th_kernel(int m, int64_t n, int64_t nsplit,
+                                                           const T* __restrict__ part,
+                                                           const T* __restrict__ Ts,
+                                                           int64_t ldt, int64_t sT,
+                                                           T* __restrict__ W2, int64_t sW2) {
+  __shared__ T w1[QR_NB][kThCols + 1];
+  __shared__ T th[QR_NB][QR_NB + 1];
+  const int64_t e = blockIdx.y;
+  const int64_t c0 = int64_t(blockIdx.x) * kThCols;
+  const int tid = threadIdx.x;
+  const int64_t len = int64_t(m) * n;
+  const T* pe = part + e * nsplit * len;
+  Ts += e * sT;
+  for (int idx = tid; idx < m * m; idx += 256) th[idx % m][idx / m] = Ts[idx % m + (idx / m) * ldt];
+  for (int idx = tid; idx < m * kThCols; idx += 256) {
+    const int i = idx % m, j = idx / m;
+    T s = T(0);
+    if (c0 + j < n) {
+      const T* pp = pe + i + (c0 + j) * m;
+      int64_t q = 0;
+      for (; q + 8 <= nsplit; q += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pp[(q + u) * len];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+      }
+      for (; q < nsplit; ++q) s += pp[q * len];
+    }
+    w1[i][j] = s;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < m * kThCols; idx += 256) {
+    const int i = idx % m, j = idx / m;
+    if (c0 + j >= n) continue;
+    T s = T(0);
+    for (int t = 0; t < m; ++t) s += conj_s(th[t][i]) * w1[t][j];
+    W2[e * sW2 + i + (c0 + j) * m] = s;
+  }
+}
+
+// Whether gemm_tn_splitk cuts the K range (see there).
+inline bool splitk_pays(int64_t m, int64_t n, int64_t kdim, int64_t batch, int64_t kchunk,
+                        int64_t part_cap) {
+  const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
+  // split only while the unsplit grid leaves CUs idle: with many matrices (the small boxes of
+  // the low levels) the partial products and their sum only add workgroups and a launch; and
+  // only when the partial products fit the caller's buffer
+  const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N) * batch;
+  return nsplit > 1 && tiles < 512 && batch * nsplit * m * n <= part_cap;
+}
+
+// The chunks of A^H B (K cut in chunks of kchunk rows) into part, ld m.
+template <typename T>
+void splitk_chunks(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
+                   int64_t sA, const T* B, int64_t ldb, int64_t sB, int64_t batch, T* part,
+                   int64_t kchunk) {
+  const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
+  const int64_t big = INT64_MAX;
+  gemm_skip_split<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), part, m,
+                     m * n, batch, (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0,
+                     nsplit, kchunk, true);
+}
+
 // C (m x n) = A^H B for tall A (K x m) and B (K x n) with a long K: the K range is cut in
 // nsplit chunks computed as independent GEMMs into `part`, then summed in a fixed order.
 // (A^H = A^T for real data.)
@@ -1975,22 +2045,31 @@ template <typename T>
 void gemm_tn_splitk(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A, int64_t lda,
                     int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                     int64_t batch, T* part, int64_t kchunk, int64_t part_cap) {
-  const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
-  // split only while the unsplit grid leaves CUs idle: with many matrices (the small boxes of
-  // the low levels) the partial products and their sum only add workgroups and a launch; and
-  // only when the partial products fit the caller's buffer
-  const int64_t tiles = ((m + GB_M - 1) / GB_M) * ((n + GB_N - 1) / GB_N) * batch;
-  if (nsplit <= 1 || ldc != m || tiles >= 512 || batch * nsplit * m * n > part_cap) {
+  if (ldc != m || !splitk_pays(m, n, kdim, batch, kchunk, part_cap)) {
     gemm_h<T>(st, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), C, ldc, sC, batch);
     return;
   }
-  const int64_t len = m * n;
-  const int64_t big = INT64_MAX;
-  gemm_skip_split<T>(st, true, false, m, n, kdim, T(1), A, lda, sA, B, ldb, sB, T(0), part, m, len,
-                     batch, (m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, big, 0, big, 0, nsplit,
-                     kchunk, true);
+  const int64_t nsplit = (kdim + kchunk - 1) / kchunk, len = m * n;
+  splitk_chunks<T>(st, m, n, kdim, A, lda, sA, B, ldb, sB, batch, part, kchunk);
   hipLaunchKernelGGL(sum_split_kernel<T>, dim3((len + 255) / 256, batch), dim3(256), 0, st, len,
                      nsplit, part, C, sC);
+}
+
+template <typename T>
+void gemm_tn_splitk_th(hipStream_t st, int64_t m, int64_t n, int64_t kdim, const T* A,
+                       int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB, const T* Ts,
+                       int64_t ldt, int64_t sT, T* W1, T* W2, int64_t sW, int64_t batch, T* part,
+                       int64_t kchunk, int64_t part_cap) {
+  if (m > QR_NB || !splitk_pays(m, n, kdim, batch, kchunk, part_cap)) {
+    gemm_tn_splitk<T>(st, m, n, kdim, A, lda, sA, B, ldb, sB, W1, m, sW, batch, part, kchunk,
+                      part_cap);
+    gemm_h<T>(st, m, n, m, T(1), Ts, ldt, sT, W1, m, sW, T(0), W2, m, sW, batch);
+    return;
+  }
+  const int64_t nsplit = (kdim + kchunk - 1) / kchunk;
+  splitk_chunks<T>(st, m, n, kdim, A, lda, sA, B, ldb, sB, batch, part, kchunk);
+  hipLaunchKernelGGL(sum_split_th_kernel<T>, dim3((n + kThCols - 1) / kThCols, batch), dim3(256),
+                     0, st, int(m), n, nsplit, part, Ts, ldt, sT, W2, sW);
 }
 
 int64_t ls_part_elems(int64_t M, int64_t n, int64_t nbo, int64_t kchunk, int64_t kchunk_thin) {
@@ -2085,6 +2164,9 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void gemm_tn_splitk<T>(hipStream_t, int64_t, int64_t, int64_t, const T*, int64_t,       \
                                   int64_t, const T*, int64_t, int64_t, T*, int64_t, int64_t,       \
                                   int64_t, T*, int64_t, int64_t);                                  \
+  template void gemm_tn_splitk_th<T>(hipStream_t, int64_t, int64_t, int64_t, const T*, int64_t,    \
+                                     int64_t, const T*, int64_t, int64_t, const T*, int64_t,       \
+                                     int64_t, T*, T*, int64_t, int64_t, T*, int64_t, int64_t);     \
   template void add2d<T>(hipStream_t, int64_t, int64_t, T, const T*, int64_t, int64_t, T, const T*, \
                          int64_t, int64_t, T*, int64_t, int64_t, int64_t);
 

@@ -911,10 +911,9 @@ void InterpBuildT<S>::add(int l) {
         const int64_t nin = jbo - s0 - jb + (merge ? std::max<int64_t>(nc, 0) : 0);
         if (nin <= 0) continue;
         S* Cin = A + j1 + (j1 + jb) * M;
-        k::gemm_tn_splitk<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, w.W1, jb, jb * nin, E,
-                             w.part, kchunk_thin, part_cap);
-        k::gemm_h<S>(st, jb, nin, jb, S(1), Tss, nb, kOuterNb * nb, w.W1, jb, jb * nin, S(0),
-                     w.W2, jb, jb * nin, E);
+        // W2 = Ts^H V^H C_in
+        k::gemm_tn_splitk_th<S>(st, jb, nin, Mps, Vs, Mp, sV, Cin, M, sA, Tss, nb, kOuterNb * nb,
+                                w.W1, w.W2, jb * nin, E, w.part, kchunk_thin, part_cap);
         k::gemm<S>(st, false, false, Mps, nin, jb, S(-1), Vs, Mp, sV, w.W2, jb, jb * nin,
                    S(1), Cin, M, sA, E);
       }
