@@ -35,11 +35,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The tree build runs its levels on streams of their own; the HIP runtime maps a process's
-# streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and the kernels of streams
-# sharing a queue run one after the other.  Sixteen queues let the levels run side by side
-# (512^2 Helmholtz tree: 0.303 s with 4, 0.289 s with 8, 0.282 s with 16).  Read when the
-# runtime starts, so it is set before torch is imported.
+# The tree build queues its levels on four streams of its own, beside the null stream (the
+# skeleton IDs, uploads) and the caller's; the HIP runtime maps a process's streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default) and the kernels of streams sharing a queue
+# run one after the other, so sixteen queues keep them apart.  Read when the runtime starts,
+# so it is set before torch is imported.
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "c128": 78.6, "c64": 157.3}  # MI355X dense MFMA

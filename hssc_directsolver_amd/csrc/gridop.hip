@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gridop.h"
@@ -282,6 +283,9 @@ int hssc_grid_op_create(const hssc_params* p, hssc_grid_op** out) {
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
     return fail(HSSC_ERR_HIP, "no HIP device is available");
   return guarded_grid([&]() {
+    // typically created while a tree build still runs on its streams: allocations take a
+    // cached block or new memory rather than waiting for the device
+    pool::NoSync ns;
     auto* op = new hssc_grid_op();
     static std::atomic<uint64_t> next_uid{1};
     op->uid = next_uid.fetch_add(1);
@@ -296,9 +300,23 @@ int hssc_grid_op_create(const hssc_params* p, hssc_grid_op** out) {
       // per-point weights (row b, column c) of X_source
       std::vector<double> X(2 * op->N), bw(op->N), cw(op->N);
       source_points(p->Ng, p->np, X.data());
-      for (int64_t g = 0; g < op->N; ++g) {
-        bw[g] = point_weight(*p, X[g], X[op->N + g], true);
-        cw[g] = point_weight(*p, X[g], X[op->N + g], false);
+      {
+        // the weights are independent per point: contiguous chunks on host threads (this
+        // runs while the GPU still works on a tree build queued just before)
+        const int64_t N = op->N;
+        const int nw = int(std::max<int64_t>(
+            1, std::min<int64_t>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())),
+                                 N / 4096)));
+        auto chunk = [&](int w) {
+          for (int64_t g = N * w / nw; g < N * (w + 1) / nw; ++g) {
+            bw[g] = point_weight(*p, X[g], X[N + g], true);
+            cw[g] = point_weight(*p, X[g], X[N + g], false);
+          }
+        };
+        std::vector<std::thread> th;
+        for (int w = 1; w < nw; ++w) th.emplace_back(chunk, w);
+        chunk(0);
+        for (auto& x : th) x.join();
       }
       op->b = galloc<double>(op->N);
       op->c = galloc<double>(op->N);
