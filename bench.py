@@ -51,7 +51,7 @@ PEAK_HBM_GBS = 8000.0
 # for a plain 1 GiB device copy (`bench.py --calibrate-copy`: FETCH_SIZE 512 MiB -> x2,
 # WRITE_SIZE 1 GiB -> x1).  profiles/hss2d_512_helmholtz_traffic.md holds the runs.
 MEASURED_TRAFFIC = {
-    "gemm_c128_nn": 2 * 78164.6 * 1024 + 65175.7 * 1024,
+    "gemm_c128_nn": 2 * 77905.1 * 1024 + 59022.3 * 1024,
 }
 
 
