@@ -152,6 +152,11 @@ void tikhonov_rows(hipStream_t st, int64_t Mq, int64_t k, T* A, int64_t lda, int
 template <typename T>
 void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
                  int64_t sW, int64_t batch);
+// Every 32 x 32 diagonal block of the leading kk x kk upper triangle, in one launch: block q
+// (rows [32 q, min(32 q + 32, kk))) into W + e sW + q 32^2 (ld its size); sW >= ceil(kk / 32) 32^2.
+template <typename T>
+void trinv_upper_all(hipStream_t st, int64_t kk, const T* R, int64_t ldr, int64_t sR, T* W,
+                     int64_t sW, int64_t batch);
 
 // y = alpha op(A) x + gamma z ; work needs batch * ceil(inner/512) * out elements.
 template <typename T>

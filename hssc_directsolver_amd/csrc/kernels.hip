@@ -1064,16 +1064,21 @@ void invert_batched(hipStream_t st, int64_t n, T* A, int64_t lda, int64_t sA, in
 
 constexpr int QR_NB = 32;  // panel width (the trailing update runs as GEMMs)
 
-// Triangular solve helper: in-place inverse of an upper-triangular nb x nb diagonal block
-// (used by the blocked back substitution for T = R11^-1 C).
+// Triangular solve helper: inverses of the upper-triangular diagonal blocks of R (used by the
+// blocked back substitution for T = R11^-1 C).  Workgroup (e, q) inverts the block of rows and
+// columns [p, p + nb), p = p0 + 32 q, nb = min(32, pend - p), of matrix e into
+// W + e sW + q 32^2 (nb x nb, ld nb).
 template <typename T>
-__global__ void __launch_bounds__(256) trinv_upper_kernel(int64_t p, int nb, const T* __restrict__ R,
-                                                          int64_t ldr, int64_t sR,
-                                                          T* __restrict__ W, int64_t sW) {
+__global__ void __launch_bounds__(256) trinv_upper_kernel(int64_t p0, int64_t pend,
+                                                          const T* __restrict__ R, int64_t ldr,
+                                                          int64_t sR, T* __restrict__ W,
+                                                          int64_t sW) {
   __shared__ T S[QR_NB][QR_NB + 1];
   __shared__ T X[QR_NB][QR_NB + 1];
+  const int64_t p = p0 + int64_t(blockIdx.y) * QR_NB;
+  const int nb = int(pend - p < QR_NB ? pend - p : QR_NB);
   R += int64_t(blockIdx.x) * sR;
-  W += int64_t(blockIdx.x) * sW;
+  W += int64_t(blockIdx.x) * sW + int64_t(blockIdx.y) * QR_NB * QR_NB;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int i = idx % nb, j = idx / nb;
@@ -1100,8 +1105,16 @@ __global__ void __launch_bounds__(256) trinv_upper_kernel(int64_t p, int nb, con
 template <typename T>
 void trinv_upper(hipStream_t st, int64_t p, int nb, const T* R, int64_t ldr, int64_t sR, T* W,
                  int64_t sW, int64_t batch) {
-  hipLaunchKernelGGL(trinv_upper_kernel<T>, dim3(batch), dim3(256), 0, st, p, nb, R, ldr, sR, W,
-                     sW);
+  hipLaunchKernelGGL(trinv_upper_kernel<T>, dim3(batch, 1), dim3(256), 0, st, p, p + nb, R, ldr,
+                     sR, W, sW);
+}
+
+template <typename T>
+void trinv_upper_all(hipStream_t st, int64_t kk, const T* R, int64_t ldr, int64_t sR, T* W,
+                     int64_t sW, int64_t batch) {
+  if (kk <= 0) return;
+  hipLaunchKernelGGL(trinv_upper_kernel<T>, dim3(batch, (kk + QR_NB - 1) / QR_NB), dim3(256), 0,
+                     st, int64_t(0), kk, R, ldr, sR, W, sW);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2151,6 +2164,8 @@ void sumsq(hipStream_t st, int64_t len, const T* x, double* part, int nblocks) {
   template void sym_lower_fill<T>(hipStream_t, int64_t, T*, int64_t, int64_t, int64_t);            \
   template void trinv_upper<T>(hipStream_t, int64_t, int, const T*, int64_t, int64_t, T*, int64_t, \
                                int64_t);                                                           \
+  template void trinv_upper_all<T>(hipStream_t, int64_t, const T*, int64_t, int64_t, T*, int64_t,  \
+                                   int64_t);                                                       \
   template void cqr_panel<T>(hipStream_t, int64_t, int64_t, int, T*, int64_t, int64_t, T*, int64_t, \
                              int64_t, T*, int64_t, wide_t<T>*, int64_t);                           \
   template void t_assemble<T>(hipStream_t, int, const T*, int64_t, const T*, int64_t, int64_t, T*,  \

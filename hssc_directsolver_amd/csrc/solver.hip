@@ -929,15 +929,24 @@ void InterpBuildT<S>::add(int l) {
                  S(1), C, M, sA, E);
     });
   }
-  // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.
+  // Back substitution R11 X = C1 (C1 = A[0:k, k:n]) from the bottom block up.  The inverses
+  // of R11's 32 x 32 diagonal blocks are formed first, all in one launch, into W1 (free by
+  // then), so the chain of dependent launches is two products per block; a level too small
+  // for that layout inverts each block in its own step.
   S* C1 = A + kk * M;
   const int64_t last = ((kk - 1) / nb) * nb;
+  const int64_t sInv = ((kk + nb - 1) / nb) * nb * nb;  // per matrix
+  const bool inv_all = sInv <= nbo * n;
+  if (inv_all)
+    sl.push_back([=](hipStream_t st) { k::trinv_upper_all<S>(st, kk, A, M, sA, w.W1, nbo * n, E); });
   for (int64_t p = last; p >= 0; p -= nb) {
     sl.push_back([=](hipStream_t st) {
       const int bsz = int(std::min(nb, kk - p));
-      k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
+      const S* Rinv = inv_all ? w.W1 + (p / nb) * nb * nb : w.Tf;
+      const int64_t sRinv = inv_all ? nbo * n : nb * nb;
+      if (!inv_all) k::trinv_upper<S>(st, p, bsz, A, M, sA, w.Tf, nb * nb, E);
       // X_b = inv(R_bb) C1_b  (in place; one workgroup reads its columns before storing)
-      k::gemm<S>(st, false, false, bsz, r, bsz, S(1), w.Tf, bsz, nb * nb, C1 + p, M, sA, S(0),
+      k::gemm<S>(st, false, false, bsz, r, bsz, S(1), Rinv, bsz, sRinv, C1 + p, M, sA, S(0),
                  C1 + p, M, sA, E);
       if (p > 0)
         k::gemm<S>(st, false, false, p, r, bsz, S(-1), A + p * M, M, sA, C1 + p, M, sA, S(1),
