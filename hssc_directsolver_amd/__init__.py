@@ -13,6 +13,7 @@ from .hss2d import (  # noqa: F401
     applyinv_device_noalloc, dense_inv_device, levHSS, solve_device, solve_exact_device,
 )
 from .interp import ID, LRID_rand  # noqa: F401
+from .export import a_hss, export_mat, info_tree  # noqa: F401
 from .hss1d import (  # noqa: F401
     Curve, HSS1D_apply_nsym, HSS1D_applyinv_nsym, HSS1D_compress_batched, HSS1D_compress_nsym,
     HSS1D_invert_nsym, HSS1D_solve, split, star_contour,
