@@ -9,7 +9,7 @@ from ._lib import HsscError, cached_memory, lib, release_cached_memory  # noqa: 
 from .hss2d import (  # noqa: F401
     HSS2D_apply_fsym, HSS2D_apply_TI, HSS2D_applyinv_fsym, HSS2D_applyinv_TI,
     HSS2D_build_index_sets, HSS2D_build_tree_fsym, HSS2D_compress_fsym, HSS2D_invert_fsym,
-    GridOperator, HSS_tree_parameters, Kernel_Eval, apply_device, applyinv_device,
+    GridOperator, HSS_tree_parameters, Kernel_Eval, apply_device, applyinv_device, bicgstab,
     applyinv_device_noalloc, dense_inv_device, levHSS, solve_device, solve_exact_device,
 )
 from .interp import ID, LRID_rand  # noqa: F401

@@ -475,3 +475,65 @@ def solve_exact_device(A_HSS: Inverse, A: GridOperator, B, Y, tol=1e-10, maxit=1
                                               C.c_void_p(Y.data_ptr()), m, float(tol), int(maxit),
                                               C.byref(it), C.byref(rel), _stream_ptr(stream)))
     return it.value, rel.value
+
+
+def bicgstab(A, b, tol=1e-10, maxit=200, M=None):
+    """MATLAB's bicgstab(A, b, tol, maxit, M) on GPU tensors (the preconditioned iterative
+    solve of HSS2D/Test/Test_HSS2D_InverseCompression.m:201-228: A the compressed NTI operator,
+    M the HSS inverse of the translation-invariant operator).  A and M are callables
+    y = A(x), y = M(x) (M applies the preconditioner's inverse) on tensors of shape (N,).
+    Van der Vorst's iteration from x0 = 0, the preconditioner applied to the search and the
+    intermediate directions; stops when the true relative residual ||b - A x|| / ||b|| is at
+    most tol, after a half step (s) or a full one.  Returns (x, flag, relres, it) like MATLAB:
+    flag 0 = converged, 1 = maxit reached, 4 = breakdown; it counts half steps as 0.5."""
+    import torch
+    nb = torch.linalg.vector_norm(b).item()
+    x = torch.zeros_like(b)
+    if nb == 0:
+        return x, 0, 0.0, 0
+    Mi = M if M is not None else (lambda v: v)
+    r = b.clone()
+    rt = r.clone()
+    rho = alpha = omega = 1.0 + 0j
+    p = torch.zeros_like(b)
+    v = torch.zeros_like(b)
+
+    def true_res(y):
+        return torch.linalg.vector_norm(b - A(y)).item() / nb
+
+    relres = 1.0
+    for i in range(1, maxit + 1):
+        rho1, rho = rho, torch.vdot(rt, r).item()
+        if rho == 0:
+            return x, 4, relres, i - 1
+        if i == 1:
+            p = r.clone()
+        else:
+            p = r + ((rho / rho1) * (alpha / omega)) * (p - omega * v)
+        ph = Mi(p)
+        v = A(ph)
+        den = torch.vdot(rt, v).item()
+        if den == 0:
+            return x, 4, relres, i - 1
+        alpha = rho / den
+        xh = x + alpha * ph
+        s = r - alpha * v
+        if torch.linalg.vector_norm(s).item() / nb <= tol:
+            relres = true_res(xh)
+            if relres <= tol:
+                return xh, 0, relres, i - 0.5
+        sh = Mi(s)
+        t = A(sh)
+        tt = torch.vdot(t, t).real.item()
+        if tt == 0:
+            return xh, 4, relres, i - 0.5
+        omega = torch.vdot(t, s).item() / tt
+        x = xh + omega * sh
+        r = s - omega * t
+        if torch.linalg.vector_norm(r).item() / nb <= tol:
+            relres = true_res(x)
+            if relres <= tol:
+                return x, 0, relres, i
+        if omega == 0:
+            return x, 4, relres, i
+    return x, 1, true_res(x), maxit
