@@ -875,13 +875,17 @@ void InterpBuildT<S>::add(int l) {
       k::zero2d<S>(st, kk, n, A + Mq, M, sA, E);
       k::tikhonov_rows<S>(st, Mq, kk, A, M, sA, E, kTikhonovEta);
     }
+    k::zero2d<S>(st, M, nbo, w.V, M, M * nbo, E);  // V's rows the panels never write
   });
   for (int64_t j0 = 0; j0 < kk; j0 += nbo) {
     sl.push_back([=](hipStream_t st) {
       const int jbo = int(std::min(nbo, kk - j0));
       const int64_t nc = n - j0 - jbo;
-      const int64_t Mp = M - j0;  // leading dimension of V
-      const int64_t sV = Mp * nbo;
+      // V keeps leading dimension M for every outer panel: the rows above each sub-panel's
+      // diagonal block and the regularisation rows below its reach are then never written
+      // and stay zero from the level's first step
+      const int64_t Mp = M;
+      const int64_t sV = M * nbo;
       // Rows that can be nonzero below row j of the columns >= j once the columns before
       // jend are factored: every data row, and of the Tikhonov rows only those of the
       // factored columns (the reflector of column j reaches the Tikhonov rows of the
@@ -892,7 +896,6 @@ void InterpBuildT<S>::add(int l) {
       // a narrow trailing block (the last panels of the small levels) takes each
       // sub-panel's reflectors directly, without the outer panel's block reflector
       const bool merge = nc <= kMergeCols;
-      k::zero2d<S>(st, Mpe, jbo, w.V, Mp, sV, E);
       for (int s0 = 0; s0 < jbo; s0 += int(nb)) {
         const int64_t j1 = j0 + s0;
         const int jb = int(std::min<int64_t>(nb, jbo - s0));
